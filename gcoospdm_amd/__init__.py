@@ -1,0 +1,336 @@
+"""gcoospdm_amd -- MI355X-native GCOOSpDM (grouped-COO sparse x dense multiply).
+
+Python binding of the C ABI in ``include/gcoo_spdm.h`` (libgcoospdm.so,
+HIP kernels for gfx950).  The classes mirror the reference's plugin API
+(reference src/ShGemm.h:7-24, src/SpGemmBuilder.h:12-23,
+src/ShGCOOSpDM.h:8-33) so that code written against the reference reads the
+same:
+
+    algo = SpGemmBuilder.getShGemmAlgo(S, M, K, "gcoospdm")
+    algo.mutiplyBy(D, K, N, C)          # convert S + multiply
+    algo.build(); algo.runKernelTest(D, K, N, C)
+
+Matrices are torch tensors on a HIP device (torch supplies device memory and
+streams only; every computation runs in libgcoospdm.so).  There is no CPU
+fallback: importing this package on a machine without the built library
+raises, and every call fails loudly when the device path is unavailable.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgcoospdm.so")
+
+GCOO_OK = 0
+GCOO_ERR_INVALID = 1
+GCOO_ERR_HIP = 2
+GCOO_ERR_ALLOC = 3
+GCOO_ERR_OVERFLOW = 4
+GCOO_ERR_UNSUPPORTED = 5
+GCOO_ERR_BUDGET = 6
+GCOO_ERR_STATE = 7
+
+# Every symbol include/gcoo_spdm.h declares: (name, restype, argtypes).
+_c = ctypes
+_P = _c.c_void_p
+_I = _c.c_int
+_I64 = _c.c_int64
+_U64 = _c.c_uint64
+_SZ = _c.c_size_t
+_F = _c.c_float
+_PP = _c.POINTER(_c.c_void_p)
+_SIGNATURES = [
+    ("gcoo_last_error", _c.c_char_p, []),
+    ("gcoo_version", _c.c_char_p, []),
+    ("gcoo_device_count", _I, [_c.POINTER(_I)]),
+    ("gcoo_set_device", _I, [_I]),
+    ("gcoo_dev_malloc", _I, [_PP, _SZ]),
+    ("gcoo_dev_free", _I, [_P]),
+    ("gcoo_memcpy_h2d", _I, [_P, _P, _SZ, _P]),
+    ("gcoo_memcpy_d2h", _I, [_P, _P, _SZ, _P]),
+    ("gcoo_memset_async", _I, [_P, _I, _SZ, _P]),
+    ("gcoo_stream_create", _I, [_PP]),
+    ("gcoo_stream_destroy", _I, [_P]),
+    ("gcoo_stream_sync", _I, [_P]),
+    ("gcoo_device_sync", _I, []),
+    ("gcoo_timer_create", _I, [_PP]),
+    ("gcoo_timer_destroy", _I, [_P]),
+    ("gcoo_timer_start", _I, [_P, _P]),
+    ("gcoo_timer_stop", _I, [_P, _P]),
+    ("gcoo_timer_elapsed_ms", _I, [_P, _c.POINTER(_F)]),
+    ("gcoo_fill_uniform_async", _I, [_P, _I64, _U64, _U64, _F, _P]),
+    ("gcoo_fill_sparse_async", _I, [_P, _I64, _U64, _U64, _F, _U64, _P]),
+    ("gcoo_create", _I, [_I, _I, _I, _PP]),
+    ("gcoo_destroy", _I, [_P]),
+    ("gcoo_convert_dense", _I, [_P, _P, _P]),
+    ("gcoo_convert_coo", _I, [_P, _P, _P, _P, _I64, _P]),
+    ("gcoo_get_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I),
+                           _c.POINTER(_I), _c.POINTER(_I64)]),
+    ("gcoo_get_arrays", _I, [_P, _PP, _PP, _PP, _PP, _PP]),
+    ("gcoo_spdm_f32_async", _I, [_P, _P, _I, _I, _P, _P]),
+    ("gcoo_spdm_f32", _I, [_P, _P, _I, _I, _P, _P]),
+    ("gcoo_multiply_by", _I, [_P, _P, _P, _I, _I, _P, _P]),
+]
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+class GCOOError(RuntimeError):
+    def __init__(self, code: int, where: str, msg: str):
+        super().__init__(f"{where} failed ({code}): {msg}")
+        self.code = code
+
+
+def lib() -> ctypes.CDLL:
+    """Load libgcoospdm.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is not built; run `make -C gcoospdm_amd` "
+                "(or __graft_entry__.build()) first")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGNATURES:
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def declared_symbols():
+    return [s[0] for s in _SIGNATURES]
+
+
+def _check(rc: int, where: str):
+    if rc != GCOO_OK:
+        raise GCOOError(rc, where, lib().gcoo_last_error().decode())
+
+
+def _ptr(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def _stream(stream=None) -> int:
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return int(stream.cuda_stream)
+
+
+def _require_dev(t, name, dtype=None):
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a HIP device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous (row-major)")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+
+
+class GCOOMatrix:
+    """A sparse S (M x K) in grouped COO, owned by libgcoospdm.so."""
+
+    def __init__(self, M: int, K: int, cpg: int = 4):
+        h = ctypes.c_void_p()
+        _check(lib().gcoo_create(M, K, cpg, ctypes.byref(h)), "gcoo_create")
+        self._h = h
+        self.M, self.K, self.cpg = M, K, cpg
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().gcoo_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def convert_dense(self, S, stream=None):
+        import torch
+        _require_dev(S, "S", torch.float32)
+        if tuple(S.shape) != (self.M, self.K):
+            raise ValueError(f"S shape {tuple(S.shape)} != {(self.M, self.K)}")
+        _check(lib().gcoo_convert_dense(self._h, _ptr(S), _stream(stream)),
+               "gcoo_convert_dense")
+        return self
+
+    def convert_coo(self, rows, cols, vals, stream=None):
+        import torch
+        _require_dev(rows, "rows", torch.int32)
+        _require_dev(cols, "cols", torch.int32)
+        _require_dev(vals, "vals", torch.float32)
+        n = rows.numel()
+        if cols.numel() != n or vals.numel() != n:
+            raise ValueError("rows, cols, vals must have the same length")
+        _check(lib().gcoo_convert_coo(self._h, _ptr(rows), _ptr(cols), _ptr(vals), n,
+                                      _stream(stream)), "gcoo_convert_coo")
+        return self
+
+    def info(self):
+        M, K, cpg, ng = (ctypes.c_int() for _ in range(4))
+        nnz = ctypes.c_int64()
+        _check(lib().gcoo_get_info(self._h, ctypes.byref(M), ctypes.byref(K),
+                                   ctypes.byref(cpg), ctypes.byref(ng),
+                                   ctypes.byref(nnz)), "gcoo_get_info")
+        return dict(M=M.value, K=K.value, cpg=cpg.value, nGroup=ng.value, nnz=nnz.value)
+
+    @property
+    def nnz(self) -> int:
+        return self.info()["nnz"]
+
+    @property
+    def nGroup(self) -> int:
+        return self.info()["nGroup"]
+
+    def arrays_host(self):
+        """Copy the GCOO arrays to host numpy arrays (for parity checks).
+
+        Returns dict(vals f32[nnz], rows i32[nnz], cols i32[nnz],
+        groupIndex i32[nGroup+1], nnzPerGroup i32[nGroup+1])."""
+        import numpy as np
+        ptrs = [ctypes.c_void_p() for _ in range(5)]
+        _check(lib().gcoo_get_arrays(self._h, *[ctypes.byref(p) for p in ptrs]),
+               "gcoo_get_arrays")
+        inf = self.info()
+        nnz, ng = inf["nnz"], inf["nGroup"]
+        out = {}
+        for name, p, n, dt in zip(
+                ["vals", "rows", "cols", "groupIndex", "nnzPerGroup"], ptrs,
+                [nnz, nnz, nnz, ng + 1, ng + 1],
+                [np.float32, np.int32, np.int32, np.int32, np.int32]):
+            a = np.empty(n, dtype=dt)
+            if n:
+                _check(lib().gcoo_memcpy_d2h(a.ctypes.data, p, a.nbytes, _stream()),
+                       "gcoo_memcpy_d2h")
+            out[name] = a
+        return out
+
+    def spdm(self, D, C, stream=None, sync=True):
+        """C = S * D (FP32, bit-identical to the reference kernel)."""
+        import torch
+        _require_dev(D, "D", torch.float32)
+        _require_dev(C, "C", torch.float32)
+        K, N = D.shape
+        if tuple(C.shape) != (self.M, N):
+            raise ValueError(f"C shape {tuple(C.shape)} != {(self.M, N)}")
+        fn = lib().gcoo_spdm_f32 if sync else lib().gcoo_spdm_f32_async
+        _check(fn(self._h, _ptr(D), K, N, _ptr(C), _stream(stream)), "gcoo_spdm_f32")
+        return C
+
+    def multiply_by(self, S, D, C, stream=None):
+        """Convert S then C = S * D (the reference's mutiplyBy)."""
+        import torch
+        _require_dev(S, "S", torch.float32)
+        _require_dev(D, "D", torch.float32)
+        _require_dev(C, "C", torch.float32)
+        K, N = D.shape
+        _check(lib().gcoo_multiply_by(self._h, _ptr(S), _ptr(D), K, N, _ptr(C),
+                                      _stream(stream)), "gcoo_multiply_by")
+        return C
+
+
+class ShGemm:
+    """Reference src/ShGemm.h:7-24 (abstract)."""
+
+    def __init__(self, nRows: int, nCols: int):
+        self.m_nRows = nRows
+        self.m_nCols = nCols
+        self.m_pDenseMatrix = None
+
+    def mutiplyBy(self, A, nRows, nCols, C):
+        raise NotImplementedError
+
+    def mutiply(self, B, nRows, nCols, C):
+        raise NotImplementedError
+
+    def build(self):
+        raise NotImplementedError
+
+    def runKernelTest(self, A, nRows, nCols, C):
+        raise NotImplementedError
+
+
+class ShGCOOSpDM(ShGemm):
+    """Reference src/ShGCOOSpDM.h:8-33 over the C ABI.
+
+    denseMatrix: device tensor S (nRows x nCols).  mutiplyBy / runKernelTest
+    take the dense operand A (nRows_A x nCols_A with nRows_A == nCols) and
+    write C (nRows x nCols_A)."""
+
+    def __init__(self, denseMatrix, nRows: int, nCols: int, cpg: int = 4):
+        super().__init__(nRows, nCols)
+        self.m_pDenseMatrix = denseMatrix
+        self._m = GCOOMatrix(nRows, nCols, cpg)
+        self.m_nGroup = (nRows + cpg - 1) // cpg
+
+    def _shape_check(self, A, nRows, nCols):
+        if tuple(A.shape) != (nRows, nCols):
+            raise ValueError(f"A shape {tuple(A.shape)} != {(nRows, nCols)}")
+
+    def mutiplyBy(self, A, nRows, nCols, C):
+        self._shape_check(A, nRows, nCols)
+        self._m.multiply_by(self.m_pDenseMatrix, A, C)
+        return 0.0
+
+    def mutiply(self, B, nRows, nCols, C):
+        return self.mutiplyBy(B, nRows, nCols, C)
+
+    def build(self):
+        self._m.convert_dense(self.m_pDenseMatrix)
+
+    def runKernelTest(self, A, nRows, nCols, C):
+        self._shape_check(A, nRows, nCols)
+        self._m.spdm(A, C)
+        return 0.0
+
+    @property
+    def matrix(self) -> GCOOMatrix:
+        return self._m
+
+
+class SpGemmBuilder:
+    """Reference src/SpGemmBuilder.h:12-23."""
+
+    @staticmethod
+    def getShGemmAlgo(denseMatrix, nRows: int, nCols: int, desc: str):
+        if desc == "gcoospdm":
+            return ShGCOOSpDM(denseMatrix, nRows, nCols)
+        return None
+
+
+def keep_threshold(sparsity: float) -> int:
+    """floor(sparsity * 2^40): the Bernoulli cut of gcoo_fill_sparse_async."""
+    import math
+    if sparsity <= 0.0:
+        return 0
+    if sparsity >= 1.0:
+        return 1 << 40
+    return int(math.floor(sparsity * 1099511627776.0))
+
+
+def fill_uniform(t, seed: int, stream_id: int, scale: float = 1.0, stream=None):
+    import torch
+    _require_dev(t, "t", torch.float32)
+    _check(lib().gcoo_fill_uniform_async(_ptr(t), t.numel(), seed, stream_id, scale,
+                                         _stream(stream)), "gcoo_fill_uniform_async")
+    return t
+
+
+def fill_sparse(t, seed: int, stream_id: int, sparsity: float, scale: float = 1.0,
+                stream=None):
+    import torch
+    _require_dev(t, "t", torch.float32)
+    _check(lib().gcoo_fill_sparse_async(_ptr(t), t.numel(), seed, stream_id, scale,
+                                        keep_threshold(sparsity), _stream(stream)),
+           "gcoo_fill_sparse_async")
+    return t

@@ -1,0 +1,382 @@
+// gcoo_api.hip -- the extern "C" boundary of libgcoospdm.so
+// (declared in include/gcoo_spdm.h).
+#include "gcoo_internal.h"
+
+#include <climits>
+#include <cstdarg>
+#include <cstdio>
+#include <new>
+
+namespace gcoo {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+const char *last_error_message() { return g_err; }
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+void clear_error() { g_err[0] = '\0'; }
+
+int ensure_capacity(gcoo_matrix *h, int64_t need)
+{
+    if (need <= h->cap) return GCOO_OK;
+    // Slack of 1/16 so that re-converting a same-sized matrix of similar
+    // density does not reallocate.
+    int64_t cap = need + need / 16 + 1024;
+    if (cap > (int64_t)INT32_MAX) cap = INT32_MAX;
+    if (h->vals) GCOO_HIP_TRY(hipFree(h->vals));
+    if (h->rowcol) GCOO_HIP_TRY(hipFree(h->rowcol));
+    h->vals = nullptr;
+    h->rowcol = nullptr;
+    h->cap = 0;
+    if (hipMalloc(&h->vals, sizeof(float) * (size_t)cap) != hipSuccess ||
+        hipMalloc(&h->rowcol, sizeof(int) * 2 * (size_t)cap) != hipSuccess) {
+        (void)hipGetLastError();
+        if (h->vals) (void)hipFree(h->vals);
+        h->vals = nullptr;
+        return set_error(GCOO_ERR_ALLOC, "cannot allocate GCOO arrays for %lld entries",
+                         (long long)cap);
+    }
+    h->cap = cap;
+    return GCOO_OK;
+}
+
+}  // namespace gcoo
+
+using namespace gcoo;
+
+namespace {
+
+inline hipStream_t as_stream(gcoo_stream s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Reads the conversion status word back (one small D2H copy plus a stream
+// synchronisation: the replacement of the reference's nnz readback,
+// src/ShGCOOSpDM.cu:83-90).
+int read_status(gcoo_handle h, hipStream_t s)
+{
+    GCOO_HIP_TRY(hipMemcpyAsync(h->h_status, h->d_status, sizeof(ConvStatus),
+                                hipMemcpyDeviceToHost, s));
+    GCOO_HIP_TRY(hipStreamSynchronize(s));
+    if (h->h_status->overflow)
+        return set_error(GCOO_ERR_OVERFLOW, "nnz %lld does not fit in int32",
+                         (long long)h->h_status->total);
+    return GCOO_OK;
+}
+
+// Enqueue count + scan + scatter with the current capacity; when the scan's
+// total exceeds it, grow and scatter again.  On return the conversion is
+// complete and h->nnz is set.  When D is given, the multiply is enqueued
+// right behind the scatter (guarded by the status word) so that a conversion
+// that fits the existing capacity costs a single synchronisation.
+int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
+                               int N, float *C, hipStream_t s)
+{
+    h->converted = false;
+    int rc = launch_count_and_scan(h, S, s);
+    if (rc) return rc;
+    if (h->cap > 0) {
+        if ((rc = launch_scatter(h, S, s))) return rc;
+        if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
+    }
+    if ((rc = read_status(h, s))) return rc;
+    const int64_t total = h->h_status->total;
+    if (total > h->cap || h->cap == 0) {
+        if ((rc = ensure_capacity(h, total))) return rc;
+        if ((rc = launch_scatter(h, S, s))) return rc;
+        if (D && (rc = launch_spdm_f32(h, D, N, C, nullptr, s))) return rc;
+        GCOO_HIP_TRY(hipStreamSynchronize(s));
+    }
+    h->nnz = total;
+    h->converted = true;
+    return GCOO_OK;
+}
+
+int check_spdm_args(gcoo_handle h, const float *D, int K, int N, float *C)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (K != h->K)
+        return set_error(GCOO_ERR_INVALID, "dense operand has %d rows, sparse has %d columns",
+                         K, h->K);
+    if (N < 0) return set_error(GCOO_ERR_INVALID, "N = %d", N);
+    if (N > 0 && h->M > 0 && (!D && h->K > 0)) return set_error(GCOO_ERR_INVALID, "null D");
+    if (N > 0 && h->M > 0 && !C) return set_error(GCOO_ERR_INVALID, "null C");
+    if ((long long)h->M * N > (long long)INT32_MAX * 64)
+        return set_error(GCOO_ERR_OVERFLOW, "output too large");
+    return GCOO_OK;
+}
+
+}  // namespace
+
+struct gcoo_timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+extern "C" {
+
+const char *gcoo_last_error(void) { return last_error_message(); }
+
+const char *gcoo_version(void) { return "gcoospdm-mi355x 0.1 (gfx950)"; }
+
+int gcoo_device_count(int *count)
+{
+    if (!count) return set_error(GCOO_ERR_INVALID, "null count");
+    GCOO_HIP_TRY(hipGetDeviceCount(count));
+    return GCOO_OK;
+}
+
+int gcoo_set_device(int device)
+{
+    GCOO_HIP_TRY(hipSetDevice(device));
+    return GCOO_OK;
+}
+
+int gcoo_dev_malloc(void **ptr, size_t bytes)
+{
+    if (!ptr) return set_error(GCOO_ERR_INVALID, "null ptr");
+    *ptr = nullptr;
+    if (bytes == 0) return GCOO_OK;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        return set_error(GCOO_ERR_ALLOC, "hipMalloc of %zu bytes failed", bytes);
+    }
+    return GCOO_OK;
+}
+
+int gcoo_dev_free(void *ptr)
+{
+    if (ptr) GCOO_HIP_TRY(hipFree(ptr));
+    return GCOO_OK;
+}
+
+int gcoo_memcpy_h2d(void *dst, const void *src, size_t bytes, gcoo_stream s)
+{
+    if (bytes == 0) return GCOO_OK;
+    if (!dst || !src) return set_error(GCOO_ERR_INVALID, "null pointer");
+    GCOO_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, as_stream(s)));
+    GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_memcpy_d2h(void *dst, const void *src, size_t bytes, gcoo_stream s)
+{
+    if (bytes == 0) return GCOO_OK;
+    if (!dst || !src) return set_error(GCOO_ERR_INVALID, "null pointer");
+    GCOO_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, as_stream(s)));
+    GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_memset_async(void *dst, int value, size_t bytes, gcoo_stream s)
+{
+    if (bytes == 0) return GCOO_OK;
+    if (!dst) return set_error(GCOO_ERR_INVALID, "null pointer");
+    GCOO_HIP_TRY(hipMemsetAsync(dst, value, bytes, as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_stream_create(gcoo_stream *s)
+{
+    if (!s) return set_error(GCOO_ERR_INVALID, "null stream out");
+    hipStream_t st;
+    GCOO_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *s = st;
+    return GCOO_OK;
+}
+
+int gcoo_stream_destroy(gcoo_stream s)
+{
+    if (s) GCOO_HIP_TRY(hipStreamDestroy(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_stream_sync(gcoo_stream s)
+{
+    GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_device_sync(void)
+{
+    GCOO_HIP_TRY(hipDeviceSynchronize());
+    return GCOO_OK;
+}
+
+int gcoo_timer_create(gcoo_timer_t *t)
+{
+    if (!t) return set_error(GCOO_ERR_INVALID, "null timer out");
+    auto *x = new (std::nothrow) gcoo_timer;
+    if (!x) return set_error(GCOO_ERR_ALLOC, "timer");
+    if (hipEventCreate(&x->a) != hipSuccess || hipEventCreate(&x->b) != hipSuccess) {
+        delete x;
+        return set_error(GCOO_ERR_HIP, "hipEventCreate failed");
+    }
+    *t = x;
+    return GCOO_OK;
+}
+
+int gcoo_timer_destroy(gcoo_timer_t t)
+{
+    if (!t) return GCOO_OK;
+    (void)hipEventDestroy(t->a);
+    (void)hipEventDestroy(t->b);
+    delete t;
+    return GCOO_OK;
+}
+
+int gcoo_timer_start(gcoo_timer_t t, gcoo_stream s)
+{
+    if (!t) return set_error(GCOO_ERR_INVALID, "null timer");
+    GCOO_HIP_TRY(hipEventRecord(t->a, as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_timer_stop(gcoo_timer_t t, gcoo_stream s)
+{
+    if (!t) return set_error(GCOO_ERR_INVALID, "null timer");
+    GCOO_HIP_TRY(hipEventRecord(t->b, as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_timer_elapsed_ms(gcoo_timer_t t, float *ms)
+{
+    if (!t || !ms) return set_error(GCOO_ERR_INVALID, "null argument");
+    GCOO_HIP_TRY(hipEventSynchronize(t->b));
+    GCOO_HIP_TRY(hipEventElapsedTime(ms, t->a, t->b));
+    return GCOO_OK;
+}
+
+int gcoo_fill_uniform_async(float *dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                            float scale, gcoo_stream s)
+{
+    if (n < 0 || (n > 0 && !dst)) return set_error(GCOO_ERR_INVALID, "bad fill args");
+    return launch_fill(dst, n, seed, stream_id, scale, 0, false, as_stream(s));
+}
+
+int gcoo_fill_sparse_async(float *dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                           float scale, uint64_t keep_threshold, gcoo_stream s)
+{
+    if (n < 0 || (n > 0 && !dst)) return set_error(GCOO_ERR_INVALID, "bad fill args");
+    return launch_fill(dst, n, seed, stream_id, scale, keep_threshold, true, as_stream(s));
+}
+
+int gcoo_create(int M, int K, int cpg, gcoo_handle *out)
+{
+    if (!out) return set_error(GCOO_ERR_INVALID, "null handle out");
+    *out = nullptr;
+    if (M < 0 || K < 0) return set_error(GCOO_ERR_INVALID, "negative shape %d x %d", M, K);
+    if (cpg != 1 && cpg != 2 && cpg != 4)
+        return set_error(GCOO_ERR_INVALID, "cpg must be 1, 2 or 4 (got %d)", cpg);
+    auto *h = new (std::nothrow) gcoo_matrix;
+    if (!h) return set_error(GCOO_ERR_ALLOC, "handle");
+    h->M = M;
+    h->K = K;
+    h->cpg = cpg;
+    h->nGroup = (M + cpg - 1) / cpg;
+    (void)hipGetDevice(&h->device);
+    if (hipMalloc(&h->idx, sizeof(int) * 2 * ((size_t)h->nGroup + 1)) != hipSuccess ||
+        hipMalloc(&h->d_status, sizeof(ConvStatus)) != hipSuccess ||
+        hipHostMalloc(&h->h_status, sizeof(ConvStatus), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        gcoo_destroy(h);
+        return set_error(GCOO_ERR_ALLOC, "cannot allocate GCOO index arrays");
+    }
+    h->h_status->total = 0;
+    h->h_status->overflow = 0;
+    *out = h;
+    return GCOO_OK;
+}
+
+int gcoo_destroy(gcoo_handle h)
+{
+    if (!h) return GCOO_OK;
+    if (h->vals) (void)hipFree(h->vals);
+    if (h->rowcol) (void)hipFree(h->rowcol);
+    if (h->idx) (void)hipFree(h->idx);
+    if (h->d_status) (void)hipFree(h->d_status);
+    if (h->h_status) (void)hipHostFree(h->h_status);
+    if (h->coo_tmp) (void)hipFree(h->coo_tmp);
+    delete h;
+    return GCOO_OK;
+}
+
+int gcoo_convert_dense(gcoo_handle h, const float *S, gcoo_stream s)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (!S && (long long)h->M * h->K > 0) return set_error(GCOO_ERR_INVALID, "null S");
+    return convert_and_maybe_multiply(h, S, nullptr, 0, nullptr, as_stream(s));
+}
+
+int gcoo_convert_coo(gcoo_handle h, const int *rows, const int *cols, const float *vals,
+                     int64_t nnz, gcoo_stream s)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (nnz < 0 || (nnz > 0 && (!rows || !cols || !vals)))
+        return set_error(GCOO_ERR_INVALID, "bad COO arguments");
+    if (nnz > INT32_MAX) return set_error(GCOO_ERR_OVERFLOW, "nnz %lld > int32", (long long)nnz);
+    h->converted = false;
+    int rc = convert_coo(h, rows, cols, vals, nnz, as_stream(s));
+    if (rc) return rc;
+    h->converted = true;
+    return GCOO_OK;
+}
+
+int gcoo_get_info(gcoo_handle h, int *M, int *K, int *cpg, int *nGroup, int64_t *nnz)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (M) *M = h->M;
+    if (K) *K = h->K;
+    if (cpg) *cpg = h->cpg;
+    if (nGroup) *nGroup = h->nGroup;
+    if (nnz) *nnz = h->nnz;
+    return GCOO_OK;
+}
+
+int gcoo_get_arrays(gcoo_handle h, const float **vals, const int **rows, const int **cols,
+                    const int **groupIndex, const int **nnzPerGroup)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (!h->converted) return set_error(GCOO_ERR_STATE, "matrix not converted");
+    if (vals) *vals = h->vals;
+    if (rows) *rows = h->rows();
+    if (cols) *cols = h->cols();
+    if (groupIndex) *groupIndex = h->groupIndex();
+    if (nnzPerGroup) *nnzPerGroup = h->nnzPerGroup();
+    return GCOO_OK;
+}
+
+int gcoo_spdm_f32_async(gcoo_handle h, const float *D, int K, int N, float *C, gcoo_stream s)
+{
+    int rc = check_spdm_args(h, D, K, N, C);
+    if (rc) return rc;
+    if (!h->converted) return set_error(GCOO_ERR_STATE, "matrix not converted");
+    return launch_spdm_f32(h, D, N, C, nullptr, as_stream(s));
+}
+
+int gcoo_spdm_f32(gcoo_handle h, const float *D, int K, int N, float *C, gcoo_stream s)
+{
+    int rc = gcoo_spdm_f32_async(h, D, K, N, C, s);
+    if (rc) return rc;
+    GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K, int N, float *C,
+                     gcoo_stream s)
+{
+    int rc = check_spdm_args(h, D, K, N, C);
+    if (rc) return rc;
+    if (!S && (long long)h->M * h->K > 0) return set_error(GCOO_ERR_INVALID, "null S");
+    return convert_and_maybe_multiply(h, S, D, N, C, as_stream(s));
+}
+
+}  // extern "C"

@@ -1,0 +1,133 @@
+// gcoo_internal.h -- shared declarations of the HIP translation units of
+// libgcoospdm.so.  Not part of the public ABI (that is include/gcoo_spdm.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/gcoo_spdm.h"
+
+namespace gcoo {
+
+// Native clang vectors (HIP's float4 wrapper is not accepted by the
+// nontemporal builtins).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Thread-local last-error message and status helpers.
+int set_error(int code, const char *fmt, ...);
+void clear_error();
+
+#define GCOO_HIP_TRY(expr)                                                    \
+    do {                                                                      \
+        hipError_t e_ = (expr);                                               \
+        if (e_ != hipSuccess)                                                 \
+            return ::gcoo::set_error(GCOO_ERR_HIP, "%s failed: %s (%s:%d)",   \
+                                     #expr, hipGetErrorString(e_), __FILE__,  \
+                                     __LINE__);                               \
+    } while (0)
+
+#define GCOO_CHECK_LAUNCH(name)                                               \
+    do {                                                                      \
+        hipError_t e_ = hipGetLastError();                                    \
+        if (e_ != hipSuccess)                                                 \
+            return ::gcoo::set_error(GCOO_ERR_HIP, "launch of %s failed: %s", \
+                                     name, hipGetErrorString(e_));            \
+    } while (0)
+
+// Device-side status word written by the scan: total nnz (64-bit) and an
+// overflow flag (total exceeds int32 or the handle's entry capacity).
+struct ConvStatus {
+    long long total;
+    int overflow;
+    int pad;
+};
+
+}  // namespace gcoo
+
+// The handle.  Array layout mirrors the reference (src/ShGCOOSpDM.cu:33-35,
+// 94-97): rows and cols share one allocation, groupIndex and nnzPerGroup
+// share another.  Capacity is kept across conversions so a repeated
+// conversion of same-sized inputs allocates nothing.
+struct gcoo_matrix {
+    int M = 0, K = 0, cpg = 4, nGroup = 0;
+    int64_t nnz = 0;
+    bool converted = false;
+    int device = 0;
+
+    float *vals = nullptr;     // [cap]
+    int *rowcol = nullptr;     // [2 * cap]: rows at 0, cols at cap
+    int *idx = nullptr;        // [2 * (nGroup + 1)]: groupIndex, nnzPerGroup
+    int64_t cap = 0;           // entry capacity of vals / rowcol
+
+    gcoo::ConvStatus *d_status = nullptr;  // device
+    gcoo::ConvStatus *h_status = nullptr;  // pinned host mirror
+
+    // scratch for the COO path (keys / permutation), grown on demand
+    void *coo_tmp = nullptr;
+    size_t coo_tmp_bytes = 0;
+
+    int *rows() const { return rowcol; }
+    int *cols() const { return rowcol + cap; }
+    int *groupIndex() const { return idx; }
+    int *nnzPerGroup() const { return idx + (nGroup + 1); }
+};
+
+namespace gcoo {
+
+// ---- conversion (gcoo_convert.hip) ----
+// Enqueue count + scan into h->idx and h->d_status.
+int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s);
+// Enqueue the scatter.  Writes nothing when d_status reports overflow.
+int launch_scatter(gcoo_matrix *h, const float *S, hipStream_t s);
+// Grow entry capacity to at least need (discarding contents).
+int ensure_capacity(gcoo_matrix *h, int64_t need);
+// COO path (gcoo_coo.hip).
+int convert_coo(gcoo_matrix *h, const int *rows, const int *cols,
+                const float *vals, int64_t nnz, hipStream_t s);
+
+// ---- SpDM (gcoo_spdm.hip) ----
+// guard: when non-null, the kernel exits if guard->total > h->cap.
+int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
+                    const ConvStatus *guard, hipStream_t s);
+
+// ---- generators (gcoo_gen.hip) ----
+int launch_fill(float *dst, int64_t n, uint64_t seed, uint64_t stream_id,
+                float scale, uint64_t keep_threshold, bool sparse,
+                hipStream_t s);
+
+// Number of XCDs a launch is spread over (round-robin dispatch of
+// workgroups).  gfx950 (MI355X) has 8.
+constexpr int kNumXcd = 8;
+
+// Workgroup index remapped so that consecutive logical indices run on the
+// same XCD: hardware places workgroup b on XCD b % kNumXcd.  Returns a
+// bijection of [0, nblocks).
+__device__ __forceinline__ int xcd_remap(int b, int nblocks)
+{
+    int q = nblocks / kNumXcd, r = nblocks % kNumXcd;
+    int x = b % kNumXcd, i = b / kNumXcd;
+    return (x < r) ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Wave64 inclusive prefix sum of an int.
+__device__ __forceinline__ int wave_inclusive_sum(int v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        int t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+}  // namespace gcoo

@@ -1,0 +1,150 @@
+/*
+ * gcoo_spdm.h -- C ABI of the MI355X GCOOSpDM library (libgcoospdm.so).
+ *
+ * The library implements the reference's hot path -- dense -> grouped-COO
+ * (GCOO) conversion and the grouped-COO sparse x dense multiply -- as HIP
+ * kernels for gfx950.  Every entry point takes plain pointers and sizes; all
+ * matrix pointers are DEVICE pointers owned by the caller unless stated.
+ * Storage is row-major FP32: S is the sparse operand (M x K), D the dense
+ * operand (K x N), C = S * D (M x N); this is the product the reference
+ * computes (reference src/main.cu:131 with src/utils.cpp:30-46).
+ *
+ * Errors: every function returns GCOO_OK (0) or a GCOO_ERR_* code and never
+ * exits the process (the reference prints and exits, src/ShGCOOSpDM.cu via
+ * checkCudaErrors); gcoo_last_error() gives the message of the calling
+ * thread's last failure.
+ *
+ * Streams: gcoo_stream is a hipStream_t passed as void* (NULL = the null
+ * stream).  Entry points named *_async only enqueue work; the others return
+ * after the work is complete, like the reference's methods, which end in
+ * cudaDeviceSynchronize.
+ */
+#ifndef GCOO_SPDM_H
+#define GCOO_SPDM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gcoo_matrix *gcoo_handle;
+typedef void *gcoo_stream;
+
+enum gcoo_status {
+    GCOO_OK = 0,
+    GCOO_ERR_INVALID = 1,     /* bad argument (size, pointer, cpg, ...)     */
+    GCOO_ERR_HIP = 2,         /* a HIP runtime call failed                  */
+    GCOO_ERR_ALLOC = 3,       /* device or host allocation failed           */
+    GCOO_ERR_OVERFLOW = 4,    /* nnz or an index does not fit in int32      */
+    GCOO_ERR_UNSUPPORTED = 5, /* algorithm / option not available           */
+    GCOO_ERR_BUDGET = 6,      /* request exceeds the serving HBM budget     */
+    GCOO_ERR_STATE = 7        /* handle not in the state the call needs     */
+};
+
+/* Message of the calling thread's last error ("" if none). */
+const char *gcoo_last_error(void);
+/* Library version string. */
+const char *gcoo_version(void);
+
+/* ---- Device memory, copies, synchronisation and timing --------------------
+ * For hosts that do not link the HIP runtime themselves.  They replace the
+ * driver's direct CUDA runtime calls: cudaMalloc / cudaMemcpy / cudaFree
+ * (reference src/main.cu:33-38,85,91-93) and the cudaEvent timing of the
+ * repeat loop (src/main.cu:63-81). */
+int gcoo_device_count(int *count);
+int gcoo_set_device(int device);
+int gcoo_dev_malloc(void **ptr, size_t bytes);
+int gcoo_dev_free(void *ptr);
+int gcoo_memcpy_h2d(void *dst, const void *src, size_t bytes, gcoo_stream s);
+int gcoo_memcpy_d2h(void *dst, const void *src, size_t bytes, gcoo_stream s);
+int gcoo_memset_async(void *dst, int value, size_t bytes, gcoo_stream s);
+int gcoo_stream_create(gcoo_stream *s);
+int gcoo_stream_destroy(gcoo_stream s);
+int gcoo_stream_sync(gcoo_stream s);
+int gcoo_device_sync(void);
+typedef struct gcoo_timer *gcoo_timer_t;
+int gcoo_timer_create(gcoo_timer_t *t);
+int gcoo_timer_destroy(gcoo_timer_t t);
+int gcoo_timer_start(gcoo_timer_t t, gcoo_stream s);
+int gcoo_timer_stop(gcoo_timer_t t, gcoo_stream s);
+/* Waits for the stop event and returns start->stop in milliseconds. */
+int gcoo_timer_elapsed_ms(gcoo_timer_t t, float *ms);
+
+/* ---- Synthetic inputs ------------------------------------------------------
+ * Device-side generators for the benchmark workloads.  They play the role of
+ * the driver's randomInit (reference src/utils.cpp:5-19) with a portable,
+ * counter-based generator instead of glibc rand():
+ *   h(i)  = mix64(key + (i + 1) * 0x9E3779B97F4A7C15),
+ *   key   = mix64(seed * 0x9E3779B97F4A7C15 + stream_id),
+ *   mix64 = the SplitMix64 finaliser;
+ *   value = (2 * (h >> 40) - 2^24 + 1) * 2^-24 * scale   (odd numerator, so
+ *           never 0, uniform in (-scale, scale));
+ *   kept  = (h & (2^40 - 1)) >= keep_threshold           (Bernoulli mask).
+ * keep_threshold = floor(sparsity * 2^40); an element is zero when not kept. */
+int gcoo_fill_uniform_async(float *dst, int64_t n, uint64_t seed,
+                            uint64_t stream_id, float scale, gcoo_stream s);
+int gcoo_fill_sparse_async(float *dst, int64_t n, uint64_t seed,
+                           uint64_t stream_id, float scale,
+                           uint64_t keep_threshold, gcoo_stream s);
+
+/* ---- The GCOO matrix -------------------------------------------------------
+ * A handle owns the grouped-COO arrays of one sparse matrix S (M x K) with
+ * cpg consecutive rows per group (nGroup = ceil(M / cpg)), in the reference
+ * layout (reference src/ShGCOOSpDM.h:25-32, src/ShGCOOSpDM.cu:33-35,94-97):
+ *   vals        f32[nnz]   S[r][k] bit copies,
+ *   rows, cols  i32[nnz]   absolute row r and column k,
+ *   groupIndex  i32[nGroup+1], exclusive scan of nnzPerGroup (last = nnz),
+ *   nnzPerGroup i32[nGroup+1], last entry 0,
+ * entries of group g stored at [groupIndex[g], groupIndex[g+1]) sorted by
+ * (k, r); S[r][k] is a nonzero iff S[r][k] != 0.0 (so -0.0 is dropped and NaN
+ * kept).  cpg must be 1, 2 or 4 (the reference's COLUMN_PER_GROUP range,
+ * src/constants.h:4 and src/gcoo_spdm_kernels.cu:141). */
+
+/* Empty handle for an M x K sparse matrix.  Replaces the ShGCOOSpDM
+ * constructor (reference src/ShGCOOSpDM.cu:13-17). */
+int gcoo_create(int M, int K, int cpg, gcoo_handle *out);
+int gcoo_destroy(gcoo_handle h);
+
+/* Dense -> GCOO: per-group count, wave64/LDS exclusive scan, scatter.
+ * Replaces ShGCOOSpDM::convertToGroupCOOFormat and build()
+ * (reference src/ShGCOOSpDM.cu:19-126,182-193; kernels
+ * src/gcoo_spdm_kernels.cu:4-90,160-168).  S is a device pointer to the
+ * dense row-major M x K matrix.  Synchronous. */
+int gcoo_convert_dense(gcoo_handle h, const float *S, gcoo_stream s);
+
+/* COO -> GCOO for matrices too large to hold densely (the reference's int
+ * offsets overflow at M*K >= 2^31, src/gcoo_spdm_kernels.cu:16,57).  rows,
+ * cols, vals are device arrays of nnz entries in any order, duplicates not
+ * allowed; entries with value == 0.0 are dropped as in the dense path.
+ * The result is identical to gcoo_convert_dense of the equivalent dense S. */
+int gcoo_convert_coo(gcoo_handle h, const int *rows, const int *cols,
+                     const float *vals, int64_t nnz, gcoo_stream s);
+
+/* Shape and arrays of a converted handle (device pointers, owned by h). */
+int gcoo_get_info(gcoo_handle h, int *M, int *K, int *cpg, int *nGroup,
+                  int64_t *nnz);
+int gcoo_get_arrays(gcoo_handle h, const float **vals, const int **rows,
+                    const int **cols, const int **groupIndex,
+                    const int **nnzPerGroup);
+
+/* C = S * D with the converted S.  D is K x N, C is M x N, row-major FP32,
+ * device pointers.  Every C[r][j] is one FP32 fmaf chain from +0.0f over
+ * row r's nonzeros in ascending k: bit-identical to the reference kernel
+ * sparse_dense_groupcoo_mat_mul_kernel (src/gcoo_spdm_kernels.cu:92-158).
+ * Replaces ShGCOOSpDM::runKernelTest (src/ShGCOOSpDM.cu:195-214). */
+int gcoo_spdm_f32_async(gcoo_handle h, const float *D, int K, int N, float *C,
+                        gcoo_stream s);
+int gcoo_spdm_f32(gcoo_handle h, const float *D, int K, int N, float *C,
+                  gcoo_stream s);
+
+/* Convert S then multiply: ShGCOOSpDM::mutiplyBy (src/ShGCOOSpDM.cu:131-180),
+ * without the reference's per-call allocations once capacity is reached. */
+int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,
+                     int N, float *C, gcoo_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

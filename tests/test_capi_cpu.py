@@ -1,0 +1,70 @@
+"""Host-side checks of the C ABI and the `main` driver (no GPU needed):
+the library loads, exports every function include/gcoo_spdm.h declares, and
+argument validation fails with a status code instead of exiting."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import gcoospdm_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gcoo_spdm.h")
+MAIN = os.path.join(ROOT, "gcoospdm_amd", "host", "main")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gcoo_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree():
+    assert header_functions() == sorted(G.declared_symbols())
+
+
+def test_library_exports_every_declared_symbol():
+    lib = G.lib()
+    for name in header_functions():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", G.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (gcoo_\w+)", out))
+    assert set(header_functions()) <= exported
+
+
+def test_version_and_error_paths():
+    lib = G.lib()
+    assert b"gfx950" in lib.gcoo_version()
+    h = ctypes.c_void_p()
+    assert lib.gcoo_create(8, 8, 3, ctypes.byref(h)) == G.GCOO_ERR_INVALID
+    assert b"cpg" in lib.gcoo_last_error()
+    assert lib.gcoo_create(-1, 8, 4, ctypes.byref(h)) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_create(8, 8, 4, None) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_spdm_f32(None, None, 8, 8, None, None) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_convert_dense(None, None, None) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_get_arrays(None, None, None, None, None, None) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_destroy(None) == G.GCOO_OK
+    assert lib.gcoo_fill_uniform_async(None, -1, 0, 0, 1.0, None) == G.GCOO_ERR_INVALID
+
+
+def test_python_mirror_factory_keys():
+    # unknown and out-of-scope keys return None like the reference's NULL
+    for key in ("cusparse", "cublas", "csr", "nope"):
+        assert G.SpGemmBuilder.getShGemmAlgo(None, 4, 4, key) is None
+
+
+@pytest.mark.skipif(not os.path.exists(MAIN), reason="main driver not built")
+def test_main_driver_flag_validation():
+    r = subprocess.run([MAIN, "-help"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "-algo" in r.stdout
+    r = subprocess.run([MAIN, "-nRowsA=8"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "algo" in r.stderr
+    r = subprocess.run([MAIN, "-algo=gcoospdm", "-nRowsA=8", "-nColsB=9"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "nColsB" in r.stderr
+    r = subprocess.run([MAIN, "-algo=gcoospdm", "-file=x.mtx"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 2

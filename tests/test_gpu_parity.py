@@ -1,0 +1,218 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bars: GCOO arrays bit-exact; FP32 product bit-exact with the reference's
+fmaf chain (NaN payload included); relative Frobenius error vs FP64 <= 1e-5
+(the reference's sdkCompareL2fe tolerance, src/main.cu:136)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import gcoospdm_amd as G
+import oracle
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+FP32_REL_TOL = 1e-5
+
+
+def to_dev(a, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def check_arrays(m, g_ref):
+    got = m.arrays_host()
+    assert m.nnz == g_ref["nnz"]
+    for key in ("rows", "cols", "groupIndex", "nnzPerGroup"):
+        np.testing.assert_array_equal(got[key], g_ref[key], err_msg=key)
+    assert H.bits_equal(got["vals"], g_ref["vals"])
+
+
+EDGE_SHAPES = [
+    (13, 7, 5, 0.5),      # M % 4 != 0, K % 4 != 0 (scalar loads)
+    (41, 8, 3, 0.05),     # empty groups, N odd
+    (9, 16, 64, 1.0),     # dense
+    (8, 12, 6, 0.0),      # all zero
+    (1, 17, 1, 0.4),
+    (64, 1000, 300, 0.02),
+    (257, 515, 129, 0.1),
+]
+
+
+@pytest.mark.parametrize("cpg", [1, 2, 4])
+@pytest.mark.parametrize("M,K,N,dens", EDGE_SHAPES)
+def test_convert_and_spdm_bit_exact(dev, M, K, N, dens, cpg):
+    import torch
+    print(f"case M={M} K={K} N={N} dens={dens} cpg={cpg}", flush=True)
+    rng = np.random.default_rng(M * 7 + K * 3 + N + cpg)
+    S = H.rng_sparse(rng, M, K, dens)
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    g = oracle.convert(S, cpg)
+    m = G.GCOOMatrix(M, K, cpg).convert_dense(to_dev(S, dev))
+    check_arrays(m, g)
+    C = torch.full((M, N), float("nan"), device=dev)
+    m.spdm(to_dev(D, dev), C)
+    assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+def test_edge_values(dev):
+    import torch
+    S = np.zeros((7, 9), np.float32)
+    S[0, 1] = -0.0
+    S[0, 2] = 1.5
+    S[1, 0] = np.nan
+    S[2, 8] = np.float32(1e-40)
+    S[3, 3] = -np.float32(1e-45)
+    S[4, 4] = np.inf
+    S[6, 8] = 3.0
+    D = np.random.default_rng(0).uniform(-1, 1, (9, 5)).astype(np.float32)
+    D[8, 0] = np.float32(1e-39)  # denormal * normal products
+    g = oracle.convert(S, 4)
+    m = G.GCOOMatrix(7, 9, 4).convert_dense(to_dev(S, dev))
+    check_arrays(m, g)
+    C = torch.zeros((7, 5), device=dev)
+    m.spdm(to_dev(D, dev), C)
+    assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+def test_unaligned_operands(dev):
+    """Sub-tensor views that are not 16-byte aligned take the scalar paths."""
+    import torch
+    rng = np.random.default_rng(11)
+    M, K, N = 33, 20, 12
+    S = H.rng_sparse(rng, M, K, 0.2)
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    Sbuf = torch.zeros(M * K + 1, device=dev)
+    Sbuf[1:] = to_dev(S.reshape(-1), dev)
+    Sv = Sbuf[1:].view(M, K)
+    Dbuf = torch.zeros(K * N + 1, device=dev)
+    Dbuf[1:] = to_dev(D.reshape(-1), dev)
+    Dv = Dbuf[1:].view(K, N)
+    Cbuf = torch.zeros(M * N + 1, device=dev)
+    Cv = Cbuf[1:].view(M, N)
+    g = oracle.convert(S, 4)
+    m = G.GCOOMatrix(M, K, 4).convert_dense(Sv)
+    check_arrays(m, g)
+    m.spdm(Dv, Cv)
+    assert H.bits_equal(Cv.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+def test_config1_800_s09_vs_fp64(dev):
+    """Plain method, small: 800 x 800, p(nonzero) = 0.1, U[-1,1], fixed seed;
+    checked bit-exact vs the fmaf chain and against FP64 (rel <= 1e-5)."""
+    import torch
+    n = 800
+    S = H.counter_fill(n * n, 1, 1, 1.0, G.keep_threshold(0.9), True).reshape(n, n)
+    D = H.counter_fill(n * n, 1, 0, 1.0).reshape(n, n)
+    Sd = torch.empty((n, n), device=dev)
+    Dd = torch.empty((n, n), device=dev)
+    G.fill_sparse(Sd, 1, 1, 0.9)
+    G.fill_uniform(Dd, 1, 0)
+    # the device generator is bit-identical to its host definition
+    assert H.bits_equal(Sd.cpu().numpy(), S) and H.bits_equal(Dd.cpu().numpy(), D)
+    algo = G.SpGemmBuilder.getShGemmAlgo(Sd, n, n, "gcoospdm")
+    C = torch.empty((n, n), device=dev)
+    algo.mutiplyBy(Dd, n, n, C)
+    g = oracle.convert(S, 4)
+    check_arrays(algo.matrix, g)
+    Cn = C.cpu().numpy()
+    assert H.bits_equal(Cn, oracle.spdm_f32(g, D, threads=8))
+    Cd, _ = oracle.spdm_fp64(g, D, threads=8)
+    err = oracle.rel_frobenius(Cn, Cd)
+    print(f"config1 rel_frobenius={err:.3e}")
+    assert err <= FP32_REL_TOL
+
+
+@pytest.mark.parametrize("sparsity", [0.99, 0.995, 0.999])
+def test_config2_n8000_full_size(dev, sparsity):
+    """n = 8000: conversion bit-exact at full size; the product bit-exact on a
+    sample of row groups spread over the matrix and all columns, plus FP64
+    rel-Frobenius on the same sample."""
+    import torch
+    n = 8000
+    print(f"config2 n={n} s={sparsity}", flush=True)
+    Sd = torch.empty((n, n), device=dev)
+    Dd = torch.empty((n, n), device=dev)
+    G.fill_sparse(Sd, 7, 1, sparsity)
+    G.fill_uniform(Dd, 7, 0)
+    C = torch.empty((n, n), device=dev)
+    m = G.GCOOMatrix(n, n, 4)
+    m.multiply_by(Sd, Dd, C)
+    S = Sd.cpu().numpy()
+    D = Dd.cpu().numpy()
+    g = oracle.convert(S, 4)
+    check_arrays(m, g)
+    groups = np.unique(np.linspace(0, n // 4 - 1, 48).astype(int))
+    rows = np.concatenate([np.arange(4 * q, 4 * q + 4) for q in groups])
+    sub = oracle.convert(S[rows], 4)
+    Csub = oracle.spdm_f32(sub, D, threads=oracle.max_threads())
+    Cn = C.cpu().numpy()[rows]
+    assert H.bits_equal(Cn, Csub)
+    Cd, _ = oracle.spdm_fp64(sub, D, threads=oracle.max_threads())
+    assert oracle.rel_frobenius(Cn, Cd) <= FP32_REL_TOL
+
+
+def test_repeat_and_capacity_growth(dev):
+    """mutiplyBy on a handle re-used with denser inputs (capacity grows) and
+    then sparser ones (capacity kept) matches a fresh conversion each time."""
+    import torch
+    M, K, N = 200, 300, 96
+    m = G.GCOOMatrix(M, K, 4)
+    rng = np.random.default_rng(3)
+    for dens in (0.01, 0.2, 0.5, 0.05, 0.5):
+        S = H.rng_sparse(rng, M, K, dens)
+        D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        C = torch.empty((M, N), device=dev)
+        m.multiply_by(to_dev(S, dev), to_dev(D, dev), C)
+        g = oracle.convert(S, 4)
+        check_arrays(m, g)
+        assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+def test_reference_api_mirror(dev):
+    """build() + runKernelTest == mutiplyBy == mutiply (reference
+    src/main.cu:53-75 call sequence)."""
+    import torch
+    M, K, N = 96, 64, 80
+    rng = np.random.default_rng(9)
+    S = to_dev(H.rng_sparse(rng, M, K, 0.1), dev)
+    D = to_dev(rng.uniform(-1, 1, (K, N)).astype(np.float32), dev)
+    algo = G.SpGemmBuilder.getShGemmAlgo(S, M, K, "gcoospdm")
+    C1, C2, C3 = (torch.empty((M, N), device=dev) for _ in range(3))
+    algo.build()
+    algo.runKernelTest(D, K, N, C1)
+    algo.mutiplyBy(D, K, N, C2)
+    algo.mutiply(D, K, N, C3)
+    assert torch.equal(C1.view(torch.int32), C2.view(torch.int32))
+    assert torch.equal(C1.view(torch.int32), C3.view(torch.int32))
+    with pytest.raises(G.GCOOError):
+        algo.runKernelTest(D[:-1], K - 1, N, C1)   # K mismatch -> error, no crash
+
+
+MAIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                    "gcoospdm_amd", "host", "main")
+
+
+@pytest.mark.parametrize("kernelonly", [0, 1])
+def test_main_driver_check(dev, kernelonly):
+    """`main -algo=gcoospdm` on the config-1 size with its FP64 check."""
+    r = subprocess.run([MAIN, "-algo=gcoospdm", "-nRowsA=800", "-nColsA=800",
+                        "-nRowsB=800", "-nColsB=800", "-sparsity=0.9", "-dist=uniform",
+                        "-check=1", "-iters=5", f"-kernelonly={kernelonly}"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    assert "Test passed" in r.stdout and "Time:" in r.stdout
+
+
+def test_main_driver_reference_inputs(dev):
+    """The reference's own generator (glibc rand, values in [0,1])."""
+    r = subprocess.run([MAIN, "-algo=gcoospdm", "-nRowsA=512", "-nColsA=512",
+                        "-nRowsB=512", "-nColsB=512", "-sparsity=0.95", "-check=1",
+                        "-iters=3"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    assert "Test passed" in r.stdout
