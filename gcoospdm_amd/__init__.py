@@ -73,9 +73,21 @@ _SIGNATURES = [
     ("gcoo_spdm_f32_async", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_spdm_f32", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_multiply_by", _I, [_P, _P, _P, _I, _I, _P, _P]),
+    ("gcoo_set_spdm_config", _I, [_P, _P]),
+    ("gcoo_get_spdm_config", _I, [_P, _P]),
+    ("gcoo_f32_to_bf16_async", _I, [_P, _P, _I64, _P]),
+    ("gcoo_spdm_bf16_async", _I, [_P, _P, _I, _I, _P, _P]),
+    ("gcoo_spdm_bf16", _I, [_P, _P, _I, _I, _P, _P]),
 ]
 
 _lib: Optional[ctypes.CDLL] = None
+
+
+class SpdmConfig(ctypes.Structure):
+    """gcoo_spdm_config (include/gcoo_spdm.h): variant, vec, waves, chunk;
+    zero fields mean automatic."""
+    _fields_ = [("variant", ctypes.c_int), ("vec", ctypes.c_int),
+                ("waves", ctypes.c_int), ("chunk", ctypes.c_int)]
 
 
 class GCOOError(RuntimeError):
@@ -227,6 +239,32 @@ class GCOOMatrix:
         _check(fn(self._h, _ptr(D), K, N, _ptr(C), _stream(stream)), "gcoo_spdm_f32")
         return C
 
+    def spdm_bf16(self, D16, C, stream=None, sync=True):
+        """C = S * D with BF16 storage of S's values and of D (a torch.int16
+        or torch.bfloat16 tensor of BF16 bits), FP32 accumulation."""
+        import torch
+        if D16.dtype not in (torch.int16, torch.bfloat16):
+            raise ValueError("D16 must hold BF16 bits (torch.bfloat16 or torch.int16)")
+        _require_dev(D16, "D16")
+        _require_dev(C, "C", torch.float32)
+        K, N = D16.shape
+        if tuple(C.shape) != (self.M, N):
+            raise ValueError(f"C shape {tuple(C.shape)} != {(self.M, N)}")
+        fn = lib().gcoo_spdm_bf16 if sync else lib().gcoo_spdm_bf16_async
+        _check(fn(self._h, _ptr(D16), K, N, _ptr(C), _stream(stream)), "gcoo_spdm_bf16")
+        return C
+
+    def set_config(self, variant=0, vec=0, waves=0, chunk=0):
+        """Multiply-kernel variant / tile shape (0 = automatic)."""
+        cfg = SpdmConfig(variant, vec, waves, chunk)
+        _check(lib().gcoo_set_spdm_config(self._h, ctypes.byref(cfg)), "gcoo_set_spdm_config")
+        return self
+
+    def get_config(self):
+        cfg = SpdmConfig()
+        _check(lib().gcoo_get_spdm_config(self._h, ctypes.byref(cfg)), "gcoo_get_spdm_config")
+        return dict(variant=cfg.variant, vec=cfg.vec, waves=cfg.waves, chunk=cfg.chunk)
+
     def multiply_by(self, S, D, C, stream=None):
         """Convert S then C = S * D (the reference's mutiplyBy)."""
         import torch
@@ -316,6 +354,19 @@ def keep_threshold(sparsity: float) -> int:
     if sparsity >= 1.0:
         return 1 << 40
     return int(math.floor(sparsity * 1099511627776.0))
+
+
+def to_bf16(x, out=None, stream=None):
+    """RNE FP32 -> BF16 on the device (gcoo_f32_to_bf16_async); returns a
+    torch.bfloat16 tensor of the same shape."""
+    import torch
+    _require_dev(x, "x", torch.float32)
+    if out is None:
+        out = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device)
+    _require_dev(out, "out")
+    _check(lib().gcoo_f32_to_bf16_async(_ptr(x), _ptr(out), x.numel(), _stream(stream)),
+           "gcoo_f32_to_bf16_async")
+    return out
 
 
 def fill_uniform(t, seed: int, stream_id: int, scale: float = 1.0, stream=None):

@@ -35,14 +35,23 @@ int ensure_capacity(gcoo_matrix *h, int64_t need)
     if (cap > (int64_t)INT32_MAX) cap = INT32_MAX;
     if (h->vals) GCOO_HIP_TRY(hipFree(h->vals));
     if (h->rowcol) GCOO_HIP_TRY(hipFree(h->rowcol));
+    if (h->seg) GCOO_HIP_TRY(hipFree(h->seg));
+    if (h->seg16) GCOO_HIP_TRY(hipFree(h->seg16));
     h->vals = nullptr;
     h->rowcol = nullptr;
+    h->seg = nullptr;
+    h->seg16 = nullptr;
+    h->seg16_valid = false;
     h->cap = 0;
     if (hipMalloc(&h->vals, sizeof(float) * (size_t)cap) != hipSuccess ||
-        hipMalloc(&h->rowcol, sizeof(int) * 2 * (size_t)cap) != hipSuccess) {
+        hipMalloc(&h->rowcol, sizeof(int) * 2 * (size_t)cap) != hipSuccess ||
+        hipMalloc(&h->seg, sizeof(int2) * (size_t)cap) != hipSuccess) {
         (void)hipGetLastError();
         if (h->vals) (void)hipFree(h->vals);
+        if (h->rowcol) (void)hipFree(h->rowcol);
         h->vals = nullptr;
+        h->rowcol = nullptr;
+        h->seg = nullptr;
         return set_error(GCOO_ERR_ALLOC, "cannot allocate GCOO arrays for %lld entries",
                          (long long)cap);
     }
@@ -81,6 +90,7 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
                                int N, float *C, hipStream_t s)
 {
     h->converted = false;
+    h->seg16_valid = false;
     int rc = launch_count_and_scan(h, S, s);
     if (rc) return rc;
     if (h->cap > 0) {
@@ -284,6 +294,8 @@ int gcoo_create(int M, int K, int cpg, gcoo_handle *out)
     h->nGroup = (M + cpg - 1) / cpg;
     (void)hipGetDevice(&h->device);
     if (hipMalloc(&h->idx, sizeof(int) * 2 * ((size_t)h->nGroup + 1)) != hipSuccess ||
+        hipMalloc(&h->rowptr, sizeof(int) * ((size_t)M + 1)) != hipSuccess ||
+        hipMalloc(&h->rowcnt, sizeof(int) * ((size_t)M + 1)) != hipSuccess ||
         hipMalloc(&h->d_status, sizeof(ConvStatus)) != hipSuccess ||
         hipHostMalloc(&h->h_status, sizeof(ConvStatus), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
@@ -302,6 +314,10 @@ int gcoo_destroy(gcoo_handle h)
     if (h->vals) (void)hipFree(h->vals);
     if (h->rowcol) (void)hipFree(h->rowcol);
     if (h->idx) (void)hipFree(h->idx);
+    if (h->seg) (void)hipFree(h->seg);
+    if (h->seg16) (void)hipFree(h->seg16);
+    if (h->rowptr) (void)hipFree(h->rowptr);
+    if (h->rowcnt) (void)hipFree(h->rowcnt);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->coo_tmp) (void)hipFree(h->coo_tmp);
@@ -324,6 +340,7 @@ int gcoo_convert_coo(gcoo_handle h, const int *rows, const int *cols, const floa
         return set_error(GCOO_ERR_INVALID, "bad COO arguments");
     if (nnz > INT32_MAX) return set_error(GCOO_ERR_OVERFLOW, "nnz %lld > int32", (long long)nnz);
     h->converted = false;
+    h->seg16_valid = false;
     int rc = convert_coo(h, rows, cols, vals, nnz, as_stream(s));
     if (rc) return rc;
     h->converted = true;
@@ -367,6 +384,65 @@ int gcoo_spdm_f32(gcoo_handle h, const float *D, int K, int N, float *C, gcoo_st
     int rc = gcoo_spdm_f32_async(h, D, K, N, C, s);
     if (rc) return rc;
     GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_f32_to_bf16_async(const float *x, uint16_t *y, int64_t n, gcoo_stream s)
+{
+    if (n < 0 || (n > 0 && (!x || !y))) return set_error(GCOO_ERR_INVALID, "bad arguments");
+    return launch_f32_to_bf16(x, y, n, as_stream(s));
+}
+
+int gcoo_spdm_bf16_async(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
+                         gcoo_stream s)
+{
+    int rc = check_spdm_args(h, reinterpret_cast<const float *>(D), K, N, C);
+    if (rc) return rc;
+    if (!h->converted) return set_error(GCOO_ERR_STATE, "matrix not converted");
+    if (!h->seg16_valid && h->nnz > 0) {
+        if (!h->seg16) {
+            if (hipMalloc(&h->seg16, sizeof(int2) * (size_t)h->cap) != hipSuccess) {
+                (void)hipGetLastError();
+                h->seg16 = nullptr;
+                return set_error(GCOO_ERR_ALLOC, "cannot allocate the BF16 segments");
+            }
+        }
+        if ((rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, as_stream(s)))) return rc;
+    }
+    h->seg16_valid = true;
+    return launch_spdm_bf16(h, D, N, C, as_stream(s));
+}
+
+int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C, gcoo_stream s)
+{
+    int rc = gcoo_spdm_bf16_async(h, D, K, N, C, s);
+    if (rc) return rc;
+    GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
+    return GCOO_OK;
+}
+
+int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (!cfg) {
+        h->spdm_cfg = gcoo_spdm_config{0, 0, 0, 0};
+        return GCOO_OK;
+    }
+    if (cfg->variant < 0 || cfg->variant > 3)
+        return set_error(GCOO_ERR_INVALID, "variant %d", cfg->variant);
+    if (cfg->vec != 0 && cfg->vec != 1 && cfg->vec != 2 && cfg->vec != 4 && cfg->vec != 8)
+        return set_error(GCOO_ERR_INVALID, "vec %d (0, 1, 2, 4 or 8)", cfg->vec);
+    if (cfg->waves < 0 || cfg->waves > 16)
+        return set_error(GCOO_ERR_INVALID, "waves %d (0..16)", cfg->waves);
+    if (cfg->chunk < 0) return set_error(GCOO_ERR_INVALID, "chunk %d", cfg->chunk);
+    h->spdm_cfg = *cfg;
+    return GCOO_OK;
+}
+
+int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg)
+{
+    if (!h || !cfg) return set_error(GCOO_ERR_INVALID, "null argument");
+    *cfg = h->spdm_cfg;
     return GCOO_OK;
 }
 

@@ -3,14 +3,20 @@
 // Three kernels replace the reference's K1 / K2 / K3
 // (src/gcoo_spdm_kernels.cu:4-28, :160-168, :30-90; launched from
 // ShGCOOSpDM::convertToGroupCOOFormat, src/ShGCOOSpDM.cu:19-126):
-//   count_kernel   one workgroup per group, float4 streaming reads of the
-//                  group's cpg rows, wave64 shuffle + LDS reduction;
-//   scan_kernel    one 1024-thread workgroup: exclusive scan of the counts
-//                  (the reference runs this serially on one thread), 64-bit
-//                  total and overflow flag into a device status word;
-//   scatter_kernel one workgroup per group, k-chunks of 1024 columns, wave64
-//                  prefix sums + LDS carry across the 4 waves, entries written
-//                  in (k, r) order -- the reference's order, bit for bit.
+//   count_kernel   one workgroup per group, 16-byte streaming reads of the
+//                  group's cpg rows, per-row wave64 shuffle + LDS reductions;
+//   scan_kernel    one 1024-thread workgroup: exclusive scan of the per-row
+//                  counts into row pointers (the reference scans per group on
+//                  a single thread), from which groupIndex / nnzPerGroup are
+//                  read off; 64-bit total and overflow flag in a status word;
+//   scatter_kernel one workgroup per group, k-chunks of 1024 columns, one
+//                  64-bit wave64 prefix sum carrying the cpg per-row counts in
+//                  16-bit fields; writes every entry twice:
+//                    - in the reference GCOO order (k, r) -- bit for bit the
+//                      reference's vals / rows / cols;
+//                    - in row segments: (k, val) pairs of row r at
+//                      rowptr[r] + rank, the order the multiply kernel walks
+//                      (each group's entries split by row, k ascending).
 // All three are HBM-streaming integer/byte work: no MFMA, coalesced 16-byte
 // loads along rows of S.
 #include "gcoo_internal.h"
@@ -31,12 +37,14 @@ __device__ __forceinline__ int nz(float v) { return v != 0.0f ? 1 : 0; }
 
 template <int CPG, bool VEC4>
 __global__ __launch_bounds__(kConvThreads) void count_kernel(
-    const float *__restrict__ S, int M, int K, int *__restrict__ nnzPerGroup)
+    const float *__restrict__ S, int M, int K, int *__restrict__ rowcnt)
 {
     const int g = blockIdx.x;
     const int r0 = g * CPG;
     const int nrows = min(CPG, M - r0);
-    int c = 0;
+    int c[CPG];
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) c[j] = 0;
     if (VEC4) {
         const int K4 = K >> 2;
         const f32x4 *base = reinterpret_cast<const f32x4 *>(S + (size_t)r0 * K);
@@ -46,7 +54,7 @@ __global__ __launch_bounds__(kConvThreads) void count_kernel(
             for (int j = 0; j < CPG; ++j) {
                 if (j < nrows) {
                     f32x4 v = __builtin_nontemporal_load(base + (size_t)j * K4 + i);
-                    c += nz(v.x) + nz(v.y) + nz(v.z) + nz(v.w);
+                    c[j] += nz(v.x) + nz(v.y) + nz(v.z) + nz(v.w);
                 }
             }
         }
@@ -55,38 +63,44 @@ __global__ __launch_bounds__(kConvThreads) void count_kernel(
         for (int i = threadIdx.x; i < K; i += kConvThreads) {
 #pragma unroll
             for (int j = 0; j < CPG; ++j)
-                if (j < nrows) c += nz(base[(size_t)j * K + i]);
+                if (j < nrows) c[j] += nz(base[(size_t)j * K + i]);
         }
     }
-    __shared__ int part[kConvWaves];
-    c = wave_sum(c);
-    if (__lane_id() == 0) part[threadIdx.x >> 6] = c;
+    __shared__ int part[kConvWaves][CPG];
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) {
+        int t = wave_sum(c[j]);
+        if (__lane_id() == 0) part[threadIdx.x >> 6][j] = t;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < nrows) {
         int t = 0;
 #pragma unroll
-        for (int w = 0; w < kConvWaves; ++w) t += part[w];
-        nnzPerGroup[g] = t;
+        for (int w = 0; w < kConvWaves; ++w) t += part[w][threadIdx.x];
+        rowcnt[r0 + threadIdx.x] = t;
     }
 }
 
-// Exclusive scan of nnzPerGroup[0..nGroup) into groupIndex[0..nGroup];
-// groupIndex[nGroup] = total, nnzPerGroup[nGroup] = 0 (the reference's memset
-// of the shared allocation, src/ShGCOOSpDM.cu:37).
+// rowptr[0..M] = exclusive scan of rowcnt[0..M) (rowptr[M] = total); then
+// groupIndex[g] = rowptr[min(g*cpg, M)] for g <= nGroup and
+// nnzPerGroup[g] = groupIndex[g+1] - groupIndex[g], nnzPerGroup[nGroup] = 0
+// (the reference's scan of the group counts, src/gcoo_spdm_kernels.cu:160-168,
+// and the memset of the shared allocation, src/ShGCOOSpDM.cu:37).
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(
-    int *__restrict__ nnzPerGroup, int *__restrict__ groupIndex, int nGroup,
-    ConvStatus *__restrict__ st)
+    const int *__restrict__ rowcnt, int *__restrict__ rowptr, int M, int cpg, int nGroup,
+    int *__restrict__ groupIndex, int *__restrict__ nnzPerGroup,
+    ConvStatus *__restrict__ st, int clear_invalid)
 {
     __shared__ long long wtot[kScanThreads / 64];
     const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
     long long carry = 0;
-    for (int base = 0; base < nGroup; base += kScanThreads * kScanItems) {
+    for (int base = 0; base < M; base += kScanThreads * kScanItems) {
         int v[kScanItems];
         long long tsum = 0;
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) {
             int idx = base + tid * kScanItems + i;
-            v[i] = idx < nGroup ? nnzPerGroup[idx] : 0;
+            v[i] = idx < M ? rowcnt[idx] : 0;
             tsum += v[i];
         }
         long long incl = tsum;
@@ -108,44 +122,61 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) {
             int idx = base + tid * kScanItems + i;
-            if (idx < nGroup) groupIndex[idx] = (int)run;
+            if (idx < M) rowptr[idx] = (int)run;
             run += v[i];
         }
         carry += tile;
         __syncthreads();
     }
     if (tid == 0) {
-        groupIndex[nGroup] = (int)carry;
-        nnzPerGroup[nGroup] = 0;
+        rowptr[M] = (int)carry;
         st->total = carry;
         st->overflow = carry > (long long)INT_MAX ? 1 : 0;
+        if (clear_invalid) st->invalid = 0;
+    }
+    __syncthreads();
+    for (int g = tid; g <= nGroup; g += kScanThreads) {
+        const long long r = (long long)g * cpg;
+        const int a = rowptr[r < M ? r : M];
+        groupIndex[g] = a;
+        if (g < nGroup) {
+            const long long r1 = r + cpg;
+            nnzPerGroup[g] = rowptr[r1 < M ? r1 : M] - a;
+        } else {
+            nnzPerGroup[g] = 0;
+        }
     }
 }
 
 // One workgroup per group.  Chunk of kConvThreads * KPT columns; thread t owns
-// columns [chunk + KPT*t, chunk + KPT*t + KPT) and emits its nonzeros in
-// (k, r) order at an offset given by a workgroup-wide exclusive scan.
+// columns [chunk + KPT*t, chunk + KPT*t + KPT).  Its per-row counts are packed
+// in 16-bit fields of one 64-bit word (a chunk holds at most 1024 entries per
+// row) so that one workgroup-wide exclusive scan yields both the group-order
+// offset (sum of the fields) and every row segment's offset (each field).
 template <int CPG, bool VEC4>
 __global__ __launch_bounds__(kConvThreads) void scatter_kernel(
-    const float *__restrict__ S, int M, int K,
-    const int *__restrict__ groupIndex, const ConvStatus *__restrict__ st,
-    long long cap, float *__restrict__ vals, int *__restrict__ rows,
-    int *__restrict__ cols)
+    const float *__restrict__ S, int M, int K, const int *__restrict__ groupIndex,
+    const int *__restrict__ rowptr, const ConvStatus *__restrict__ st, long long cap,
+    float *__restrict__ vals, int *__restrict__ rows, int *__restrict__ cols,
+    int2 *__restrict__ seg)
 {
     if (st->total > cap || st->overflow) return;
     constexpr int KPT = VEC4 ? 4 : 1;
     constexpr int CHUNK = kConvThreads * KPT;
-    __shared__ int wtot[kConvWaves];
+    __shared__ unsigned long long wtot[kConvWaves];
     const int g = blockIdx.x;
     const int r0 = g * CPG;
     const int nrows = min(CPG, M - r0);
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    int base = groupIndex[g];
+    int gbase = groupIndex[g];
+    int rbase[CPG];
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) rbase[j] = j < nrows ? rowptr[r0 + j] : 0;
     const float *grow = S + (size_t)r0 * K;
     for (int k0 = 0; k0 < K; k0 += CHUNK) {
         const int kb = k0 + threadIdx.x * KPT;
         float v[CPG][KPT];
-        int cnt = 0;
+        unsigned long long packed = 0;
 #pragma unroll
         for (int j = 0; j < CPG; ++j) {
             if (VEC4) {
@@ -162,35 +193,57 @@ __global__ __launch_bounds__(kConvThreads) void scatter_kernel(
             } else {
                 v[j][0] = (j < nrows && kb < K) ? grow[(size_t)j * K + kb] : 0.f;
             }
+            int cj = 0;
 #pragma unroll
-            for (int q = 0; q < KPT; ++q) cnt += nz(v[j][q]);
+            for (int q = 0; q < KPT; ++q) cj += nz(v[j][q]);
+            packed |= (unsigned long long)cj << (16 * j);
         }
-        int incl = wave_inclusive_sum(cnt);
+        unsigned long long incl = packed;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            unsigned long long t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
         if (lane == 63) wtot[wave] = incl;
         __syncthreads();
-        int before = 0, total = 0;
+        unsigned long long before = 0, total = 0;
 #pragma unroll
         for (int w = 0; w < kConvWaves; ++w) {
-            int x = wtot[w];
-            before += (w < wave) ? x : 0;
+            unsigned long long x = wtot[w];
+            before += (w < wave) ? x : 0ull;
             total += x;
         }
-        if (cnt) {
-            int pos = base + before + incl - cnt;
+        const unsigned long long excl = before + incl - packed;
+        if (packed) {
+            int posr[CPG];
+            int posg = gbase;
+#pragma unroll
+            for (int j = 0; j < CPG; ++j) {
+                const int e = (int)((excl >> (16 * j)) & 0xFFFFu);
+                posr[j] = rbase[j] + e;
+                posg += e;
+            }
 #pragma unroll
             for (int q = 0; q < KPT; ++q) {
 #pragma unroll
                 for (int j = 0; j < CPG; ++j) {
                     if (nz(v[j][q])) {
-                        vals[pos] = v[j][q];
-                        rows[pos] = r0 + j;
-                        cols[pos] = kb + q;
-                        ++pos;
+                        vals[posg] = v[j][q];
+                        rows[posg] = r0 + j;
+                        cols[posg] = kb + q;
+                        ++posg;
+                        seg[posr[j]] = make_int2(kb + q, __float_as_int(v[j][q]));
+                        ++posr[j];
                     }
                 }
             }
         }
-        base += total;
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) {
+            const int tj = (int)((total >> (16 * j)) & 0xFFFFu);
+            rbase[j] += tj;
+            gbase += tj;
+        }
         __syncthreads();
     }
 }
@@ -201,10 +254,10 @@ int launch_count_t(gcoo_matrix *h, const float *S, bool vec4, hipStream_t s)
     dim3 grid(h->nGroup), block(kConvThreads);
     if (vec4)
         hipLaunchKernelGGL((count_kernel<CPG, true>), grid, block, 0, s, S, h->M,
-                           h->K, h->nnzPerGroup());
+                           h->K, h->rowcnt);
     else
         hipLaunchKernelGGL((count_kernel<CPG, false>), grid, block, 0, s, S,
-                           h->M, h->K, h->nnzPerGroup());
+                           h->M, h->K, h->rowcnt);
     GCOO_CHECK_LAUNCH("count_kernel");
     return GCOO_OK;
 }
@@ -215,12 +268,12 @@ int launch_scatter_t(gcoo_matrix *h, const float *S, bool vec4, hipStream_t s)
     dim3 grid(h->nGroup), block(kConvThreads);
     if (vec4)
         hipLaunchKernelGGL((scatter_kernel<CPG, true>), grid, block, 0, s, S,
-                           h->M, h->K, h->groupIndex(), h->d_status,
-                           (long long)h->cap, h->vals, h->rows(), h->cols());
+                           h->M, h->K, h->groupIndex(), h->rowptr, h->d_status,
+                           (long long)h->cap, h->vals, h->rows(), h->cols(), h->seg);
     else
         hipLaunchKernelGGL((scatter_kernel<CPG, false>), grid, block, 0, s, S,
-                           h->M, h->K, h->groupIndex(), h->d_status,
-                           (long long)h->cap, h->vals, h->rows(), h->cols());
+                           h->M, h->K, h->groupIndex(), h->rowptr, h->d_status,
+                           (long long)h->cap, h->vals, h->rows(), h->cols(), h->seg);
     GCOO_CHECK_LAUNCH("scatter_kernel");
     return GCOO_OK;
 }
@@ -245,8 +298,14 @@ int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s)
         }
         if (rc) return rc;
     }
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s,
-                       h->nnzPerGroup(), h->groupIndex(), h->nGroup, h->d_status);
+    return launch_row_scan(h, s, true);
+}
+
+int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid)
+{
+    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, h->rowcnt, h->rowptr,
+                       h->M, h->cpg, h->nGroup, h->groupIndex(), h->nnzPerGroup(),
+                       h->d_status, clear_invalid ? 1 : 0);
     GCOO_CHECK_LAUNCH("scan_kernel");
     return GCOO_OK;
 }

@@ -36,12 +36,13 @@ void clear_error();
                                      name, hipGetErrorString(e_));            \
     } while (0)
 
-// Device-side status word written by the scan: total nnz (64-bit) and an
-// overflow flag (total exceeds int32 or the handle's entry capacity).
+// Device-side status word of a conversion: total nnz (64-bit, written by the
+// scan), an overflow flag (total exceeds int32) and an invalid-input flag
+// (COO entries out of range or duplicated).
 struct ConvStatus {
     long long total;
     int overflow;
-    int pad;
+    int invalid;
 };
 
 }  // namespace gcoo
@@ -59,7 +60,22 @@ struct gcoo_matrix {
     float *vals = nullptr;     // [cap]
     int *rowcol = nullptr;     // [2 * cap]: rows at 0, cols at cap
     int *idx = nullptr;        // [2 * (nGroup + 1)]: groupIndex, nnzPerGroup
-    int64_t cap = 0;           // entry capacity of vals / rowcol
+    int64_t cap = 0;           // entry capacity of vals / rowcol / seg
+
+    // Row segments: the same entries grouped by row, k ascending within a
+    // row, as (k, bits of val) pairs; row r's segment is
+    // seg[rowptr[r] .. rowptr[r+1]).  Group g's segments are contiguous and
+    // span exactly [groupIndex[g], groupIndex[g+1]).
+    int2 *seg = nullptr;       // [cap]
+    // BF16 storage form of the same segments (value bits in the low 16 bits
+    // of .y); allocated on first use, rebuilt after every conversion.
+    int2 *seg16 = nullptr;     // [cap]
+    bool seg16_valid = false;
+    int *rowptr = nullptr;     // [M + 1]
+    int *rowcnt = nullptr;     // [M] per-row counts (conversion scratch)
+
+    // multiply-kernel configuration (gcoo_set_spdm_config)
+    gcoo_spdm_config spdm_cfg = {0, 0, 0, 0};
 
     gcoo::ConvStatus *d_status = nullptr;  // device
     gcoo::ConvStatus *h_status = nullptr;  // pinned host mirror
@@ -79,6 +95,8 @@ namespace gcoo {
 // ---- conversion (gcoo_convert.hip) ----
 // Enqueue count + scan into h->idx and h->d_status.
 int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s);
+// Enqueue the row-pointer scan alone (rowcnt -> rowptr, groupIndex, ...).
+int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid);
 // Enqueue the scatter.  Writes nothing when d_status reports overflow.
 int launch_scatter(gcoo_matrix *h, const float *S, hipStream_t s);
 // Grow entry capacity to at least need (discarding contents).
@@ -91,11 +109,18 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols,
 // guard: when non-null, the kernel exits if guard->total > h->cap.
 int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s);
+// BF16 storage form: h->seg16 must hold the BF16 entry values.
+int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
+                     hipStream_t s);
 
-// ---- generators (gcoo_gen.hip) ----
+// ---- generators and element conversions (gcoo_gen.hip) ----
 int launch_fill(float *dst, int64_t n, uint64_t seed, uint64_t stream_id,
                 float scale, uint64_t keep_threshold, bool sparse,
                 hipStream_t s);
+// y = RNE bf16 of x (NaN kept quiet).
+int launch_f32_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);
+// seg16[i] = (seg[i].x, bf16 bits of seg[i].y) for i < n.
+int launch_seg_to_bf16(const int2 *seg, int2 *seg16, int64_t n, hipStream_t s);
 
 // Number of XCDs a launch is spread over (round-robin dispatch of
 // workgroups).  gfx950 (MI355X) has 8.

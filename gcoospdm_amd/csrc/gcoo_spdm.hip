@@ -174,6 +174,362 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_f32_wave_kernel(
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Storage types of D and of the entry values.  FP32: the reference's form.
+// BF16: the upper 16 bits of an FP32 (RNE-rounded by gcoo_f32_to_bf16_async);
+// widened exactly to FP32 before every FMA, so the arithmetic (and its order)
+// is unchanged and only the bytes moved are halved.
+// ---------------------------------------------------------------------------
+struct F32 {
+    static constexpr int bytes = 4;
+    __device__ static float val(int bits) { return __int_as_float(bits); }
+};
+struct BF16 {
+    static constexpr int bytes = 2;
+    __device__ static float val(int bits) { return __int_as_float(bits << 16); }
+};
+
+constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer, gfx9 dword3
+
+__device__ __forceinline__ float lo_bf16(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi_bf16(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+// VEC consecutive elements of type E at byte offset voff + soff of the
+// buffer, widened to FP32.
+template <typename E, int VEC>
+__device__ __forceinline__ void buf_load(float (&d)[VEC], __amdgpu_buffer_rsrc_t r, int voff,
+                                         int soff)
+{
+    constexpr int B = VEC * E::bytes;
+    if constexpr (B == 2) {
+        const unsigned short x = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+        d[0] = lo_bf16(x);
+    } else if constexpr (B == 4) {
+        const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        if constexpr (VEC == 1) {
+            d[0] = __uint_as_float(x);
+        } else {
+            d[0] = lo_bf16(x);
+            d[VEC - 1] = hi_bf16(x);
+        }
+    } else if constexpr (B == 8) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        const unsigned w[2] = {(unsigned)x[0], (unsigned)x[1]};
+        if constexpr (VEC == 2) {
+            d[0] = __uint_as_float(w[0]);
+            d[1] = __uint_as_float(w[1]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                d[2 * i] = lo_bf16(w[i]);
+                d[2 * i + 1] = hi_bf16(w[i]);
+            }
+        }
+    } else {
+        static_assert(B == 16, "16-byte loads at most");
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        const unsigned w[4] = {(unsigned)x[0], (unsigned)x[1], (unsigned)x[2], (unsigned)x[3]};
+        if constexpr (VEC == 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d[i] = __uint_as_float(w[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                d[2 * i] = lo_bf16(w[i]);
+                d[2 * i + 1] = hi_bf16(w[i]);
+            }
+        }
+    }
+}
+
+// Same, from LDS.
+template <typename E, int VEC>
+__device__ __forceinline__ void lds_load(float (&d)[VEC], const unsigned char *p)
+{
+    constexpr int B = VEC * E::bytes;
+    if constexpr (B == 2) {
+        d[0] = lo_bf16(*reinterpret_cast<const unsigned short *>(p));
+    } else if constexpr (B == 4) {
+        const unsigned x = *reinterpret_cast<const unsigned *>(p);
+        if constexpr (VEC == 1) {
+            d[0] = __uint_as_float(x);
+        } else {
+            d[0] = lo_bf16(x);
+            d[VEC - 1] = hi_bf16(x);
+        }
+    } else if constexpr (B == 8) {
+        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 x = *reinterpret_cast<const u32x2 *>(p);
+        if constexpr (VEC == 2) {
+            d[0] = __uint_as_float(x.x);
+            d[1] = __uint_as_float(x.y);
+        } else {
+            d[0] = lo_bf16(x.x); d[1] = hi_bf16(x.x);
+            d[2] = lo_bf16(x.y); d[3] = hi_bf16(x.y);
+        }
+    } else {
+        typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 x = *reinterpret_cast<const u32x4 *>(p);
+        const unsigned w[4] = {x.x, x.y, x.z, x.w};
+        if constexpr (VEC == 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d[i] = __uint_as_float(w[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                d[2 * i] = lo_bf16(w[i]);
+                d[2 * i + 1] = hi_bf16(w[i]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 2: row segments.  One wave per (group, column tile) as above, but
+// each of the group's rows is walked over its own k-sorted segment
+// (rowptr / seg, written by the conversion scatter) with one accumulator set:
+// no per-entry row dispatch.  D is read with buffer loads: the entry's row
+// offset k*N*bytes is a scalar offset, the lane's column offset a vector
+// offset, and lanes past N read zeros from the hardware range check.
+// ---------------------------------------------------------------------------
+template <typename E, int VEC, int U>
+__global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
+    const int *__restrict__ rowptr, const int2 *__restrict__ seg, int nGroup, int cpg,
+    int M, const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C,
+    int nGroupBlocks, const ConvStatus *__restrict__ guard, long long cap)
+{
+    if (guard != nullptr && guard->total > cap) return;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = L / nGroupBlocks;
+    const int gb = L - ct * nGroupBlocks;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = __lane_id();
+    const int g = gb * kSpdmWaves + wave;
+    if (g >= nGroup) return;
+    const int j = ct * (64 * VEC) + lane * VEC;
+    const bool active = j < N;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
+    const int voff = active ? j * E::bytes : (int)dBytes;  // out of range -> zeros
+    const int rowBytes = N * E::bytes;
+    const int r0 = g * cpg;
+    const int r1 = min(r0 + cpg, M);
+    for (int r = r0; r < r1; ++r) {
+        float acc[VEC];
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[v] = 0.0f;
+        const int p0 = rowptr[r];
+        const int p1 = rowptr[r + 1];
+        int p = p0;
+        for (; p + U <= p1; p += U) {
+            float a[U];
+            float d[U][VEC];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int2 kv = seg[p + u];
+                a[u] = E::val(kv.y);
+                buf_load<E, VEC>(d[u], rsrc, voff, kv.x * rowBytes);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) fmac(acc[v], a[u], d[u][v]);
+        }
+        for (; p < p1; ++p) {
+            const int2 kv = seg[p];
+            float d[VEC];
+            buf_load<E, VEC>(d, rsrc, voff, kv.x * rowBytes);
+            const float a = E::val(kv.y);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) fmac(acc[v], a, d[v]);
+        }
+        if (active) {
+            if constexpr (VEC <= 4) {
+                store_vec<VEC>(C + (size_t)r * N + j, acc);
+            } else {
+                const float (&lo)[4] = *reinterpret_cast<const float(*)[4]>(&acc[0]);
+                const float (&hi)[4] = *reinterpret_cast<const float(*)[4]>(&acc[4]);
+                store_vec<4>(C + (size_t)r * N + j, lo);
+                store_vec<4>(C + (size_t)r * N + j + 4, hi);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Variant 3: LDS-staged D panel.  A workgroup of WAVES waves owns a panel of
+// WAVES*RPW consecutive rows and a TN = 64*VEC column tile.  It walks k in
+// chunks of TK: every chunk's D rows D[k0 .. k0+TK)[tile] are staged once in
+// LDS (double-buffered: the next chunk's global loads are in flight while the
+// current one is consumed) and read from there by every row of the panel, so
+// each staged value serves panel_rows * density FMAs instead of one.  Each
+// wave keeps RPW rows' accumulators in VGPRs and walks every row's k-sorted
+// segment across the chunks in order: per row the FMA chain is still k
+// ascending, bit for bit the reference's.  The next entry of every row is
+// held in SGPRs so a chunk with no entry for a row costs one compare.
+// ---------------------------------------------------------------------------
+template <typename E, int VEC, int RPW, int WAVES, int TK>
+__global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
+    const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int K,
+    const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C,
+    int nPanels, const ConvStatus *__restrict__ guard, long long cap)
+{
+    constexpr int TN = 64 * VEC;                 // columns per tile
+    constexpr int ROWB = TN * E::bytes;          // bytes of one staged D row
+    constexpr int NT = WAVES * 64;
+    constexpr int CHUNK_P = TK * ROWB / 16;      // 16-byte pieces per chunk
+    constexpr int PER_T = CHUNK_P / NT;          // pieces per thread per chunk
+    constexpr int PIECES_ROW = ROWB / 16;
+    static_assert(CHUNK_P % NT == 0, "chunk must split evenly over threads");
+    __shared__ f32x4 sD[2][CHUNK_P];
+
+    if (guard != nullptr && guard->total > cap) return;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = L / nPanels;
+    const int panel = L - ct * nPanels;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = __lane_id();
+    const int col0 = ct * TN;
+    const int j = col0 + lane * VEC;
+    const bool active = j < N;
+
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
+    constexpr int EPP = 16 / E::bytes;           // elements per 16-byte piece
+    int s_voff[PER_T];
+    int s_row[PER_T];
+#pragma unroll
+    for (int i = 0; i < PER_T; ++i) {
+        const int f = threadIdx.x + i * NT;
+        s_row[i] = f / PIECES_ROW;
+        const int c = col0 + (f % PIECES_ROW) * EPP;
+        s_voff[i] = c < N ? c * E::bytes : (int)dBytes;  // past N -> zeros
+    }
+    const int rowBytes = N * E::bytes;
+
+    const int rbase = panel * (WAVES * RPW) + wave * RPW;
+    int p[RPW], pe[RPW], kn[RPW];
+    float vn[RPW];
+    float acc[RPW][VEC];
+#pragma unroll
+    for (int q = 0; q < RPW; ++q) {
+        const int r = rbase + q;
+        p[q] = r < M ? rowptr[r] : 0;
+        pe[q] = r < M ? rowptr[r + 1] : 0;
+        if (p[q] < pe[q]) {
+            const int2 e = seg[p[q]];
+            kn[q] = e.x;
+            vn[q] = E::val(e.y);
+        } else {
+            kn[q] = INT_MAX;
+            vn[q] = 0.0f;
+        }
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) acc[q][v] = 0.0f;
+    }
+
+    f32x4 stage[PER_T];
+    auto issue = [&](int k0) {
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) {
+            const int k = k0 + s_row[i];
+            stage[i] = __builtin_bit_cast(
+                f32x4, __builtin_amdgcn_raw_buffer_load_b128(
+                           rsrc, k < K ? s_voff[i] : (int)dBytes, k < K ? k * rowBytes : 0, 0));
+        }
+    };
+    auto commit = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) sD[buf][threadIdx.x + i * NT] = stage[i];
+    };
+
+    issue(0);
+    commit(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += TK) {
+        const int kend = k0 + TK;
+        const bool more = kend < K;
+        if (more) issue(kend);
+        const unsigned char *sb =
+            reinterpret_cast<const unsigned char *>(sD[buf]) + lane * VEC * E::bytes;
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            while (kn[q] < kend) {
+                float d[VEC];
+                lds_load<E, VEC>(d, sb + (kn[q] - k0) * ROWB);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) fmac(acc[q][v], vn[q], d[v]);
+                ++p[q];
+                if (p[q] < pe[q]) {
+                    const int2 e = seg[p[q]];
+                    kn[q] = e.x;
+                    vn[q] = E::val(e.y);
+                } else {
+                    kn[q] = INT_MAX;
+                }
+            }
+        }
+        if (more) commit(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            if (rbase + q < M) {
+                float *dst = C + (size_t)(rbase + q) * N + j;
+                if constexpr (VEC <= 4) {
+                    store_vec<VEC>(dst, acc[q]);
+                } else {
+                    const float (&lo)[4] = *reinterpret_cast<const float(*)[4]>(&acc[q][0]);
+                    const float (&hi)[4] = *reinterpret_cast<const float(*)[4]>(&acc[q][4]);
+                    store_vec<4>(dst, lo);
+                    store_vec<4>(dst + 4, hi);
+                }
+            }
+        }
+    }
+}
+
+template <typename E, int VEC, int RPW, int WAVES, int TK>
+int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+               const ConvStatus *guard, hipStream_t s)
+{
+    const int panelRows = WAVES * RPW;
+    const int nPanels = (h->M + panelRows - 1) / panelRows;
+    const int nColTiles = (N + 64 * VEC - 1) / (64 * VEC);
+    const long long nblocks = (long long)nPanels * nColTiles;
+    if (nblocks > INT32_MAX)
+        return set_error(GCOO_ERR_OVERFLOW, "grid of %lld workgroups too large", nblocks);
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    hipLaunchKernelGGL((spdm_lds_kernel<E, VEC, RPW, WAVES, TK>), dim3((unsigned)nblocks),
+                       dim3(WAVES * 64), 0, s, h->rowptr, seg, h->M, h->K, D,
+                       (unsigned)dBytes, N, C, nPanels, guard, (long long)h->cap);
+    GCOO_CHECK_LAUNCH("spdm_lds_kernel");
+    return GCOO_OK;
+}
+
+template <typename E, int VEC>
+int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                  const ConvStatus *guard, hipStream_t s)
+{
+    constexpr int U = 8;
+    const int nGroupBlocks = (h->nGroup + kSpdmWaves - 1) / kSpdmWaves;
+    const int tileCols = 64 * VEC;
+    const int nColTiles = (N + tileCols - 1) / tileCols;
+    const long long nblocks = (long long)nGroupBlocks * nColTiles;
+    if (nblocks > INT32_MAX)
+        return set_error(GCOO_ERR_OVERFLOW, "grid of %lld workgroups too large", nblocks);
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    hipLaunchKernelGGL((spdm_rowseg_kernel<E, VEC, U>), dim3((unsigned)nblocks),
+                       dim3(kSpdmThreads), 0, s, h->rowptr, seg, h->nGroup, h->cpg, h->M, D,
+                       (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap);
+    GCOO_CHECK_LAUNCH("spdm_rowseg_kernel");
+    return GCOO_OK;
+}
+
 template <int CPG, int VEC>
 int launch_wave(const gcoo_matrix *h, const float *D, int N, float *C,
                 const ConvStatus *guard, hipStream_t s)
@@ -194,13 +550,73 @@ int launch_wave(const gcoo_matrix *h, const float *D, int N, float *C,
 }
 
 template <int CPG>
-int launch_cpg(const gcoo_matrix *h, const float *D, int N, float *C,
-               const ConvStatus *guard, hipStream_t s)
+int launch_group_order(const gcoo_matrix *h, int vec, const float *D, int N, float *C,
+                       const ConvStatus *guard, hipStream_t s)
 {
-    const uintptr_t al = reinterpret_cast<uintptr_t>(D) | reinterpret_cast<uintptr_t>(C);
-    if (N % 4 == 0 && (al & 15u) == 0) return launch_wave<CPG, 4>(h, D, N, C, guard, s);
-    if (N % 2 == 0 && (al & 7u) == 0) return launch_wave<CPG, 2>(h, D, N, C, guard, s);
+    if (vec >= 4) return launch_wave<CPG, 4>(h, D, N, C, guard, s);
+    if (vec == 2) return launch_wave<CPG, 2>(h, D, N, C, guard, s);
     return launch_wave<CPG, 1>(h, D, N, C, guard, s);
+}
+
+// Widest vector of `elemBytes`-byte elements (at most 16 bytes per lane) that
+// N and the alignment of D and C allow.
+int max_vec(const void *D, int N, const float *C, int elemBytes)
+{
+    const uintptr_t aD = reinterpret_cast<uintptr_t>(D);
+    const uintptr_t aC = reinterpret_cast<uintptr_t>(C);
+    for (int v = 16 / elemBytes; v > 1; v >>= 1) {
+        const unsigned dAl = (unsigned)(v * elemBytes) - 1;
+        const unsigned cAl = (unsigned)(v * 4 > 16 ? 16 : v * 4) - 1;
+        if (N % v == 0 && (aD & dAl) == 0 && (aC & cAl) == 0) return v;
+    }
+    return 1;
+}
+
+template <typename E>
+int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                 const ConvStatus *guard, hipStream_t s)
+{
+    const gcoo_spdm_config &cfg = h->spdm_cfg;
+    const int vmax = max_vec(D, N, C, E::bytes);
+    const int vdef = E::bytes == 4 ? 2 : 4;
+    int vec = cfg.vec ? cfg.vec : vdef;
+    if (vec > vmax) vec = vmax;
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    if (dBytes >= 0x7fffffffull)
+        return set_error(GCOO_ERR_UNSUPPORTED,
+                         "dense operand of %llu bytes: the buffer-addressed kernels need < 2 GiB",
+                         dBytes);
+    int variant = cfg.variant ? cfg.variant : 2;
+    if (variant == 1) variant = 2;
+    // variant 3 stages 16-byte pieces of D rows: it needs whole pieces per
+    // row (N * bytes % 16 == 0) and a 16-byte aligned D
+    if (variant == 3 && ((N * E::bytes) % 16 != 0 || (reinterpret_cast<uintptr_t>(D) & 15u) != 0))
+        variant = 2;
+    if (variant == 3) {
+        const int waves = cfg.waves ? cfg.waves : 16;
+        if constexpr (E::bytes == 4) {
+            if (vec >= 4) {
+                if (waves <= 8) return launch_lds<E, 4, 16, 8, 64>(h, seg, D, N, C, guard, s);
+                return launch_lds<E, 4, 16, 16, 64>(h, seg, D, N, C, guard, s);
+            }
+            if (vec == 2) return launch_lds<E, 2, 16, 16, 128>(h, seg, D, N, C, guard, s);
+            return launch_lds<E, 1, 32, 16, 256>(h, seg, D, N, C, guard, s);
+        } else {
+            if (vec >= 8) return launch_lds<E, 8, 8, 16, 64>(h, seg, D, N, C, guard, s);
+            if (vec == 4) return launch_lds<E, 4, 16, 16, 128>(h, seg, D, N, C, guard, s);
+            return launch_lds<E, 2, 16, 16, 256>(h, seg, D, N, C, guard, s);
+        }
+    }
+    if constexpr (E::bytes == 4) {
+        if (vec >= 4) return launch_rowseg<E, 4>(h, seg, D, N, C, guard, s);
+        if (vec == 2) return launch_rowseg<E, 2>(h, seg, D, N, C, guard, s);
+        return launch_rowseg<E, 1>(h, seg, D, N, C, guard, s);
+    } else {
+        if (vec >= 8) return launch_rowseg<E, 8>(h, seg, D, N, C, guard, s);
+        if (vec == 4) return launch_rowseg<E, 4>(h, seg, D, N, C, guard, s);
+        if (vec == 2) return launch_rowseg<E, 2>(h, seg, D, N, C, guard, s);
+        return launch_rowseg<E, 1>(h, seg, D, N, C, guard, s);
+    }
 }
 
 }  // namespace
@@ -209,12 +625,28 @@ int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s)
 {
     if (h->nGroup == 0 || N == 0) return GCOO_OK;
-    switch (h->cpg) {
-    case 1: return launch_cpg<1>(h, D, N, C, guard, s);
-    case 2: return launch_cpg<2>(h, D, N, C, guard, s);
-    case 4: return launch_cpg<4>(h, D, N, C, guard, s);
-    default: return set_error(GCOO_ERR_INVALID, "cpg %d not supported", h->cpg);
+    const gcoo_spdm_config &cfg = h->spdm_cfg;
+    const unsigned long long dBytes = (unsigned long long)h->K * N * 4;
+    // Variant 1 (group order, 64-bit addressing) is also the path for dense
+    // operands of 2 GiB or more.
+    if (cfg.variant == 1 || dBytes >= 0x7fffffffull) {
+        int vec = cfg.vec ? cfg.vec : 4;
+        const int vmax = max_vec(D, N, C, 4);
+        if (vec > vmax) vec = vmax;
+        switch (h->cpg) {
+        case 1: return launch_group_order<1>(h, vec, D, N, C, guard, s);
+        case 2: return launch_group_order<2>(h, vec, D, N, C, guard, s);
+        case 4: return launch_group_order<4>(h, vec, D, N, C, guard, s);
+        default: return set_error(GCOO_ERR_INVALID, "cpg %d not supported", h->cpg);
+        }
     }
+    return launch_typed<F32>(h, h->seg, D, N, C, guard, s);
+}
+
+int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C, hipStream_t s)
+{
+    if (h->nGroup == 0 || N == 0) return GCOO_OK;
+    return launch_typed<BF16>(h, h->seg16, D, N, C, nullptr, s);
 }
 
 }  // namespace gcoo

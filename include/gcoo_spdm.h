@@ -139,6 +139,42 @@ int gcoo_spdm_f32_async(gcoo_handle h, const float *D, int K, int N, float *C,
 int gcoo_spdm_f32(gcoo_handle h, const float *D, int K, int N, float *C,
                   gcoo_stream s);
 
+/* BF16 storage form (no counterpart in the reference, which is FP32 only):
+ * S's entry values and D stored as BF16 -- the upper 16 bits of an FP32,
+ * rounded to nearest-even by gcoo_f32_to_bf16_async -- widened exactly to
+ * FP32 and accumulated with the same per-output fmaf chain (k ascending) in
+ * FP32; C is FP32.  Moves half the bytes of D.  The BF16 values of S are
+ * derived from the converted FP32 entries on first use after a conversion.
+ * D is K x N BF16 (uint16 bits), device pointer. */
+int gcoo_f32_to_bf16_async(const float *x, uint16_t *y, int64_t n, gcoo_stream s);
+int gcoo_spdm_bf16_async(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
+                         gcoo_stream s);
+int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
+                   gcoo_stream s);
+
+/* Multiply-kernel tuning of one handle (the reference fixes these at compile
+ * time: COLUMN_PER_GROUP / BS in src/constants.h:4-5, grid-searched by
+ * src/autotune.py:61-86).  Zero fields mean "choose automatically".
+ *   variant 1  group order: walks the group's (k, r) list with one wave per
+ *              group and per-row accumulators;
+ *   variant 2  row segments: walks each row's k-sorted segment with one
+ *              accumulator set per row (no per-entry row dispatch);
+ *   variant 3  LDS-staged: a workgroup stages k-chunks of a D column panel
+ *              in LDS, shared by all rows of a row panel.
+ *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
+ *              D per lane; a wave covers 64*vec columns.
+ *   waves      waves per workgroup (1..16).
+ *   chunk      k-chunk length of variant 3.
+ * All variants give bit-identical C. */
+typedef struct gcoo_spdm_config {
+    int variant;
+    int vec;
+    int waves;
+    int chunk;
+} gcoo_spdm_config;
+int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg);
+int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
+
 /* Convert S then multiply: ShGCOOSpDM::mutiplyBy (src/ShGCOOSpDM.cu:131-180),
  * without the reference's per-call allocations once capacity is reached. */
 int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,

@@ -192,6 +192,103 @@ def test_reference_api_mirror(dev):
         algo.runKernelTest(D[:-1], K - 1, N, C1)   # K mismatch -> error, no crash
 
 
+@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("vec", [1, 2, 4])
+@pytest.mark.parametrize("M,K,N,dens", [(130, 257, 520, 0.05), (61, 1030, 200, 0.3),
+                                        (1000, 900, 64, 0.01)])
+def test_kernel_variants_bit_exact(dev, variant, vec, M, K, N, dens):
+    """Every multiply variant / tile width gives the reference's bits."""
+    import torch
+    print(f"variant={variant} vec={vec} M={M} K={K} N={N}", flush=True)
+    rng = np.random.default_rng(M + K + N)
+    S = H.rng_sparse(rng, M, K, dens)
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    g = oracle.convert(S, 4)
+    m = G.GCOOMatrix(M, K, 4).convert_dense(to_dev(S, dev))
+    m.set_config(variant=variant, vec=vec)
+    C = torch.full((M, N), float("nan"), device=dev)
+    m.spdm(to_dev(D, dev), C)
+    assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+def test_row_segments_match_gcoo(dev):
+    """The per-row segments the conversion emits hold exactly the GCOO
+    entries of each row, k ascending (checked through the product of
+    S = permutation-like sparse matrices)."""
+    import torch
+    M, K = 50, 40
+    S = np.zeros((M, K), np.float32)
+    rng = np.random.default_rng(12)
+    for r in range(M):
+        ks = rng.choice(K, size=rng.integers(0, 6), replace=False)
+        S[r, ks] = rng.uniform(-1, 1, ks.size)
+    D = np.eye(K, dtype=np.float32)   # C = S exactly
+    m = G.GCOOMatrix(M, K, 4).convert_dense(to_dev(S, dev))
+    m.set_config(variant=2)
+    C = torch.empty((M, K), device=dev)
+    m.spdm(to_dev(D, dev), C)
+    assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(oracle.convert(S, 4), D))
+
+
+def _coo_of(S, rng, extra_zeros=0):
+    r, k = np.nonzero(S != 0.0)
+    v = S[r, k]
+    if extra_zeros:   # explicit zeros in the COO input are dropped
+        zr = rng.integers(0, S.shape[0], extra_zeros)
+        zk = rng.integers(0, S.shape[1], extra_zeros)
+        keep = S[zr, zk] == 0.0
+        zr, zk = zr[keep], zk[keep]
+        key = np.unique(zr.astype(np.int64) * S.shape[1] + zk)
+        zr, zk = (key // S.shape[1]).astype(np.int64), (key % S.shape[1]).astype(np.int64)
+        r = np.concatenate([r, zr])
+        k = np.concatenate([k, zk])
+        v = np.concatenate([v, np.zeros(zr.size, np.float32)])
+    perm = rng.permutation(r.size)
+    return (r[perm].astype(np.int32), k[perm].astype(np.int32), v[perm].astype(np.float32))
+
+
+@pytest.mark.parametrize("cpg", [1, 2, 4])
+@pytest.mark.parametrize("M,K,dens", [(37, 50, 0.1), (200, 3000, 0.05), (9, 20000, 0.5),
+                                      (64, 64, 0.0)])
+def test_coo_conversion_matches_dense(dev, M, K, dens, cpg):
+    """Shuffled COO input (with explicit zeros) gives the dense path's
+    arrays bit for bit; rows of > 64 and > 16384 entries take the LDS and
+    global-memory sorts."""
+    import torch
+    print(f"coo M={M} K={K} dens={dens} cpg={cpg}", flush=True)
+    rng = np.random.default_rng(M * 31 + K + cpg)
+    S = H.rng_sparse(rng, M, K, dens)
+    r, k, v = _coo_of(S, rng, extra_zeros=17)
+    g = oracle.convert(S, cpg)
+    m = G.GCOOMatrix(M, K, cpg).convert_coo(to_dev(r, dev), to_dev(k, dev), to_dev(v, dev))
+    check_arrays(m, g)
+    N = 70
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    C = torch.empty((M, N), device=dev)
+    m.spdm(to_dev(D, dev), C)
+    assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+def test_coo_rejects_bad_input(dev):
+    r = np.array([0, 1, 1], np.int32)
+    k = np.array([2, 3, 3], np.int32)
+    v = np.array([1.0, 2.0, 3.0], np.float32)
+    m = G.GCOOMatrix(4, 5, 4)
+    with pytest.raises(G.GCOOError) as e:
+        m.convert_coo(to_dev(r, dev), to_dev(k, dev), to_dev(v, dev))
+    assert e.value.code == G.GCOO_ERR_INVALID
+    k2 = np.array([2, 3, 5], np.int32)   # column 5 outside K = 5
+    with pytest.raises(G.GCOOError) as e:
+        m.convert_coo(to_dev(r, dev), to_dev(k2, dev), to_dev(v, dev))
+    assert e.value.code == G.GCOO_ERR_INVALID
+    # the handle is usable after a rejected input
+    r3 = np.array([0, 3], np.int32)
+    k3 = np.array([4, 0], np.int32)
+    v3 = np.array([1.5, -2.0], np.float32)
+    m.convert_coo(to_dev(r3, dev), to_dev(k3, dev), to_dev(v3, dev))
+    assert m.nnz == 2
+
+
 MAIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                     "gcoospdm_amd", "host", "main")
 
