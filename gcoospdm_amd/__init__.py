@@ -78,6 +78,16 @@ _SIGNATURES = [
     ("gcoo_f32_to_bf16_async", _I, [_P, _P, _I64, _P]),
     ("gcoo_spdm_bf16_async", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_spdm_bf16", _I, [_P, _P, _I, _I, _P, _P]),
+    ("gcoo_batch_create", _I, [_PP]),
+    ("gcoo_batch_destroy", _I, [_P]),
+    ("gcoo_batch_clear", _I, [_P]),
+    ("gcoo_batch_add", _I, [_P, _P, _P, _I, _I, _P, _I, _I, _c.POINTER(_I)]),
+    ("gcoo_batch_commit", _I, [_P, _P]),
+    ("gcoo_batch_run_async", _I, [_P, _P]),
+    ("gcoo_batch_graph_capture", _I, [_P, _P]),
+    ("gcoo_batch_graph_launch", _I, [_P, _P]),
+    ("gcoo_batch_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I64), _c.POINTER(_I)]),
+    ("gcoo_batch_job_tile", _I, [_P, _I, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I)]),
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -275,6 +285,81 @@ class GCOOMatrix:
         _check(lib().gcoo_multiply_by(self._h, _ptr(S), _ptr(D), K, N, _ptr(C),
                                       _stream(stream)), "gcoo_multiply_by")
         return C
+
+
+class Batch:
+    """Many independent multiplies in one persistent launch (gcoo_batch_*).
+
+    add() keeps references to the handle and tensors so they outlive the
+    batch's runs."""
+
+    def __init__(self):
+        b = ctypes.c_void_p()
+        _check(lib().gcoo_batch_create(ctypes.byref(b)), "gcoo_batch_create")
+        self._b = b
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_b", None) is not None and self._b.value:
+            lib().gcoo_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def clear(self):
+        _check(lib().gcoo_batch_clear(self._b), "gcoo_batch_clear")
+        self._keep = []
+
+    def add(self, m: "GCOOMatrix", D, C, rows=0, vec=0) -> int:
+        import torch
+        _require_dev(C, "C", torch.float32)
+        _require_dev(D, "D")
+        if D.dtype == torch.float32:
+            dtype = 0
+        elif D.dtype in (torch.bfloat16, torch.int16):
+            dtype = 1
+        else:
+            raise ValueError("D must be float32 or BF16 bits")
+        K, N = D.shape
+        if K != m.K or tuple(C.shape) != (m.M, N):
+            raise ValueError("shape mismatch")
+        idx = ctypes.c_int()
+        _check(lib().gcoo_batch_add(self._b, m.handle, _ptr(D), dtype, N, _ptr(C), rows, vec,
+                                    ctypes.byref(idx)), "gcoo_batch_add")
+        self._keep.append((m, D, C))
+        return idx.value
+
+    def commit(self, stream=None):
+        _check(lib().gcoo_batch_commit(self._b, _stream(stream)), "gcoo_batch_commit")
+        return self
+
+    def run(self, stream=None):
+        _check(lib().gcoo_batch_run_async(self._b, _stream(stream)), "gcoo_batch_run_async")
+
+    def capture(self, stream):
+        _check(lib().gcoo_batch_graph_capture(self._b, _stream(stream)),
+               "gcoo_batch_graph_capture")
+
+    def replay(self, stream=None):
+        _check(lib().gcoo_batch_graph_launch(self._b, _stream(stream)),
+               "gcoo_batch_graph_launch")
+
+    def info(self):
+        j, g = ctypes.c_int(), ctypes.c_int()
+        it = ctypes.c_int64()
+        _check(lib().gcoo_batch_info(self._b, ctypes.byref(j), ctypes.byref(it),
+                                     ctypes.byref(g)), "gcoo_batch_info")
+        return dict(jobs=j.value, items=it.value, grid_blocks=g.value)
+
+    def job_tile(self, job):
+        r, v, ln = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().gcoo_batch_job_tile(self._b, job, ctypes.byref(r), ctypes.byref(v),
+                                         ctypes.byref(ln)), "gcoo_batch_job_tile")
+        return dict(rows=r.value, vec=v.value, lanes=ln.value)
 
 
 class ShGemm:

@@ -1,0 +1,436 @@
+// gcoo_batch.hip -- many independent SpDM jobs in one persistent launch.
+//
+// The reference runs one multiply per launch with a fixed tile
+// (src/ShGCOOSpDM.cu:153-167, src/constants.h:4-5).  Here a batch of jobs
+// C_i = S_i * D_i (each S_i a converted handle, D_i FP32 or BF16, any
+// M_i / K_i / N_i) is cut into work items -- `rows` consecutive rows x one
+// column tile of 64*vec columns (or two or four row streams of 32 / 16 lanes
+// when N_i is narrow) -- and a persistent kernel sized to the machine pulls
+// items from a device counter until none are left.  Every output element is
+// the same fmaf chain (row r's entries, k ascending) as in the plain kernels,
+// so a job's C is bit-identical whether it runs alone or in any batch.
+//
+// The run is one counter-reset kernel followed by the persistent kernel; it
+// is graph-capturable (no memset, no host synchronisation, no allocation).
+#include "gcoo_internal.h"
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+namespace gcoo {
+namespace {
+
+constexpr int kBatchWaves = 4;
+constexpr int kBatchThreads = 64 * kBatchWaves;
+
+// Device descriptor of one job (16-byte aligned).
+struct JobDesc {
+    const int *rowptr;
+    const int2 *seg;
+    const void *D;
+    float *C;
+    int M, K, N;
+    int bytes;        // 4 (FP32) or 2 (BF16)
+    int vec;          // columns per lane
+    int lanes;        // lanes per row stream: 64, 32 or 16
+    int rows;         // rows per work item (multiple of 64 / lanes)
+    int colTiles;     // ceil(N / (lanes * vec))
+    unsigned dBytes;  // K * N * bytes (< 2^31)
+    int pad;
+};
+
+__device__ __forceinline__ float lo16(unsigned w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float hi16(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
+
+// Up to 8 widened elements at byte offset voff (per lane) of the buffer.
+__device__ __forceinline__ void load_elems(float (&d)[8], __amdgpu_buffer_rsrc_t r, int voff,
+                                           int bytes, int vec)
+{
+    const int B = bytes * vec;
+    if (B == 16) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+        if (bytes == 4) {
+            d[0] = __uint_as_float(x[0]); d[1] = __uint_as_float(x[1]);
+            d[2] = __uint_as_float(x[2]); d[3] = __uint_as_float(x[3]);
+        } else {
+            d[0] = lo16(x[0]); d[1] = hi16(x[0]); d[2] = lo16(x[1]); d[3] = hi16(x[1]);
+            d[4] = lo16(x[2]); d[5] = hi16(x[2]); d[6] = lo16(x[3]); d[7] = hi16(x[3]);
+        }
+    } else if (B == 8) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
+        if (bytes == 4) {
+            d[0] = __uint_as_float(x[0]); d[1] = __uint_as_float(x[1]);
+        } else {
+            d[0] = lo16(x[0]); d[1] = hi16(x[0]); d[2] = lo16(x[1]); d[3] = hi16(x[1]);
+        }
+    } else if (B == 4) {
+        const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0);
+        if (bytes == 4) {
+            d[0] = __uint_as_float(x);
+        } else {
+            d[0] = lo16(x); d[1] = hi16(x);
+        }
+    } else {
+        d[0] = lo16(__builtin_amdgcn_raw_buffer_load_b16(r, voff, 0, 0));
+    }
+}
+
+__device__ __forceinline__ void fmac_v(float &acc, float a, float d)
+{
+    asm("v_fmac_f32 %0, %1, %2" : "+v"(acc) : "v"(a), "v"(d));
+}
+
+// One row stream of a work item: lane (sub, lc) walks row r's segment and
+// accumulates columns j .. j+vec-1.
+template <int VEC>
+__device__ __forceinline__ void run_row(const JobDesc &jd, __amdgpu_buffer_rsrc_t rsrc, int r,
+                                        int j, bool active)
+{
+    constexpr int U = 4;
+    float acc[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] = 0.0f;
+    const bool valid = r < jd.M;
+    int p = valid ? jd.rowptr[r] : 0;
+    const int pend = valid ? jd.rowptr[r + 1] : 0;
+    // byte offsets are unsigned: a lane past N starts at dBytes, so every one
+    // of its loads is out of range and returns zeros
+    const unsigned rowBytes = (unsigned)jd.N * (unsigned)jd.bytes;
+    const unsigned voffc = active ? (unsigned)j * (unsigned)jd.bytes : jd.dBytes;
+    const bool bf = jd.bytes == 2;
+    while (__builtin_amdgcn_ballot_w64(p < pend) != 0) {
+        int2 e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            e[u] = (p + u < pend) ? jd.seg[p + u] : make_int2(0, 0);
+        float d[U][8];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned voff =
+                (p + u < pend) ? voffc + (unsigned)e[u].x * rowBytes : jd.dBytes;
+            load_elems(d[u], rsrc, (int)voff, jd.bytes, VEC);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (p + u < pend) {
+                const float a = bf ? __int_as_float(e[u].y << 16) : __int_as_float(e[u].y);
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) fmac_v(acc[v], a, d[u][v]);
+            }
+        }
+        p += U;
+    }
+    if (valid && active) {
+        float *dst = jd.C + (size_t)r * jd.N + j;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) __builtin_nontemporal_store(acc[v], dst + v);
+    }
+}
+
+__global__ __launch_bounds__(64) void batch_reset_kernel(unsigned *counter) { *counter = 0; }
+
+__global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
+    const JobDesc *__restrict__ jobs, const int *__restrict__ itemStart, int nJobs,
+    int totalItems, unsigned *__restrict__ counter)
+{
+    const int lane = __lane_id();
+    for (;;) {
+        int it = 0;
+        if (lane == 0) it = (int)atomicAdd(counter, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);
+        if (it >= totalItems) return;
+        // job = last index with itemStart[job] <= it (scalar binary search)
+        int lo = 0, hi = nJobs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (itemStart[mid] <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const JobDesc jd = jobs[lo];
+        const int local = it - itemStart[lo];
+        const int rb = local / jd.colTiles;
+        const int ct = local - rb * jd.colTiles;
+        const int sub = lane / jd.lanes;
+        const int lc = lane - sub * jd.lanes;
+        const int streams = 64 / jd.lanes;
+        const int j = (ct * jd.lanes + lc) * jd.vec;
+        const bool active = j < jd.N;
+        const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<void *>(jd.D), (short)0, (int)jd.dBytes, 0x00020000);
+        const int r0 = rb * jd.rows;
+        for (int q = 0; q < jd.rows; q += streams) {
+            const int r = r0 + q + sub;
+            switch (jd.vec) {
+            case 1: run_row<1>(jd, rsrc, r, j, active); break;
+            case 2: run_row<2>(jd, rsrc, r, j, active); break;
+            case 4: run_row<4>(jd, rsrc, r, j, active); break;
+            default: run_row<8>(jd, rsrc, r, j, active); break;
+            }
+        }
+    }
+}
+
+}  // namespace
+}  // namespace gcoo
+
+using namespace gcoo;
+
+struct gcoo_batch {
+    std::vector<JobDesc> jobs;
+    std::vector<int> itemStart;  // nJobs + 1
+    std::vector<gcoo_handle> handles;
+    JobDesc *d_jobs = nullptr;
+    int *d_itemStart = nullptr;
+    unsigned *d_counter = nullptr;
+    size_t capJobs = 0;
+    bool committed = false;
+    int gridBlocks = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+};
+
+namespace {
+
+int pick_lanes(int N, int vec)
+{
+    const int cols = (N + vec - 1) / vec;
+    if (cols <= 16) return 16;
+    if (cols <= 32) return 32;
+    return 64;
+}
+
+void destroy_graph(gcoo_batch *b)
+{
+    if (b->graphExec) (void)hipGraphExecDestroy(b->graphExec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->graphExec = nullptr;
+    b->graph = nullptr;
+}
+
+int enqueue_run(gcoo_batch *b, hipStream_t s)
+{
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
+    GCOO_CHECK_LAUNCH("batch_reset_kernel");
+    const int nJobs = (int)b->jobs.size();
+    hipLaunchKernelGGL(batch_persistent_kernel, dim3(b->gridBlocks), dim3(kBatchThreads), 0, s,
+                       b->d_jobs, b->d_itemStart, nJobs, b->itemStart[nJobs], b->d_counter);
+    GCOO_CHECK_LAUNCH("batch_persistent_kernel");
+    return GCOO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcoo_batch_create(gcoo_batch_t *out)
+{
+    if (!out) return set_error(GCOO_ERR_INVALID, "null batch out");
+    auto *b = new (std::nothrow) gcoo_batch;
+    if (!b) return set_error(GCOO_ERR_ALLOC, "batch");
+    if (hipMalloc(&b->d_counter, 256) != hipSuccess) {
+        (void)hipGetLastError();
+        delete b;
+        return set_error(GCOO_ERR_ALLOC, "batch counter");
+    }
+    b->itemStart.push_back(0);
+    *out = b;
+    return GCOO_OK;
+}
+
+int gcoo_batch_destroy(gcoo_batch_t b)
+{
+    if (!b) return GCOO_OK;
+    destroy_graph(b);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->d_itemStart) (void)hipFree(b->d_itemStart);
+    if (b->d_counter) (void)hipFree(b->d_counter);
+    delete b;
+    return GCOO_OK;
+}
+
+int gcoo_batch_clear(gcoo_batch_t b)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    destroy_graph(b);
+    b->jobs.clear();
+    b->handles.clear();
+    b->itemStart.assign(1, 0);
+    b->committed = false;
+    return GCOO_OK;
+}
+
+int gcoo_batch_add(gcoo_batch_t b, gcoo_handle h, const void *D, int dtype, int N, float *C,
+                   int rows, int vec, int *job_index)
+{
+    if (!b || !h) return set_error(GCOO_ERR_INVALID, "null batch or handle");
+    if (!h->converted) return set_error(GCOO_ERR_STATE, "matrix not converted");
+    if (dtype != 0 && dtype != 1) return set_error(GCOO_ERR_INVALID, "dtype %d (0 f32, 1 bf16)", dtype);
+    if (N < 0 || (N > 0 && h->M > 0 && (!C || (!D && h->K > 0))))
+        return set_error(GCOO_ERR_INVALID, "bad operands");
+    const int bytes = dtype == 0 ? 4 : 2;
+    const unsigned long long dBytes = (unsigned long long)h->K * N * bytes;
+    if (dBytes >= 0x7fffffffull) return set_error(GCOO_ERR_UNSUPPORTED, "dense operand >= 2 GiB");
+    if (rows < 0 || rows > 64) return set_error(GCOO_ERR_INVALID, "rows %d (0..64)", rows);
+    const int vmaxElems = 16 / bytes;
+    if (vec == 0) vec = bytes == 4 ? (N >= 256 ? 4 : N >= 128 ? 2 : 1) : (N >= 256 ? 4 : 2);
+    if (vec != 1 && vec != 2 && vec != 4 && vec != 8) return set_error(GCOO_ERR_INVALID, "vec %d", vec);
+    if (vec > vmaxElems) vec = vmaxElems;
+    // D is read vec elements at a time: N and D's alignment must allow it
+    const uintptr_t al = reinterpret_cast<uintptr_t>(D);
+    while (vec > 1 && (N % vec != 0 || (al & (uintptr_t)(vec * bytes - 1)) != 0)) vec >>= 1;
+    JobDesc jd{};
+    jd.rowptr = h->rowptr;
+    jd.seg = dtype == 0 ? h->seg : nullptr;  // BF16 segments bound at commit
+    jd.D = D;
+    jd.C = C;
+    jd.M = h->M;
+    jd.K = h->K;
+    jd.N = N;
+    jd.bytes = bytes;
+    jd.vec = vec;
+    jd.lanes = pick_lanes(N, vec);
+    const int streams = 64 / jd.lanes;
+    if (rows == 0) rows = 4;
+    rows = (rows + streams - 1) / streams * streams;
+    jd.rows = rows;
+    jd.colTiles = N > 0 ? (N + jd.lanes * vec - 1) / (jd.lanes * vec) : 0;
+    jd.dBytes = (unsigned)dBytes;
+    const long long items = (N > 0 && h->M > 0) ? (long long)((h->M + rows - 1) / rows) * jd.colTiles : 0;
+    const long long start = b->itemStart.back();
+    if (start + items > INT32_MAX) return set_error(GCOO_ERR_OVERFLOW, "too many work items");
+    b->jobs.push_back(jd);
+    b->handles.push_back(h);
+    b->itemStart.push_back((int)(start + items));
+    b->committed = false;
+    destroy_graph(b);
+    if (job_index) *job_index = (int)b->jobs.size() - 1;
+    return GCOO_OK;
+}
+
+int gcoo_batch_commit(gcoo_batch_t b, gcoo_stream stream)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // BF16 jobs need their handles' BF16 segments
+    for (size_t i = 0; i < b->jobs.size(); ++i) {
+        if (b->jobs[i].bytes != 2) continue;
+        gcoo_handle h = b->handles[i];
+        if (!h->seg16_valid && h->nnz > 0) {
+            if (!h->seg16 && hipMalloc(&h->seg16, sizeof(int2) * (size_t)h->cap) != hipSuccess) {
+                (void)hipGetLastError();
+                h->seg16 = nullptr;
+                return set_error(GCOO_ERR_ALLOC, "cannot allocate BF16 segments");
+            }
+            int rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, s);
+            if (rc) return rc;
+        }
+        h->seg16_valid = true;
+        b->jobs[i].seg = h->seg16;
+    }
+    const size_t n = b->jobs.size();
+    if (n > b->capJobs) {
+        if (b->d_jobs) GCOO_HIP_TRY(hipFree(b->d_jobs));
+        if (b->d_itemStart) GCOO_HIP_TRY(hipFree(b->d_itemStart));
+        b->d_jobs = nullptr;
+        b->d_itemStart = nullptr;
+        const size_t cap = std::max<size_t>(n, 64);
+        if (hipMalloc(&b->d_jobs, sizeof(JobDesc) * cap) != hipSuccess ||
+            hipMalloc(&b->d_itemStart, sizeof(int) * (cap + 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            b->capJobs = 0;
+            return set_error(GCOO_ERR_ALLOC, "batch descriptors");
+        }
+        b->capJobs = cap;
+    }
+    if (n > 0) {
+        GCOO_HIP_TRY(hipMemcpyAsync(b->d_jobs, b->jobs.data(), sizeof(JobDesc) * n,
+                                    hipMemcpyHostToDevice, s));
+        GCOO_HIP_TRY(hipMemcpyAsync(b->d_itemStart, b->itemStart.data(), sizeof(int) * (n + 1),
+                                    hipMemcpyHostToDevice, s));
+    }
+    GCOO_HIP_TRY(hipStreamSynchronize(s));
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const long long items = b->itemStart.back();
+    // persistent: as many workgroups as can be resident at once
+    int perCu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, batch_persistent_kernel,
+                                                     kBatchThreads, 0) != hipSuccess ||
+        perCu < 1) {
+        (void)hipGetLastError();
+        perCu = 4;
+    }
+    long long blocks = (long long)cus * perCu;
+    const long long needed = (items + kBatchWaves - 1) / kBatchWaves;
+    if (blocks > needed) blocks = needed;
+    if (blocks < 1) blocks = 1;
+    b->gridBlocks = (int)blocks;
+    b->committed = true;
+    destroy_graph(b);
+    return GCOO_OK;
+}
+
+int gcoo_batch_run_async(gcoo_batch_t b, gcoo_stream stream)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    if (!b->committed) return set_error(GCOO_ERR_STATE, "batch not committed");
+    if (b->jobs.empty() || b->itemStart.back() == 0) return GCOO_OK;
+    return enqueue_run(b, reinterpret_cast<hipStream_t>(stream));
+}
+
+int gcoo_batch_graph_capture(gcoo_batch_t b, gcoo_stream stream)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    if (!b->committed) return set_error(GCOO_ERR_STATE, "batch not committed");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!s) return set_error(GCOO_ERR_INVALID, "graph capture needs a non-null stream");
+    destroy_graph(b);
+    GCOO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int rc = GCOO_OK;
+    if (!b->jobs.empty() && b->itemStart.back() > 0) rc = enqueue_run(b, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return set_error(GCOO_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    b->graph = g;
+    e = hipGraphInstantiate(&b->graphExec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        destroy_graph(b);
+        return set_error(GCOO_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    return GCOO_OK;
+}
+
+int gcoo_batch_graph_launch(gcoo_batch_t b, gcoo_stream stream)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    if (!b->graphExec) return set_error(GCOO_ERR_STATE, "no captured graph");
+    GCOO_HIP_TRY(hipGraphLaunch(b->graphExec, reinterpret_cast<hipStream_t>(stream)));
+    return GCOO_OK;
+}
+
+int gcoo_batch_info(gcoo_batch_t b, int *jobs, int64_t *items, int *grid_blocks)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    if (jobs) *jobs = (int)b->jobs.size();
+    if (items) *items = b->itemStart.back();
+    if (grid_blocks) *grid_blocks = b->gridBlocks;
+    return GCOO_OK;
+}
+
+int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes)
+{
+    if (!b || job < 0 || job >= (int)b->jobs.size())
+        return set_error(GCOO_ERR_INVALID, "bad batch or job index");
+    if (rows) *rows = b->jobs[job].rows;
+    if (vec) *vec = b->jobs[job].vec;
+    if (lanes) *lanes = b->jobs[job].lanes;
+    return GCOO_OK;
+}
+
+}  // extern "C"

@@ -180,6 +180,34 @@ int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
 int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,
                      int N, float *C, gcoo_stream s);
 
+/* ---- Batches: many independent multiplies in one persistent launch -------
+ * No counterpart in the reference (one multiply per launch,
+ * src/ShGCOOSpDM.cu:153-167).  A batch holds jobs C_i = S_i * D_i, each S_i
+ * a converted handle, D_i FP32 (dtype 0) or BF16 bits (dtype 1, the handle's
+ * BF16 values are derived at commit), any shapes.  Each job is cut into work
+ * items of `rows` rows x one column tile of `vec` columns per lane (0 = the
+ * built-in choice; tuned values come from gcoospdm_amd/autotune.py).  A run
+ * is a counter-reset kernel plus one persistent kernel whose waves pull work
+ * items until none are left; it allocates nothing and synchronises nothing,
+ * so it can be captured into a HIP graph and replayed.  Every C element is
+ * the same fmaf chain as in gcoo_spdm_f32 / gcoo_spdm_bf16: results are
+ * bit-identical to running each job alone.  The batch keeps the pointers it
+ * was given; handles and operands must outlive its runs. */
+typedef struct gcoo_batch *gcoo_batch_t;
+int gcoo_batch_create(gcoo_batch_t *b);
+int gcoo_batch_destroy(gcoo_batch_t b);
+int gcoo_batch_clear(gcoo_batch_t b);
+int gcoo_batch_add(gcoo_batch_t b, gcoo_handle h, const void *D, int dtype, int N, float *C,
+                   int rows, int vec, int *job_index);
+/* Upload the job table (synchronous); required after the last add. */
+int gcoo_batch_commit(gcoo_batch_t b, gcoo_stream s);
+int gcoo_batch_run_async(gcoo_batch_t b, gcoo_stream s);
+/* Capture one run into a HIP graph on stream s (non-null), then replay it. */
+int gcoo_batch_graph_capture(gcoo_batch_t b, gcoo_stream s);
+int gcoo_batch_graph_launch(gcoo_batch_t b, gcoo_stream s);
+int gcoo_batch_info(gcoo_batch_t b, int *jobs, int64_t *items, int *grid_blocks);
+int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes);
+
 #ifdef __cplusplus
 }
 #endif
