@@ -1,0 +1,188 @@
+"""Autotuner for the multiply tiles.
+
+The reference grid-searches its compile-time COLUMN_PER_GROUP / BS by
+substituting them into templates, recompiling and timing `main`
+(src/autotune.py:21-86, src/constants.h.t:4-5).  Here the tile parameters
+are runtime arguments, so the search needs no rebuild: each candidate is
+timed with HIP events on the device and the fastest is written to a table.
+
+Two tables:
+  batch[dtype][class] = {rows, vec}   for gcoo_batch_add, where
+      class = (N bucket, entries-per-row bucket) of a job;
+  plain[dtype][class] = {variant, vec} for gcoo_set_spdm_config.
+
+Run:  python -m gcoospdm_amd.autotune [--out gcoospdm_amd/tuning.json]
+The table shipped in the package was produced by that command on an MI355X.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_TABLE = os.path.join(_HERE, "tuning.json")
+
+N_BUCKETS = [32, 64, 128, 256]            # upper bounds (last open-ended)
+ROW_BUCKETS = [8, 32, 128]                # entries per row: <8, <32, <128, >=128
+
+
+def n_class(N: int) -> int:
+    for i, b in enumerate(N_BUCKETS):
+        if N <= b:
+            return i
+    return len(N_BUCKETS) - 1
+
+
+def row_class(per_row: float) -> int:
+    for i, b in enumerate(ROW_BUCKETS):
+        if per_row < b:
+            return i
+    return len(ROW_BUCKETS)
+
+
+def class_key(N: int, per_row: float) -> str:
+    return f"n{n_class(N)}r{row_class(per_row)}"
+
+
+_table_cache = {}
+
+
+def load(path: str = DEFAULT_TABLE) -> dict:
+    if path not in _table_cache:
+        if os.path.exists(path):
+            with open(path) as f:
+                _table_cache[path] = json.load(f)
+        else:
+            _table_cache[path] = {}
+    return _table_cache[path]
+
+
+def batch_tile(N: int, per_row: float, bf16: bool, table: dict | None = None):
+    """(rows, vec) for a job of width N with `per_row` entries per row;
+    (0, 0) = the library's built-in choice."""
+    t = (table if table is not None else load()).get("batch", {}).get("bf16" if bf16 else "f32", {})
+    e = t.get(class_key(N, per_row))
+    return (e["rows"], e["vec"]) if e else (0, 0)
+
+
+def plain_config(N: int, per_row: float, bf16: bool, table: dict | None = None):
+    t = (table if table is not None else load()).get("plain", {}).get("bf16" if bf16 else "f32", {})
+    e = t.get(class_key(N, per_row))
+    return (e["variant"], e["vec"]) if e else (0, 0)
+
+
+def _time(fn, reps=5):
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def tune_batch(jobs_by_class, bf16: bool, log):
+    """jobs_by_class: {key: [(m, D, C, N), ...]} -> {key: {rows, vec}}"""
+    import gcoospdm_amd as G
+    out = {}
+    vecs = [2, 4, 8] if bf16 else [1, 2, 4]
+    for key, jobs in sorted(jobs_by_class.items()):
+        best = None
+        for rows in (1, 2, 4, 8, 16):
+            for vec in vecs:
+                b = G.Batch()
+                for m, D, C, N in jobs:
+                    b.add(m, D, C, rows=rows, vec=vec)
+                b.commit()
+                ms = _time(b.run)
+                log(dict(kind="batch", dtype="bf16" if bf16 else "f32", cls=key, rows=rows,
+                         vec=vec, ms=round(ms, 4), jobs=len(jobs)))
+                if best is None or ms < best[0]:
+                    best = (ms, rows, vec)
+                b.close()
+        out[key] = {"rows": best[1], "vec": best[2], "ms": round(best[0], 4)}
+    return out
+
+
+def tune_plain(bf16: bool, log, device="cuda:0"):
+    """Representative square-ish shapes per class; variant 2 vs 3, vec."""
+    import torch
+    import gcoospdm_amd as G
+    out = {}
+    shapes = {  # class key -> (M, K, N, sparsity)
+        class_key(32, 4): (4096, 4096, 32, 0.999), class_key(256, 4): (4096, 4096, 256, 0.999),
+        class_key(256, 20): (4096, 4096, 256, 0.995), class_key(256, 80): (8000, 8000, 256, 0.99),
+        class_key(64, 20): (4096, 4096, 64, 0.995), class_key(128, 20): (4096, 4096, 128, 0.995),
+        class_key(256, 400): (4096, 4096, 256, 0.9),
+    }
+    vecs = [2, 4, 8] if bf16 else [1, 2, 4]
+    for key, (M, K, N, s) in shapes.items():
+        S = torch.empty((M, K), device=device)
+        G.fill_sparse(S, 5, 1, s)
+        m = G.GCOOMatrix(M, K, 4).convert_dense(S)
+        del S
+        D = torch.empty((K, N), device=device)
+        G.fill_uniform(D, 5, 0)
+        if bf16:
+            D = G.to_bf16(D)
+        C = torch.empty((M, N), device=device)
+        best = None
+        for variant in (2, 3):
+            for vec in vecs:
+                m.set_config(variant=variant, vec=vec)
+                fn = (lambda: m.spdm_bf16(D, C, sync=False)) if bf16 else \
+                     (lambda: m.spdm(D, C, sync=False))
+                ms = _time(fn)
+                log(dict(kind="plain", dtype="bf16" if bf16 else "f32", cls=key,
+                         variant=variant, vec=vec, ms=round(ms, 4)))
+                if best is None or ms < best[0]:
+                    best = (ms, variant, vec)
+        out[key] = {"variant": best[1], "vec": best[2], "ms": round(best[0], 4)}
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_TABLE)
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--per-class", type=int, default=24)
+    ap.add_argument("--log", default=None)
+    args = ap.parse_args()
+    from . import workloads as W
+
+    logf = open(args.log, "a") if args.log else None
+
+    def log(rec):
+        line = json.dumps(rec)
+        print(line, flush=True)
+        if logf:
+            logf.write(line + "\n")
+
+    t0 = time.time()
+    table = {"device": "MI355X (gfx950)", "batch": {}, "plain": {}}
+    specs = W.batch_jobs(args.jobs, 2024)
+    for bf16 in (False, True):
+        by_class = {}
+        for sp in specs:
+            key = class_key(sp.N, sp.K * (1.0 - sp.sparsity))
+            lst = by_class.setdefault(key, [])
+            if len(lst) < args.per_class:
+                m, D, C = W.materialize(sp, bf16=bf16)
+                lst.append((m, D, C, sp.N))
+        table["batch"]["bf16" if bf16 else "f32"] = tune_batch(by_class, bf16, log)
+        del by_class
+        table["plain"]["bf16" if bf16 else "f32"] = tune_plain(bf16, log)
+    table["seconds"] = round(time.time() - t0, 1)
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    print(f"wrote {args.out}", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,92 @@
+"""Measurement of the batched path on the many-jobs workload.
+
+512 independent jobs (M, K log-uniform in [256, 8192], N in {32..256},
+sparsity in [0.95, 0.999], see workloads.batch_jobs), FP32 or BF16:
+  sequential  one plain multiply launch per job, back to back on one stream;
+  batched     one HIP-graph replay of the persistent batch kernel with the
+              autotuned tiles.
+Both are timed with HIP events on the stream they run on; GFLOP/s counts
+2 * nnz * N per job.  Used by bench.py; also runnable alone:
+    python -m gcoospdm_amd.batchbench [--jobs 512] [--bf16]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+
+
+def run(count=512, bf16=False, reps=10, seed=2024, table=None):
+    import torch
+    import gcoospdm_amd as G
+    from . import autotune, workloads as W
+
+    specs = W.batch_jobs(count, seed)
+    jobs = [W.materialize(sp, bf16=bf16) for sp in specs]
+    nnz = [m.nnz for m, _, _ in jobs]
+    flops = sum(2.0 * z * sp.N for z, sp in zip(nnz, specs))
+    algo_bytes = sum(W.spdm_bytes(z, sp.M, sp.K, sp.N, bf16) for z, sp in zip(nnz, specs))
+    stream = torch.cuda.Stream()
+
+    def seq():
+        for m, D, C in jobs:
+            if bf16:
+                m.spdm_bf16(D, C, stream=stream, sync=False)
+            else:
+                m.spdm(D, C, stream=stream, sync=False)
+
+    # plain tiles from the table as well, so both sides are tuned
+    for (m, D, C), sp in zip(jobs, specs):
+        v, vec = autotune.plain_config(sp.N, sp.K * (1 - sp.sparsity), bf16, table)
+        m.set_config(variant=v, vec=vec)
+
+    def timed(fn):
+        with torch.cuda.stream(stream):
+            fn()
+        stream.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps / 1e3
+
+    t_seq = timed(seq)
+    ref = [C.clone() for _, _, C in jobs]
+
+    b = G.Batch()
+    for (m, D, C), sp in zip(jobs, specs):
+        rows, vec = autotune.batch_tile(sp.N, sp.K * (1 - sp.sparsity), bf16, table)
+        b.add(m, D, C, rows=rows, vec=vec)
+    b.commit()
+    b.capture(stream)
+    t_batch = timed(lambda: b.replay(stream))
+    same = all(torch.equal(C.view(torch.int32), r.view(torch.int32))
+               for (_, _, C), r in zip(jobs, ref))
+    info = b.info()
+    return {
+        "workload": f"batch_{count}_jobs", "dtype": "bf16" if bf16 else "f32",
+        "jobs": count, "nnz": int(sum(nnz)), "gflop": round(flops / 1e9, 3),
+        "algorithmic_bytes": int(algo_bytes),
+        "sequential_ms": round(t_seq * 1e3, 4), "batched_ms": round(t_batch * 1e3, 4),
+        "sequential_gflops": round(flops / t_seq / 1e9, 2),
+        "batched_gflops": round(flops / t_batch / 1e9, 2),
+        "speedup": round(t_seq / t_batch, 3),
+        "batched_hbm_frac": round(algo_bytes / t_batch / 8e12, 4),
+        "work_items": info["items"], "grid_blocks": info["grid_blocks"],
+        "bit_identical_to_alone": bool(same),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--jobs", type=int, default=512)
+    ap.add_argument("--bf16", action="store_true")
+    ap.add_argument("--reps", type=int, default=10)
+    args = ap.parse_args()
+    print(json.dumps(run(args.jobs, args.bf16, args.reps)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
