@@ -88,6 +88,15 @@ _SIGNATURES = [
     ("gcoo_batch_graph_launch", _I, [_P, _P]),
     ("gcoo_batch_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I64), _c.POINTER(_I)]),
     ("gcoo_batch_job_tile", _I, [_P, _I, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I)]),
+    ("gcoo_server_create", _I, [_P, _PP]),
+    ("gcoo_server_destroy", _I, [_P]),
+    ("gcoo_server_register", _I, [_P, _P, _P, _I, _I, _I, _I, _c.POINTER(_I)]),
+    ("gcoo_server_submit", _I, [_P, _I, _c.POINTER(_U64)]),
+    ("gcoo_server_wait", _I, [_P, _U64, _c.POINTER(_c.c_double)]),
+    ("gcoo_server_result", _I, [_P, _U64, _PP, _c.POINTER(_I), _c.POINTER(_I)]),
+    ("gcoo_server_release", _I, [_P, _U64]),
+    ("gcoo_server_stats", _I, [_P, _P]),
+    ("gcoo_server_reset_stats", _I, [_P]),
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -360,6 +369,84 @@ class Batch:
         _check(lib().gcoo_batch_job_tile(self._b, job, ctypes.byref(r), ctypes.byref(v),
                                          ctypes.byref(ln)), "gcoo_batch_job_tile")
         return dict(rows=r.value, vec=v.value, lanes=ln.value)
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int),
+                ("max_batch_jobs", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("completed", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double), ("max_ms", ctypes.c_double),
+                ("wall_s", ctypes.c_double), ("peak_bytes", ctypes.c_size_t),
+                ("budget_bytes", ctypes.c_size_t)]
+
+
+class Server:
+    """Continuous-batching request loop (gcoo_server_*), operands resident."""
+
+    def __init__(self, hbm_budget_bytes: int, slots: int = 4, max_batch_jobs: int = 128):
+        cfg = ServerConfig(hbm_budget_bytes, slots, max_batch_jobs)
+        s = ctypes.c_void_p()
+        _check(lib().gcoo_server_create(ctypes.byref(cfg), ctypes.byref(s)), "gcoo_server_create")
+        self._s = s
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_s", None) is not None and self._s.value:
+            lib().gcoo_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def register(self, m: "GCOOMatrix", D, rows=0, vec=0) -> int:
+        import torch
+        _require_dev(D, "D")
+        dtype = 0 if D.dtype == torch.float32 else 1
+        op = ctypes.c_int()
+        _check(lib().gcoo_server_register(self._s, m.handle, _ptr(D), dtype, D.shape[1], rows, vec,
+                                          ctypes.byref(op)), "gcoo_server_register")
+        self._keep.append((m, D))
+        return op.value
+
+    def submit(self, op: int) -> int:
+        rid = ctypes.c_uint64()
+        _check(lib().gcoo_server_submit(self._s, op, ctypes.byref(rid)), "gcoo_server_submit")
+        return rid.value
+
+    def wait(self, rid: int) -> float:
+        lat = ctypes.c_double()
+        _check(lib().gcoo_server_wait(self._s, rid, ctypes.byref(lat)), "gcoo_server_wait")
+        return lat.value
+
+    def result_host(self, rid: int):
+        """Copy a completed request's C to a host numpy array."""
+        import numpy as np
+        p = ctypes.c_void_p()
+        M, N = ctypes.c_int(), ctypes.c_int()
+        _check(lib().gcoo_server_result(self._s, rid, ctypes.byref(p), ctypes.byref(M),
+                                        ctypes.byref(N)), "gcoo_server_result")
+        out = np.empty((M.value, N.value), np.float32)
+        if out.size:
+            _check(lib().gcoo_memcpy_d2h(out.ctypes.data, p, out.nbytes, None), "gcoo_memcpy_d2h")
+        return out
+
+    def release(self, rid: int):
+        _check(lib().gcoo_server_release(self._s, rid), "gcoo_server_release")
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().gcoo_server_stats(self._s, ctypes.byref(st)), "gcoo_server_stats")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def reset_stats(self):
+        _check(lib().gcoo_server_reset_stats(self._s), "gcoo_server_reset_stats")
 
 
 class ShGemm:

@@ -59,6 +59,23 @@ int ensure_capacity(gcoo_matrix *h, int64_t need)
     return GCOO_OK;
 }
 
+int ensure_seg16(gcoo_matrix *h, hipStream_t s)
+{
+    if (!h->seg16_valid && h->nnz > 0) {
+        if (!h->seg16) {
+            if (hipMalloc(&h->seg16, sizeof(int2) * (size_t)h->cap) != hipSuccess) {
+                (void)hipGetLastError();
+                h->seg16 = nullptr;
+                return set_error(GCOO_ERR_ALLOC, "cannot allocate the BF16 segments");
+            }
+        }
+        int rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, s);
+        if (rc) return rc;
+    }
+    h->seg16_valid = true;
+    return GCOO_OK;
+}
+
 }  // namespace gcoo
 
 using namespace gcoo;
@@ -399,17 +416,7 @@ int gcoo_spdm_bf16_async(gcoo_handle h, const uint16_t *D, int K, int N, float *
     int rc = check_spdm_args(h, reinterpret_cast<const float *>(D), K, N, C);
     if (rc) return rc;
     if (!h->converted) return set_error(GCOO_ERR_STATE, "matrix not converted");
-    if (!h->seg16_valid && h->nnz > 0) {
-        if (!h->seg16) {
-            if (hipMalloc(&h->seg16, sizeof(int2) * (size_t)h->cap) != hipSuccess) {
-                (void)hipGetLastError();
-                h->seg16 = nullptr;
-                return set_error(GCOO_ERR_ALLOC, "cannot allocate the BF16 segments");
-            }
-        }
-        if ((rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, as_stream(s)))) return rc;
-    }
-    h->seg16_valid = true;
+    if ((rc = ensure_seg16(h, as_stream(s)))) return rc;
     return launch_spdm_bf16(h, D, N, C, as_stream(s));
 }
 

@@ -183,6 +183,8 @@ struct gcoo_batch {
     JobDesc *d_jobs = nullptr;
     int *d_itemStart = nullptr;
     unsigned *d_counter = nullptr;
+    JobDesc *h_jobs = nullptr;     // pinned staging of the descriptor upload
+    int *h_itemStart = nullptr;
     size_t capJobs = 0;
     bool committed = false;
     int gridBlocks = 0;
@@ -245,6 +247,8 @@ int gcoo_batch_destroy(gcoo_batch_t b)
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->d_itemStart) (void)hipFree(b->d_itemStart);
     if (b->d_counter) (void)hipFree(b->d_counter);
+    if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+    if (b->h_itemStart) (void)hipHostFree(b->h_itemStart);
     delete b;
     return GCOO_OK;
 }
@@ -310,33 +314,46 @@ int gcoo_batch_add(gcoo_batch_t b, gcoo_handle h, const void *D, int dtype, int 
 
 int gcoo_batch_commit(gcoo_batch_t b, gcoo_stream stream)
 {
+    int rc = gcoo::batch_commit_async(b, stream);
+    if (rc) return rc;
+    GCOO_HIP_TRY(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+    return GCOO_OK;
+}
+
+}  // extern "C"
+
+// Upload the job table through pinned staging on `stream` without waiting:
+// the staging belongs to the batch, so the caller must not commit this batch
+// again before the upload has executed (the serving loop commits a slot's
+// batch only after that slot's previous run completed).
+int gcoo::batch_commit_async(gcoo_batch_t b, gcoo_stream stream)
+{
     if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     // BF16 jobs need their handles' BF16 segments
     for (size_t i = 0; i < b->jobs.size(); ++i) {
         if (b->jobs[i].bytes != 2) continue;
         gcoo_handle h = b->handles[i];
-        if (!h->seg16_valid && h->nnz > 0) {
-            if (!h->seg16 && hipMalloc(&h->seg16, sizeof(int2) * (size_t)h->cap) != hipSuccess) {
-                (void)hipGetLastError();
-                h->seg16 = nullptr;
-                return set_error(GCOO_ERR_ALLOC, "cannot allocate BF16 segments");
-            }
-            int rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, s);
-            if (rc) return rc;
-        }
-        h->seg16_valid = true;
+        int rc = ensure_seg16(h, s);
+        if (rc) return rc;
         b->jobs[i].seg = h->seg16;
     }
     const size_t n = b->jobs.size();
     if (n > b->capJobs) {
         if (b->d_jobs) GCOO_HIP_TRY(hipFree(b->d_jobs));
         if (b->d_itemStart) GCOO_HIP_TRY(hipFree(b->d_itemStart));
+        if (b->h_jobs) GCOO_HIP_TRY(hipHostFree(b->h_jobs));
+        if (b->h_itemStart) GCOO_HIP_TRY(hipHostFree(b->h_itemStart));
         b->d_jobs = nullptr;
         b->d_itemStart = nullptr;
+        b->h_jobs = nullptr;
+        b->h_itemStart = nullptr;
         const size_t cap = std::max<size_t>(n, 64);
         if (hipMalloc(&b->d_jobs, sizeof(JobDesc) * cap) != hipSuccess ||
-            hipMalloc(&b->d_itemStart, sizeof(int) * (cap + 1)) != hipSuccess) {
+            hipMalloc(&b->d_itemStart, sizeof(int) * (cap + 1)) != hipSuccess ||
+            hipHostMalloc(&b->h_jobs, sizeof(JobDesc) * cap, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc(&b->h_itemStart, sizeof(int) * (cap + 1), hipHostMallocDefault) !=
+                hipSuccess) {
             (void)hipGetLastError();
             b->capJobs = 0;
             return set_error(GCOO_ERR_ALLOC, "batch descriptors");
@@ -344,12 +361,13 @@ int gcoo_batch_commit(gcoo_batch_t b, gcoo_stream stream)
         b->capJobs = cap;
     }
     if (n > 0) {
-        GCOO_HIP_TRY(hipMemcpyAsync(b->d_jobs, b->jobs.data(), sizeof(JobDesc) * n,
+        std::copy(b->jobs.begin(), b->jobs.end(), b->h_jobs);
+        std::copy(b->itemStart.begin(), b->itemStart.end(), b->h_itemStart);
+        GCOO_HIP_TRY(hipMemcpyAsync(b->d_jobs, b->h_jobs, sizeof(JobDesc) * n,
                                     hipMemcpyHostToDevice, s));
-        GCOO_HIP_TRY(hipMemcpyAsync(b->d_itemStart, b->itemStart.data(), sizeof(int) * (n + 1),
+        GCOO_HIP_TRY(hipMemcpyAsync(b->d_itemStart, b->h_itemStart, sizeof(int) * (n + 1),
                                     hipMemcpyHostToDevice, s));
     }
-    GCOO_HIP_TRY(hipStreamSynchronize(s));
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -371,6 +389,8 @@ int gcoo_batch_commit(gcoo_batch_t b, gcoo_stream stream)
     destroy_graph(b);
     return GCOO_OK;
 }
+
+extern "C" {
 
 int gcoo_batch_run_async(gcoo_batch_t b, gcoo_stream stream)
 {

@@ -113,6 +113,14 @@ int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
 int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
                      hipStream_t s);
 
+// Enqueue (on s) the BF16 copy of h's segments unless it is current; the
+// copy is usable by later work on s (or by anyone after s is synchronised).
+int ensure_seg16(gcoo_matrix *h, hipStream_t s);
+
+// ---- batches (gcoo_batch.hip) ----
+// gcoo_batch_commit without the final stream synchronisation.
+int batch_commit_async(gcoo_batch_t b, gcoo_stream s);
+
 // ---- generators and element conversions (gcoo_gen.hip) ----
 int launch_fill(float *dst, int64_t n, uint64_t seed, uint64_t stream_id,
                 float scale, uint64_t keep_threshold, bool sparse,

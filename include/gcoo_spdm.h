@@ -208,6 +208,45 @@ int gcoo_batch_graph_launch(gcoo_batch_t b, gcoo_stream s);
 int gcoo_batch_info(gcoo_batch_t b, int *jobs, int64_t *items, int *grid_blocks);
 int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes);
 
+/* ---- Serving: continuous batching of requests on resident operands -------
+ * No counterpart in the reference.  Operands (a converted handle and its D,
+ * FP32 or BF16) are registered once and stay resident.  Requests
+ * "C = S_op * D_op" are submitted from any thread; a scheduler thread inside
+ * the server admits them in arrival order while their outputs fit the HBM
+ * budget (one device arena of hbm_budget_bytes, first fit; a request larger
+ * than the arena is rejected with GCOO_ERR_BUDGET), and whenever one of
+ * `slots` streams is idle launches all admitted requests (at most
+ * max_batch_jobs) as one persistent batch on it, so requests arriving while
+ * other batches run are batched on another stream and overlap on the device.
+ * Latency is arrival (submit) to completion.  Results are bit-identical to
+ * the same multiply run alone; they stay in the arena until released. */
+typedef struct gcoo_server *gcoo_server_t;
+typedef struct gcoo_server_config_s {
+    size_t hbm_budget_bytes; /* output arena size: the admission budget */
+    int slots;               /* concurrent batches / streams, 1..8 */
+    int max_batch_jobs;      /* requests per launched batch */
+} gcoo_server_config;
+typedef struct gcoo_server_stats_s {
+    uint64_t completed;
+    uint64_t batches;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    double wall_s;           /* first arrival to last completion */
+    size_t peak_bytes;       /* most arena bytes in use at once */
+    size_t budget_bytes;
+} gcoo_server_stats_t;
+int gcoo_server_create(const gcoo_server_config *cfg, gcoo_server_t *s);
+int gcoo_server_destroy(gcoo_server_t s);
+/* rows / vec: batch tile of this operand's requests (0 = automatic). */
+int gcoo_server_register(gcoo_server_t s, gcoo_handle h, const void *D, int dtype, int N,
+                         int rows, int vec, int *op_id);
+int gcoo_server_submit(gcoo_server_t s, int op_id, uint64_t *req_id);
+int gcoo_server_wait(gcoo_server_t s, uint64_t req_id, double *latency_ms);
+/* Device pointer to the completed request's C (M x N FP32, row-major). */
+int gcoo_server_result(gcoo_server_t s, uint64_t req_id, const float **C, int *M, int *N);
+int gcoo_server_release(gcoo_server_t s, uint64_t req_id);
+int gcoo_server_stats(gcoo_server_t s, gcoo_server_stats_t *st);
+int gcoo_server_reset_stats(gcoo_server_t s);
+
 #ifdef __cplusplus
 }
 #endif
