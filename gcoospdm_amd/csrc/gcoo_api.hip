@@ -123,6 +123,7 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
         GCOO_HIP_TRY(hipStreamSynchronize(s));
     }
     h->nnz = total;
+    h->maxRow = h->h_status->maxRow;
     h->converted = true;
     return GCOO_OK;
 }
@@ -331,6 +332,8 @@ int gcoo_destroy(gcoo_handle h)
     if (h->vals) (void)hipFree(h->vals);
     if (h->rowcol) (void)hipFree(h->rowcol);
     if (h->idx) (void)hipFree(h->idx);
+    if (h->longList) (void)hipFree(h->longList);
+    if (h->longCtr) (void)hipFree(h->longCtr);
     if (h->seg) (void)hipFree(h->seg);
     if (h->seg16) (void)hipFree(h->seg16);
     if (h->rowptr) (void)hipFree(h->rowptr);
@@ -435,7 +438,7 @@ int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
         h->spdm_cfg = gcoo_spdm_config{0, 0, 0, 0};
         return GCOO_OK;
     }
-    if (cfg->variant < 0 || cfg->variant > 3)
+    if (cfg->variant < 0 || cfg->variant > 4)
         return set_error(GCOO_ERR_INVALID, "variant %d", cfg->variant);
     if (cfg->vec != 0 && cfg->vec != 1 && cfg->vec != 2 && cfg->vec != 4 && cfg->vec != 8)
         return set_error(GCOO_ERR_INVALID, "vec %d (0, 1, 2, 4 or 8)", cfg->vec);

@@ -92,8 +92,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
     ConvStatus *__restrict__ st, int clear_invalid)
 {
     __shared__ long long wtot[kScanThreads / 64];
+    __shared__ int wmax[kScanThreads / 64];
     const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
     long long carry = 0;
+    int vmax = 0;
     for (int base = 0; base < M; base += kScanThreads * kScanItems) {
         int v[kScanItems];
         long long tsum = 0;
@@ -102,6 +104,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
             int idx = base + tid * kScanItems + i;
             v[i] = idx < M ? rowcnt[idx] : 0;
             tsum += v[i];
+            vmax = max(vmax, v[i]);
         }
         long long incl = tsum;
 #pragma unroll
@@ -128,10 +131,17 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
         carry += tile;
         __syncthreads();
     }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) vmax = max(vmax, __shfl_xor(vmax, d, 64));
+    if (lane == 0) wmax[wave] = vmax;
+    __syncthreads();
     if (tid == 0) {
+        int m = 0;
+        for (int w = 0; w < kScanThreads / 64; ++w) m = max(m, wmax[w]);
         rowptr[M] = (int)carry;
         st->total = carry;
         st->overflow = carry > (long long)INT_MAX ? 1 : 0;
+        st->maxRow = m;
         if (clear_invalid) st->invalid = 0;
     }
     __syncthreads();

@@ -245,6 +245,7 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols, const float *v
     if (h->h_status->invalid)
         return set_error(GCOO_ERR_INVALID, "duplicate (row, col) entries in the COO input");
     h->nnz = total;
+    h->maxRow = h->h_status->maxRow;
     return GCOO_OK;
 }
 

@@ -43,6 +43,8 @@ struct ConvStatus {
     long long total;
     int overflow;
     int invalid;
+    int maxRow;  // most entries in one row
+    int pad;
 };
 
 }  // namespace gcoo
@@ -73,6 +75,11 @@ struct gcoo_matrix {
     bool seg16_valid = false;
     int *rowptr = nullptr;     // [M + 1]
     int *rowcnt = nullptr;     // [M] per-row counts (conversion scratch)
+    int maxRow = 0;            // most entries in one row (last conversion)
+
+    // long-row path scratch, allocated on first use by the multiply
+    mutable int *longList = nullptr;       // [M + 1] rows of the long-row kernel
+    mutable unsigned *longCtr = nullptr;   // [0] listed rows, [1] next work item
 
     // multiply-kernel configuration (gcoo_set_spdm_config)
     gcoo_spdm_config spdm_cfg = {0, 0, 0, 0};

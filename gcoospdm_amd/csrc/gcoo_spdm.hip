@@ -297,7 +297,7 @@ template <typename E, int VEC, int U>
 __global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int nGroup, int cpg,
     int M, const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C,
-    int nGroupBlocks, const ConvStatus *__restrict__ guard, long long cap)
+    int nGroupBlocks, const ConvStatus *__restrict__ guard, long long cap, int skipLen)
 {
     if (guard != nullptr && guard->total > cap) return;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
@@ -321,6 +321,7 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
         for (int v = 0; v < VEC; ++v) acc[v] = 0.0f;
         const int p0 = rowptr[r];
         const int p1 = rowptr[r + 1];
+        if (p1 - p0 >= skipLen) continue;  // a long row: the long-row kernel owns it
         int p = p0;
         for (; p + U <= p1; p += U) {
             float a[U];
@@ -493,6 +494,134 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Long rows.  A row with thousands of entries walked by one wave is a serial
+// chain of dependent load -> FMA steps: latency-bound, and on skewed matrices
+// it is the tail of the whole launch.  Long rows are instead listed (order
+// irrelevant: rows are independent) and pulled by a persistent kernel in
+// which one workgroup owns one (long row, TN-column tile) at a time: for T
+// entries at a time, all NT threads load the T D-row pieces into LDS at once
+// (T * TN elements in flight per workgroup, the entries of the chunk after
+// next already staged), then thread t runs column t's fmaf chain over the T
+// entries from LDS -- still row r's entries in ascending k, so the bits are
+// the reference's.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void longrow_classify_kernel(
+    const int *__restrict__ rowptr, int M, int minLen, int *__restrict__ list,
+    unsigned *__restrict__ ctr, const ConvStatus *__restrict__ guard, long long cap)
+{
+    // behind a conversion that overflowed its capacity the segments are
+    // stale: list nothing (the host re-runs the multiply)
+    if (guard != nullptr && guard->total > cap) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < M && rowptr[r + 1] - rowptr[r] >= minLen) list[atomicAdd(&ctr[0], 1u)] = r;
+}
+
+__global__ __launch_bounds__(64) void longrow_reset_kernel(unsigned *ctr)
+{
+    if (threadIdx.x < 4) ctr[threadIdx.x] = 0;
+}
+
+template <typename E, int W, int T>
+__global__ __launch_bounds__(W * 64) void spdm_longrow_kernel(
+    const int *__restrict__ rowptr, const int2 *__restrict__ seg, const int *__restrict__ list,
+    unsigned *__restrict__ ctr, int colTiles, const void *__restrict__ D, unsigned dBytes, int N,
+    float *__restrict__ C)
+{
+    constexpr int NT = W * 64;              // threads = columns per tile
+    constexpr int ROWB = NT * E::bytes;     // bytes of one staged D row piece
+    constexpr int PIECES_ROW = ROWB / 16;
+    constexpr int CHUNK_P = T * PIECES_ROW;
+    constexpr int PER_T = CHUNK_P / NT;
+    static_assert(CHUNK_P % NT == 0 && T <= NT, "chunk geometry");
+    __shared__ f32x4 sD[2][CHUNK_P];
+    __shared__ int2 sE[2][T];
+    __shared__ int sItem;
+
+    const int tid = threadIdx.x;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
+    const unsigned rowBytes = (unsigned)N * E::bytes;
+    constexpr int EPP = 16 / E::bytes;
+    for (;;) {
+        if (tid == 0) sItem = (int)atomicAdd(&ctr[1], 1u);
+        __syncthreads();
+        const int item = sItem;
+        __syncthreads();
+        if (item >= (int)ctr[0] * colTiles) return;
+        const int row = list[item / colTiles];
+        const int ct = item % colTiles;
+        const int col0 = ct * NT;
+        const int p0 = rowptr[row], p1 = rowptr[row + 1];
+        const int nChunks = (p1 - p0 + T - 1) / T;
+        // per-thread staging geometry
+        unsigned s_voff[PER_T];
+        int s_ent[PER_T];
+#pragma unroll
+        for (int i = 0; i < PER_T; ++i) {
+            const int f = tid + i * NT;
+            s_ent[i] = f / PIECES_ROW;
+            const int c = col0 + (f % PIECES_ROW) * EPP;
+            s_voff[i] = c < N ? (unsigned)c * E::bytes : dBytes;
+        }
+        auto load_entries = [&](int chunk, int buf) {
+            const int p = p0 + chunk * T + tid;
+            if (tid < T) sE[buf][tid] = (chunk < nChunks && p < p1) ? seg[p] : make_int2(-1, 0);
+        };
+        f32x4 stage[PER_T];
+        auto issue = [&](int buf) {  // D pieces of the chunk whose entries are in sE[buf]
+#pragma unroll
+            for (int i = 0; i < PER_T; ++i) {
+                const int k = sE[buf][s_ent[i]].x;
+                const unsigned off = k >= 0 ? s_voff[i] + (unsigned)k * rowBytes : dBytes;
+                stage[i] = __builtin_bit_cast(
+                    f32x4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off, 0, 0));
+            }
+        };
+        auto commit = [&](int buf) {
+#pragma unroll
+            for (int i = 0; i < PER_T; ++i) sD[buf][tid + i * NT] = stage[i];
+        };
+        load_entries(0, 0);
+        load_entries(1, 1);
+        __syncthreads();
+        issue(0);
+        commit(0);
+        __syncthreads();
+        float acc = 0.0f;
+        const int j = col0 + tid;
+        const unsigned char *colp = nullptr;
+        for (int c = 0; c < nChunks; ++c) {
+            const int buf = c & 1;
+            const bool more = c + 1 < nChunks;
+            int2 eNext = make_int2(-1, 0);
+            if (more) {
+                issue(buf ^ 1);                       // D of chunk c+1
+                const int p = p0 + (c + 2) * T + tid;  // entries of chunk c+2
+                if (tid < T && c + 2 < nChunks && p < p1) eNext = seg[p];
+            }
+            const int n = min(T, p1 - (p0 + c * T));
+            colp = reinterpret_cast<const unsigned char *>(sD[buf]) + tid * E::bytes;
+            for (int e = 0; e < n; ++e) {
+                const float a = E::val(sE[buf][e].y);
+                float d;
+                if constexpr (E::bytes == 4)
+                    d = *reinterpret_cast<const float *>(colp + e * ROWB);
+                else
+                    d = lo_bf16(*reinterpret_cast<const unsigned short *>(colp + e * ROWB));
+                acc = __builtin_fmaf(a, d, acc);
+            }
+            __syncthreads();  // everyone is done with sD[buf] / sE[buf]
+            if (more) {
+                commit(buf ^ 1);
+                if (tid < T) sE[buf][tid] = eNext;
+            }
+            __syncthreads();
+        }
+        if (j < N) C[(size_t)row * N + j] = acc;
+    }
+}
+
 template <typename E, int VEC, int RPW, int WAVES, int TK>
 int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
                const ConvStatus *guard, hipStream_t s)
@@ -513,7 +642,7 @@ int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, floa
 
 template <typename E, int VEC>
 int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                  const ConvStatus *guard, hipStream_t s)
+                  const ConvStatus *guard, hipStream_t s, int skipLen = INT_MAX)
 {
     constexpr int U = 8;
     const int nGroupBlocks = (h->nGroup + kSpdmWaves - 1) / kSpdmWaves;
@@ -525,9 +654,49 @@ int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, f
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
     hipLaunchKernelGGL((spdm_rowseg_kernel<E, VEC, U>), dim3((unsigned)nblocks),
                        dim3(kSpdmThreads), 0, s, h->rowptr, seg, h->nGroup, h->cpg, h->M, D,
-                       (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap);
+                       (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap, skipLen);
     GCOO_CHECK_LAUNCH("spdm_rowseg_kernel");
     return GCOO_OK;
+}
+
+// Rows with at least minLen entries through the long-row kernel (listed on
+// the device, no host synchronisation), the rest through the row-segment
+// kernel.  Scratch (the row list and its counters) lives in the handle.
+template <typename E, int VEC>
+int launch_split(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                 const ConvStatus *guard, hipStream_t s, int minLen)
+{
+    if (!h->longList) {
+        if (hipMalloc(&h->longList, sizeof(int) * ((size_t)h->M + 1)) != hipSuccess ||
+            hipMalloc(&h->longCtr, 256) != hipSuccess) {
+            (void)hipGetLastError();
+            if (h->longList) (void)hipFree(h->longList);
+            h->longList = nullptr;
+            return set_error(GCOO_ERR_ALLOC, "long-row scratch");
+        }
+    }
+    constexpr int W = 4, T = 32;
+    const int colTiles = (N + W * 64 - 1) / (W * 64);
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    hipLaunchKernelGGL(longrow_reset_kernel, dim3(1), dim3(64), 0, s, h->longCtr);
+    GCOO_CHECK_LAUNCH("longrow_reset_kernel");
+    hipLaunchKernelGGL(longrow_classify_kernel, dim3((h->M + 255) / 256), dim3(256), 0, s,
+                       h->rowptr, h->M, minLen, h->longList, h->longCtr, guard,
+                       (long long)h->cap);
+    GCOO_CHECK_LAUNCH("longrow_classify_kernel");
+    int dev = 0, cus = 256, perCu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, spdm_longrow_kernel<E, W, T>,
+                                                     W * 64, 0) != hipSuccess || perCu < 1) {
+        (void)hipGetLastError();
+        perCu = 1;
+    }
+    hipLaunchKernelGGL((spdm_longrow_kernel<E, W, T>), dim3(cus * perCu), dim3(W * 64), 0, s,
+                       h->rowptr, seg, h->longList, h->longCtr, colTiles, D, (unsigned)dBytes, N,
+                       C);
+    GCOO_CHECK_LAUNCH("spdm_longrow_kernel");
+    return launch_rowseg<E, VEC>(h, seg, D, N, C, guard, s, minLen);
 }
 
 template <int CPG, int VEC>
@@ -605,6 +774,25 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
             if (vec >= 8) return launch_lds<E, 8, 8, 16, 64>(h, seg, D, N, C, guard, s);
             if (vec == 4) return launch_lds<E, 4, 16, 16, 128>(h, seg, D, N, C, guard, s);
             return launch_lds<E, 2, 16, 16, 256>(h, seg, D, N, C, guard, s);
+        }
+    }
+    // Long rows through the long-row kernel (variant 4, chosen explicitly or
+    // by the autotuner's table; measured slower than variant 2 on the Zipf
+    // workload so far, so never chosen automatically).  The long-row kernel
+    // stages 16-byte pieces of D rows.
+    const int minLen = cfg.chunk > 0 ? cfg.chunk : 256;
+    const bool splitOk =
+        (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0;
+    if (splitOk && variant == 4) {
+        if constexpr (E::bytes == 4) {
+            if (vec >= 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen);
+            if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen);
+            return launch_split<E, 1>(h, seg, D, N, C, guard, s, minLen);
+        } else {
+            if (vec >= 8) return launch_split<E, 8>(h, seg, D, N, C, guard, s, minLen);
+            if (vec == 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen);
+            if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen);
+            return launch_split<E, 1>(h, seg, D, N, C, guard, s, minLen);
         }
     }
     if constexpr (E::bytes == 4) {

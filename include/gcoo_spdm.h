@@ -161,10 +161,14 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              accumulator set per row (no per-entry row dispatch);
  *   variant 3  LDS-staged: a workgroup stages k-chunks of a D column panel
  *              in LDS, shared by all rows of a row panel.
+ *   variant 4  long rows split off: rows with >= chunk entries (default
+ *              256) go to a persistent kernel in which a workgroup stages the
+ *              D rows of 32 entries at a time in LDS and walks the row's
+ *              chain from there; the others take variant 2.
  *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
  *              D per lane; a wave covers 64*vec columns.
  *   waves      waves per workgroup (1..16).
- *   chunk      k-chunk length of variant 3.
+ *   chunk      k-chunk length of variant 3; long-row threshold of variant 4.
  * All variants give bit-identical C. */
 typedef struct gcoo_spdm_config {
     int variant;
