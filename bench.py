@@ -1,22 +1,29 @@
 """Benchmark of the GCOOSpDM hot path on one MI355X.
 
-A step is one `mutiplyBy` -- dense S -> GCOO conversion plus the grouped-COO
-multiply C = S * D -- over the synthetic n = 8000 square workload (S with
-i.i.d. Bernoulli sparsity 0.99, values U(-1,1); D dense U(-1,1); generated
-in HBM before timing).  This is the path the reference's own results time
-(src/bench.py:9 KERNEL_ONLY=False).
+Headline (the JSON line's `value`): a step is one `mutiplyBy` -- dense S ->
+GCOO conversion plus the grouped-COO multiply C = S * D -- on the square
+n = 8000 workload with sparsity 0.99 (S Bernoulli, values U(-1,1); D dense
+U(-1,1); generated in HBM before timing), FP32.  This is the path the
+reference's own results time (src/bench.py:9 KERNEL_ONLY=False); `value` is
+2 * nnz * N / t in GFLOP/s over the timed steps.
 
-Printed: one JSON line with
-  value      effective GFLOP/s = 2 * nnz * N / t over the timed steps,
-  roofline   the multiply kernel alone: algorithmic bytes
-             12*nnz + 8*(nGroup+1) + 4*K*N + 4*M*N per launch over its mean
-             duration (HIP events on its stream), against 8 TB/s,
-  cpu_baseline  the oracle's C restatement (conversion + fmaf chain) on the
-             host cores, on a bounded sample of the same workload.
+`roofline` is the multiply kernel alone (the dominant kernel of the step):
+algorithmic bytes 12*nnz + 8*(nGroup+1) + 4*K*N + 4*M*N per launch over its
+mean duration, measured with HIP events on the stream it is launched on,
+against MI355X's 8 TB/s.  `traffic` is the HBM bytes per launch measured by
+rocprofv3 counters (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), read
+from profiles/traffic.json when that summary exists.
+
+`cpu_baseline` times the oracle's C restatement (OpenMP) on the host cores on
+a bounded sample.  Extra objects report the other workloads: the kernel alone
+at every sparsity (FP32 and BF16), the 800 x 800 case, the skewed Zipf matrix
+built from COO, 512 batched jobs against 512 sequential launches, and the
+continuous-batching server's p50 / p99 latency against a plain FIFO.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -25,13 +32,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
-# Reference full-path result at n=8000, s=0.99 on P100 (87.824 ms),
-# results/result.syth.p100.raw:34729 -> 2 * 640000 * 8000 / 87.824 ms.
+# Reference full-path results at n = 8000 on P100 (ms -> GFLOP/s with
+# nnz = (1 - s) n^2): results/result.syth.p100.raw:34729,34730,34738.
 REF_P100_GFLOPS = {0.99: 116.6, 0.995: 111.7, 0.999: 79.9}
 
 
-def spdm_bytes(nnz, M, K, N, cpg=4):
+def spdm_bytes(nnz, M, K, N, bf16=False, cpg=4):
     ng = (M + cpg - 1) // cpg
+    if bf16:
+        return 10 * nnz + 8 * (ng + 1) + 2 * K * N + 4 * M * N
     return 12 * nnz + 8 * (ng + 1) + 4 * K * N + 4 * M * N
 
 
@@ -40,17 +49,28 @@ def conv_bytes(nnz, M, K, cpg=4):
     return 4 * M * K + 12 * nnz + 8 * (ng + 1)
 
 
+def event_time(fn, iters, stream):
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(iters):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / 1e3 / iters
+
+
 def cpu_baseline(S_host, D_host, nnz_total, N, budget_s=15.0):
-    """Oracle (C, OpenMP) conversion of the full S, plus the fmaf-chain
-    multiply on a bounded sample of row groups, scaled to GFLOP/s."""
-    import numpy as np
+    """The oracle (C, OpenMP) on the host: dense -> GCOO of the full S plus
+    the fmaf-chain multiply of a sample of whole rows, scaled to GFLOP/s of
+    the full step."""
     import oracle
     threads = oracle.max_threads()
     t0 = time.perf_counter()
-    g = oracle.convert(S_host, 4)
+    oracle.convert(S_host, 4)
     t_conv = time.perf_counter() - t0
     M = S_host.shape[0]
-    # sample: first `rows` rows (whole groups), sized from a probe to ~budget_s
     rows = 64
     while True:
         sub = oracle.convert(S_host[:rows], 4)
@@ -66,18 +86,165 @@ def cpu_baseline(S_host, D_host, nnz_total, N, budget_s=15.0):
         oracle.spdm_f32(sub, D_host, threads=threads)
     dt = (time.perf_counter() - t0) / reps
     spdm_gflops = 2.0 * sub["nnz"] * N / dt / 1e9
-    # full-path estimate: full conversion + the multiply at the sampled rate
     t_full = t_conv + (2.0 * nnz_total * N / 1e9) / spdm_gflops
     return {
         "value": round(2.0 * nnz_total * N / t_full / 1e9, 3),
         "unit": "GFLOP/s",
         "cores": threads,
         "kind": "port",
-        "sample": (f"oracle C restatement (OpenMP): dense->GCOO of the full {M}x{S_host.shape[1]} S "
-                   f"({t_conv * 1e3:.1f} ms) + fmaf-chain multiply of the first {rows} rows "
-                   f"x all {N} columns ({sub['nnz']} nnz, {dt * 1e3:.1f} ms/rep, {reps} reps) "
-                   f"scaled to all {nnz_total} nnz; multiply-only rate {spdm_gflops:.2f} GFLOP/s"),
+        "sample": (f"oracle C restatement (OpenMP, {threads} threads): dense->GCOO of the full "
+                   f"{M}x{S_host.shape[1]} S ({t_conv * 1e3:.1f} ms) + fmaf-chain multiply of the "
+                   f"first {rows} rows x all {N} columns ({sub['nnz']} nnz, {dt * 1e3:.1f} ms/rep, "
+                   f"{reps} reps), scaled to all {nnz_total} nnz; multiply-only rate "
+                   f"{spdm_gflops:.2f} GFLOP/s"),
     }
+
+
+def traffic_from_profiles():
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
+def kernel_sweep(G, dev, n, seed):
+    """Multiply kernel alone at each sparsity, FP32 and BF16, plus the
+    conversion; HIP events on the launch stream."""
+    import torch
+    out = []
+    stream = torch.cuda.current_stream()
+    for s in (0.99, 0.995, 0.999):
+        S = torch.empty((n, n), device=dev)
+        D = torch.empty((n, n), device=dev)
+        C = torch.empty((n, n), device=dev)
+        G.fill_sparse(S, seed, 1, s)
+        G.fill_uniform(D, seed, 0)
+        m = G.GCOOMatrix(n, n, 4).convert_dense(S)
+        nnz = m.nnz
+        D16 = G.to_bf16(D)
+        rec = {"sparsity": s, "nnz": nnz}
+        for dt in ("f32", "bf16"):
+            fn = (lambda: m.spdm(D, C, sync=False)) if dt == "f32" else \
+                 (lambda: m.spdm_bf16(D16, C, sync=False))
+            fn()
+            t = event_time(fn, 20, stream)
+            by = spdm_bytes(nnz, n, n, n, dt == "bf16")
+            rec[dt] = {"us": round(t * 1e6, 1), "gflops": round(2 * nnz * n / t / 1e9, 1),
+                       "hbm_frac": round(by / t / 8e12, 4)}
+        t = event_time(lambda: m.convert_dense(S), 5, stream)
+        rec["convert_us"] = round(t * 1e6, 1)
+        rec["convert_gbs"] = round(conv_bytes(nnz, n, n) / t / 1e9, 1)
+        out.append(rec)
+        del S, D, C, D16, m
+    return out
+
+
+def small_case(G, dev, seed):
+    """800 x 800, p(nonzero) = 0.1: full mutiplyBy path, as the reference's
+    `main -algo=gcoospdm` times it (launch-bound at this size)."""
+    import torch
+    n = 800
+    S = torch.empty((n, n), device=dev)
+    D = torch.empty((n, n), device=dev)
+    C = torch.empty((n, n), device=dev)
+    G.fill_sparse(S, seed, 1, 0.9)
+    G.fill_uniform(D, seed, 0)
+    algo = G.SpGemmBuilder.getShGemmAlgo(S, n, n, "gcoospdm")
+    for _ in range(5):
+        algo.mutiplyBy(D, n, n, C)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        algo.mutiplyBy(D, n, n, C)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / 100
+    nnz = algo.matrix.nnz
+    k = event_time(lambda: algo.matrix.spdm(D, C, sync=False), 50, torch.cuda.current_stream())
+    return {"workload": "square_n800_s0.9", "nnz": nnz, "full_path_us": round(t * 1e6, 1),
+            "full_path_gflops": round(2 * nnz * n / t / 1e9, 1),
+            "kernel_us": round(k * 1e6, 2), "kernel_gflops": round(2 * nnz * n / k / 1e9, 1),
+            "reference_gflops_titanx_full_path": 82.4}
+
+
+def zipf_case(G, dev):
+    """65536 x 65536 S with Zipf(1.5) rows (mean 16), built from COO; D is
+    65536 x 256 N(0,1)."""
+    import numpy as np
+    import torch
+    from gcoospdm_amd import workloads as W
+    M = K = 65536
+    N = 256
+    rows, cols, vals = W.zipf_coo(M, K, seed=11)
+    r, c, v = (torch.from_numpy(x).to(dev) for x in (rows, cols, vals))
+    m = G.GCOOMatrix(M, K, 4)
+    m.convert_coo(r, c, v)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        m.convert_coo(r, c, v)
+    torch.cuda.synchronize()
+    t_conv = (time.perf_counter() - t0) / 3
+    nnz = m.nnz
+    D = torch.from_numpy(np.random.default_rng(12).standard_normal((K, N)).astype(np.float32)).to(dev)
+    D16 = G.to_bf16(D)
+    C = torch.empty((M, N), device=dev)
+    stream = torch.cuda.current_stream()
+    from gcoospdm_amd import autotune
+    out = {"workload": "zipf1.5_65536x65536_mean16_N256", "nnz": nnz,
+           "coo_convert_us": round(t_conv * 1e6, 1)}
+    for dt in ("f32", "bf16"):
+        v_, vec = autotune.plain_config(N, nnz / M, dt == "bf16")
+        m.set_config(variant=v_, vec=vec)
+        fn = (lambda: m.spdm(D, C, sync=False)) if dt == "f32" else \
+             (lambda: m.spdm_bf16(D16, C, sync=False))
+        fn()
+        t = event_time(fn, 20, stream)
+        by = spdm_bytes(nnz, M, K, N, dt == "bf16")
+        out[dt] = {"us": round(t * 1e6, 1), "gflops": round(2 * nnz * N / t / 1e9, 1),
+                   "hbm_frac": round(by / t / 8e12, 4)}
+    return out
+
+
+def serve_case():
+    """The continuous-batching server (C++ driver gcoo_serve) on 512
+    resident jobs, Poisson arrivals at 0.95 x saturated capacity."""
+    exe = os.path.join(ROOT, "gcoospdm_amd", "host", "gcoo_serve")
+    if not os.path.exists(exe):
+        return {"error": "gcoo_serve not built"}
+    r = subprocess.run([exe, "-jobs=512", "-requests=2000", "-loads=0.95",
+                        "-table=" + os.path.join(ROOT, "gcoospdm_amd", "tuning.json")],
+                       capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-400:]}
+    recs = [json.loads(line) for line in r.stdout.splitlines() if line.startswith("{")]
+    out = {}
+    for rec in recs:
+        key = f"{rec['kind']}_{rec['dtype']}" + (f"_{rec['mode']}" if "mode" in rec else "")
+        out[key] = {k: rec[k] for k in rec if k not in ("kind", "dtype", "mode")}
+    return out
+
+
+def profile_kernel(n, s, seed, iters):
+    """A few launches of the dominant kernel and of a copy kernel of known
+    size, for rocprofv3 counter runs (no JSON)."""
+    import torch
+    import gcoospdm_amd as G
+    dev = torch.device("cuda:0")
+    S = torch.empty((n, n), device=dev)
+    D = torch.empty((n, n), device=dev)
+    C = torch.empty((n, n), device=dev)
+    G.fill_sparse(S, seed, 1, s)
+    G.fill_uniform(D, seed, 0)
+    m = G.GCOOMatrix(n, n, 4).convert_dense(S)
+    for _ in range(iters):
+        m.spdm(D, C, sync=False)
+    X = torch.empty(64 << 20, device=dev)   # 256 MiB
+    Y = torch.empty_like(X)
+    for _ in range(iters):
+        Y.copy_(X)                          # calibration: 256 MiB read + 256 MiB written
+    torch.cuda.synchronize()
+    print(f"profiled spdm n={n} s={s} nnz={m.nnz} x{iters}; copy 256MiB x{iters}", flush=True)
 
 
 def main():
@@ -89,15 +256,21 @@ def main():
     ap.add_argument("--sparsity", type=float, default=0.99)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--kernel-iters", type=int, default=50)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--profile-kernel", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "the GCOO multiply shards by row groups with no collective on the hot path, "
-              "so the one-GPU figure is the per-GPU figure.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the multiply "
+              "shards by row groups with no collective on the hot path, so the one-GPU figure "
+              "is the per-GPU figure; multi-GPU runs are not part of this build.",
+              file=sys.stderr)
         sys.exit(2)
+    if args.profile_kernel:
+        profile_kernel(args.n, args.sparsity, args.seed, 10)
+        return
 
     import torch
     import gcoospdm_amd as G
@@ -125,37 +298,41 @@ def main():
     t_step = (time.perf_counter() - t0) / args.steps
     gflops = 2.0 * nnz * N / t_step / 1e9
 
-    # multiply kernel alone: HIP events on the stream it is launched on
     m = algo.matrix
     stream = torch.cuda.current_stream()
     for _ in range(3):
-        m.spdm(D, C, sync=False)
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for _ in range(args.kernel_iters):
         m.spdm(D, C, stream=stream, sync=False)
-    ev1.record(stream)
-    ev1.synchronize()
-    t_kern = ev0.elapsed_time(ev1) / 1e3 / args.kernel_iters
+    t_kern = event_time(lambda: m.spdm(D, C, stream=stream, sync=False), args.kernel_iters, stream)
     bytes_k = spdm_bytes(nnz, M, K, N)
     achieved = bytes_k / t_kern / 1e9
+    torch.cuda.synchronize()
+    t_conv = event_time(lambda: m.convert_dense(S), 10, stream)
 
-    # conversion alone (count + scan + scatter + status readback)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    reps = max(3, args.kernel_iters // 5)
-    for _ in range(reps):
-        m.convert_dense(S)
-    torch.cuda.synchronize()
-    t_conv = (time.perf_counter() - t0) / reps
+    traffic = None
+    tr = traffic_from_profiles()
+    if tr and tr.get("workload") == f"square_n{n}_s{s}":
+        traffic = tr.get("hbm_bytes_per_launch")
 
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(S.cpu().numpy(), D.cpu().numpy(), nnz, N, args.cpu_budget)
 
+    extras = {}
+    if not args.no_extras:
+        del S, D, C, algo, m
+        torch.cuda.empty_cache()
+        extras["kernel_sweep_n8000"] = kernel_sweep(G, dev, n, args.seed)
+        extras["square_n800"] = small_case(G, dev, args.seed)
+        extras["zipf_coo"] = zipf_case(G, dev)
+        from gcoospdm_amd import batchbench
+        extras["batch_512_f32"] = batchbench.run(512, False, 10)
+        extras["batch_512_bf16"] = batchbench.run(512, True, 10)
+        torch.cuda.empty_cache()
+        extras["serving"] = serve_case()
+
     ref = REF_P100_GFLOPS.get(round(s, 3)) if n == 8000 else None
     out = {
-        "metric": "SpDM GFLOP/s (2*nnz*N/t), full mutiplyBy path (dense->GCOO + multiply)",
+        "metric": "SpDM GFLOP/s (2*nnz*N/t) of the full mutiplyBy step (dense->GCOO + multiply)",
         "value": round(gflops, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -167,22 +344,23 @@ def main():
         "vs_baseline": round(gflops / ref, 3) if ref else None,
         "dtype": "f32",
         "data": "synthetic (device counter RNG: S Bernoulli keep 1-s, U(-1,1); D U(-1,1))",
-        "config": {"workload": f"gcoo_spdm_square_n{n}_s{s}", "M": M, "K": K, "N": N,
-                   "sparsity": s, "cpg": 4, "nnz": nnz,
-                   "path": "mutiplyBy: dense->GCOO conversion + grouped-COO multiply",
+        "config": {"workload": f"square_n{n}_s{s}", "M": M, "K": K, "N": N, "sparsity": s,
+                   "cpg": 4, "nnz": nnz,
+                   "path": "mutiplyBy: dense->GCOO conversion + grouped-COO multiply, FP32",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "spdm_f32_wave_kernel",
+        "roofline": {"bound": "hbm", "kernel": "spdm_rowseg_kernel (FP32 multiply)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": bytes_k, "mean_launch_us": round(t_kern * 1e6, 2),
                      "fp32_tflops": round(2.0 * nnz * N / t_kern / 1e12, 3),
                      "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS},
         "kernel_only_gflops": round(2.0 * nnz * N / t_kern / 1e9, 3),
-        "convert_ms": round(t_conv * 1e3, 4),
+        "convert_us": round(t_conv * 1e6, 1),
         "convert_gbs": round(conv_bytes(nnz, M, K) / t_conv / 1e9, 1),
-        "baseline_ref": "reference full path on P100 (results/result.syth.p100.raw:34729)",
+        "baseline_ref": "reference full mutiplyBy path on P100, results/result.syth.p100.raw:34729",
         "cpu_baseline": cpu,
     }
+    out.update(extras)
     print(json.dumps(out), flush=True)
 
 

@@ -103,10 +103,11 @@ _lib: Optional[ctypes.CDLL] = None
 
 
 class SpdmConfig(ctypes.Structure):
-    """gcoo_spdm_config (include/gcoo_spdm.h): variant, vec, waves, chunk;
-    zero fields mean automatic."""
+    """gcoo_spdm_config (include/gcoo_spdm.h): variant, vec, waves, chunk,
+    unroll; zero fields mean automatic."""
     _fields_ = [("variant", ctypes.c_int), ("vec", ctypes.c_int),
-                ("waves", ctypes.c_int), ("chunk", ctypes.c_int)]
+                ("waves", ctypes.c_int), ("chunk", ctypes.c_int),
+                ("unroll", ctypes.c_int)]
 
 
 class GCOOError(RuntimeError):
@@ -273,16 +274,17 @@ class GCOOMatrix:
         _check(fn(self._h, _ptr(D16), K, N, _ptr(C), _stream(stream)), "gcoo_spdm_bf16")
         return C
 
-    def set_config(self, variant=0, vec=0, waves=0, chunk=0):
+    def set_config(self, variant=0, vec=0, waves=0, chunk=0, unroll=0):
         """Multiply-kernel variant / tile shape (0 = automatic)."""
-        cfg = SpdmConfig(variant, vec, waves, chunk)
+        cfg = SpdmConfig(variant, vec, waves, chunk, unroll)
         _check(lib().gcoo_set_spdm_config(self._h, ctypes.byref(cfg)), "gcoo_set_spdm_config")
         return self
 
     def get_config(self):
         cfg = SpdmConfig()
         _check(lib().gcoo_get_spdm_config(self._h, ctypes.byref(cfg)), "gcoo_get_spdm_config")
-        return dict(variant=cfg.variant, vec=cfg.vec, waves=cfg.waves, chunk=cfg.chunk)
+        return dict(variant=cfg.variant, vec=cfg.vec, waves=cfg.waves, chunk=cfg.chunk,
+                    unroll=cfg.unroll)
 
     def multiply_by(self, S, D, C, stream=None):
         """Convert S then C = S * D (the reference's mutiplyBy)."""

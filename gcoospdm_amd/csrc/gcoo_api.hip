@@ -435,7 +435,7 @@ int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
 {
     if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
     if (!cfg) {
-        h->spdm_cfg = gcoo_spdm_config{0, 0, 0, 0};
+        h->spdm_cfg = gcoo_spdm_config{0, 0, 0, 0, 0};
         return GCOO_OK;
     }
     if (cfg->variant < 0 || cfg->variant > 4)
@@ -445,6 +445,8 @@ int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
     if (cfg->waves < 0 || cfg->waves > 16)
         return set_error(GCOO_ERR_INVALID, "waves %d (0..16)", cfg->waves);
     if (cfg->chunk < 0) return set_error(GCOO_ERR_INVALID, "chunk %d", cfg->chunk);
+    if (cfg->unroll != 0 && cfg->unroll != 8 && cfg->unroll != 16 && cfg->unroll != 32)
+        return set_error(GCOO_ERR_INVALID, "unroll %d (0, 8, 16 or 32)", cfg->unroll);
     h->spdm_cfg = *cfg;
     return GCOO_OK;
 }

@@ -82,7 +82,7 @@ struct gcoo_matrix {
     mutable unsigned *longCtr = nullptr;   // [0] listed rows, [1] next work item
 
     // multiply-kernel configuration (gcoo_set_spdm_config)
-    gcoo_spdm_config spdm_cfg = {0, 0, 0, 0};
+    gcoo_spdm_config spdm_cfg = {0, 0, 0, 0, 0};
 
     gcoo::ConvStatus *d_status = nullptr;  // device
     gcoo::ConvStatus *h_status = nullptr;  // pinned host mirror

@@ -640,11 +640,10 @@ int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, floa
     return GCOO_OK;
 }
 
-template <typename E, int VEC>
-int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                  const ConvStatus *guard, hipStream_t s, int skipLen = INT_MAX)
+template <typename E, int VEC, int U>
+int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                    const ConvStatus *guard, hipStream_t s, int skipLen)
 {
-    constexpr int U = 8;
     const int nGroupBlocks = (h->nGroup + kSpdmWaves - 1) / kSpdmWaves;
     const int tileCols = 64 * VEC;
     const int nColTiles = (N + tileCols - 1) / tileCols;
@@ -657,6 +656,19 @@ int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, f
                        (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap, skipLen);
     GCOO_CHECK_LAUNCH("spdm_rowseg_kernel");
     return GCOO_OK;
+}
+
+// unroll (entries in flight per wave): 8, 16 or 32; 8 by default (measured
+// best or equal at n = 8000 for every sparsity with vec 2)
+template <typename E, int VEC>
+int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                  const ConvStatus *guard, hipStream_t s, int skipLen = INT_MAX)
+{
+    const int u = h->spdm_cfg.unroll ? h->spdm_cfg.unroll : 8;
+    if (u <= 8) return launch_rowseg_u<E, VEC, 8>(h, seg, D, N, C, guard, s, skipLen);
+    if (u <= 16 || VEC * E::bytes > 8)
+        return launch_rowseg_u<E, VEC, 16>(h, seg, D, N, C, guard, s, skipLen);
+    return launch_rowseg_u<E, VEC, 32>(h, seg, D, N, C, guard, s, skipLen);
 }
 
 // Rows with at least minLen entries through the long-row kernel (listed on

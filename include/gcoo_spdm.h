@@ -169,12 +169,15 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              D per lane; a wave covers 64*vec columns.
  *   waves      waves per workgroup (1..16).
  *   chunk      k-chunk length of variant 3; long-row threshold of variant 4.
+ *   unroll     entries whose D loads are in flight per wave in variant 2
+ *              (8, 16 or 32).
  * All variants give bit-identical C. */
 typedef struct gcoo_spdm_config {
     int variant;
     int vec;
     int waves;
     int chunk;
+    int unroll;
 } gcoo_spdm_config;
 int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg);
 int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
