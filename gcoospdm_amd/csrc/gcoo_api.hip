@@ -116,8 +116,9 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
     }
     if ((rc = read_status(h, s))) return rc;
     const int64_t total = h->h_status->total;
-    if (total > h->cap || h->cap == 0) {
-        if ((rc = ensure_capacity(h, total))) return rc;
+    const int64_t need = total + h->h_status->maxRow + kSegSlack;
+    if (need > h->cap || h->cap == 0) {
+        if ((rc = ensure_capacity(h, need))) return rc;
         if ((rc = launch_scatter(h, S, s))) return rc;
         if (D && (rc = launch_spdm_f32(h, D, N, C, nullptr, s))) return rc;
         GCOO_HIP_TRY(hipStreamSynchronize(s));
@@ -438,7 +439,7 @@ int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
         h->spdm_cfg = gcoo_spdm_config{0, 0, 0, 0, 0};
         return GCOO_OK;
     }
-    if (cfg->variant < 0 || cfg->variant > 4)
+    if (cfg->variant < 0 || cfg->variant > 5)
         return set_error(GCOO_ERR_INVALID, "variant %d", cfg->variant);
     if (cfg->vec != 0 && cfg->vec != 1 && cfg->vec != 2 && cfg->vec != 4 && cfg->vec != 8)
         return set_error(GCOO_ERR_INVALID, "vec %d (0, 1, 2, 4 or 8)", cfg->vec);

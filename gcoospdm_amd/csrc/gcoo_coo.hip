@@ -220,7 +220,7 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols, const float *v
         return set_error(GCOO_ERR_OVERFLOW, "nnz %lld does not fit in int32",
                          (long long)h->h_status->total);
     const int64_t total = h->h_status->total;
-    if ((rc = ensure_capacity(h, total))) return rc;
+    if ((rc = ensure_capacity(h, total + h->h_status->maxRow + kSegSlack))) return rc;
     if (total > 0) {
         GCOO_HIP_TRY(hipMemsetAsync(h->rowcnt, 0, sizeof(int) * ((size_t)h->M + 1), s));
         hipLaunchKernelGGL(coo_place_kernel, dim3(grid_for(nnz)), dim3(kThreads), 0, s, rows,

@@ -137,6 +137,10 @@ int launch_f32_to_bf16(const float *x, uint16_t *y, int64_t n, hipStream_t s);
 // seg16[i] = (seg[i].x, bf16 bits of seg[i].y) for i < n.
 int launch_seg_to_bf16(const int2 *seg, int2 *seg16, int64_t n, hipStream_t s);
 
+// Entries of segment capacity beyond nnz + maxRow that every conversion
+// provides, so kernels may read a few entries past a row's end.
+constexpr int kSegSlack = 64;
+
 // Number of XCDs a launch is spread over (round-robin dispatch of
 // workgroups).  gfx950 (MI355X) has 8.
 constexpr int kNumXcd = 8;

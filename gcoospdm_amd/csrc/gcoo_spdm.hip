@@ -359,6 +359,121 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Variant 5: a group's rows in parallel.  One wave owns 4 consecutive rows
+// (a cpg = 4 group) and a 16 * VEC-column tile: lane l serves row
+// (l >> 4) and columns tile + VEC * (l & 15).  Every step the 4 row streams
+// each take their row's next entry, so one buffer load moves 4 D-row pieces
+// of 16*VEC elements -- 16 bytes per lane, the widest gather -- while the
+// column tile (and so the D panel a tile sweeps) stays narrow enough for the
+// XCD's L2.  Entries come from 4 scalar streams (one s_load_dwordx16 per 8
+// entries per row); a row that has run out takes the entry (a = -0.0, k past
+// the end): the load returns +0.0 and fmaf(-0.0, +0.0, acc) == acc exactly
+// for every acc (+/-0, NaN, inf included), so no predication is needed and
+// each row's chain is its own entries in ascending k.
+// ---------------------------------------------------------------------------
+template <typename E, int VEC>
+__global__ __launch_bounds__(kSpdmThreads) void spdm_grouprows_kernel(
+    const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int nQuads,
+    const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C,
+    const ConvStatus *__restrict__ guard, long long cap)
+{
+    constexpr int U = 8;                        // entries per scalar batch per row
+    constexpr int TN = 16 * VEC;                // columns per tile
+    // a row that runs out keeps reading (unused) entries up to the longest
+    // sibling's length past its start: the segments must have nnz + maxRow +
+    // kSegSlack entries of capacity (every conversion provides that)
+    if (guard != nullptr && guard->total + guard->maxRow + kSegSlack > cap) return;
+    const int nQuadBlocks = (nQuads + kSpdmWaves - 1) / kSpdmWaves;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = L / nQuadBlocks;
+    const int qb = L - ct * nQuadBlocks;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int quad = qb * kSpdmWaves + wave;
+    if (quad >= nQuads) return;
+    const int lane = __lane_id();
+    const int sub = lane >> 4;
+    const int j = ct * TN + (lane & 15) * VEC;
+    const bool active = j < N;
+    const int r0 = quad * 4;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
+    const unsigned rowBytes = (unsigned)N * E::bytes;
+    const unsigned voffLane = active ? (unsigned)j * E::bytes : dBytes;
+    // the 4 row streams (scalar)
+    int p[4], pe[4];
+    int steps = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = r0 + q;
+        p[q] = r < M ? rowptr[r] : 0;
+        pe[q] = r < M ? rowptr[r + 1] : 0;
+        steps = max(steps, pe[q] - p[q]);
+    }
+    float acc[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] = 0.0f;
+    for (int s = 0; s < steps; s += U) {
+        // U entries of each row (reads past a row's end stay inside seg: the
+        // allocation has at least 1024 entries of slack beyond nnz)
+        int kq[4][U];
+        float aq[4][U];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int2 e = seg[p[q] + s + u];
+                const bool live = p[q] + s + u < pe[q];
+                kq[q][u] = live ? e.x : -1;
+                aq[q][u] = live ? E::val(e.y) : -0.0f;
+            }
+        }
+        float d[U][VEC];
+        float a[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = sub == 0 ? kq[0][u] : sub == 1 ? kq[1][u] : sub == 2 ? kq[2][u] : kq[3][u];
+            a[u] = sub == 0 ? aq[0][u] : sub == 1 ? aq[1][u] : sub == 2 ? aq[2][u] : aq[3][u];
+            const unsigned off = k >= 0 ? voffLane + (unsigned)k * rowBytes : dBytes;
+            buf_load<E, VEC>(d[u], rsrc, (int)off, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) acc[v] = __builtin_fmaf(a[u], d[u][v], acc[v]);
+    }
+    const int r = r0 + sub;
+    if (active && r < M) {
+        float *dst = C + (size_t)r * N + j;
+        if constexpr (VEC <= 4) {
+            store_vec<VEC>(dst, acc);
+        } else {
+            const float (&lo)[4] = *reinterpret_cast<const float(*)[4]>(&acc[0]);
+            const float (&hi)[4] = *reinterpret_cast<const float(*)[4]>(&acc[4]);
+            store_vec<4>(dst, lo);
+            store_vec<4>(dst + 4, hi);
+        }
+    }
+}
+
+template <typename E, int VEC>
+int launch_grouprows(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                     const ConvStatus *guard, hipStream_t s)
+{
+    const int nQuads = (h->M + 3) / 4;
+    const int nQuadBlocks = (nQuads + kSpdmWaves - 1) / kSpdmWaves;
+    const int nColTiles = (N + 16 * VEC - 1) / (16 * VEC);
+    const long long nblocks = (long long)nQuadBlocks * nColTiles;
+    if (nblocks > INT32_MAX)
+        return set_error(GCOO_ERR_OVERFLOW, "grid of %lld workgroups too large", nblocks);
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    hipLaunchKernelGGL((spdm_grouprows_kernel<E, VEC>), dim3((unsigned)nblocks), dim3(kSpdmThreads),
+                       0, s, h->rowptr, seg, h->M, nQuads, D, (unsigned)dBytes, N, C, guard,
+                       (long long)h->cap);
+    GCOO_CHECK_LAUNCH("spdm_grouprows_kernel");
+    return GCOO_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Variant 3: LDS-staged D panel.  A workgroup of WAVES waves owns a panel of
 // WAVES*RPW consecutive rows and a TN = 64*VEC column tile.  It walks k in
 // chunks of TK: every chunk's D rows D[k0 .. k0+TK)[tile] are staged once in
@@ -786,6 +901,19 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
             if (vec >= 8) return launch_lds<E, 8, 8, 16, 64>(h, seg, D, N, C, guard, s);
             if (vec == 4) return launch_lds<E, 4, 16, 16, 128>(h, seg, D, N, C, guard, s);
             return launch_lds<E, 2, 16, 16, 256>(h, seg, D, N, C, guard, s);
+        }
+    }
+    if (variant == 5) {
+        // widest 16-byte-per-lane vector the operands allow
+        if constexpr (E::bytes == 4) {
+            if (vmax >= 4) return launch_grouprows<E, 4>(h, seg, D, N, C, guard, s);
+            if (vmax == 2) return launch_grouprows<E, 2>(h, seg, D, N, C, guard, s);
+            return launch_grouprows<E, 1>(h, seg, D, N, C, guard, s);
+        } else {
+            if (vmax >= 8) return launch_grouprows<E, 8>(h, seg, D, N, C, guard, s);
+            if (vmax == 4) return launch_grouprows<E, 4>(h, seg, D, N, C, guard, s);
+            if (vmax == 2) return launch_grouprows<E, 2>(h, seg, D, N, C, guard, s);
+            return launch_grouprows<E, 1>(h, seg, D, N, C, guard, s);
         }
     }
     // Long rows through the long-row kernel (variant 4, chosen explicitly or

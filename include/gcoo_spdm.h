@@ -165,6 +165,8 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              256) go to a persistent kernel in which a workgroup stages the
  *              D rows of 32 entries at a time in LDS and walks the row's
  *              chain from there; the others take variant 2.
+ *   variant 5  group rows: one wave walks the 4 rows of a row group at once,
+ *              16 lanes per row, each lane loading 16 bytes of D per entry.
  *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
  *              D per lane; a wave covers 64*vec columns.
  *   waves      waves per workgroup (1..16).

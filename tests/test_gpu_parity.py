@@ -192,7 +192,7 @@ def test_reference_api_mirror(dev):
         algo.runKernelTest(D[:-1], K - 1, N, C1)   # K mismatch -> error, no crash
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("vec", [1, 2, 4])
 @pytest.mark.parametrize("M,K,N,dens", [(130, 257, 520, 0.05), (61, 1030, 200, 0.3),
                                         (1000, 900, 64, 0.01)])
