@@ -37,6 +37,9 @@ int ensure_capacity(gcoo_matrix *h, int64_t need)
     if (h->rowcol) GCOO_HIP_TRY(hipFree(h->rowcol));
     if (h->seg) GCOO_HIP_TRY(hipFree(h->seg));
     if (h->seg16) GCOO_HIP_TRY(hipFree(h->seg16));
+    if (h->plist) GCOO_HIP_TRY(hipFree(h->plist));
+    h->plist = nullptr;
+    invalidate_panel_lists(h);
     h->vals = nullptr;
     h->rowcol = nullptr;
     h->seg = nullptr;
@@ -108,6 +111,7 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
 {
     h->converted = false;
     h->seg16_valid = false;
+    invalidate_panel_lists(h);
     int rc = launch_count_and_scan(h, S, s);
     if (rc) return rc;
     if (h->cap > 0) {
@@ -119,6 +123,8 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
     const int64_t need = total + h->h_status->maxRow + kSegSlack;
     if (need > h->cap || h->cap == 0) {
         if ((rc = ensure_capacity(h, need))) return rc;
+        h->nnz = total;  // the multiply's automatic variant choice reads it
+        h->maxRow = h->h_status->maxRow;
         if ((rc = launch_scatter(h, S, s))) return rc;
         if (D && (rc = launch_spdm_f32(h, D, N, C, nullptr, s))) return rc;
         GCOO_HIP_TRY(hipStreamSynchronize(s));
@@ -337,6 +343,8 @@ int gcoo_destroy(gcoo_handle h)
     if (h->longCtr) (void)hipFree(h->longCtr);
     if (h->seg) (void)hipFree(h->seg);
     if (h->seg16) (void)hipFree(h->seg16);
+    if (h->plist) (void)hipFree(h->plist);
+    if (h->poff) (void)hipFree(h->poff);
     if (h->rowptr) (void)hipFree(h->rowptr);
     if (h->rowcnt) (void)hipFree(h->rowcnt);
     if (h->d_status) (void)hipFree(h->d_status);
@@ -362,6 +370,7 @@ int gcoo_convert_coo(gcoo_handle h, const int *rows, const int *cols, const floa
     if (nnz > INT32_MAX) return set_error(GCOO_ERR_OVERFLOW, "nnz %lld > int32", (long long)nnz);
     h->converted = false;
     h->seg16_valid = false;
+    invalidate_panel_lists(h);
     int rc = convert_coo(h, rows, cols, vals, nnz, as_stream(s));
     if (rc) return rc;
     h->converted = true;
@@ -439,7 +448,7 @@ int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
         h->spdm_cfg = gcoo_spdm_config{0, 0, 0, 0, 0};
         return GCOO_OK;
     }
-    if (cfg->variant < 0 || cfg->variant > 5)
+    if (cfg->variant < 0 || cfg->variant > 6)
         return set_error(GCOO_ERR_INVALID, "variant %d", cfg->variant);
     if (cfg->vec != 0 && cfg->vec != 1 && cfg->vec != 2 && cfg->vec != 4 && cfg->vec != 8)
         return set_error(GCOO_ERR_INVALID, "vec %d (0, 1, 2, 4 or 8)", cfg->vec);

@@ -81,6 +81,16 @@ struct gcoo_matrix {
     mutable int *longList = nullptr;       // [M + 1] rows of the long-row kernel
     mutable unsigned *longCtr = nullptr;   // [0] listed rows, [1] next work item
 
+    // Panel lists of the LDS-staged multiply (variant 6), built on first use
+    // after a conversion: the entries of each (row panel, k chunk) pair,
+    // ordered by (row, k), as ((row slot << 16) | k - chunk start, value
+    // bits), and their start offsets (see gcoo_panel.hip).
+    mutable uint2 *plist = nullptr;        // [cap]
+    mutable int *poff = nullptr;           // [poff_elems]
+    mutable size_t poff_elems = 0;
+    mutable const int2 *plist_src = nullptr;  // segments the lists hold (null: stale)
+    mutable int plist_tk = 0;                 // chunk length they were built for
+
     // multiply-kernel configuration (gcoo_set_spdm_config)
     gcoo_spdm_config spdm_cfg = {0, 0, 0, 0, 0};
 
@@ -123,6 +133,26 @@ int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
 // Enqueue (on s) the BF16 copy of h's segments unless it is current; the
 // copy is usable by later work on s (or by anyone after s is synchronised).
 int ensure_seg16(gcoo_matrix *h, hipStream_t s);
+
+// ---- panel lists of the LDS-staged multiply (gcoo_panel.hip) ----
+// A panel is kPanelRows consecutive rows; wave w of the multiply owns panel
+// rows [w * kPanelRowsPerWave, (w + 1) * kPanelRowsPerWave).
+constexpr int kPanelRows = 256;
+constexpr int kPanelRowsPerWave = 16;
+// Enqueue (on s) the panel lists of h's entries from seg (h->seg or
+// h->seg16) for k chunks of tk rows, unless they are current.  Kernels exit
+// early when guard is non-null and guard->total > h->cap.
+int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk,
+                       const ConvStatus *guard, hipStream_t s);
+// Ints between the offset arrays of consecutive panels: nC * kPanelRows + 1
+// offsets, padded to a multiple of 4 so that every panel's array is 16-byte
+// aligned.
+__host__ __device__ inline size_t panel_stride(int nC)
+{
+    return ((size_t)nC * kPanelRows + 1 + 3) & ~(size_t)3;
+}
+// Forget the panel lists (after a conversion or a capacity change).
+inline void invalidate_panel_lists(const gcoo_matrix *h) { h->plist_src = nullptr; }
 
 // ---- batches (gcoo_batch.hip) ----
 // gcoo_batch_commit without the final stream synchronisation.

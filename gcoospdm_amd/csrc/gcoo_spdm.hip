@@ -18,6 +18,8 @@
 //   * workgroups are remapped so each XCD sweeps whole column tiles of C:
 //     the D column panel of a tile (K x 64*VEC floats) then stays in that
 //     XCD's L2 while all groups stream through it.
+#include <type_traits>
+
 #include "gcoo_internal.h"
 
 namespace gcoo {
@@ -610,6 +612,183 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Variant 6: panel-staged.  One workgroup of 16 waves owns a panel of
+// kPanelRows (256) rows and a 256-column tile of C (4 columns per lane);
+// wave w keeps the accumulators of panel rows [16w, 16w + 16) in VGPRs.  K is
+// walked in chunks of TK rows of D: a chunk's tile piece is copied to LDS once,
+// by direct buffer-to-LDS loads issued one chunk ahead into the other of two
+// buffers, and serves every entry of the panel in that chunk -- on average
+// 256 * density entries per staged D row instead of one L2 gather per entry.
+//   * Waves 0..14 stage the chunks; wave 15 stages nothing and, in one
+//     workgroup of each column tile per chunk (panel == chunk % panels),
+//     touches the tile piece of the chunk kPanelPrefetch chunks ahead so that
+//     it is in L2 when the others stage it.  The panels of a tile run in step,
+//     so without this every chunk would be a simultaneous L2 miss of all of
+//     them.  Wave 15 never waits for those loads inside the loop: vector
+//     memory completions are counted in issue order, and a wait would stall
+//     on the miss it is there to absorb.
+//   * Each wave takes its rows' entries of the chunk from the panel list
+//     (built by gcoo_panel.hip from the row segments, ordered by row then k),
+//     8 at a time from scalar loads: 8 LDS reads in flight, then 8 fmaf steps.
+//   * The step's row is wave-uniform but varies per entry.  The 64
+//     accumulators sit in v0..v63 (component v of row slot q in v[16v + q])
+//     and each step is 4 v_fma_f32 in VGPR-index mode, source 2 and
+//     destination relative to M0 = q: acc[q] = fmaf(x, d, acc[q]) with no
+//     moves and no branches.
+// Every row's chain is its entries in ascending k, as in all variants.
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int kPanelPrefetch = 4;  // chunks between the L2 touch and the staging
+constexpr double kPanelMinDensity = 0.007;  // automatic choice of variant 6 (FP32)
+
+// a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform q < 16.
+// M0 (the index register, also the LDS address of buffer-to-LDS loads) is
+// restored before the block ends.
+__device__ __forceinline__ void panel_fma(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
+                                          int q, float x, const float (&d)[4])
+{
+    int saved;
+    asm volatile(
+        "s_mov_b32 %4, m0\n\t"
+        "s_set_gpr_idx_on %5, gpr_idx(SRC2,DST)\n\t"
+        "v_fma_f32 v0, %6, %7, v0\n\t"
+        "v_fma_f32 v16, %6, %8, v16\n\t"
+        "v_fma_f32 v32, %6, %9, v32\n\t"
+        "v_fma_f32 v48, %6, %10, v48\n\t"
+        "s_set_gpr_idx_off\n\t"
+        "s_mov_b32 m0, %4"
+        : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3), "=&s"(saved)
+        : "s"(q & 15), "s"(x), "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]));
+}
+
+template <typename E, int TK>
+__global__ __launch_bounds__(1024) void spdm_panel_kernel(
+    const int *__restrict__ poff, const uint2 *__restrict__ plist, int M, int K, int nC,
+    const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C, int nPanels,
+    const ConvStatus *__restrict__ guard, long long cap)
+{
+    constexpr int VEC = 4;
+    constexpr int RPW = kPanelRowsPerWave;
+    static_assert(RPW == 16 && kPanelRows == 256, "16 waves of 16 rows");
+    constexpr int TN = 64 * VEC;              // columns per tile
+    constexpr int LB = VEC * E::bytes;        // bytes of a D row per lane
+    constexpr int ROWB = 64 * LB;             // bytes of a staged D row
+    constexpr int CHUNKB = TK * ROWB;         // bytes of a staged chunk
+    constexpr int NLOAD = CHUNKB / 1024;      // 1 KB wave loads per chunk
+    constexpr int SW = 15;                    // staging waves
+    static_assert(CHUNKB % 1024 == 0 && NLOAD % SW == 0, "chunk loads must split over 15 waves");
+    constexpr int LPW = NLOAD / SW;           // wave loads per staging wave per chunk
+    constexpr int PPR = ROWB / 16;            // 16-byte pieces per staged D row
+    constexpr int RPL = 64 / PPR;             // D rows per wave load
+    constexpr int LINES = ROWB / 128;         // 128-byte lines per staged D row
+    constexpr int RPP = 64 / LINES;           // D rows per prefetch load
+    constexpr int NPF = (TK + RPP - 1) / RPP; // prefetch loads per chunk
+    constexpr int DUMMY = 2 * CHUNKB / 16;    // 256-byte target of the prefetch loads
+    static_assert(2 * CHUNKB + 256 <= 160 * 1024, "LDS");
+    __shared__ f32x4 sD[DUMMY + 16];
+
+    if (guard != nullptr && guard->total > cap) return;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = L / nPanels;
+    const int panel = L - ct * nPanels;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = __lane_id();
+    const int col0 = ct * TN;
+    const int rowBytes = N * E::bytes;
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
+
+    // Staging: this lane's 16-byte piece of every wave load, D row (first
+    // row of the load + prow), columns from pcol; pieces past N or K read
+    // zeros.
+    const int prow = lane / PPR;
+    const int pcol = col0 + (lane % PPR) * (16 / E::bytes);
+    const int pvoff = pcol < N ? pcol * E::bytes + prow * rowBytes : (int)dBytes;
+    auto stage = [&](int c, int buf) {
+        if (wave < SW) {
+#pragma unroll
+            for (int i = 0; i < LPW; ++i) {
+                const int li = wave * LPW + i;     // wave load index in the chunk
+                const int k0 = c * TK + li * RPL;  // its first D row
+                const int voff = k0 + prow < K ? pvoff : (int)dBytes;
+                const int soff = k0 < K ? k0 * rowBytes : 0;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rsrc,
+                    (__attribute__((address_space(3))) void *)&sD[(buf * CHUNKB + li * 1024) / 16],
+                    16, voff, soff, 0, 0);
+            }
+        }
+    };
+    // L2 touch of chunk c by wave 15: 4 bytes of every 128-byte line of the
+    // tile piece (lane: row lane / LINES, line lane % LINES).
+    const int frow = lane / LINES;
+    const int fbyte = col0 * E::bytes + (lane % LINES) * 128;
+    auto touch = [&](int c) {
+#pragma unroll
+        for (int i = 0; i < NPF; ++i) {
+            const int r = i * RPP + frow;
+            const bool ok = r < TK && c * TK + r < K && fbyte < rowBytes;
+            const int voff = ok ? fbyte + frow * rowBytes : (int)dBytes;
+            const int soff = ok ? (c * TK + i * RPP) * rowBytes : 0;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                rsrc, (__attribute__((address_space(3))) void *)&sD[DUMMY], 4, voff, soff, 0, 0);
+        }
+    };
+
+    f32x16 a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    const int *op = poff + (size_t)panel * panel_stride(nC) + wave * RPW;
+    const unsigned char *ldsLane = reinterpret_cast<const unsigned char *>(sD) + lane * LB;
+
+    stage(0, 0);
+    int pb = op[0], pe = op[RPW];
+    for (int c = 0; c < nC; ++c) {
+        // chunk c has landed in LDS for every wave, and every wave is done
+        // with chunk c - 1, whose buffer the next stage overwrites (a bare
+        // barrier: a fence here would wait for wave 15's touches too)
+        if (wave < SW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        asm volatile("s_barrier" ::: "memory");
+        if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
+        if (wave == SW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
+        const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
+        const int cn = c + 1 < nC ? c + 1 : c;
+        const int pbn = op[(size_t)cn * kPanelRows];
+        const int pen = op[(size_t)cn * kPanelRows + RPW];
+        auto run = [&](auto bn, int p) {
+            constexpr int B = decltype(bn)::value;
+            uint2 m[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) m[u] = plist[p + u];
+            float d[B][VEC];
+#pragma unroll
+            for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + (m[u].x & 0xffffu) * ROWB);
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                panel_fma(a0, a1, a2, a3, (int)(m[u].x >> 16), E::val((int)m[u].y), d[u]);
+        };
+        int p = pb;
+        for (; p + 8 <= pe; p += 8) run(std::integral_constant<int, 8>{}, p);
+        if (p + 4 <= pe) { run(std::integral_constant<int, 4>{}, p); p += 4; }
+        if (p + 2 <= pe) { run(std::integral_constant<int, 2>{}, p); p += 2; }
+        if (p < pe) run(std::integral_constant<int, 1>{}, p);
+        pb = pbn;
+        pe = pen;
+    }
+    // no LDS write may land after the workgroup's LDS is released
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+
+    const int j = col0 + lane * VEC;
+    if (j < N) {
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = panel * kPanelRows + wave * RPW + q;
+            const float out[VEC] = {a0[q], a1[q], a2[q], a3[q]};
+            if (r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Long rows.  A row with thousands of entries walked by one wave is a serial
 // chain of dependent load -> FMA steps: latency-bound, and on skewed matrices
 // it is the tail of the whole launch.  Long rows are instead listed (order
@@ -755,6 +934,24 @@ int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, floa
     return GCOO_OK;
 }
 
+template <typename E, int TK>
+int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                 const ConvStatus *guard, hipStream_t s)
+{
+    int rc = ensure_panel_lists(h, seg, TK, guard, s);
+    if (rc) return rc;
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    const int nPanels = (h->M + kPanelRows - 1) / kPanelRows;
+    const int nC = (h->K + TK - 1) / TK;
+    const int tiles = (N + 255) / 256;
+    const long long nblocks = (long long)nPanels * tiles;
+    hipLaunchKernelGGL((spdm_panel_kernel<E, TK>), dim3((unsigned)nblocks), dim3(1024), 0, s,
+                       h->poff, h->plist, h->M, h->K, nC, D, (unsigned)dBytes, N, C, nPanels,
+                       guard, (long long)h->cap);
+    GCOO_CHECK_LAUNCH("spdm_panel_kernel");
+    return GCOO_OK;
+}
+
 template <typename E, int VEC, int U>
 int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s, int skipLen)
@@ -882,7 +1079,18 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         return set_error(GCOO_ERR_UNSUPPORTED,
                          "dense operand of %llu bytes: the buffer-addressed kernels need < 2 GiB",
                          dBytes);
-    int variant = cfg.variant ? cfg.variant : 2;
+    int variant = cfg.variant;
+    if (variant == 0) {
+        // Panel staging costs a fixed K x N read of D per 256-row panel, row
+        // segments cost one D-row gather per entry: the panel kernel wins from
+        // about 0.7 % density on (measured at n = 8000, FP32).  BF16 gathers
+        // move half the bytes and stay ahead.  The density is the handle's
+        // last completed conversion (a first optimistic launch does not know
+        // the new count yet); every variant gives the same bits.
+        const double cells = (double)h->M * (double)h->K;
+        const bool dense = cells > 0 && (double)h->nnz >= kPanelMinDensity * cells;
+        variant = (E::bytes == 4 && dense && h->M >= 2 * kPanelRows && N >= 256) ? 6 : 2;
+    }
     if (variant == 1) variant = 2;
     // variant 3 stages 16-byte pieces of D rows: it needs whole pieces per
     // row (N * bytes % 16 == 0) and a 16-byte aligned D
@@ -903,6 +1111,14 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
             return launch_lds<E, 2, 16, 16, 256>(h, seg, D, N, C, guard, s);
         }
     }
+    // variant 6 stages 16-byte pieces of D rows (N * bytes % 16 == 0, a
+    // 16-byte aligned D) and stores 4 columns per lane (vec 4 allowed)
+    if (variant == 6 && (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0 &&
+        vmax >= 4) {
+        if constexpr (E::bytes == 4) return launch_panel<E, 75>(h, seg, D, N, C, guard, s);
+        else return launch_panel<E, 150>(h, seg, D, N, C, guard, s);
+    }
+    if (variant == 6) variant = 2;
     if (variant == 5) {
         // widest 16-byte-per-lane vector the operands allow
         if constexpr (E::bytes == 4) {

@@ -167,6 +167,10 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              chain from there; the others take variant 2.
  *   variant 5  group rows: one wave walks the 4 rows of a row group at once,
  *              16 lanes per row, each lane loading 16 bytes of D per entry.
+ *   variant 6  panel-staged: a 1024-thread workgroup owns 256 rows x 256
+ *              columns; k-chunks of the D tile are staged in LDS once and
+ *              serve all 256 rows (needs N * element bytes % 16 == 0, a
+ *              16-byte aligned D and C; otherwise variant 2 runs).
  *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
  *              D per lane; a wave covers 64*vec columns.
  *   waves      waves per workgroup (1..16).

@@ -34,7 +34,7 @@ def test_bf16_rounding_matches_oracle(dev):
     assert (y[~finite] & 0x7F80 == 0x7F80).all() and (y[~finite] & 0x7F != 0).all()
 
 
-@pytest.mark.parametrize("variant", [2, 3, 5])
+@pytest.mark.parametrize("variant", [2, 3, 5, 6])
 @pytest.mark.parametrize("vec", [1, 2, 4, 8])
 @pytest.mark.parametrize("M,K,N,dens", [(130, 257, 512, 0.05), (61, 1030, 96, 0.3),
                                         (300, 77, 1030, 0.2)])

@@ -126,20 +126,21 @@ def test_config1_800_s09_vs_fp64(dev):
     assert err <= FP32_REL_TOL
 
 
+@pytest.mark.parametrize("variant", [0, 6])
 @pytest.mark.parametrize("sparsity", [0.99, 0.995, 0.999])
-def test_config2_n8000_full_size(dev, sparsity):
+def test_config2_n8000_full_size(dev, sparsity, variant):
     """n = 8000: conversion bit-exact at full size; the product bit-exact on a
     sample of row groups spread over the matrix and all columns, plus FP64
     rel-Frobenius on the same sample."""
     import torch
     n = 8000
-    print(f"config2 n={n} s={sparsity}", flush=True)
+    print(f"config2 n={n} s={sparsity} variant={variant}", flush=True)
     Sd = torch.empty((n, n), device=dev)
     Dd = torch.empty((n, n), device=dev)
     G.fill_sparse(Sd, 7, 1, sparsity)
     G.fill_uniform(Dd, 7, 0)
     C = torch.empty((n, n), device=dev)
-    m = G.GCOOMatrix(n, n, 4)
+    m = G.GCOOMatrix(n, n, 4).set_config(variant=variant)
     m.multiply_by(Sd, Dd, C)
     S = Sd.cpu().numpy()
     D = Dd.cpu().numpy()
@@ -155,12 +156,14 @@ def test_config2_n8000_full_size(dev, sparsity):
     assert oracle.rel_frobenius(Cn, Cd) <= FP32_REL_TOL
 
 
-def test_repeat_and_capacity_growth(dev):
+@pytest.mark.parametrize("variant", [0, 6])
+def test_repeat_and_capacity_growth(dev, variant):
     """mutiplyBy on a handle re-used with denser inputs (capacity grows) and
     then sparser ones (capacity kept) matches a fresh conversion each time."""
     import torch
     M, K, N = 200, 300, 96
-    m = G.GCOOMatrix(M, K, 4)
+    print(f"capacity growth variant={variant}", flush=True)
+    m = G.GCOOMatrix(M, K, 4).set_config(variant=variant)
     rng = np.random.default_rng(3)
     for dens in (0.01, 0.2, 0.5, 0.05, 0.5):
         S = H.rng_sparse(rng, M, K, dens)
@@ -192,7 +195,7 @@ def test_reference_api_mirror(dev):
         algo.runKernelTest(D[:-1], K - 1, N, C1)   # K mismatch -> error, no crash
 
 
-@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("variant", [1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("vec", [1, 2, 4])
 @pytest.mark.parametrize("M,K,N,dens", [(130, 257, 520, 0.05), (61, 1030, 200, 0.3),
                                         (1000, 900, 64, 0.01)])
@@ -209,6 +212,33 @@ def test_kernel_variants_bit_exact(dev, variant, vec, M, K, N, dens):
     C = torch.full((M, N), float("nan"), device=dev)
     m.spdm(to_dev(D, dev), C)
     assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
+@pytest.mark.parametrize("M,K,N,dens", [
+    (300, 1000, 512, 0.02),    # 2 panels (one partial), 16 chunks
+    (257, 129, 260, 0.3),      # dense chunks: the 8-entry batches and every tail size
+    (513, 70, 1024, 0.0),      # no entries
+    (5, 3, 8, 0.5),            # one partial panel, one partial chunk, one partial tile
+    (1000, 5000, 256, 0.005),  # most (row, chunk) lists empty
+    (64, 2000, 1030, 0.05),    # N * 4 not a multiple of 16: row-segment fallback
+])
+def test_panel_variant_shapes(dev, M, K, N, dens):
+    """The panel-staged multiply (variant 6) across panel, chunk and tile
+    edges gives the reference's bits, and follows re-conversions of the
+    handle (its panel lists are rebuilt)."""
+    import torch
+    print(f"panel M={M} K={K} N={N} dens={dens}", flush=True)
+    rng = np.random.default_rng(M * 7 + K + N)
+    m = G.GCOOMatrix(M, K, 4).set_config(variant=6)
+    for d in (dens, dens / 2):
+        S = H.rng_sparse(rng, M, K, d)
+        S[M // 2, : min(K, 700)] = rng.uniform(-1, 1, min(K, 700))  # a long row
+        D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        g = oracle.convert(S, 4)
+        m.convert_dense(to_dev(S, dev))
+        C = torch.full((M, N), float("nan"), device=dev)
+        m.spdm(to_dev(D, dev), C)
+        assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
 
 
 def test_row_segments_match_gcoo(dev):
