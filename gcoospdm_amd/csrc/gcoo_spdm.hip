@@ -640,7 +640,8 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPanelPrefetch = 4;  // chunks between the L2 touch and the staging
-constexpr double kPanelMinDensity = 0.007;  // automatic choice of variant 6 (FP32)
+constexpr double kPanelMinDensity = 0.007;  // automatic choice of variant 6 (FP32) ...
+constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
 // a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform q < 16.
 // M0 (the index register, also the LDS address of buffer-to-LDS loads) is
@@ -1087,9 +1088,11 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         // move half the bytes and stay ahead.  The density is the handle's
         // last completed conversion (a first optimistic launch does not know
         // the new count yet); every variant gives the same bits.
+        // It also needs a workgroup per CU at least (256 x 256 tiles of C).
         const double cells = (double)h->M * (double)h->K;
         const bool dense = cells > 0 && (double)h->nnz >= kPanelMinDensity * cells;
-        variant = (E::bytes == 4 && dense && h->M >= 2 * kPanelRows && N >= 256) ? 6 : 2;
+        const long long wgs = (long long)((h->M + kPanelRows - 1) / kPanelRows) * ((N + 255) / 256);
+        variant = (E::bytes == 4 && dense && wgs >= kPanelMinWorkgroups) ? 6 : 2;
     }
     if (variant == 1) variant = 2;
     // variant 3 stages 16-byte pieces of D rows: it needs whole pieces per
