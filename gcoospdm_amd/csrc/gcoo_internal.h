@@ -83,13 +83,14 @@ struct gcoo_matrix {
 
     // Panel lists of the LDS-staged multiply (variant 6), built on first use
     // after a conversion: the entries of each (row panel, k chunk) pair,
-    // ordered by (row, k), as ((row slot << 16) | k - chunk start, value
-    // bits), and their start offsets (see gcoo_panel.hip).
+    // ordered by (row, k), as ((LDS offset of the D row << 8) | row slot,
+    // value bits), and their start offsets (see gcoo_panel.hip).
     mutable uint2 *plist = nullptr;        // [cap]
     mutable int *poff = nullptr;           // [poff_elems]
     mutable size_t poff_elems = 0;
     mutable const int2 *plist_src = nullptr;  // segments the lists hold (null: stale)
     mutable int plist_tk = 0;                 // chunk length they were built for
+    mutable int plist_rowb = 0;               // bytes of a staged D row they encode
 
     // multiply-kernel configuration (gcoo_set_spdm_config)
     gcoo_spdm_config spdm_cfg = {0, 0, 0, 0, 0};
@@ -140,9 +141,10 @@ int ensure_seg16(gcoo_matrix *h, hipStream_t s);
 constexpr int kPanelRows = 256;
 constexpr int kPanelRowsPerWave = 16;
 // Enqueue (on s) the panel lists of h's entries from seg (h->seg or
-// h->seg16) for k chunks of tk rows, unless they are current.  Kernels exit
-// early when guard is non-null and guard->total > h->cap.
-int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk,
+// h->seg16) for k chunks of tk rows staged at rowb bytes per D row, unless
+// they are current.  Kernels exit early when guard is non-null and
+// guard->total > h->cap.
+int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
                        const ConvStatus *guard, hipStream_t s);
 // Ints between the offset arrays of consecutive panels: nC * kPanelRows + 1
 // offsets, padded to a multiple of 4 so that every panel's array is 16-byte

@@ -14,7 +14,9 @@
 //      panel's first entry -- poff then holds each (panel, chunk, row) list
 //      start, and the element after the last chunk holds the panel's end;
 //   3. panel_rows_kernel<true>: every entry moves to its list slot as
-//      ((row slot in its wave << 16) | (k - chunk start), value bits).
+//      (((k - chunk start) * staged row bytes << 8) | row slot in its wave,
+//      value bits): the LDS byte offset of its D row within the staged chunk
+//      and the accumulator index, ready for the multiply's scalar unit.
 // Passes 1 and 3 use one wave per row, 64 entries at a time; a row's entries
 // of one chunk are a contiguous run of its k-sorted segment, so an entry's
 // rank in its (row, chunk) list is its distance from the run's first entry.
@@ -25,9 +27,9 @@ namespace {
 
 template <bool SCATTER>
 __global__ __launch_bounds__(256) void panel_rows_kernel(
-    const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int nC,
-    int *__restrict__ poff, uint2 *__restrict__ plist, const ConvStatus *__restrict__ guard,
-    long long cap)
+    const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int rowb,
+    int nC, int *__restrict__ poff, uint2 *__restrict__ plist,
+    const ConvStatus *__restrict__ guard, long long cap)
 {
     if (guard != nullptr && guard->total > cap) return;
     const int wave = threadIdx.x >> 6;
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(256) void panel_rows_kernel(
     const int p = row / kPanelRows;
     const int rl = row - p * kPanelRows;
     int *base = poff + (size_t)p * panel_stride(nC) + rl;
-    const unsigned slot = (unsigned)(rl % kPanelRowsPerWave) << 16;
+    const unsigned slot = (unsigned)(rl % kPanelRowsPerWave);
     const int b = rowptr[row];
     const int e = rowptr[row + 1];
     int carry = b;  // first entry of the run still open at the previous batch's end
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(256) void panel_rows_kernel(
             if (c != cn) base[(size_t)c * kPanelRows] = i - s + 1;
         } else {
             const int pos = base[(size_t)c * kPanelRows] + (i - s);
-            plist[pos] = make_uint2(slot | (unsigned)(kv.x - c * tk), (unsigned)kv.y);
+            plist[pos] = make_uint2(((unsigned)((kv.x - c * tk) * rowb) << 8) | slot, (unsigned)kv.y);
         }
     }
 }
@@ -142,10 +144,12 @@ __global__ __launch_bounds__(1024) void panel_tile_scan_kernel(
 
 }  // namespace
 
-int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, const ConvStatus *guard,
-                       hipStream_t s)
+int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
+                       const ConvStatus *guard, hipStream_t s)
 {
-    if (h->plist_src == seg && h->plist_tk == tk) return GCOO_OK;
+    if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb) return GCOO_OK;
+    if ((long long)tk * rowb >= (1ll << 24))
+        return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
     const int nP = (h->M + kPanelRows - 1) / kPanelRows;
     const int nC = (h->K + tk - 1) / tk;
     const size_t L = (size_t)nC * kPanelRows + 1;
@@ -175,7 +179,7 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, const Conv
         GCOO_HIP_TRY(hipMemsetAsync(h->poff, 0, sizeof(int) * (size_t)nP * stride, s));
         const unsigned rowBlocks = (unsigned)((h->M + 3) / 4);
         hipLaunchKernelGGL(panel_rows_kernel<false>, dim3(rowBlocks), dim3(256), 0, s, h->rowptr,
-                           seg, h->M, tk, nC, h->poff, h->plist, guard, (long long)h->cap);
+                           seg, h->M, tk, rowb, nC, h->poff, h->plist, guard, (long long)h->cap);
         GCOO_CHECK_LAUNCH("panel_rows_kernel<count>");
         int *tsum = h->poff + (size_t)nP * stride;
         const unsigned tiles = (unsigned)((size_t)nP * nT);
@@ -186,11 +190,12 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, const Conv
                            h->poff, nC, nT, tsum, guard, (long long)h->cap);
         GCOO_CHECK_LAUNCH("panel_tile_scan_kernel");
         hipLaunchKernelGGL(panel_rows_kernel<true>, dim3(rowBlocks), dim3(256), 0, s, h->rowptr,
-                           seg, h->M, tk, nC, h->poff, h->plist, guard, (long long)h->cap);
+                           seg, h->M, tk, rowb, nC, h->poff, h->plist, guard, (long long)h->cap);
         GCOO_CHECK_LAUNCH("panel_rows_kernel<scatter>");
     }
     h->plist_src = seg;
     h->plist_tk = tk;
+    h->plist_rowb = rowb;
     return GCOO_OK;
 }
 

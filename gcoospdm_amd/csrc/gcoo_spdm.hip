@@ -634,7 +634,9 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 //     accumulators sit in v0..v63 (component v of row slot q in v[16v + q])
 //     and each step is 4 v_fma_f32 in VGPR-index mode, source 2 and
 //     destination relative to M0 = q: acc[q] = fmaf(x, d, acc[q]) with no
-//     moves and no branches.
+//     moves and no branches.  List entries carry the D row's LDS offset and
+//     the row slot ready for the scalar unit, and four steps share one
+//     index-mode block, so a step costs about four scalar instructions.
 // Every row's chain is its entries in ascending k, as in all variants.
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -643,24 +645,72 @@ constexpr int kPanelPrefetch = 4;  // chunks between the L2 touch and the stagin
 constexpr double kPanelMinDensity = 0.007;  // automatic choice of variant 6 (FP32) ...
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
-// a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform q < 16.
-// M0 (the index register, also the LDS address of buffer-to-LDS loads) is
-// restored before the block ends.
-__device__ __forceinline__ void panel_fma(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
-                                          int q, float x, const float (&d)[4])
+// a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform q < 16: the
+// accumulators are pinned in v0..v63 (component v of row slot q in
+// v[16v + q]) and the step runs in VGPR-index mode with source 2 and the
+// destination relative to M0[7:0] = q.  The index mode writes M0, which the
+// buffer-to-LDS loads also use; callers save M0 around a run of steps
+// (panel_m0_save / panel_m0_restore).
+__device__ __forceinline__ void panel_fma1(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
+                                           int q, float x, const float (&d)[4])
 {
-    int saved;
     asm volatile(
-        "s_mov_b32 %4, m0\n\t"
-        "s_set_gpr_idx_on %5, gpr_idx(SRC2,DST)\n\t"
-        "v_fma_f32 v0, %6, %7, v0\n\t"
-        "v_fma_f32 v16, %6, %8, v16\n\t"
-        "v_fma_f32 v32, %6, %9, v32\n\t"
-        "v_fma_f32 v48, %6, %10, v48\n\t"
-        "s_set_gpr_idx_off\n\t"
-        "s_mov_b32 m0, %4"
-        : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3), "=&s"(saved)
-        : "s"(q & 15), "s"(x), "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]));
+        "s_set_gpr_idx_on %4, gpr_idx(SRC2,DST)\n\t"
+        "v_fma_f32 v0, %5, %6, v0\n\t"
+        "v_fma_f32 v16, %5, %7, v16\n\t"
+        "v_fma_f32 v32, %5, %8, v32\n\t"
+        "v_fma_f32 v48, %5, %9, v48\n\t"
+        "s_set_gpr_idx_off"
+        : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
+        : "s"(q), "s"(x), "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]));
+}
+
+// Four steps in order, the row index of each later step set with
+// s_set_gpr_idx_idx (index only, mode kept).
+__device__ __forceinline__ void panel_fma4(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
+                                           const int (&q)[4], const float (&x)[4],
+                                           const float (&d0)[4], const float (&d1)[4],
+                                           const float (&d2)[4], const float (&d3)[4])
+{
+    asm volatile(
+        "s_set_gpr_idx_on %4, gpr_idx(SRC2,DST)\n\t"
+        "v_fma_f32 v0, %8, %12, v0\n\t"
+        "v_fma_f32 v16, %8, %13, v16\n\t"
+        "v_fma_f32 v32, %8, %14, v32\n\t"
+        "v_fma_f32 v48, %8, %15, v48\n\t"
+        "s_set_gpr_idx_idx %5\n\t"
+        "v_fma_f32 v0, %9, %16, v0\n\t"
+        "v_fma_f32 v16, %9, %17, v16\n\t"
+        "v_fma_f32 v32, %9, %18, v32\n\t"
+        "v_fma_f32 v48, %9, %19, v48\n\t"
+        "s_set_gpr_idx_idx %6\n\t"
+        "v_fma_f32 v0, %10, %20, v0\n\t"
+        "v_fma_f32 v16, %10, %21, v16\n\t"
+        "v_fma_f32 v32, %10, %22, v32\n\t"
+        "v_fma_f32 v48, %10, %23, v48\n\t"
+        "s_set_gpr_idx_idx %7\n\t"
+        "v_fma_f32 v0, %11, %24, v0\n\t"
+        "v_fma_f32 v16, %11, %25, v16\n\t"
+        "v_fma_f32 v32, %11, %26, v32\n\t"
+        "v_fma_f32 v48, %11, %27, v48\n\t"
+        "s_set_gpr_idx_off"
+        : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
+        : "s"(q[0]), "s"(q[1]), "s"(q[2]), "s"(q[3]), "s"(x[0]), "s"(x[1]), "s"(x[2]), "s"(x[3]),
+          "v"(d0[0]), "v"(d0[1]), "v"(d0[2]), "v"(d0[3]), "v"(d1[0]), "v"(d1[1]), "v"(d1[2]),
+          "v"(d1[3]), "v"(d2[0]), "v"(d2[1]), "v"(d2[2]), "v"(d2[3]), "v"(d3[0]), "v"(d3[1]),
+          "v"(d3[2]), "v"(d3[3]));
+}
+
+__device__ __forceinline__ int panel_m0_save()
+{
+    int m0;
+    asm volatile("s_mov_b32 %0, m0" : "=s"(m0) : : "memory");
+    return m0;
+}
+
+__device__ __forceinline__ void panel_m0_restore(int m0)
+{
+    asm volatile("s_mov_b32 m0, %0" : : "s"(m0) : "memory");
 }
 
 template <typename E, int TK>
@@ -755,6 +805,7 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         const int cn = c + 1 < nC ? c + 1 : c;
         const int pbn = op[(size_t)cn * kPanelRows];
         const int pen = op[(size_t)cn * kPanelRows + RPW];
+        // B entries: list entry x = (LDS byte offset << 8) | row slot
         auto run = [&](auto bn, int p) {
             constexpr int B = decltype(bn)::value;
             uint2 m[B];
@@ -762,10 +813,21 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             for (int u = 0; u < B; ++u) m[u] = plist[p + u];
             float d[B][VEC];
 #pragma unroll
-            for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + (m[u].x & 0xffffu) * ROWB);
+            for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + (m[u].x >> 8));
+            const int m0 = panel_m0_save();
+            int u = 0;
 #pragma unroll
-            for (int u = 0; u < B; ++u)
-                panel_fma(a0, a1, a2, a3, (int)(m[u].x >> 16), E::val((int)m[u].y), d[u]);
+            for (; u + 4 <= B; u += 4) {
+                const int q[4] = {(int)(m[u].x & 15u), (int)(m[u + 1].x & 15u),
+                                  (int)(m[u + 2].x & 15u), (int)(m[u + 3].x & 15u)};
+                const float x[4] = {E::val((int)m[u].y), E::val((int)m[u + 1].y),
+                                    E::val((int)m[u + 2].y), E::val((int)m[u + 3].y)};
+                panel_fma4(a0, a1, a2, a3, q, x, d[u], d[u + 1], d[u + 2], d[u + 3]);
+            }
+#pragma unroll
+            for (; u < B; ++u)
+                panel_fma1(a0, a1, a2, a3, (int)(m[u].x & 15u), E::val((int)m[u].y), d[u]);
+            panel_m0_restore(m0);
         };
         int p = pb;
         for (; p + 8 <= pe; p += 8) run(std::integral_constant<int, 8>{}, p);
@@ -939,7 +1001,8 @@ template <typename E, int TK>
 int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
                  const ConvStatus *guard, hipStream_t s)
 {
-    int rc = ensure_panel_lists(h, seg, TK, guard, s);
+    constexpr int ROWB = 256 * E::bytes;  // bytes of a staged D row (256 columns)
+    int rc = ensure_panel_lists(h, seg, TK, ROWB, guard, s);
     if (rc) return rc;
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
     const int nPanels = (h->M + kPanelRows - 1) / kPanelRows;
