@@ -183,10 +183,10 @@ __global__ __launch_bounds__(kConvThreads) void scatter_kernel(
 #pragma unroll
     for (int j = 0; j < CPG; ++j) rbase[j] = j < nrows ? rowptr[r0 + j] : 0;
     const float *grow = S + (size_t)r0 * K;
-    for (int k0 = 0; k0 < K; k0 += CHUNK) {
+    // this thread's elements of the group's rows in the chunk at k0 (zeros
+    // past the matrix)
+    auto load_chunk = [&](int k0, float (&v)[CPG][KPT]) {
         const int kb = k0 + threadIdx.x * KPT;
-        float v[CPG][KPT];
-        unsigned long long packed = 0;
 #pragma unroll
         for (int j = 0; j < CPG; ++j) {
             if (VEC4) {
@@ -203,6 +203,23 @@ __global__ __launch_bounds__(kConvThreads) void scatter_kernel(
             } else {
                 v[j][0] = (j < nrows && kb < K) ? grow[(size_t)j * K + kb] : 0.f;
             }
+        }
+    };
+    float vn[CPG][KPT];
+    load_chunk(0, vn);
+    for (int k0 = 0; k0 < K; k0 += CHUNK) {
+        const int kb = k0 + threadIdx.x * KPT;
+        float v[CPG][KPT];
+#pragma unroll
+        for (int j = 0; j < CPG; ++j)
+#pragma unroll
+            for (int q = 0; q < KPT; ++q) v[j][q] = vn[j][q];
+        // the next chunk's loads are in flight while this one is scanned and
+        // written
+        if (k0 + CHUNK < K) load_chunk(k0 + CHUNK, vn);
+        unsigned long long packed = 0;
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) {
             int cj = 0;
 #pragma unroll
             for (int q = 0; q < KPT; ++q) cj += nz(v[j][q]);
