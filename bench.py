@@ -14,7 +14,7 @@ against MI355X's 8 TB/s.  `traffic` is the HBM bytes per launch measured by
 rocprofv3 counters (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), read
 from profiles/traffic.json when that summary exists.
 
-`cpu_baseline` times the oracle's C restatement (OpenMP) on the host cores on
+`cpu_baseline` times the oracle's C restatement (OpenMP, and on one core) on
 a bounded sample.  Extra objects report the other workloads: the kernel alone
 at every sparsity (FP32 and BF16), the 800 x 800 case, the skewed Zipf matrix
 built from COO, 512 batched jobs against 512 sequential launches, and the
@@ -61,10 +61,55 @@ def event_time(fn, iters, stream):
     return e0.elapsed_time(e1) / 1e3 / iters
 
 
+def cpu_single_core(S_host, D_host, nnz_total, N, t_conv, budget_s):
+    """The same restatement on one core: the multiply of a row sample on one
+    thread, scaled to the full step with the (single-threaded) conversion."""
+    import oracle
+    rows = 16
+    while True:
+        sub = oracle.convert(S_host[:rows], 4)
+        t0 = time.perf_counter()
+        oracle.spdm_f32(sub, D_host, threads=1)
+        dt = time.perf_counter() - t0
+        if dt > budget_s / 4 or rows >= S_host.shape[0]:
+            break
+        rows = min(S_host.shape[0], rows * 4)
+    gf = 2.0 * sub["nnz"] * N / dt / 1e9
+    t_full = t_conv + (2.0 * nnz_total * N / 1e9) / gf
+    return {"value": round(2.0 * nnz_total * N / t_full / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+            "multiply_gflops": round(gf, 3),
+            "sample": f"first {rows} rows x all {N} columns ({sub['nnz']} nnz) on one thread"}
+
+
+def reference_fp64_c1():
+    """The reference's own matrixMulCPU (src/utils.cpp:30-46, built from its
+    sources into oracle/_ref) on the 800 x 800 x 800 case, single-threaded as
+    written: the FP64 check of the small configuration."""
+    import numpy as np
+    import oracle
+    if not oracle.ref_available():
+        return None
+    rng = np.random.default_rng(5)
+    n = 800
+    S = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    S[rng.random((n, n)) >= 0.1] = 0.0
+    D = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    C = np.zeros((n, n), np.float32)
+    R = oracle.ref()
+    t0 = time.perf_counter()
+    R.ref_matrix_mul_cpu(C, S, D, n, n, n)
+    dt = time.perf_counter() - t0
+    return {"ms": round(dt * 1e3, 1), "cores": 1, "kind": "reference",
+            "gflops_dense": round(2.0 * n ** 3 / dt / 1e9, 3),
+            "gflops_effective": round(2.0 * int((S != 0).sum()) * n / dt / 1e9, 3),
+            "sample": "matrixMulCPU(C, A, B, 800, 800, 800), A p(nonzero)=0.1"}
+
+
 def cpu_baseline(S_host, D_host, nnz_total, N, budget_s=15.0):
     """The oracle (C, OpenMP) on the host: dense -> GCOO of the full S plus
     the fmaf-chain multiply of a sample of whole rows, scaled to GFLOP/s of
-    the full step."""
+    the full step; the same on one core; and the reference's own FP64 check
+    routine at 800^3."""
     import oracle
     threads = oracle.max_threads()
     t0 = time.perf_counter()
@@ -92,6 +137,8 @@ def cpu_baseline(S_host, D_host, nnz_total, N, budget_s=15.0):
         "unit": "GFLOP/s",
         "cores": threads,
         "kind": "port",
+        "single_core": cpu_single_core(S_host, D_host, nnz_total, N, t_conv, budget_s / 3),
+        "reference_fp64_check_800": reference_fp64_c1(),
         "sample": (f"oracle C restatement (OpenMP, {threads} threads): dense->GCOO of the full "
                    f"{M}x{S_host.shape[1]} S ({t_conv * 1e3:.1f} ms) + fmaf-chain multiply of the "
                    f"first {rows} rows x all {N} columns ({sub['nnz']} nnz, {dt * 1e3:.1f} ms/rep, "
@@ -219,11 +266,12 @@ def zipf_case(G, dev):
 
 def serve_case():
     """The continuous-batching server (C++ driver gcoo_serve) on 512
-    resident jobs, Poisson arrivals at 0.95 x saturated capacity."""
+    resident jobs, Poisson arrivals at 0.5, 0.8 and 0.95 x saturated
+    capacity."""
     exe = os.path.join(ROOT, "gcoospdm_amd", "host", "gcoo_serve")
     if not os.path.exists(exe):
         return {"error": "gcoo_serve not built"}
-    r = subprocess.run([exe, "-jobs=512", "-requests=2000", "-loads=0.95",
+    r = subprocess.run([exe, "-jobs=512", "-requests=2000", "-loads=0.5,0.8,0.95",
                         "-table=" + os.path.join(ROOT, "gcoospdm_amd", "tuning.json")],
                        capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
@@ -232,6 +280,8 @@ def serve_case():
     out = {}
     for rec in recs:
         key = f"{rec['kind']}_{rec['dtype']}" + (f"_{rec['mode']}" if "mode" in rec else "")
+        if rec["kind"] == "poisson":
+            key += f"_load{rec['load']:g}"
         out[key] = {k: rec[k] for k in rec if k not in ("kind", "dtype", "mode")}
     return out
 

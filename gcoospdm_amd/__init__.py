@@ -510,13 +510,32 @@ class ShGCOOSpDM(ShGemm):
         return self._m
 
 
+class ShCsrSpGemm(ShGCOOSpDM):
+    """Factory key "csr": the reference's row-wise CSR SpDM ablation
+    (src/ShCsrSpGemm.cu:9-146) as the GCOO path with one row per group --
+    whose row segments are CSR -- and the row-segment kernel."""
+
+    def __init__(self, denseMatrix, nRows: int, nCols: int):
+        super().__init__(denseMatrix, nRows, nCols, cpg=1)
+        self._m.set_config(variant=2)
+
+
 class SpGemmBuilder:
-    """Reference src/SpGemmBuilder.h:12-23."""
+    """Reference src/SpGemmBuilder.h:12-23: "gcoospdm", "csr", and the vendor
+    baselines "cusparse" (rocSPARSE) and "cublas" (rocBLAS); None otherwise."""
 
     @staticmethod
     def getShGemmAlgo(denseMatrix, nRows: int, nCols: int, desc: str):
         if desc == "gcoospdm":
             return ShGCOOSpDM(denseMatrix, nRows, nCols)
+        if desc == "csr":
+            return ShCsrSpGemm(denseMatrix, nRows, nCols)
+        if desc == "cusparse":
+            from .vendor import ShRocsparseGemm
+            return ShRocsparseGemm(denseMatrix, nRows, nCols)
+        if desc == "cublas":
+            from .vendor import ShRocblasGemm
+            return ShRocblasGemm(denseMatrix, nRows, nCols)
         return None
 
 

@@ -1,15 +1,17 @@
 // SpGemmBuilder.h -- string factory of ShGemm algorithms (reference
-// src/SpGemmBuilder.h:12-23).  "gcoospdm" is the GCOO backend.  The
-// reference's vendor baselines ("cusparse", "cublas") and its "csr" kernel
-// are not part of this library; their keys return NULL exactly like an
-// unknown key does in the reference.
+// src/SpGemmBuilder.h:12-23).  "gcoospdm" is the GCOO backend; "csr" the
+// row-wise CSR ablation (the same library, one row per group); "cusparse"
+// and "cublas" the vendor baselines on rocSPARSE and rocBLAS
+// (libgcoo_vendor.so).  Unknown keys return NULL, as in the reference.
 #ifndef GCOO_SPGEMMBUILDER_H
 #define GCOO_SPGEMMBUILDER_H
 
 #include <string>
 
+#include "ShCsrSpGemm.h"
 #include "ShGCOOSpDM.h"
 #include "ShGemm.h"
+#include "ShVendorGemm.h"
 
 class SpGemmBuilder
 {
@@ -18,6 +20,9 @@ public:
                                  const std::string &desc)
     {
         if (desc == "gcoospdm") return new ShGCOOSpDM(denseMatrix, nRows, nCols);
+        if (desc == "csr") return new ShCsrSpGemm(denseMatrix, nRows, nCols);
+        if (desc == "cusparse") return new ShRocsparseGemm(denseMatrix, nRows, nCols);
+        if (desc == "cublas") return new ShRocblasGemm(denseMatrix, nRows, nCols);
         return nullptr;
     }
 };

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -131,6 +132,61 @@ std::string Flags::get_str(const char *name, const std::string &def) const
     const char *v = find(name);
     if (!v || *v != '=') return def;
     return std::string(v + 1);
+}
+
+bool read_matrix_market(const std::string &path, MtxMatrix &out, std::string &err)
+{
+    FILE *fp = std::fopen(path.c_str(), "r");
+    if (!fp) {
+        err = "cannot find file: " + path;
+        return false;
+    }
+    std::vector<char> line(4096);
+    auto next = [&]() { return std::fgets(line.data(), (int)line.size(), fp) != nullptr; };
+    auto fail = [&](const std::string &why) {
+        std::fclose(fp);
+        err = path + ": " + why;
+        return false;
+    };
+    if (!next()) return fail("empty file");
+    const bool general = std::strstr(line.data(), "general") != nullptr;
+    bool more;
+    do {
+        more = next();
+    } while (more && line[0] == '%');
+    if (!more) return fail("no size line");
+    long long r = 0, c = 0, n = 0;
+    if (std::sscanf(line.data(), "%lld %lld %lld", &r, &c, &n) != 3 || r < 0 || c < 0 ||
+        r > 2147483647ll || c > 2147483647ll)
+        return fail("bad size line");
+    out.nRows = (int)r;
+    out.nCols = (int)c;
+    out.nnz = 0;
+    out.dense.assign((size_t)r * (size_t)c, 0.0f);
+    while (next()) {
+        char *ch = line.data();
+        if (std::strspn(ch, " \t\r\n") == std::strlen(ch)) continue;  // blank line
+        const long long row = std::atoll(ch) - 1;
+        ch = std::strchr(ch, ' ');
+        if (!ch) return fail("entry line without a column index");
+        ++ch;
+        const long long col = std::atoll(ch) - 1;
+        ch = std::strchr(ch, ' ');
+        float val = (float)std::rand();  // drawn for every entry, used when none is given
+        if (ch) val = (float)std::atof(ch + 1);
+        if (row < 0 || row >= r || col < 0 || col >= c)
+            return fail("entry (" + std::to_string(row + 1) + ", " + std::to_string(col + 1) +
+                        ") outside the matrix");
+        out.dense[(size_t)row * (size_t)c + (size_t)col] = val;
+        ++out.nnz;
+        if (!general) {
+            if (col >= r || row >= c) return fail("mirrored entry outside the matrix");
+            out.dense[(size_t)col * (size_t)c + (size_t)row] = val;
+            if (col != row) ++out.nnz;
+        }
+    }
+    std::fclose(fp);
+    return true;
 }
 
 }  // namespace gcoo_host

@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <vector>
 
 namespace gcoo_host {
 
@@ -29,6 +30,22 @@ long long fill_counter(float *data, size_t size, uint64_t seed, uint64_t stream_
 // std::thread.
 double rel_l2_vs_fp64(const float *S, const float *D, const float *C, int M, int K,
                       int N, int threads);
+
+// A Matrix Market file read the way the reference's MatrixReader does
+// (src/MatrixReader.cpp:11-84): the header's "general" keyword selects an
+// unsymmetric matrix, anything else is mirrored (a(c, r) = a(r, c), an
+// off-diagonal entry counted twice); comment lines start with '%'; each entry
+// line is "row col [value]" (1-based, fields split at single spaces), and an
+// entry with no value takes rand() as its value -- rand() is drawn for every
+// entry line, as in the reference.  Returns false with a message on a
+// missing or malformed file or an index outside the stated size (the
+// reference writes out of bounds there).
+struct MtxMatrix {
+    int nRows = 0, nCols = 0;
+    long long nnz = 0;           // the reference's count (m_nnz)
+    std::vector<float> dense;    // row-major nRows x nCols
+};
+bool read_matrix_market(const std::string &path, MtxMatrix &out, std::string &err);
 
 class Flags
 {

@@ -12,6 +12,10 @@
 //   -cpg=N             rows per group (1, 2, 4)
 //   -check=1           FP64 check, relative L2 <= 1e-5 (the reference's
 //                      disabled check, src/main.cu:128-142)
+//   -file=path.mtx     the reference's timeFromFile (src/main.cu:149-178): the
+//                      Matrix Market matrix is both the sparse S and the dense
+//                      operand (square matrices only)
+//   -dump=path         write C (row-major float32, no header) after the run
 // Output keeps the reference's lines ("nnz cpu:", "A[..] * B[..]",
 // "Algorithm:", "Time:" in microseconds) so its harness can parse it, plus a
 // GFLOPS line (2 * nnz * N / t).
@@ -53,9 +57,10 @@ void usage()
     std::printf("      -nRowsB=nRowsB\n");
     std::printf("      -nColsB=nColsB\n");
     std::printf("      -sparsity=sparsity (Sparsity of matrix B)\n");
-    std::printf("      -algo=gcoospdm\n");
+    std::printf("      -algo=gcoospdm|csr|cusparse|cublas\n");
     std::printf("      -kernelonly=0 (Benchmark kernel only)\n");
     std::printf("      -dist=ref|uniform -seed=N -iters=N -cpg=4 -check=0\n");
+    std::printf("      -file=matrix.mtx (S and the dense operand) -dump=C.bin\n");
     std::printf("C = B * A with B sparse: nColsB must equal nRowsA\n");
 }
 
@@ -85,7 +90,8 @@ Result bench(const float *A, size_t nRowsA, size_t nColsA, const float *B, size_
     else
         algoObj.reset(SpGemmBuilder::getShGemmAlgo(dB, (int)nRowsB, (int)nColsB, algo));
     if (!algoObj) {
-        std::fprintf(stderr, "unknown or unavailable algorithm '%s' (available: gcoospdm)\n",
+        std::fprintf(stderr,
+                     "unknown algorithm '%s' (available: gcoospdm, csr, cusparse, cublas)\n",
                      algo.c_str());
         std::exit(2);
     }
@@ -122,6 +128,15 @@ Result bench(const float *A, size_t nRowsA, size_t nColsA, const float *B, size_
     return res;
 }
 
+bool dump_matrix(const std::string &path, const std::vector<float> &C)
+{
+    if (path.empty()) return true;
+    FILE *fp = std::fopen(path.c_str(), "wb");
+    if (!fp) return false;
+    const bool ok = std::fwrite(C.data(), sizeof(float), C.size(), fp) == C.size();
+    return std::fclose(fp) == 0 && ok;
+}
+
 }  // namespace
 
 int main(int argc, char **argv)
@@ -144,13 +159,53 @@ int main(int argc, char **argv)
     const bool check = f.get_int("check", 0) == 1;
     int iters = (int)f.get_int("iters", nRowsA > 8000 ? 20 : 100);
     if (algo.empty()) {
-        std::fprintf(stderr, "missing -algo= (available: gcoospdm)\n");
+        std::fprintf(stderr, "missing -algo= (available: gcoospdm, csr, cusparse, cublas)\n");
         usage();
         return 2;
     }
     if (f.has("file")) {
-        std::fprintf(stderr, "-file= (Matrix Market input) is not supported by this build\n");
-        return 2;
+        const std::string path = f.get_str("file", "");
+        gcoo_host::MtxMatrix mm;
+        std::string err;
+        if (!gcoo_host::read_matrix_market(path, mm, err)) {
+            std::fprintf(stderr, "%s\n", err.c_str());
+            return 2;
+        }
+        std::printf("Read mtx file: %s\n", path.c_str());
+        if (mm.nRows != mm.nCols) {
+            std::fprintf(stderr, "%s: %d x %d is not square (S is also the dense operand)\n",
+                         path.c_str(), mm.nRows, mm.nCols);
+            return 2;
+        }
+        const size_t n = (size_t)mm.nRows;
+        std::printf("nnz cpu: %lld, density: %.4f\n", mm.nnz,
+                    n ? mm.nnz * 1.0 / ((double)n * (double)n) : 0.0);
+        long long nzv = 0;  // nonzero values, the multiply's work (duplicates collapse)
+        for (float v : mm.dense) nzv += v != 0.0f;
+        std::vector<float> C(n * n);
+        Result r;
+        try {
+            r = bench(mm.dense.data(), n, n, mm.dense.data(), n, n, C.data(), algo, kernelonly,
+                      iters, cpg);
+        } catch (const std::exception &e) {
+            std::fprintf(stderr, "error: %s\n", e.what());
+            return 1;
+        }
+        if (r.us_per_iter > 0)
+            std::printf("GFLOPS:%f\n", 2.0 * (double)nzv * (double)n / (r.us_per_iter * 1e3));
+        if (!dump_matrix(f.get_str("dump", ""), C)) {
+            std::fprintf(stderr, "cannot write -dump file\n");
+            return 1;
+        }
+        if (check) {
+            const int threads = (int)std::max(1u, std::thread::hardware_concurrency());
+            const double e = gcoo_host::rel_l2_vs_fp64(mm.dense.data(), mm.dense.data(), C.data(),
+                                                       (int)n, (int)n, (int)n, threads);
+            std::printf("RelL2:%.3e\n", e);
+            std::printf(e <= 1e-5 ? "Test passed\n" : "Test failed\n");
+            if (e > 1e-5) return 1;
+        }
+        return 0;
     }
     if (nColsB != nRowsA) {
         std::fprintf(stderr, "nColsB (%zu) must equal nRowsA (%zu)\n", nColsB, nRowsA);
@@ -188,6 +243,10 @@ int main(int argc, char **argv)
     }
     if (r.us_per_iter > 0)
         std::printf("GFLOPS:%f\n", 2.0 * (double)nnz * (double)nColsA / (r.us_per_iter * 1e3));
+    if (!dump_matrix(f.get_str("dump", ""), C)) {
+        std::fprintf(stderr, "cannot write -dump file\n");
+        return 1;
+    }
     if (check) {
         const int threads = (int)std::max(1u, std::thread::hardware_concurrency());
         const double err = gcoo_host::rel_l2_vs_fp64(B.data(), A.data(), C.data(), (int)nRowsB,

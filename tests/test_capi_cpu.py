@@ -51,9 +51,36 @@ def test_version_and_error_paths():
 
 
 def test_python_mirror_factory_keys():
-    # unknown and out-of-scope keys return None like the reference's NULL
-    for key in ("cusparse", "cublas", "csr", "nope"):
+    # unknown keys return None like the reference's NULL (the known ones
+    # build device state and are exercised by the GPU tests)
+    for key in ("nope", "groupspgemm", ""):
         assert G.SpGemmBuilder.getShGemmAlgo(None, 4, 4, key) is None
+
+
+VENDOR_HEADER = os.path.join(ROOT, "include", "gcoo_vendor.h")
+
+
+def vendor_header_functions():
+    txt = open(VENDOR_HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(gcoo_vendor_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_vendor_header_binding_and_exports():
+    from gcoospdm_amd import vendor as V
+    assert vendor_header_functions() == sorted(V.declared_vendor_symbols())
+    lib = V.vendor_lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", V.VENDOR_LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (gcoo_vendor_\w+)", out))
+    assert set(vendor_header_functions()) <= exported
+    h = ctypes.c_void_p()
+    assert lib.gcoo_vendor_csr_create(-1, 4, ctypes.byref(h)) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_vendor_csr_create(4, 4, None) == G.GCOO_ERR_INVALID
+    assert lib.gcoo_vendor_csr_destroy(None) == G.GCOO_OK
+    assert lib.gcoo_vendor_blas_destroy(None) == G.GCOO_OK
+    assert lib.gcoo_vendor_sgemm(None, None, None, 1, 1, 1, None, None) == G.GCOO_ERR_INVALID
+    assert b"argument" in lib.gcoo_vendor_last_error()
 
 
 @pytest.mark.skipif(not os.path.exists(MAIN), reason="main driver not built")
