@@ -16,7 +16,8 @@ from profiles/traffic.json when that summary exists.
 
 `cpu_baseline` times the oracle's C restatement (OpenMP, and on one core) on
 a bounded sample.  Extra objects report the other workloads: the kernel alone
-at every sparsity (FP32 and BF16), the 800 x 800 case, the skewed Zipf matrix
+at every sparsity (FP32 and BF16), rocSPARSE and rocBLAS on the same
+inputs, the 800 x 800 case, the skewed Zipf matrix
 built from COO, 512 batched jobs against 512 sequential launches, and the
 continuous-batching server's p50 / p99 latency against a plain FIFO.
 """
@@ -195,6 +196,45 @@ def kernel_sweep(G, dev, n, seed):
         rec["convert_gbs"] = round(conv_bytes(nnz, n, n) / t / 1e9, 1)
         out.append(rec)
         del S, D, C, D16, m
+    return out
+
+
+def vendor_compare(G, dev, n, seed):
+    """The reference's comparison columns (src/bench.py:52-89: gcoospdm,
+    cusparse, cublas) at n = 8000: the full call (conversion + multiply,
+    `mutiplyBy`) and the multiply alone (`build` then `runKernelTest`), in ms,
+    for this build's GCOO path, rocSPARSE CSR SpMM and rocBLAS SGEMM."""
+    import torch
+    out = []
+    S = torch.empty((n, n), device=dev)
+    D = torch.empty((n, n), device=dev)
+    C = torch.empty((n, n), device=dev)
+    G.fill_uniform(D, seed, 0)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    for sp in (0.99, 0.995, 0.999):
+        G.fill_sparse(S, seed, 1, sp)
+        rec = {"sparsity": sp}
+        for algo in ("gcoospdm", "cusparse", "cublas"):
+            try:
+                a = G.SpGemmBuilder.getShGemmAlgo(S, n, n, algo)
+                full = timed(lambda: a.mutiplyBy(D, n, n, C), 3)
+                a.build()
+                kern = timed(lambda: a.runKernelTest(D, n, n, C), 5)
+                rec[algo] = {"full_ms": round(full, 3), "kernel_ms": round(kern, 3)}
+                del a
+            except Exception as e:  # a missing vendor library is reported, not fatal
+                rec[algo] = {"error": str(e)[:200]}
+        out.append(rec)
+    del S, D, C
     return out
 
 
@@ -384,6 +424,7 @@ def main():
         del S, D, C, algo, m
         torch.cuda.empty_cache()
         extras["kernel_sweep_n8000"] = kernel_sweep(G, dev, n, args.seed)
+        extras["vendor_baselines_n8000"] = vendor_compare(G, dev, n, args.seed)
         extras["square_n800"] = small_case(G, dev, args.seed)
         extras["zipf_coo"] = zipf_case(G, dev)
         from gcoospdm_amd import batchbench
