@@ -347,6 +347,7 @@ int gcoo_destroy(gcoo_handle h)
     if (h->poff) (void)hipFree(h->poff);
     if (h->rowptr) (void)hipFree(h->rowptr);
     if (h->rowcnt) (void)hipFree(h->rowcnt);
+    if (h->mask) (void)hipFree(h->mask);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->coo_tmp) (void)hipFree(h->coo_tmp);

@@ -19,6 +19,14 @@
 //                      (each group's entries split by row, k ascending).
 // All three are HBM-streaming integer/byte work: no MFMA, coalesced 16-byte
 // loads along rows of S.
+//
+// On the 16-byte path (K % 4 == 0, S 16-byte aligned) the count pass also
+// writes a nonzero mask of S -- per row and 256-column block, the four wave
+// ballots of `v != 0` over the block's 16-byte pieces -- and the scatter takes
+// its per-thread counts from the mask and loads only the pieces of S that
+// hold an entry.  At 1% density about a quarter of S's 128-byte lines hold
+// one, so the scatter reads a fraction of S instead of all of it a second
+// time.  Same entries, same order: only where the values come from changes.
 #include "gcoo_internal.h"
 
 #include <climits>
@@ -71,6 +79,55 @@ __global__ __launch_bounds__(kConvThreads) void count_kernel(
     for (int j = 0; j < CPG; ++j) {
         int t = wave_sum(c[j]);
         if (__lane_id() == 0) part[threadIdx.x >> 6][j] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < nrows) {
+        int t = 0;
+#pragma unroll
+        for (int w = 0; w < kConvWaves; ++w) t += part[w][threadIdx.x];
+        rowcnt[r0 + threadIdx.x] = t;
+    }
+}
+
+// Count pass with the nonzero mask (16-byte path).  Wave w takes the
+// 256-column blocks w, w + 4, ... of the group's rows; lane l loads the piece
+// of columns [256 kb + 4 l, 256 kb + 4 l + 4) (zeros past K), the ballots of
+// its four components are the block's mask words, and their popcounts the
+// block's entries.
+template <int CPG>
+__global__ __launch_bounds__(kConvThreads) void count_mask_kernel(
+    const float *__restrict__ S, int M, int K, int KB, int *__restrict__ rowcnt,
+    unsigned long long *__restrict__ mask)
+{
+    const int g = blockIdx.x;
+    const int r0 = g * CPG;
+    const int nrows = min(CPG, M - r0);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = __lane_id();
+    const int K4 = K >> 2;
+    const f32x4 *base = reinterpret_cast<const f32x4 *>(S + (size_t)r0 * K);
+    int c[CPG];
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) c[j] = 0;
+    for (int kb = wave; kb < KB; kb += kConvWaves) {
+        const int i = kb * 64 + lane;
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) {
+            if (j < nrows) {
+                f32x4 v = {0.f, 0.f, 0.f, 0.f};
+                if (i < K4) v = __builtin_nontemporal_load(base + (size_t)j * K4 + i);
+                const unsigned long long b0 = __ballot(nz(v.x)), b1 = __ballot(nz(v.y));
+                const unsigned long long b2 = __ballot(nz(v.z)), b3 = __ballot(nz(v.w));
+                c[j] += __popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3);
+                const unsigned long long b = lane == 0 ? b0 : lane == 1 ? b1 : lane == 2 ? b2 : b3;
+                if (lane < 4) mask[((size_t)(r0 + j) * KB + kb) * 4 + lane] = b;
+            }
+        }
+    }
+    __shared__ int part[kConvWaves][CPG];
+    if (lane == 0) {
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) part[wave][j] = c[j];
     }
     __syncthreads();
     if (threadIdx.x < nrows) {
@@ -275,10 +332,145 @@ __global__ __launch_bounds__(kConvThreads) void scatter_kernel(
     }
 }
 
+// The scatter on the 16-byte path with the count pass's mask: one workgroup
+// per (group, chunk of 1024 columns), so that no workgroup walks a row.  The
+// chunk's starting offsets are read off the mask: wave j counts the set bits
+// of row j's mask words before the chunk (the entries of that row to the
+// left), and the group-order base is the sum over the group's rows.  Within
+// the chunk, wave w takes its w-th 256-column block; lane l's nibble of row j
+// (bit q: column 256 kb + 4 l + q holds an entry) is bit l of the block's four
+// mask words, S is read only where the nibble is not zero, and the counts, the
+// prefix sum and the writes are those of scatter_kernel.
+template <int CPG>
+__global__ __launch_bounds__(kConvThreads) void scatter_mask_kernel(
+    const float *__restrict__ S, int M, int K, int KB,
+    const unsigned long long *__restrict__ mask, const int *__restrict__ groupIndex,
+    const int *__restrict__ rowptr, const ConvStatus *__restrict__ st, long long cap,
+    float *__restrict__ vals, int *__restrict__ rows, int *__restrict__ cols,
+    int2 *__restrict__ seg)
+{
+    if (st->total > cap || st->overflow) return;
+    __shared__ unsigned long long wtot[kConvWaves];
+    __shared__ int left[CPG];
+    const int g = blockIdx.x;
+    const int cc = blockIdx.y;
+    const int r0 = g * CPG;
+    const int nrows = min(CPG, M - r0);
+    const int lane = __lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int K4 = K >> 2;
+    // entries of row `wave` left of the chunk: mask words [0, 16 cc)
+    if (wave < nrows) {
+        const unsigned long long *mr = mask + (size_t)(r0 + wave) * KB * 4;
+        int n = 0;
+        for (int w = lane; w < 16 * cc; w += 64) n += __popcll(mr[w]);
+        n = wave_sum(n);
+        if (lane == 0) left[wave] = n;
+    }
+    __syncthreads();
+    int gbase = groupIndex[g];
+    int rbase[CPG];
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) {
+        rbase[j] = 0;
+        if (j < nrows) {
+            rbase[j] = rowptr[r0 + j] + left[j];
+            gbase += left[j];
+        }
+    }
+    const f32x4 *grow = reinterpret_cast<const f32x4 *>(S + (size_t)r0 * K);
+    const int kb = 4 * cc + wave;
+    const int i = kb * 64 + lane;
+    unsigned nib[CPG];
+    float v[CPG][4];
+    unsigned long long packed = 0;
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) {
+        nib[j] = 0;
+        if (j < nrows && kb < KB) {
+            const unsigned long long *mp = mask + ((size_t)(r0 + j) * KB + kb) * 4;
+            nib[j] = (unsigned)((mp[0] >> lane) & 1ull) | (unsigned)((mp[1] >> lane) & 1ull) << 1 |
+                     (unsigned)((mp[2] >> lane) & 1ull) << 2 | (unsigned)((mp[3] >> lane) & 1ull) << 3;
+        }
+        f32x4 x = {0.f, 0.f, 0.f, 0.f};
+        if (nib[j]) x = __builtin_nontemporal_load(grow + (size_t)j * K4 + i);
+        v[j][0] = x.x;
+        v[j][1] = x.y;
+        v[j][2] = x.z;
+        v[j][3] = x.w;
+        packed |= (unsigned long long)__popc(nib[j]) << (16 * j);
+    }
+    unsigned long long incl = packed;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned long long t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    if (lane == 63) wtot[wave] = incl;
+    __syncthreads();
+    unsigned long long before = 0;
+#pragma unroll
+    for (int w = 0; w < kConvWaves; ++w) before += (w < wave) ? wtot[w] : 0ull;
+    if (!packed) return;
+    const unsigned long long excl = before + incl - packed;
+    int posr[CPG];
+    int posg = gbase;
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) {
+        const int e = (int)((excl >> (16 * j)) & 0xFFFFu);
+        posr[j] = rbase[j] + e;
+        posg += e;
+    }
+    const int kcol = 4 * i;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) {
+            if ((nib[j] >> q) & 1u) {
+                vals[posg] = v[j][q];
+                rows[posg] = r0 + j;
+                cols[posg] = kcol + q;
+                ++posg;
+                seg[posr[j]] = make_int2(kcol + q, __float_as_int(v[j][q]));
+                ++posr[j];
+            }
+        }
+    }
+}
+
+// 256-column blocks of the mask.
+int mask_blocks(const gcoo_matrix *h) { return (h->K + 255) / 256; }
+
+// Make room for the mask of an M x K conversion (false if it cannot be had:
+// the conversion then runs without it).
+bool ensure_mask(gcoo_matrix *h)
+{
+    const size_t bytes = (size_t)h->M * (size_t)mask_blocks(h) * 4 * sizeof(unsigned long long);
+    if (bytes <= h->mask_bytes) return true;
+    if (h->mask) (void)hipFree(h->mask);
+    h->mask = nullptr;
+    h->mask_bytes = 0;
+    if (hipMalloc(&h->mask, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        h->mask = nullptr;
+        return false;
+    }
+    h->mask_bytes = bytes;
+    return true;
+}
+
 template <int CPG>
 int launch_count_t(gcoo_matrix *h, const float *S, bool vec4, hipStream_t s)
 {
     dim3 grid(h->nGroup), block(kConvThreads);
+    h->mask_valid = false;
+    if (vec4 && ensure_mask(h)) {
+        hipLaunchKernelGGL((count_mask_kernel<CPG>), grid, block, 0, s, S, h->M, h->K,
+                           mask_blocks(h), h->rowcnt, h->mask);
+        GCOO_CHECK_LAUNCH("count_mask_kernel");
+        h->mask_valid = true;
+        return GCOO_OK;
+    }
     if (vec4)
         hipLaunchKernelGGL((count_kernel<CPG, true>), grid, block, 0, s, S, h->M,
                            h->K, h->rowcnt);
@@ -293,6 +485,16 @@ template <int CPG>
 int launch_scatter_t(gcoo_matrix *h, const float *S, bool vec4, hipStream_t s)
 {
     dim3 grid(h->nGroup), block(kConvThreads);
+    const unsigned chunks = (unsigned)((h->K + kConvThreads * 4 - 1) / (kConvThreads * 4));
+    if (vec4 && h->mask_valid && chunks <= 65535u) {  // chunks index grid y
+        if (chunks == 0) return GCOO_OK;
+        hipLaunchKernelGGL((scatter_mask_kernel<CPG>), dim3(h->nGroup, chunks), block, 0, s, S,
+                           h->M, h->K,
+                           mask_blocks(h), h->mask, h->groupIndex(), h->rowptr, h->d_status,
+                           (long long)h->cap, h->vals, h->rows(), h->cols(), h->seg);
+        GCOO_CHECK_LAUNCH("scatter_mask_kernel");
+        return GCOO_OK;
+    }
     if (vec4)
         hipLaunchKernelGGL((scatter_kernel<CPG, true>), grid, block, 0, s, S,
                            h->M, h->K, h->groupIndex(), h->rowptr, h->d_status,
@@ -316,6 +518,7 @@ int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s)
 {
     const bool v4 = use_vec4(h, S);
     int rc = GCOO_OK;
+    h->mask_valid = false;
     if (h->nGroup > 0) {
         switch (h->cpg) {
         case 1: rc = launch_count_t<1>(h, S, v4, s); break;

@@ -77,6 +77,14 @@ struct gcoo_matrix {
     int *rowcnt = nullptr;     // [M] per-row counts (conversion scratch)
     int maxRow = 0;            // most entries in one row (last conversion)
 
+    // Nonzero mask of the last dense conversion (16-byte path only), written
+    // by the count pass and read by the scatter so that it loads only the
+    // pieces of S that hold an entry.  Per row and 256-column block, four
+    // 64-bit words: bit l of word c is column block * 256 + 4 * l + c.
+    unsigned long long *mask = nullptr;  // [M * maskBlocks * 4]
+    size_t mask_bytes = 0;
+    bool mask_valid = false;   // written for the S of the current conversion
+
     // long-row path scratch, allocated on first use by the multiply
     mutable int *longList = nullptr;       // [M + 1] rows of the long-row kernel
     mutable unsigned *longCtr = nullptr;   // [0] listed rows, [1] next work item

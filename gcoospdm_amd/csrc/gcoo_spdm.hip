@@ -645,14 +645,16 @@ constexpr int kPanelPrefetch = 4;  // chunks between the L2 touch and the stagin
 constexpr double kPanelMinDensity = 0.007;  // automatic choice of variant 6 (FP32) ...
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
-// a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform q < 16: the
-// accumulators are pinned in v0..v63 (component v of row slot q in
-// v[16v + q]) and the step runs in VGPR-index mode with source 2 and the
-// destination relative to M0[7:0] = q.  The index mode writes M0, which the
-// buffer-to-LDS loads also use; callers save M0 around a run of steps
-// (panel_m0_save / panel_m0_restore).
+// a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform row slot
+// q < 16: the accumulators are pinned in v0..v63 (component v of row slot q
+// in v[16v + q]) and the step runs in VGPR-index mode with source 2 and the
+// destination relative to M0[7:0].  The index instructions copy bits [7:0]
+// of their operand into M0[7:0] and ignore the rest, so callers pass the
+// packed list word ((LDS offset << 8) | q) as it is.  The index mode writes
+// M0, which the buffer-to-LDS loads also use; callers save M0 around a run of
+// steps (panel_m0_save / panel_m0_restore).
 __device__ __forceinline__ void panel_fma1(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
-                                           int q, float x, const float (&d)[4])
+                                           unsigned q, float x, const float (&d)[4])
 {
     asm volatile(
         "s_set_gpr_idx_on %4, gpr_idx(SRC2,DST)\n\t"
@@ -668,7 +670,7 @@ __device__ __forceinline__ void panel_fma1(f32x16 &a0, f32x16 &a1, f32x16 &a2, f
 // Four steps in order, the row index of each later step set with
 // s_set_gpr_idx_idx (index only, mode kept).
 __device__ __forceinline__ void panel_fma4(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
-                                           const int (&q)[4], const float (&x)[4],
+                                           const unsigned (&q)[4], const float (&x)[4],
                                            const float (&d0)[4], const float (&d1)[4],
                                            const float (&d2)[4], const float (&d3)[4])
 {
@@ -751,8 +753,12 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
 
     // Staging: this lane's 16-byte piece of every wave load, D row (first
-    // row of the load + prow), columns from pcol; pieces past N or K read
-    // zeros.
+    // row of the load + prow), columns from pcol; pieces past N read zeros.
+    // Rows past K are never read by an entry: a load of whole rows past K is
+    // skipped, and a row past K inside a load reads zeros.  With one D row
+    // per load (FP32) the test is wave-uniform, so a load costs no vector
+    // instructions: the chunk's loads are issued right after the barrier,
+    // ahead of the entries.
     const int prow = lane / PPR;
     const int pcol = col0 + (lane % PPR) * (16 / E::bytes);
     const int pvoff = pcol < N ? pcol * E::bytes + prow * rowBytes : (int)dBytes;
@@ -762,12 +768,12 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             for (int i = 0; i < LPW; ++i) {
                 const int li = wave * LPW + i;     // wave load index in the chunk
                 const int k0 = c * TK + li * RPL;  // its first D row
-                const int voff = k0 + prow < K ? pvoff : (int)dBytes;
-                const int soff = k0 < K ? k0 * rowBytes : 0;
+                if (k0 >= K) break;
+                const int voff = (RPL == 1 || k0 + prow < K) ? pvoff : (int)dBytes;
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rsrc,
                     (__attribute__((address_space(3))) void *)&sD[(buf * CHUNKB + li * 1024) / 16],
-                    16, voff, soff, 0, 0);
+                    16, voff, k0 * rowBytes, 0, 0);
             }
         }
     };
@@ -781,7 +787,9 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             const int r = i * RPP + frow;
             const bool ok = r < TK && c * TK + r < K && fbyte < rowBytes;
             const int voff = ok ? fbyte + frow * rowBytes : (int)dBytes;
-            const int soff = ok ? (c * TK + i * RPP) * rowBytes : 0;
+            // soffset must be wave-uniform (a per-lane choice here would
+            // make the compiler loop over its distinct values)
+            const int soff = c * TK + i * RPP < K ? (c * TK + i * RPP) * rowBytes : 0;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(
                 rsrc, (__attribute__((address_space(3))) void *)&sD[DUMMY], 4, voff, soff, 0, 0);
         }
@@ -791,8 +799,24 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     const int *op = poff + (size_t)panel * panel_stride(nC) + wave * RPW;
     const unsigned char *ldsLane = reinterpret_cast<const unsigned char *>(sD) + lane * LB;
 
+    // This wave's list bounds of chunks [cw, cw + 64), lane i holding chunk
+    // cw + i, read with v_readlane: a scalar load of the next chunk's bounds
+    // would be waited for before the entry loop, a full round trip per chunk.
+    // The window is reloaded (and waited for) once every 64 chunks.
+    int cw = 0;
+    int vpb = 0, vpe = 0;
+    auto load_window = [&]() {
+        vpb = 0;
+        vpe = 0;
+        if (cw + lane < nC) {
+            vpb = op[(size_t)(cw + lane) * kPanelRows];
+            vpe = op[(size_t)(cw + lane) * kPanelRows + RPW];
+        }
+        asm volatile("" : "+v"(vpb), "+v"(vpe));  // the wait for the loads lands here
+    };
+    load_window();
     stage(0, 0);
-    int pb = op[0], pe = op[RPW];
+    int pb = __builtin_amdgcn_readlane(vpb, 0), pe = __builtin_amdgcn_readlane(vpe, 0);
     for (int c = 0; c < nC; ++c) {
         // chunk c has landed in LDS for every wave, and every wave is done
         // with chunk c - 1, whose buffer the next stage overwrites (a bare
@@ -802,9 +826,13 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
         if (wave == SW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
         const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
-        const int cn = c + 1 < nC ? c + 1 : c;
-        const int pbn = op[(size_t)cn * kPanelRows];
-        const int pen = op[(size_t)cn * kPanelRows + RPW];
+        if (c + 1 == cw + 64) {
+            cw += 64;
+            load_window();
+        }
+        const int li = c + 1 < nC ? c + 1 - cw : 0;
+        const int pbn = __builtin_amdgcn_readlane(vpb, li);
+        const int pen = __builtin_amdgcn_readlane(vpe, li);
         // B entries: list entry x = (LDS byte offset << 8) | row slot
         auto run = [&](auto bn, int p) {
             constexpr int B = decltype(bn)::value;
@@ -818,15 +846,14 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             int u = 0;
 #pragma unroll
             for (; u + 4 <= B; u += 4) {
-                const int q[4] = {(int)(m[u].x & 15u), (int)(m[u + 1].x & 15u),
-                                  (int)(m[u + 2].x & 15u), (int)(m[u + 3].x & 15u)};
+                const unsigned q[4] = {m[u].x, m[u + 1].x, m[u + 2].x, m[u + 3].x};
                 const float x[4] = {E::val((int)m[u].y), E::val((int)m[u + 1].y),
                                     E::val((int)m[u + 2].y), E::val((int)m[u + 3].y)};
                 panel_fma4(a0, a1, a2, a3, q, x, d[u], d[u + 1], d[u + 2], d[u + 3]);
             }
 #pragma unroll
             for (; u < B; ++u)
-                panel_fma1(a0, a1, a2, a3, (int)(m[u].x & 15u), E::val((int)m[u].y), d[u]);
+                panel_fma1(a0, a1, a2, a3, m[u].x, E::val((int)m[u].y), d[u]);
             panel_m0_restore(m0);
         };
         int p = pb;
