@@ -619,17 +619,23 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 // by direct buffer-to-LDS loads issued one chunk ahead into the other of two
 // buffers, and serves every entry of the panel in that chunk -- on average
 // 256 * density entries per staged D row instead of one L2 gather per entry.
-//   * Waves 0..14 stage the chunks; wave 15 stages nothing and, in one
-//     workgroup of each column tile per chunk (panel == chunk % panels),
-//     touches the tile piece of the chunk kPanelPrefetch chunks ahead so that
-//     it is in L2 when the others stage it.  The panels of a tile run in step,
-//     so without this every chunk would be a simultaneous L2 miss of all of
-//     them.  Wave 15 never waits for those loads inside the loop: vector
-//     memory completions are counted in issue order, and a wait would stall
-//     on the miss it is there to absorb.
+//   * Waves 0..4 stage the chunks (15 wave loads each); wave 15 stages
+//     nothing and, in one workgroup of each column tile per chunk
+//     (panel == chunk % panels), touches the tile piece of the chunk
+//     kPanelPrefetch chunks ahead so that it is in L2 when the stagers load
+//     it.  The panels of a tile run in step, so without this every chunk
+//     would be a simultaneous L2 miss of all of them.  Wave 15 never waits
+//     for those loads inside the loop: vector memory completions are counted
+//     in issue order, and a wait would stall on the miss it is there to
+//     absorb.  The hardware issues from the oldest (lowest) waves first, so
+//     the staging sits with them and the younger waves, which finish their
+//     entries last, carry no loads of their own.
 //   * Each wave takes its rows' entries of the chunk from the panel list
 //     (built by gcoo_panel.hip from the row segments, ordered by row then k),
 //     8 at a time from scalar loads: 8 LDS reads in flight, then 8 fmaf steps.
+//     Before each group of 8 a wave sets its issue priority from the entries
+//     it has left in the chunk, so that the wave the barrier waits for is
+//     served first.
 //   * The step's row is wave-uniform but varies per entry.  The 64
 //     accumulators sit in v0..v63 (component v of row slot q in v[16v + q])
 //     and each step is 4 v_fma_f32 in VGPR-index mode, source 2 and
@@ -729,8 +735,9 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     constexpr int ROWB = 64 * LB;             // bytes of a staged D row
     constexpr int CHUNKB = TK * ROWB;         // bytes of a staged chunk
     constexpr int NLOAD = CHUNKB / 1024;      // 1 KB wave loads per chunk
-    constexpr int SW = 15;                    // staging waves
-    static_assert(CHUNKB % 1024 == 0 && NLOAD % SW == 0, "chunk loads must split over 15 waves");
+    constexpr int SW = 5;                     // staging waves
+    constexpr int TW = 15;                    // L2 touch wave
+    static_assert(CHUNKB % 1024 == 0 && NLOAD % SW == 0, "chunk loads must split over the stagers");
     constexpr int LPW = NLOAD / SW;           // wave loads per staging wave per chunk
     constexpr int PPR = ROWB / 16;            // 16-byte pieces per staged D row
     constexpr int RPL = 64 / PPR;             // D rows per wave load
@@ -824,7 +831,7 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         if (wave < SW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         asm volatile("s_barrier" ::: "memory");
         if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
-        if (wave == SW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
+        if (wave == TW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
         const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
         if (c + 1 == cw + 64) {
             cw += 64;
@@ -857,10 +864,16 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             panel_m0_restore(m0);
         };
         int p = pb;
-        for (; p + 8 <= pe; p += 8) run(std::integral_constant<int, 8>{}, p);
+        for (; p + 8 <= pe; p += 8) {
+            const int rem = pe - p;
+            if (rem >= 16) __builtin_amdgcn_s_setprio(3);
+            else __builtin_amdgcn_s_setprio(2);
+            run(std::integral_constant<int, 8>{}, p);
+        }
         if (p + 4 <= pe) { run(std::integral_constant<int, 4>{}, p); p += 4; }
         if (p + 2 <= pe) { run(std::integral_constant<int, 2>{}, p); p += 2; }
         if (p < pe) run(std::integral_constant<int, 1>{}, p);
+        __builtin_amdgcn_s_setprio(0);
         pb = pbn;
         pe = pen;
     }
