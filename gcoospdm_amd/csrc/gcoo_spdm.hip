@@ -648,7 +648,7 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPanelPrefetch = 4;  // chunks between the L2 touch and the staging
-constexpr double kPanelMinDensity = 0.007;  // automatic choice of variant 6 (FP32) ...
+constexpr double kPanelMinDensity = 0.0045;  // automatic choice of variant 6 (FP32) ...
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
 // a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform row slot
@@ -1187,8 +1187,9 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
     if (variant == 0) {
         // Panel staging costs a fixed K x N read of D per 256-row panel, row
         // segments cost one D-row gather per entry: the panel kernel wins from
-        // about 0.7 % density on (measured at n = 8000, FP32).  BF16 gathers
-        // move half the bytes and stay ahead.  The density is the handle's
+        // about 0.45 % density on (measured at n = 8000, FP32: 527 vs 594 us
+        // at 0.5 %, 509 vs 483 us at 0.4 %).  BF16 gathers move half the
+        // bytes and stay ahead at every density.  The density is the handle's
         // last completed conversion (a first optimistic launch does not know
         // the new count yet); every variant gives the same bits.
         // It also needs a workgroup per CU at least (256 x 256 tiles of C).
