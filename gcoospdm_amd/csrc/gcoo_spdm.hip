@@ -21,6 +21,7 @@
 #include <type_traits>
 
 #include "gcoo_internal.h"
+#include "gcoo_panel_asm.h"
 
 namespace gcoo {
 namespace {
@@ -635,7 +636,10 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 //     8 at a time from scalar loads: 8 LDS reads in flight, then 8 fmaf steps.
 //     Before each group of 8 a wave sets its issue priority from the entries
 //     it has left in the chunk, so that the wave the barrier waits for is
-//     served first.
+//     served first.  For FP32 the whole loop is one asm block
+//     (gcoo_panel_asm.h): the next group's scalar load is issued after this
+//     group's LDS wait and runs under its steps, and the last 1..7 entries
+//     take one guarded round instead of 4/2/1 groups with a load each.
 //   * The step's row is wave-uniform but varies per entry.  The 64
 //     accumulators sit in v0..v63 (component v of row slot q in v[16v + q])
 //     and each step is 4 v_fma_f32 in VGPR-index mode, source 2 and
@@ -769,18 +773,26 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     const int prow = lane / PPR;
     const int pcol = col0 + (lane % PPR) * (16 / E::bytes);
     const int pvoff = pcol < N ? pcol * E::bytes + prow * rowBytes : (int)dBytes;
+    // The wave's loads are consecutive: the LDS destination and the D row
+    // advance by a constant per load (base + immediate, running sum), so no
+    // per-load constants are kept in scalar registers across the loop.
+    typedef __attribute__((address_space(3))) unsigned char lds_u8;
+    lds_u8 *const ldsBase = (lds_u8 *)sD + wave * LPW * 1024;
     auto stage = [&](int c, int buf) {
         if (wave < SW) {
+            const int k0 = c * TK + wave * LPW * RPL;  // the wave's first D row
+            int soff = k0 * rowBytes;
+            lds_u8 *dst = ldsBase + (buf ? CHUNKB : 0);
 #pragma unroll
             for (int i = 0; i < LPW; ++i) {
-                const int li = wave * LPW + i;     // wave load index in the chunk
-                const int k0 = c * TK + li * RPL;  // its first D row
-                if (k0 >= K) break;
-                const int voff = (RPL == 1 || k0 + prow < K) ? pvoff : (int)dBytes;
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rsrc,
-                    (__attribute__((address_space(3))) void *)&sD[(buf * CHUNKB + li * 1024) / 16],
-                    16, voff, k0 * rowBytes, 0, 0);
+                const int k = k0 + i * RPL;
+                if (k >= K) break;
+                const int voff = (RPL == 1 || k + prow < K) ? pvoff : (int)dBytes;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void *)(dst + i * 1024),
+                                                         16, voff, soff, 0, 0);
+                // an opaque add: the compiler would otherwise keep one
+                // precomputed row offset per load in a scalar register
+                asm("s_add_u32 %0, %0, %1" : "+s"(soff) : "s"(RPL * rowBytes));
             }
         }
     };
@@ -840,6 +852,23 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         const int li = c + 1 < nC ? c + 1 - cw : 0;
         const int pbn = __builtin_amdgcn_readlane(vpb, li);
         const int pen = __builtin_amdgcn_readlane(vpe, li);
+        if constexpr (E::bytes == 4) {
+            // the chunk's entries in one asm loop (gcoo_panel_asm.h): next
+            // batch's entries loaded under this batch's steps, one guarded
+            // round for the last 1..7, priority from the entries left
+            const uint2 *lp = plist + pb;
+            const int nn = pe - pb;
+            const unsigned base =
+                (unsigned)(size_t)((const __attribute__((address_space(3))) unsigned char *)sb);
+            asm volatile(GCOO_PANEL_CHUNK_ASM
+                         : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
+                         : [lp] "s"(lp), [n] "s"(nn), [base] "v"(base)
+                         : GCOO_PANEL_CHUNK_CLOBBERS);
+            __builtin_amdgcn_s_setprio(0);
+            pb = pbn;
+            pe = pen;
+            continue;
+        }
         // B entries: list entry x = (LDS byte offset << 8) | row slot
         auto run = [&](auto bn, int p) {
             constexpr int B = decltype(bn)::value;

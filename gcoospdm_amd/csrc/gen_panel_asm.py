@@ -1,0 +1,139 @@
+#!/usr/bin/env python3
+"""Writes gcoo_panel_asm.h: the entry loop of one wave over one chunk of the
+FP32 panel multiply (spdm_panel_kernel, gcoo_spdm.hip) as one inline-asm body.
+
+    python3 gen_panel_asm.py > gcoo_panel_asm.h
+
+The loop takes the wave's list entries of the chunk 8 at a time.  Entry u of a
+batch is two words in SGPRs: x = (LDS byte offset of its D row << 8) | row
+slot, and the FP32 bits of its value.  Per batch:
+  * 8 LDS reads of the entries' D rows (16 bytes per lane),
+  * the wait for them (LDS only),
+  * the scalar load of the next batch's 8 entries, issued now so that it runs
+    under this batch's steps (issued any earlier, the wait for the LDS reads
+    would also wait for it: both count in lgkmcnt),
+  * 8 steps in VGPR-index mode: acc[slot] = fmaf(value, d, acc[slot]) for the
+    4 columns, the slot taken from bits 7:0 of x by s_set_gpr_idx_on/_idx,
+  * the wait for the next batch's entries.
+Two register sets alternate between batches.  The last 1..7 entries take one
+guarded round.  Before each full batch the wave's issue priority is set from
+the entries it has left (3 from 16 on, else 2).  Entries are the wave's rows
+in row order with k ascending, so every row's chain is unchanged.
+
+Operands: %[lp] SGPR pair, address of the first entry; %[n] SGPR, the number
+of entries; %[base] VGPR, this lane's LDS byte address of the chunk buffer;
+the accumulators v0..v63 (component c of row slot q in v[16c + q]).
+Scratch: v64..v103 and s40..s75.  M0 is saved and restored.  The loop reads
+up to 7 entries past the end of the list, which the list capacity covers.
+"""
+A = list(range(40, 56))
+B = list(range(56, 72))
+ST, SREM, SOFF, SM0 = 72, 73, 74, 75
+DV, AV = 64, 96
+L = []
+
+
+def e(x):
+    L.append(x)
+
+
+def reads(S, k):
+    for u in range(k):
+        e(f"s_lshr_b32 s{ST}, s{S[2*u]}, 8")
+        e(f"v_add_u32_e32 v{AV+u}, s{ST}, %[base]")
+        e(f"ds_read_b128 v[{DV+4*u}:{DV+4*u+3}], v{AV+u}")
+
+
+def entry(S, u):
+    for c in range(4):
+        e(f"v_fma_f32 v{16*c}, s{S[2*u+1]}, v{DV+4*u+c}, v{16*c}")
+
+
+def full_steps(S):
+    for h in range(2):
+        e(f"s_set_gpr_idx_on s{S[8*h]}, gpr_idx(SRC2,DST)")
+        for q in range(4):
+            if q:
+                e(f"s_set_gpr_idx_idx s{S[2*(4*h+q)]}")
+            entry(S, 4 * h + q)
+        e("s_set_gpr_idx_off")
+
+
+def tail(S, t):
+    e(f"s_lshr_b32 s{ST}, s{S[0]}, 8")
+    e(f"v_add_u32_e32 v{AV}, s{ST}, %[base]")
+    e(f"ds_read_b128 v[{DV}:{DV+3}], v{AV}")
+    for u in range(1, 7):
+        e(f"s_cmp_gt_u32 s{SREM}, {u}")
+        e(f"s_cbranch_scc0 .Lgw{t}_%=")
+        e(f"s_lshr_b32 s{ST}, s{S[2*u]}, 8")
+        e(f"v_add_u32_e32 v{AV+u}, s{ST}, %[base]")
+        e(f"ds_read_b128 v[{DV+4*u}:{DV+4*u+3}], v{AV+u}")
+    e(f".Lgw{t}_%=:")
+    e("s_waitcnt lgkmcnt(0)")
+    e(f"s_set_gpr_idx_on s{S[0]}, gpr_idx(SRC2,DST)")
+    entry(S, 0)
+    for u in range(1, 7):
+        e(f"s_cmp_gt_u32 s{SREM}, {u}")
+        e(f"s_cbranch_scc0 .Lge{t}_%=")
+        e(f"s_set_gpr_idx_idx s{S[2*u]}")
+        entry(S, u)
+    e(f".Lge{t}_%=:")
+    e("s_set_gpr_idx_off")
+    e(".Lend_%=:" if t == "b" else "s_branch .Lend_%=")
+
+
+def batch(S, N, t, nxt):
+    e(f".Lb{t}_%=:")
+    e(f"s_cmp_lt_u32 s{SREM}, 8")
+    e(f"s_cbranch_scc1 .Lt{t}_%=")
+    e(f"s_cmp_ge_u32 s{SREM}, 16")
+    e(f"s_cbranch_scc0 .Lp{t}_%=")
+    e("s_setprio 3")
+    e(f"s_branch .Lq{t}_%=")
+    e(f".Lp{t}_%=:")
+    e("s_setprio 2")
+    e(f".Lq{t}_%=:")
+    reads(S, 8)
+    e("s_waitcnt lgkmcnt(0)")
+    e(f"s_cmp_le_u32 s{SREM}, 8")
+    e(f"s_cbranch_scc1 .Ln{t}_%=")
+    e(f"s_load_dwordx16 s[{N[0]}:{N[15]}], %[lp], s{SOFF}")
+    e(f".Ln{t}_%=:")
+    full_steps(S)
+    e(f"s_sub_u32 s{SREM}, s{SREM}, 8")
+    e(f"s_add_u32 s{SOFF}, s{SOFF}, 64")
+    e(f"s_cmp_eq_u32 s{SREM}, 0")
+    e(f"s_cbranch_scc1 .Lend_%=")
+    e("s_waitcnt lgkmcnt(0)")
+    if nxt:
+        e(f"s_branch .Lb{nxt}_%=")
+
+
+e(f"s_mov_b32 s{SM0}, m0")
+e(f"s_mov_b32 s{SREM}, %[n]")
+e(f"s_cmp_eq_u32 s{SREM}, 0")
+e(f"s_cbranch_scc1 .Lend_%=")
+e(f"s_load_dwordx16 s[{A[0]}:{A[15]}], %[lp], 0x0")
+e(f"s_movk_i32 s{SOFF}, 0x40")
+e("s_waitcnt lgkmcnt(0)")
+batch(A, B, "a", None)      # falls through to the other register set
+batch(B, A, "b", "a")
+e(".Lta_%=:")
+tail(A, "a")
+e(".Ltb_%=:")
+tail(B, "b")
+e(f"s_mov_b32 m0, s{SM0}")
+
+print("// Generated by gen_panel_asm.py (same directory); see its docstring.")
+print("// The entry loop of one wave over one chunk of the FP32 panel multiply.")
+print("#pragma once")
+print("#define GCOO_PANEL_CHUNK_ASM \\")
+for i, x in enumerate(L):
+    end = " \\" if i + 1 < len(L) else ""
+    print(f'    "{x}\\n\\t"{end}')
+print("#define GCOO_PANEL_CHUNK_CLOBBERS \\")
+regs = [f'"v{i}"' for i in range(64, 104)] + [f'"s{i}"' for i in range(40, 76)] + ['"scc"', '"memory"']
+for i in range(0, len(regs), 12):
+    end = ", \\" if i + 12 < len(regs) else ""
+    print("    " + ", ".join(regs[i:i+12]) + end)
