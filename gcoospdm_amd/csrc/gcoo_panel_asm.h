@@ -1,12 +1,113 @@
 // Generated by gen_panel_asm.py (same directory); see its docstring.
-// The entry loop of one wave over one chunk of the FP32 panel multiply.
+// One chunk of the FP32 panel multiply for one wave: barrier, staging, entry loop.
 #pragma once
 #define GCOO_PANEL_CHUNK_ASM \
     "s_mov_b32 s75, m0\n\t" \
     "s_mov_b32 s73, %[n]\n\t" \
     "s_cmp_eq_u32 s73, 0\n\t" \
-    "s_cbranch_scc1 .Lend_%=\n\t" \
+    "s_cbranch_scc1 .Lnl_%=\n\t" \
     "s_load_dwordx16 s[40:55], %[lp], 0x0\n\t" \
+    ".Lnl_%=:\n\t" \
+    "s_cmp_eq_u32 %[stg], 0\n\t" \
+    "s_cbranch_scc1 .Lsw_%=\n\t" \
+    "s_waitcnt vmcnt(0)\n\t" \
+    ".Lsw_%=:\n\t" \
+    "s_barrier\n\t" \
+    "s_mov_b32 s74, %[soff]\n\t" \
+    "s_mov_b32 s72, %[dst]\n\t" \
+    "s_cmp_le_u32 %[nld], 0\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 1\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 2\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 3\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 4\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 5\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 6\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 7\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 8\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 9\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 10\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 11\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 12\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 13\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    "s_cmp_le_u32 %[nld], 14\n\t" \
+    "s_cbranch_scc1 .Lsd_%=\n\t" \
+    "s_mov_b32 m0, s72\n\t" \
+    "s_add_u32 s72, s72, 0x400\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
+    "s_add_u32 s74, s74, %[rowb]\n\t" \
+    ".Lsd_%=:\n\t" \
+    "s_cmp_eq_u32 s73, 0\n\t" \
+    "s_cbranch_scc1 .Lend_%=\n\t" \
     "s_movk_i32 s74, 0x40\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     ".Lba_%=:\n\t" \

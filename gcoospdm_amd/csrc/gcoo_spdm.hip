@@ -636,10 +636,13 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 //     8 at a time from scalar loads: 8 LDS reads in flight, then 8 fmaf steps.
 //     Before each group of 8 a wave sets its issue priority from the entries
 //     it has left in the chunk, so that the wave the barrier waits for is
-//     served first.  For FP32 the whole loop is one asm block
-//     (gcoo_panel_asm.h): the next group's scalar load is issued after this
-//     group's LDS wait and runs under its steps, and the last 1..7 entries
-//     take one guarded round instead of 4/2/1 groups with a load each.
+//     served first.  For FP32 a chunk is one asm block (gcoo_panel_asm.h):
+//     the first group's scalar load goes out before the chunk barrier (its
+//     latency passes while the wave waits there), then the barrier and the
+//     staging of the next chunk, then the entry loop, in which the next
+//     group's scalar load is issued after this group's LDS wait and runs
+//     under its steps, and the last 1..7 entries take one guarded round
+//     instead of 4/2/1 groups with a load each.
 //   * The step's row is wave-uniform but varies per entry.  The 64
 //     accumulators sit in v0..v63 (component v of row slot q in v[16v + q])
 //     and each step is 4 v_fma_f32 in VGPR-index mode, source 2 and
@@ -837,12 +840,15 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     stage(0, 0);
     int pb = __builtin_amdgcn_readlane(vpb, 0), pe = __builtin_amdgcn_readlane(vpe, 0);
     for (int c = 0; c < nC; ++c) {
-        // chunk c has landed in LDS for every wave, and every wave is done
-        // with chunk c - 1, whose buffer the next stage overwrites (a bare
-        // barrier: a fence here would wait for wave 15's touches too)
-        if (wave < SW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        asm volatile("s_barrier" ::: "memory");
-        if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
+        if constexpr (E::bytes != 4) {
+            // chunk c has landed in LDS for every wave, and every wave is
+            // done with chunk c - 1, whose buffer the next stage overwrites
+            // (a bare barrier: a fence here would wait for wave 15's touches
+            // too).  The FP32 path does this inside its asm block.
+            if (wave < SW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            asm volatile("s_barrier" ::: "memory");
+            if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
+        }
         if (wave == TW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
         const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
         if (c + 1 == cw + 64) {
@@ -853,16 +859,33 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         const int pbn = __builtin_amdgcn_readlane(vpb, li);
         const int pen = __builtin_amdgcn_readlane(vpe, li);
         if constexpr (E::bytes == 4) {
-            // the chunk's entries in one asm loop (gcoo_panel_asm.h): next
-            // batch's entries loaded under this batch's steps, one guarded
-            // round for the last 1..7, priority from the entries left
+            // The chunk in one asm block (gcoo_panel_asm.h): the first batch's
+            // entry load issued before the barrier, the barrier, the staging
+            // of chunk c + 1, and the entry loop (next batch's entries loaded
+            // under this batch's steps, one guarded round for the last 1..7,
+            // priority from the entries left).
+            static_assert(LPW == 15 && RPL == 1, "the asm block stages 15 one-row loads");
+            typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4 rs = {(unsigned)reinterpret_cast<uintptr_t>(D),
+                              (unsigned)(reinterpret_cast<uintptr_t>(D) >> 32) & 0xffffu,
+                              (unsigned)dBytes, (unsigned)kBufferFlags};
             const uint2 *lp = plist + pb;
             const int nn = pe - pb;
             const unsigned base =
                 (unsigned)(size_t)((const __attribute__((address_space(3))) unsigned char *)sb);
+            const int k1 = (c + 1) * TK + wave * LPW;  // this wave's first row of chunk c + 1
+            int nld = 0;
+            if (wave < SW && c + 1 < nC) nld = K - k1 < 0 ? 0 : (K - k1 < LPW ? K - k1 : LPW);
+            nld = __builtin_amdgcn_readfirstlane(nld);
+            const int stg = __builtin_amdgcn_readfirstlane(wave < SW ? 1 : 0);
+            const unsigned dst = __builtin_amdgcn_readfirstlane(
+                (unsigned)(size_t)(ldsBase + (((c + 1) & 1) ? CHUNKB : 0)));
+            const int soff1 = __builtin_amdgcn_readfirstlane(k1 * rowBytes);
             asm volatile(GCOO_PANEL_CHUNK_ASM
                          : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
-                         : [lp] "s"(lp), [n] "s"(nn), [base] "v"(base)
+                         : [lp] "s"(lp), [n] "s"(nn), [base] "v"(base), [stg] "s"(stg),
+                           [nld] "s"(nld), [soff] "s"(soff1), [dst] "s"(dst), [voff] "v"(pvoff),
+                           [rsrc] "s"(rs), [rowb] "s"(rowBytes)
                          : GCOO_PANEL_CHUNK_CLOBBERS);
             __builtin_amdgcn_s_setprio(0);
             pb = pbn;

@@ -1,30 +1,42 @@
 #!/usr/bin/env python3
-"""Writes gcoo_panel_asm.h: the entry loop of one wave over one chunk of the
-FP32 panel multiply (spdm_panel_kernel, gcoo_spdm.hip) as one inline-asm body.
+"""Writes gcoo_panel_asm.h: one chunk of the FP32 panel multiply
+(spdm_panel_kernel, gcoo_spdm.hip) for one wave, as one inline-asm body --
+the chunk barrier, the staging of the next chunk and the entry loop.
 
     python3 gen_panel_asm.py > gcoo_panel_asm.h
 
-The loop takes the wave's list entries of the chunk 8 at a time.  Entry u of a
-batch is two words in SGPRs: x = (LDS byte offset of its D row << 8) | row
-slot, and the FP32 bits of its value.  Per batch:
-  * 8 LDS reads of the entries' D rows (16 bytes per lane),
-  * the wait for them (LDS only),
-  * the scalar load of the next batch's 8 entries, issued now so that it runs
-    under this batch's steps (issued any earlier, the wait for the LDS reads
-    would also wait for it: both count in lgkmcnt),
-  * 8 steps in VGPR-index mode: acc[slot] = fmaf(value, d, acc[slot]) for the
-    4 columns, the slot taken from bits 7:0 of x by s_set_gpr_idx_on/_idx,
-  * the wait for the next batch's entries.
-Two register sets alternate between batches.  The last 1..7 entries take one
-guarded round.  Before each full batch the wave's issue priority is set from
-the entries it has left (3 from 16 on, else 2).  Entries are the wave's rows
-in row order with k ascending, so every row's chain is unchanged.
+In order:
+  * the scalar load of the chunk's first 8 list entries, issued before the
+    barrier so that its latency passes while the wave waits there;
+  * the staging waves wait for their loads of this chunk (vmcnt(0)), then the
+    workgroup barrier: chunk c is in LDS and every wave is done with the
+    buffer of chunk c - 1;
+  * the staging waves issue their %[nld] buffer-to-LDS loads of chunk c + 1
+    into that buffer (unrolled; LDS destination %[dst] + 1 KB per load, D row
+    offset %[soff] + %[rowb] per load, lane offset %[voff]);
+  * the entry loop, 8 entries per batch.  Entry u of a batch is two words in
+    SGPRs: x = (LDS byte offset of its D row << 8) | row slot, and the FP32
+    bits of its value.  Per batch: 8 LDS reads of the entries' D rows (16
+    bytes per lane); the wait for them (LDS only); the scalar load of the next
+    batch's 8 entries, issued now so that it runs under this batch's steps
+    (issued any earlier, the wait for the LDS reads would also wait for it:
+    both count in lgkmcnt); 8 steps in VGPR-index mode,
+    acc[slot] = fmaf(value, d, acc[slot]) for the 4 columns, the slot taken
+    from bits 7:0 of x by s_set_gpr_idx_on/_idx; the wait for the next
+    batch's entries.  Two register sets alternate between batches.  The last
+    1..7 entries take one guarded round.  Before each full batch the wave's
+    issue priority is set from the entries it has left (3 from 16 on, else 2).
+Entries are the wave's rows in row order with k ascending, so every row's
+chain is unchanged.
 
 Operands: %[lp] SGPR pair, address of the first entry; %[n] SGPR, the number
 of entries; %[base] VGPR, this lane's LDS byte address of the chunk buffer;
-the accumulators v0..v63 (component c of row slot q in v[16c + q]).
-Scratch: v64..v103 and s40..s75.  M0 is saved and restored.  The loop reads
-up to 7 entries past the end of the list, which the list capacity covers.
+%[stg] SGPR, 1 on a staging wave; %[nld], %[dst], %[soff], %[rowb] SGPRs and
+%[voff] VGPR as above; %[rsrc] the buffer resource of D (SGPR quad); the
+accumulators v0..v63 (component c of row slot q in v[16c + q]).  Scratch:
+v64..v103 and s40..s75.  M0 is saved and restored.  The loop reads up to 7
+entries past the end of the list, which the list capacity covers.  Every
+wave runs the body exactly once per chunk (one barrier each).
 """
 A = list(range(40, 56))
 B = list(range(56, 72))
@@ -112,9 +124,31 @@ def batch(S, N, t, nxt):
 
 e(f"s_mov_b32 s{SM0}, m0")
 e(f"s_mov_b32 s{SREM}, %[n]")
+# the first batch's load goes out before the barrier
+e(f"s_cmp_eq_u32 s{SREM}, 0")
+e(f"s_cbranch_scc1 .Lnl_%=")
+e(f"s_load_dwordx16 s[{A[0]}:{A[15]}], %[lp], 0x0")
+e(".Lnl_%=:")
+# the staging waves' loads of this chunk have landed, every wave is done
+# with the previous chunk
+e("s_cmp_eq_u32 %[stg], 0")
+e("s_cbranch_scc1 .Lsw_%=")
+e("s_waitcnt vmcnt(0)")
+e(".Lsw_%=:")
+e("s_barrier")
+# the next chunk's %[nld] loads (staging waves), unrolled
+e(f"s_mov_b32 s{SOFF}, %[soff]")
+e(f"s_mov_b32 s{ST}, %[dst]")
+for i in range(15):
+    e(f"s_cmp_le_u32 %[nld], {i}")
+    e("s_cbranch_scc1 .Lsd_%=")
+    e(f"s_mov_b32 m0, s{ST}")
+    e(f"s_add_u32 s{ST}, s{ST}, 0x400")
+    e("buffer_load_dwordx4 %[voff], %[rsrc], s" + str(SOFF) + " offen lds")
+    e(f"s_add_u32 s{SOFF}, s{SOFF}, %[rowb]")
+e(".Lsd_%=:")
 e(f"s_cmp_eq_u32 s{SREM}, 0")
 e(f"s_cbranch_scc1 .Lend_%=")
-e(f"s_load_dwordx16 s[{A[0]}:{A[15]}], %[lp], 0x0")
 e(f"s_movk_i32 s{SOFF}, 0x40")
 e("s_waitcnt lgkmcnt(0)")
 batch(A, B, "a", None)      # falls through to the other register set
@@ -126,7 +160,7 @@ tail(B, "b")
 e(f"s_mov_b32 m0, s{SM0}")
 
 print("// Generated by gen_panel_asm.py (same directory); see its docstring.")
-print("// The entry loop of one wave over one chunk of the FP32 panel multiply.")
+print("// One chunk of the FP32 panel multiply for one wave: barrier, staging, entry loop.")
 print("#pragma once")
 print("#define GCOO_PANEL_CHUNK_ASM \\")
 for i, x in enumerate(L):
