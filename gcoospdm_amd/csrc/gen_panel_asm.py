@@ -95,6 +95,117 @@ def tail(S, t):
     e(".Lend_%=:" if t == "b" else "s_branch .Lend_%=")
 
 
+# ---- first 16 entries: both sets loaded before the barrier, LDS reads of
+# groups of 4 rolled over two D buffers with partial lgkmcnt waits ----
+G_D = (64, 80)            # D buffers of the rolling phase
+G_AV = (96, 100)          # their address VGPRs
+SCNT = 76                 # valid entries of a partial group
+
+
+def group_entry(g, q):
+    """(SGPR of x, SGPR of the value) of entry q of group g (0..15 overall)."""
+    S = A if g < 2 else B
+    u = 4 * (g % 2) + q
+    return S[2 * u], S[2 * u + 1]
+
+
+def group_reads(g):
+    b = g % 2
+    for q in range(4):
+        sx, _ = group_entry(g, q)
+        e(f"s_lshr_b32 s{ST}, s{sx}, 8")
+        e(f"v_add_u32_e32 v{G_AV[b]+q}, s{ST}, %[base]")
+        e(f"ds_read_b128 v[{G_D[b]+4*q}:{G_D[b]+4*q+3}], v{G_AV[b]+q}")
+
+
+def group_entry_steps(g, q):
+    _, sv = group_entry(g, q)
+    b = g % 2
+    for c in range(4):
+        e(f"v_fma_f32 v{16*c}, s{sv}, v{G_D[b]+4*q+c}, v{16*c}")
+
+
+def group_steps_full(g):
+    sx, _ = group_entry(g, 0)
+    e(f"s_set_gpr_idx_on s{sx}, gpr_idx(SRC2,DST)")
+    for q in range(4):
+        if q:
+            e(f"s_set_gpr_idx_idx s{group_entry(g, q)[0]}")
+        group_entry_steps(g, q)
+    e("s_set_gpr_idx_off")
+
+
+def group_steps_guarded(g, tag):
+    # s{SCNT} = valid entries of this group (1..4)
+    sx, _ = group_entry(g, 0)
+    e(f"s_set_gpr_idx_on s{sx}, gpr_idx(SRC2,DST)")
+    group_entry_steps(g, 0)
+    for q in range(1, 4):
+        e(f"s_cmp_gt_u32 s{SCNT}, {q}")
+        e(f"s_cbranch_scc0 .Lgg{tag}_%=")
+        e(f"s_set_gpr_idx_idx s{group_entry(g, q)[0]}")
+        group_entry_steps(g, q)
+    e(f".Lgg{tag}_%=:")
+    e("s_set_gpr_idx_off")
+
+
+def group_reads_guarded(g, tag):
+    # s{SCNT} = valid entries of the group (1..3): reads for those only
+    b = g % 2
+    for q in range(4):
+        sx, _ = group_entry(g, q)
+        if q:
+            e(f"s_cmp_gt_u32 s{SCNT}, {q}")
+            e(f"s_cbranch_scc0 .L{tag}_%=")
+        e(f"s_lshr_b32 s{ST}, s{sx}, 8")
+        e(f"v_add_u32_e32 v{G_AV[b]+q}, s{ST}, %[base]")
+        e(f"ds_read_b128 v[{G_D[b]+4*q}:{G_D[b]+4*q+3}], v{G_AV[b]+q}")
+    e(f".L{tag}_%=:")
+
+
+def roll16():
+    """Entries 0..min(n,16)-1 in groups of 4 over two D buffers: the next
+    group's reads are issued before this group's steps, and lgkmcnt(4) waits
+    for this group only (no scalar load is outstanding here).  A last group
+    of 1..3 entries takes guarded reads and steps after lgkmcnt(0)."""
+    e(f"s_cmp_ge_u32 s{SREM}, 4")
+    e(f"s_cbranch_scc1 .Lf0_%=")
+    e(f"s_mov_b32 s{SCNT}, s{SREM}")
+    group_reads_guarded(0, "p0r")
+    e("s_waitcnt lgkmcnt(0)")
+    group_steps_guarded(0, "p0s")
+    e("s_branch .Lrdone_%=")
+    e(".Lf0_%=:")
+    group_reads(0)
+    for g in range(4):
+        # group g is full and its reads are in flight
+        if g == 3:
+            e("s_waitcnt lgkmcnt(0)")
+            group_steps_full(3)
+            break
+        e(f"s_cmp_ge_u32 s{SREM}, {4*g+8}")
+        e(f"s_cbranch_scc0 .Lq{g}_%=")
+        group_reads(g + 1)
+        e("s_waitcnt lgkmcnt(4)")
+        group_steps_full(g)
+        e(f"s_branch .Lf{g+1}_%=")
+        e(f".Lq{g}_%=:")
+        e(f"s_cmp_gt_u32 s{SREM}, {4*g+4}")
+        e(f"s_cbranch_scc0 .Lz{g}_%=")
+        e(f"s_sub_u32 s{SCNT}, s{SREM}, {4*g+4}")
+        group_reads_guarded(g + 1, f"p{g+1}r")
+        e("s_waitcnt lgkmcnt(0)")
+        group_steps_full(g)
+        group_steps_guarded(g + 1, f"p{g+1}s")
+        e("s_branch .Lrdone_%=")
+        e(f".Lz{g}_%=:")
+        e("s_waitcnt lgkmcnt(0)")
+        group_steps_full(g)
+        e("s_branch .Lrdone_%=")
+        e(f".Lf{g+1}_%=:")
+    e(".Lrdone_%=:")
+
+
 def batch(S, N, t, nxt):
     e(f".Lb{t}_%=:")
     e(f"s_cmp_lt_u32 s{SREM}, 8")
@@ -124,10 +235,13 @@ def batch(S, N, t, nxt):
 
 e(f"s_mov_b32 s{SM0}, m0")
 e(f"s_mov_b32 s{SREM}, %[n]")
-# the first batch's load goes out before the barrier
+# the first 16 entries' loads go out before the barrier
 e(f"s_cmp_eq_u32 s{SREM}, 0")
 e(f"s_cbranch_scc1 .Lnl_%=")
 e(f"s_load_dwordx16 s[{A[0]}:{A[15]}], %[lp], 0x0")
+e(f"s_cmp_le_u32 s{SREM}, 8")
+e(f"s_cbranch_scc1 .Lnl_%=")
+e(f"s_load_dwordx16 s[{B[0]}:{B[15]}], %[lp], 0x40")
 e(".Lnl_%=:")
 # the staging waves' loads of this chunk have landed, every wave is done
 # with the previous chunk
@@ -149,7 +263,21 @@ for i in range(15):
 e(".Lsd_%=:")
 e(f"s_cmp_eq_u32 s{SREM}, 0")
 e(f"s_cbranch_scc1 .Lend_%=")
-e(f"s_movk_i32 s{SOFF}, 0x40")
+e("s_waitcnt lgkmcnt(0)")
+e(f"s_cmp_ge_u32 s{SREM}, 12")
+e(f"s_cbranch_scc0 .Lpr2_%=")
+e("s_setprio 3")
+e(f"s_branch .Lpr_%=")
+e(".Lpr2_%=:")
+e("s_setprio 2")
+e(".Lpr_%=:")
+roll16()
+# entries from 16 on: the batch loop, first batch into A
+e(f"s_cmp_le_u32 s{SREM}, 16")
+e(f"s_cbranch_scc1 .Lend_%=")
+e(f"s_sub_u32 s{SREM}, s{SREM}, 16")
+e(f"s_load_dwordx16 s[{A[0]}:{A[15]}], %[lp], 0x80")
+e(f"s_movk_i32 s{SOFF}, 0xc0")
 e("s_waitcnt lgkmcnt(0)")
 batch(A, B, "a", None)      # falls through to the other register set
 batch(B, A, "b", "a")
@@ -167,7 +295,7 @@ for i, x in enumerate(L):
     end = " \\" if i + 1 < len(L) else ""
     print(f'    "{x}\\n\\t"{end}')
 print("#define GCOO_PANEL_CHUNK_CLOBBERS \\")
-regs = [f'"v{i}"' for i in range(64, 104)] + [f'"s{i}"' for i in range(40, 76)] + ['"scc"', '"memory"']
+regs = [f'"v{i}"' for i in range(64, 104)] + [f'"s{i}"' for i in range(40, 77)] + ['"scc"', '"memory"']
 for i in range(0, len(regs), 12):
     end = ", \\" if i + 12 < len(regs) else ""
     print("    " + ", ".join(regs[i:i+12]) + end)
