@@ -6,26 +6,33 @@ the chunk barrier, the staging of the next chunk and the entry loop.
     python3 gen_panel_asm.py > gcoo_panel_asm.h
 
 In order:
-  * the scalar load of the chunk's first 8 list entries, issued before the
-    barrier so that its latency passes while the wave waits there;
+  * the scalar loads of the chunk's first 16 list entries (two sets of 8),
+    issued before the barrier so that their latency passes while the wave
+    waits there;
   * the staging waves wait for their loads of this chunk (vmcnt(0)), then the
     workgroup barrier: chunk c is in LDS and every wave is done with the
     buffer of chunk c - 1;
   * the staging waves issue their %[nld] buffer-to-LDS loads of chunk c + 1
     into that buffer (unrolled; LDS destination %[dst] + 1 KB per load, D row
     offset %[soff] + %[rowb] per load, lane offset %[voff]);
-  * the entry loop, 8 entries per batch.  Entry u of a batch is two words in
-    SGPRs: x = (LDS byte offset of its D row << 8) | row slot, and the FP32
-    bits of its value.  Per batch: 8 LDS reads of the entries' D rows (16
-    bytes per lane); the wait for them (LDS only); the scalar load of the next
-    batch's 8 entries, issued now so that it runs under this batch's steps
-    (issued any earlier, the wait for the LDS reads would also wait for it:
-    both count in lgkmcnt); 8 steps in VGPR-index mode,
-    acc[slot] = fmaf(value, d, acc[slot]) for the 4 columns, the slot taken
-    from bits 7:0 of x by s_set_gpr_idx_on/_idx; the wait for the next
-    batch's entries.  Two register sets alternate between batches.  The last
-    1..7 entries take one guarded round.  Before each full batch the wave's
-    issue priority is set from the entries it has left (3 from 16 on, else 2).
+  * entries 0..15 in groups of 4 over two D buffers: the LDS reads of group
+    g + 1 are issued before the steps of group g, and lgkmcnt(4) waits for
+    group g only (no scalar load is outstanding then, and LDS reads complete
+    in order).  A last group of 1..3 entries reads and steps only its valid
+    entries, after lgkmcnt(0);
+  * entries from 16 on: the batch loop, 8 entries per batch.  Per batch: 8
+    LDS reads of the entries' D rows (16 bytes per lane); the wait for them
+    (LDS only); the scalar load of the next batch's 8 entries, issued now so
+    that it runs under this batch's steps (issued any earlier, the wait for
+    the LDS reads would also wait for it: both count in lgkmcnt); 8 steps;
+    the wait for the next batch's entries.  Two register sets alternate
+    between batches.  The last 1..7 entries take one guarded round.
+Entry u is two words in SGPRs: x = (LDS byte offset of its D row << 8) | row
+slot, and the FP32 bits of its value.  A step is acc[slot] = fmaf(value, d,
+acc[slot]) for the 4 columns, in VGPR-index mode (s_set_gpr_idx_on/_idx take
+the slot from bits 7:0 of x).  The wave's issue priority is set from the
+entries it has left (3 from 12 on before the groups and from 16 on before a
+batch, else 2).
 Entries are the wave's rows in row order with k ascending, so every row's
 chain is unchanged.
 
@@ -34,8 +41,9 @@ of entries; %[base] VGPR, this lane's LDS byte address of the chunk buffer;
 %[stg] SGPR, 1 on a staging wave; %[nld], %[dst], %[soff], %[rowb] SGPRs and
 %[voff] VGPR as above; %[rsrc] the buffer resource of D (SGPR quad); the
 accumulators v0..v63 (component c of row slot q in v[16c + q]).  Scratch:
-v64..v103 and s40..s75.  M0 is saved and restored.  The loop reads up to 7
-entries past the end of the list, which the list capacity covers.  Every
+v64..v103 and s40..s76.  M0 is saved and restored.  The scalar loads read up
+to 7 entries past the end of the list, which the list capacity covers; no LDS
+read is issued for an entry past the end.  Every
 wave runs the body exactly once per chunk (one barrier each).
 """
 A = list(range(40, 56))
