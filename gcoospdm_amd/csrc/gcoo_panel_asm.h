@@ -1,977 +1,2066 @@
 // Generated by gen_panel_asm.py (same directory); see its docstring.
-// One chunk of the FP32 panel multiply for one wave: barrier, staging, entry loop.
+// A segment of chunks of the FP32 panel multiply for one wave: per chunk the
+// barrier, the next chunk's entry loads and staging, and the entry loop.
 #pragma once
-#define GCOO_PANEL_CHUNK_ASM \
-    "s_mov_b32 s75, m0\n\t" \
-    "s_mov_b32 s73, %[n]\n\t" \
-    "s_cmp_eq_u32 s73, 0\n\t" \
-    "s_cbranch_scc1 .Lnl_%=\n\t" \
-    "s_load_dwordx16 s[40:55], %[lp], 0x0\n\t" \
-    "s_cmp_le_u32 s73, 8\n\t" \
-    "s_cbranch_scc1 .Lnl_%=\n\t" \
-    "s_load_dwordx16 s[56:71], %[lp], 0x40\n\t" \
-    ".Lnl_%=:\n\t" \
-    "s_cmp_eq_u32 %[stg], 0\n\t" \
-    "s_cbranch_scc1 .Lsw_%=\n\t" \
+#define GCOO_PANEL_SEGMENT_ASM \
+    "s_mov_b32 s28, m0\n\t" \
+    "s_nop 4\n\t" \
+    "v_readlane_b32 s29, %[vpbo], %[sli]\n\t" \
+    "v_readlane_b32 s30, %[vpn], %[sli]\n\t" \
+    "s_cmp_eq_u32 %[par], 0\n\t" \
+    "s_cbranch_scc0 .Lo1_%=\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lc0_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lc0_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s24\n\t" \
+    "s_branch .Lc0_%=\n\t" \
+    ".Lo1_%=:\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lc1_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lc1_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s24\n\t" \
+    "s_branch .Lc1_%=\n\t" \
+    ".Lc0_%=:\n\t" \
+    "s_mov_b32 s25, s30\n\t" \
+    "s_add_u32 s26, s29, 0x80\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lw0_%=\n\t" \
     "s_waitcnt vmcnt(0)\n\t" \
-    ".Lsw_%=:\n\t" \
+    ".Lw0_%=:\n\t" \
     "s_barrier\n\t" \
-    "s_mov_b32 s74, %[soff]\n\t" \
-    "s_mov_b32 s72, %[dst]\n\t" \
-    "s_cmp_le_u32 %[nld], 0\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 1\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 2\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 3\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 4\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 5\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 6\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 7\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 8\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 9\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 10\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 11\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 12\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 13\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    "s_cmp_le_u32 %[nld], 14\n\t" \
-    "s_cbranch_scc1 .Lsd_%=\n\t" \
-    "s_mov_b32 m0, s72\n\t" \
-    "s_add_u32 s72, s72, 0x400\n\t" \
-    "buffer_load_dwordx4 %[voff], %[rsrc], s74 offen lds\n\t" \
-    "s_add_u32 s74, s74, %[rowb]\n\t" \
-    ".Lsd_%=:\n\t" \
-    "s_cmp_eq_u32 s73, 0\n\t" \
-    "s_cbranch_scc1 .Lend_%=\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_cmp_ge_u32 s73, 12\n\t" \
-    "s_cbranch_scc0 .Lpr2_%=\n\t" \
+    "s_add_u32 s24, %[sli], 1\n\t" \
+    "s_cmp_lt_u32 s24, %[liend]\n\t" \
+    "s_cbranch_scc0 .Lnp0_%=\n\t" \
+    "v_readlane_b32 s29, %[vpbo], s24\n\t" \
+    "v_readlane_b32 s30, %[vpn], s24\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lnp0_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lnp0_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s24\n\t" \
+    ".Lnp0_%=:\n\t" \
+    "s_cmp_le_u32 %[sso], %[lim]\n\t" \
+    "s_cbranch_scc0 .Lsg0_%=\n\t" \
+    "s_add_u32 m0, %[sdst], 0x12c00\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsd0_%=\n\t" \
+    ".Lsg0_%=:\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lsd0_%=\n\t" \
+    "s_add_u32 m0, %[sdst], 0x12c00\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    ".Lsd0_%=:\n\t" \
+    "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lpr20_%=\n\t" \
     "s_setprio 3\n\t" \
-    "s_branch .Lpr_%=\n\t" \
-    ".Lpr2_%=:\n\t" \
+    "s_branch .Lpr0_%=\n\t" \
+    ".Lpr20_%=:\n\t" \
     "s_setprio 2\n\t" \
-    ".Lpr_%=:\n\t" \
-    "s_cmp_ge_u32 s73, 4\n\t" \
-    "s_cbranch_scc1 .Lf0_%=\n\t" \
-    "s_mov_b32 s76, s73\n\t" \
-    "s_lshr_b32 s72, s40, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
+    ".Lpr0_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 4\n\t" \
+    "s_cbranch_scc1 .Lf00_%=\n\t" \
+    "s_mov_b32 s27, s25\n\t" \
+    "s_lshr_b32 s24, s36, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lp0r_%=\n\t" \
-    "s_lshr_b32 s72, s42, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "s_lshr_b32 s24, s38, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lp0r_%=\n\t" \
-    "s_lshr_b32 s72, s44, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "s_lshr_b32 s24, s40, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lp0r_%=\n\t" \
-    "s_lshr_b32 s72, s46, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "s_lshr_b32 s24, s42, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
-    ".Lp0r_%=:\n\t" \
+    ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s40, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s41, v64, v0\n\t" \
-    "v_fma_f32 v16, s41, v65, v16\n\t" \
-    "v_fma_f32 v32, s41, v66, v32\n\t" \
-    "v_fma_f32 v48, s41, v67, v48\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lggp0s_%=\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v64, v0\n\t" \
+    "v_fma_f32 v16, s37, v65, v16\n\t" \
+    "v_fma_f32 v32, s37, v66, v32\n\t" \
+    "v_fma_f32 v48, s37, v67, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v68, v0\n\t" \
+    "v_fma_f32 v16, s39, v69, v16\n\t" \
+    "v_fma_f32 v32, s39, v70, v32\n\t" \
+    "v_fma_f32 v48, s39, v71, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v72, v0\n\t" \
+    "v_fma_f32 v16, s41, v73, v16\n\t" \
+    "v_fma_f32 v32, s41, v74, v32\n\t" \
+    "v_fma_f32 v48, s41, v75, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
     "s_set_gpr_idx_idx s42\n\t" \
-    "v_fma_f32 v0, s43, v68, v0\n\t" \
-    "v_fma_f32 v16, s43, v69, v16\n\t" \
-    "v_fma_f32 v32, s43, v70, v32\n\t" \
-    "v_fma_f32 v48, s43, v71, v48\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lggp0s_%=\n\t" \
-    "s_set_gpr_idx_idx s44\n\t" \
-    "v_fma_f32 v0, s45, v72, v0\n\t" \
-    "v_fma_f32 v16, s45, v73, v16\n\t" \
-    "v_fma_f32 v32, s45, v74, v32\n\t" \
-    "v_fma_f32 v48, s45, v75, v48\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lggp0s_%=\n\t" \
-    "s_set_gpr_idx_idx s46\n\t" \
-    "v_fma_f32 v0, s47, v76, v0\n\t" \
-    "v_fma_f32 v16, s47, v77, v16\n\t" \
-    "v_fma_f32 v32, s47, v78, v32\n\t" \
-    "v_fma_f32 v48, s47, v79, v48\n\t" \
-    ".Lggp0s_%=:\n\t" \
+    "v_fma_f32 v0, s43, v76, v0\n\t" \
+    "v_fma_f32 v16, s43, v77, v16\n\t" \
+    "v_fma_f32 v32, s43, v78, v32\n\t" \
+    "v_fma_f32 v48, s43, v79, v48\n\t" \
+    ".Lggp0s0_%=:\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lf0_%=:\n\t" \
-    "s_lshr_b32 s72, s40, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf00_%=:\n\t" \
+    "s_lshr_b32 s24, s36, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s72, s42, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
+    "s_lshr_b32 s24, s38, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s72, s44, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
+    "s_lshr_b32 s24, s40, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s72, s46, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
+    "s_lshr_b32 s24, s42, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
-    "s_cmp_ge_u32 s73, 8\n\t" \
-    "s_cbranch_scc0 .Lq0_%=\n\t" \
-    "s_lshr_b32 s72, s48, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
+    "s_cmp_ge_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lq00_%=\n\t" \
+    "s_lshr_b32 s24, s44, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s72, s50, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
+    "s_lshr_b32 s24, s46, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s72, s52, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
+    "s_lshr_b32 s24, s48, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s72, s54, 8\n\t" \
-    "v_add_u32_e32 v103, s72, %[base]\n\t" \
+    "s_lshr_b32 s24, s50, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
-    "s_set_gpr_idx_on s40, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s41, v64, v0\n\t" \
-    "v_fma_f32 v16, s41, v65, v16\n\t" \
-    "v_fma_f32 v32, s41, v66, v32\n\t" \
-    "v_fma_f32 v48, s41, v67, v48\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v64, v0\n\t" \
+    "v_fma_f32 v16, s37, v65, v16\n\t" \
+    "v_fma_f32 v32, s37, v66, v32\n\t" \
+    "v_fma_f32 v48, s37, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v68, v0\n\t" \
+    "v_fma_f32 v16, s39, v69, v16\n\t" \
+    "v_fma_f32 v32, s39, v70, v32\n\t" \
+    "v_fma_f32 v48, s39, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v72, v0\n\t" \
+    "v_fma_f32 v16, s41, v73, v16\n\t" \
+    "v_fma_f32 v32, s41, v74, v32\n\t" \
+    "v_fma_f32 v48, s41, v75, v48\n\t" \
     "s_set_gpr_idx_idx s42\n\t" \
-    "v_fma_f32 v0, s43, v68, v0\n\t" \
-    "v_fma_f32 v16, s43, v69, v16\n\t" \
-    "v_fma_f32 v32, s43, v70, v32\n\t" \
-    "v_fma_f32 v48, s43, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s44\n\t" \
-    "v_fma_f32 v0, s45, v72, v0\n\t" \
-    "v_fma_f32 v16, s45, v73, v16\n\t" \
-    "v_fma_f32 v32, s45, v74, v32\n\t" \
-    "v_fma_f32 v48, s45, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s46\n\t" \
-    "v_fma_f32 v0, s47, v76, v0\n\t" \
-    "v_fma_f32 v16, s47, v77, v16\n\t" \
-    "v_fma_f32 v32, s47, v78, v32\n\t" \
-    "v_fma_f32 v48, s47, v79, v48\n\t" \
+    "v_fma_f32 v0, s43, v76, v0\n\t" \
+    "v_fma_f32 v16, s43, v77, v16\n\t" \
+    "v_fma_f32 v32, s43, v78, v32\n\t" \
+    "v_fma_f32 v48, s43, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lf1_%=\n\t" \
-    ".Lq0_%=:\n\t" \
-    "s_cmp_gt_u32 s73, 4\n\t" \
-    "s_cbranch_scc0 .Lz0_%=\n\t" \
-    "s_sub_u32 s76, s73, 4\n\t" \
-    "s_lshr_b32 s72, s48, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
+    "s_branch .Lf10_%=\n\t" \
+    ".Lq00_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lz00_%=\n\t" \
+    "s_sub_u32 s27, s25, 4\n\t" \
+    "s_lshr_b32 s24, s44, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lp1r_%=\n\t" \
-    "s_lshr_b32 s72, s50, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "s_lshr_b32 s24, s46, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lp1r_%=\n\t" \
-    "s_lshr_b32 s72, s52, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "s_lshr_b32 s24, s48, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lp1r_%=\n\t" \
-    "s_lshr_b32 s72, s54, 8\n\t" \
-    "v_add_u32_e32 v103, s72, %[base]\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "s_lshr_b32 s24, s50, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
-    ".Lp1r_%=:\n\t" \
+    ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s40, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s41, v64, v0\n\t" \
-    "v_fma_f32 v16, s41, v65, v16\n\t" \
-    "v_fma_f32 v32, s41, v66, v32\n\t" \
-    "v_fma_f32 v48, s41, v67, v48\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v64, v0\n\t" \
+    "v_fma_f32 v16, s37, v65, v16\n\t" \
+    "v_fma_f32 v32, s37, v66, v32\n\t" \
+    "v_fma_f32 v48, s37, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v68, v0\n\t" \
+    "v_fma_f32 v16, s39, v69, v16\n\t" \
+    "v_fma_f32 v32, s39, v70, v32\n\t" \
+    "v_fma_f32 v48, s39, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v72, v0\n\t" \
+    "v_fma_f32 v16, s41, v73, v16\n\t" \
+    "v_fma_f32 v32, s41, v74, v32\n\t" \
+    "v_fma_f32 v48, s41, v75, v48\n\t" \
     "s_set_gpr_idx_idx s42\n\t" \
-    "v_fma_f32 v0, s43, v68, v0\n\t" \
-    "v_fma_f32 v16, s43, v69, v16\n\t" \
-    "v_fma_f32 v32, s43, v70, v32\n\t" \
-    "v_fma_f32 v48, s43, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s44\n\t" \
-    "v_fma_f32 v0, s45, v72, v0\n\t" \
-    "v_fma_f32 v16, s45, v73, v16\n\t" \
-    "v_fma_f32 v32, s45, v74, v32\n\t" \
-    "v_fma_f32 v48, s45, v75, v48\n\t" \
+    "v_fma_f32 v0, s43, v76, v0\n\t" \
+    "v_fma_f32 v16, s43, v77, v16\n\t" \
+    "v_fma_f32 v32, s43, v78, v32\n\t" \
+    "v_fma_f32 v48, s43, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v80, v0\n\t" \
+    "v_fma_f32 v16, s45, v81, v16\n\t" \
+    "v_fma_f32 v32, s45, v82, v32\n\t" \
+    "v_fma_f32 v48, s45, v83, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
     "s_set_gpr_idx_idx s46\n\t" \
-    "v_fma_f32 v0, s47, v76, v0\n\t" \
-    "v_fma_f32 v16, s47, v77, v16\n\t" \
-    "v_fma_f32 v32, s47, v78, v32\n\t" \
-    "v_fma_f32 v48, s47, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_set_gpr_idx_on s48, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s49, v80, v0\n\t" \
-    "v_fma_f32 v16, s49, v81, v16\n\t" \
-    "v_fma_f32 v32, s49, v82, v32\n\t" \
-    "v_fma_f32 v48, s49, v83, v48\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lggp1s_%=\n\t" \
-    "s_set_gpr_idx_idx s50\n\t" \
-    "v_fma_f32 v0, s51, v84, v0\n\t" \
-    "v_fma_f32 v16, s51, v85, v16\n\t" \
-    "v_fma_f32 v32, s51, v86, v32\n\t" \
-    "v_fma_f32 v48, s51, v87, v48\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lggp1s_%=\n\t" \
-    "s_set_gpr_idx_idx s52\n\t" \
-    "v_fma_f32 v0, s53, v88, v0\n\t" \
-    "v_fma_f32 v16, s53, v89, v16\n\t" \
-    "v_fma_f32 v32, s53, v90, v32\n\t" \
-    "v_fma_f32 v48, s53, v91, v48\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lggp1s_%=\n\t" \
-    "s_set_gpr_idx_idx s54\n\t" \
-    "v_fma_f32 v0, s55, v92, v0\n\t" \
-    "v_fma_f32 v16, s55, v93, v16\n\t" \
-    "v_fma_f32 v32, s55, v94, v32\n\t" \
-    "v_fma_f32 v48, s55, v95, v48\n\t" \
-    ".Lggp1s_%=:\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lz0_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s40, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s41, v64, v0\n\t" \
-    "v_fma_f32 v16, s41, v65, v16\n\t" \
-    "v_fma_f32 v32, s41, v66, v32\n\t" \
-    "v_fma_f32 v48, s41, v67, v48\n\t" \
-    "s_set_gpr_idx_idx s42\n\t" \
-    "v_fma_f32 v0, s43, v68, v0\n\t" \
-    "v_fma_f32 v16, s43, v69, v16\n\t" \
-    "v_fma_f32 v32, s43, v70, v32\n\t" \
-    "v_fma_f32 v48, s43, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s44\n\t" \
-    "v_fma_f32 v0, s45, v72, v0\n\t" \
-    "v_fma_f32 v16, s45, v73, v16\n\t" \
-    "v_fma_f32 v32, s45, v74, v32\n\t" \
-    "v_fma_f32 v48, s45, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s46\n\t" \
-    "v_fma_f32 v0, s47, v76, v0\n\t" \
-    "v_fma_f32 v16, s47, v77, v16\n\t" \
-    "v_fma_f32 v32, s47, v78, v32\n\t" \
-    "v_fma_f32 v48, s47, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lf1_%=:\n\t" \
-    "s_cmp_ge_u32 s73, 12\n\t" \
-    "s_cbranch_scc0 .Lq1_%=\n\t" \
-    "s_lshr_b32 s72, s56, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
-    "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s72, s58, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
-    "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s72, s60, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s72, s62, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "s_set_gpr_idx_on s48, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s49, v80, v0\n\t" \
-    "v_fma_f32 v16, s49, v81, v16\n\t" \
-    "v_fma_f32 v32, s49, v82, v32\n\t" \
-    "v_fma_f32 v48, s49, v83, v48\n\t" \
-    "s_set_gpr_idx_idx s50\n\t" \
-    "v_fma_f32 v0, s51, v84, v0\n\t" \
-    "v_fma_f32 v16, s51, v85, v16\n\t" \
-    "v_fma_f32 v32, s51, v86, v32\n\t" \
-    "v_fma_f32 v48, s51, v87, v48\n\t" \
-    "s_set_gpr_idx_idx s52\n\t" \
-    "v_fma_f32 v0, s53, v88, v0\n\t" \
-    "v_fma_f32 v16, s53, v89, v16\n\t" \
-    "v_fma_f32 v32, s53, v90, v32\n\t" \
-    "v_fma_f32 v48, s53, v91, v48\n\t" \
-    "s_set_gpr_idx_idx s54\n\t" \
-    "v_fma_f32 v0, s55, v92, v0\n\t" \
-    "v_fma_f32 v16, s55, v93, v16\n\t" \
-    "v_fma_f32 v32, s55, v94, v32\n\t" \
-    "v_fma_f32 v48, s55, v95, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lf2_%=\n\t" \
-    ".Lq1_%=:\n\t" \
-    "s_cmp_gt_u32 s73, 8\n\t" \
-    "s_cbranch_scc0 .Lz1_%=\n\t" \
-    "s_sub_u32 s76, s73, 8\n\t" \
-    "s_lshr_b32 s72, s56, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
-    "ds_read_b128 v[64:67], v96\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lp2r_%=\n\t" \
-    "s_lshr_b32 s72, s58, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
-    "ds_read_b128 v[68:71], v97\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lp2r_%=\n\t" \
-    "s_lshr_b32 s72, s60, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lp2r_%=\n\t" \
-    "s_lshr_b32 s72, s62, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    ".Lp2r_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s48, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s49, v80, v0\n\t" \
-    "v_fma_f32 v16, s49, v81, v16\n\t" \
-    "v_fma_f32 v32, s49, v82, v32\n\t" \
-    "v_fma_f32 v48, s49, v83, v48\n\t" \
-    "s_set_gpr_idx_idx s50\n\t" \
-    "v_fma_f32 v0, s51, v84, v0\n\t" \
-    "v_fma_f32 v16, s51, v85, v16\n\t" \
-    "v_fma_f32 v32, s51, v86, v32\n\t" \
-    "v_fma_f32 v48, s51, v87, v48\n\t" \
-    "s_set_gpr_idx_idx s52\n\t" \
-    "v_fma_f32 v0, s53, v88, v0\n\t" \
-    "v_fma_f32 v16, s53, v89, v16\n\t" \
-    "v_fma_f32 v32, s53, v90, v32\n\t" \
-    "v_fma_f32 v48, s53, v91, v48\n\t" \
-    "s_set_gpr_idx_idx s54\n\t" \
-    "v_fma_f32 v0, s55, v92, v0\n\t" \
-    "v_fma_f32 v16, s55, v93, v16\n\t" \
-    "v_fma_f32 v32, s55, v94, v32\n\t" \
-    "v_fma_f32 v48, s55, v95, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_set_gpr_idx_on s56, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s57, v64, v0\n\t" \
-    "v_fma_f32 v16, s57, v65, v16\n\t" \
-    "v_fma_f32 v32, s57, v66, v32\n\t" \
-    "v_fma_f32 v48, s57, v67, v48\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lggp2s_%=\n\t" \
-    "s_set_gpr_idx_idx s58\n\t" \
-    "v_fma_f32 v0, s59, v68, v0\n\t" \
-    "v_fma_f32 v16, s59, v69, v16\n\t" \
-    "v_fma_f32 v32, s59, v70, v32\n\t" \
-    "v_fma_f32 v48, s59, v71, v48\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lggp2s_%=\n\t" \
-    "s_set_gpr_idx_idx s60\n\t" \
-    "v_fma_f32 v0, s61, v72, v0\n\t" \
-    "v_fma_f32 v16, s61, v73, v16\n\t" \
-    "v_fma_f32 v32, s61, v74, v32\n\t" \
-    "v_fma_f32 v48, s61, v75, v48\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lggp2s_%=\n\t" \
-    "s_set_gpr_idx_idx s62\n\t" \
-    "v_fma_f32 v0, s63, v76, v0\n\t" \
-    "v_fma_f32 v16, s63, v77, v16\n\t" \
-    "v_fma_f32 v32, s63, v78, v32\n\t" \
-    "v_fma_f32 v48, s63, v79, v48\n\t" \
-    ".Lggp2s_%=:\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lz1_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s48, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s49, v80, v0\n\t" \
-    "v_fma_f32 v16, s49, v81, v16\n\t" \
-    "v_fma_f32 v32, s49, v82, v32\n\t" \
-    "v_fma_f32 v48, s49, v83, v48\n\t" \
-    "s_set_gpr_idx_idx s50\n\t" \
-    "v_fma_f32 v0, s51, v84, v0\n\t" \
-    "v_fma_f32 v16, s51, v85, v16\n\t" \
-    "v_fma_f32 v32, s51, v86, v32\n\t" \
-    "v_fma_f32 v48, s51, v87, v48\n\t" \
-    "s_set_gpr_idx_idx s52\n\t" \
-    "v_fma_f32 v0, s53, v88, v0\n\t" \
-    "v_fma_f32 v16, s53, v89, v16\n\t" \
-    "v_fma_f32 v32, s53, v90, v32\n\t" \
-    "v_fma_f32 v48, s53, v91, v48\n\t" \
-    "s_set_gpr_idx_idx s54\n\t" \
-    "v_fma_f32 v0, s55, v92, v0\n\t" \
-    "v_fma_f32 v16, s55, v93, v16\n\t" \
-    "v_fma_f32 v32, s55, v94, v32\n\t" \
-    "v_fma_f32 v48, s55, v95, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lf2_%=:\n\t" \
-    "s_cmp_ge_u32 s73, 16\n\t" \
-    "s_cbranch_scc0 .Lq2_%=\n\t" \
-    "s_lshr_b32 s72, s64, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s72, s66, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s72, s68, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s72, s70, 8\n\t" \
-    "v_add_u32_e32 v103, s72, %[base]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "s_set_gpr_idx_on s56, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s57, v64, v0\n\t" \
-    "v_fma_f32 v16, s57, v65, v16\n\t" \
-    "v_fma_f32 v32, s57, v66, v32\n\t" \
-    "v_fma_f32 v48, s57, v67, v48\n\t" \
-    "s_set_gpr_idx_idx s58\n\t" \
-    "v_fma_f32 v0, s59, v68, v0\n\t" \
-    "v_fma_f32 v16, s59, v69, v16\n\t" \
-    "v_fma_f32 v32, s59, v70, v32\n\t" \
-    "v_fma_f32 v48, s59, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s60\n\t" \
-    "v_fma_f32 v0, s61, v72, v0\n\t" \
-    "v_fma_f32 v16, s61, v73, v16\n\t" \
-    "v_fma_f32 v32, s61, v74, v32\n\t" \
-    "v_fma_f32 v48, s61, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s62\n\t" \
-    "v_fma_f32 v0, s63, v76, v0\n\t" \
-    "v_fma_f32 v16, s63, v77, v16\n\t" \
-    "v_fma_f32 v32, s63, v78, v32\n\t" \
-    "v_fma_f32 v48, s63, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lf3_%=\n\t" \
-    ".Lq2_%=:\n\t" \
-    "s_cmp_gt_u32 s73, 12\n\t" \
-    "s_cbranch_scc0 .Lz2_%=\n\t" \
-    "s_sub_u32 s76, s73, 12\n\t" \
-    "s_lshr_b32 s72, s64, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lp3r_%=\n\t" \
-    "s_lshr_b32 s72, s66, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lp3r_%=\n\t" \
-    "s_lshr_b32 s72, s68, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lp3r_%=\n\t" \
-    "s_lshr_b32 s72, s70, 8\n\t" \
-    "v_add_u32_e32 v103, s72, %[base]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
-    ".Lp3r_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s56, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s57, v64, v0\n\t" \
-    "v_fma_f32 v16, s57, v65, v16\n\t" \
-    "v_fma_f32 v32, s57, v66, v32\n\t" \
-    "v_fma_f32 v48, s57, v67, v48\n\t" \
-    "s_set_gpr_idx_idx s58\n\t" \
-    "v_fma_f32 v0, s59, v68, v0\n\t" \
-    "v_fma_f32 v16, s59, v69, v16\n\t" \
-    "v_fma_f32 v32, s59, v70, v32\n\t" \
-    "v_fma_f32 v48, s59, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s60\n\t" \
-    "v_fma_f32 v0, s61, v72, v0\n\t" \
-    "v_fma_f32 v16, s61, v73, v16\n\t" \
-    "v_fma_f32 v32, s61, v74, v32\n\t" \
-    "v_fma_f32 v48, s61, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s62\n\t" \
-    "v_fma_f32 v0, s63, v76, v0\n\t" \
-    "v_fma_f32 v16, s63, v77, v16\n\t" \
-    "v_fma_f32 v32, s63, v78, v32\n\t" \
-    "v_fma_f32 v48, s63, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_set_gpr_idx_on s64, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s65, v80, v0\n\t" \
-    "v_fma_f32 v16, s65, v81, v16\n\t" \
-    "v_fma_f32 v32, s65, v82, v32\n\t" \
-    "v_fma_f32 v48, s65, v83, v48\n\t" \
-    "s_cmp_gt_u32 s76, 1\n\t" \
-    "s_cbranch_scc0 .Lggp3s_%=\n\t" \
-    "s_set_gpr_idx_idx s66\n\t" \
-    "v_fma_f32 v0, s67, v84, v0\n\t" \
-    "v_fma_f32 v16, s67, v85, v16\n\t" \
-    "v_fma_f32 v32, s67, v86, v32\n\t" \
-    "v_fma_f32 v48, s67, v87, v48\n\t" \
-    "s_cmp_gt_u32 s76, 2\n\t" \
-    "s_cbranch_scc0 .Lggp3s_%=\n\t" \
-    "s_set_gpr_idx_idx s68\n\t" \
-    "v_fma_f32 v0, s69, v88, v0\n\t" \
-    "v_fma_f32 v16, s69, v89, v16\n\t" \
-    "v_fma_f32 v32, s69, v90, v32\n\t" \
-    "v_fma_f32 v48, s69, v91, v48\n\t" \
-    "s_cmp_gt_u32 s76, 3\n\t" \
-    "s_cbranch_scc0 .Lggp3s_%=\n\t" \
-    "s_set_gpr_idx_idx s70\n\t" \
-    "v_fma_f32 v0, s71, v92, v0\n\t" \
-    "v_fma_f32 v16, s71, v93, v16\n\t" \
-    "v_fma_f32 v32, s71, v94, v32\n\t" \
-    "v_fma_f32 v48, s71, v95, v48\n\t" \
-    ".Lggp3s_%=:\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lz2_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s56, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s57, v64, v0\n\t" \
-    "v_fma_f32 v16, s57, v65, v16\n\t" \
-    "v_fma_f32 v32, s57, v66, v32\n\t" \
-    "v_fma_f32 v48, s57, v67, v48\n\t" \
-    "s_set_gpr_idx_idx s58\n\t" \
-    "v_fma_f32 v0, s59, v68, v0\n\t" \
-    "v_fma_f32 v16, s59, v69, v16\n\t" \
-    "v_fma_f32 v32, s59, v70, v32\n\t" \
-    "v_fma_f32 v48, s59, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s60\n\t" \
-    "v_fma_f32 v0, s61, v72, v0\n\t" \
-    "v_fma_f32 v16, s61, v73, v16\n\t" \
-    "v_fma_f32 v32, s61, v74, v32\n\t" \
-    "v_fma_f32 v48, s61, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s62\n\t" \
-    "v_fma_f32 v0, s63, v76, v0\n\t" \
-    "v_fma_f32 v16, s63, v77, v16\n\t" \
-    "v_fma_f32 v32, s63, v78, v32\n\t" \
-    "v_fma_f32 v48, s63, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lrdone_%=\n\t" \
-    ".Lf3_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s64, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s65, v80, v0\n\t" \
-    "v_fma_f32 v16, s65, v81, v16\n\t" \
-    "v_fma_f32 v32, s65, v82, v32\n\t" \
-    "v_fma_f32 v48, s65, v83, v48\n\t" \
-    "s_set_gpr_idx_idx s66\n\t" \
-    "v_fma_f32 v0, s67, v84, v0\n\t" \
-    "v_fma_f32 v16, s67, v85, v16\n\t" \
-    "v_fma_f32 v32, s67, v86, v32\n\t" \
-    "v_fma_f32 v48, s67, v87, v48\n\t" \
-    "s_set_gpr_idx_idx s68\n\t" \
-    "v_fma_f32 v0, s69, v88, v0\n\t" \
-    "v_fma_f32 v16, s69, v89, v16\n\t" \
-    "v_fma_f32 v32, s69, v90, v32\n\t" \
-    "v_fma_f32 v48, s69, v91, v48\n\t" \
-    "s_set_gpr_idx_idx s70\n\t" \
-    "v_fma_f32 v0, s71, v92, v0\n\t" \
-    "v_fma_f32 v16, s71, v93, v16\n\t" \
-    "v_fma_f32 v32, s71, v94, v32\n\t" \
-    "v_fma_f32 v48, s71, v95, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    ".Lrdone_%=:\n\t" \
-    "s_cmp_le_u32 s73, 16\n\t" \
-    "s_cbranch_scc1 .Lend_%=\n\t" \
-    "s_sub_u32 s73, s73, 16\n\t" \
-    "s_load_dwordx16 s[40:55], %[lp], 0x80\n\t" \
-    "s_movk_i32 s74, 0xc0\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    ".Lba_%=:\n\t" \
-    "s_cmp_lt_u32 s73, 8\n\t" \
-    "s_cbranch_scc1 .Lta_%=\n\t" \
-    "s_cmp_ge_u32 s73, 16\n\t" \
-    "s_cbranch_scc0 .Lpa_%=\n\t" \
-    "s_setprio 3\n\t" \
-    "s_branch .Lqa_%=\n\t" \
-    ".Lpa_%=:\n\t" \
-    "s_setprio 2\n\t" \
-    ".Lqa_%=:\n\t" \
-    "s_lshr_b32 s72, s40, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
-    "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s72, s42, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
-    "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s72, s44, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s72, s46, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "s_lshr_b32 s72, s48, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s72, s50, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s72, s52, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s72, s54, 8\n\t" \
-    "v_add_u32_e32 v103, s72, %[base]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_cmp_le_u32 s73, 8\n\t" \
-    "s_cbranch_scc1 .Lna_%=\n\t" \
-    "s_load_dwordx16 s[56:71], %[lp], s74\n\t" \
-    ".Lna_%=:\n\t" \
-    "s_set_gpr_idx_on s40, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s41, v64, v0\n\t" \
-    "v_fma_f32 v16, s41, v65, v16\n\t" \
-    "v_fma_f32 v32, s41, v66, v32\n\t" \
-    "v_fma_f32 v48, s41, v67, v48\n\t" \
-    "s_set_gpr_idx_idx s42\n\t" \
-    "v_fma_f32 v0, s43, v68, v0\n\t" \
-    "v_fma_f32 v16, s43, v69, v16\n\t" \
-    "v_fma_f32 v32, s43, v70, v32\n\t" \
-    "v_fma_f32 v48, s43, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s44\n\t" \
-    "v_fma_f32 v0, s45, v72, v0\n\t" \
-    "v_fma_f32 v16, s45, v73, v16\n\t" \
-    "v_fma_f32 v32, s45, v74, v32\n\t" \
-    "v_fma_f32 v48, s45, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s46\n\t" \
-    "v_fma_f32 v0, s47, v76, v0\n\t" \
-    "v_fma_f32 v16, s47, v77, v16\n\t" \
-    "v_fma_f32 v32, s47, v78, v32\n\t" \
-    "v_fma_f32 v48, s47, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_set_gpr_idx_on s48, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s49, v80, v0\n\t" \
-    "v_fma_f32 v16, s49, v81, v16\n\t" \
-    "v_fma_f32 v32, s49, v82, v32\n\t" \
-    "v_fma_f32 v48, s49, v83, v48\n\t" \
-    "s_set_gpr_idx_idx s50\n\t" \
-    "v_fma_f32 v0, s51, v84, v0\n\t" \
-    "v_fma_f32 v16, s51, v85, v16\n\t" \
-    "v_fma_f32 v32, s51, v86, v32\n\t" \
-    "v_fma_f32 v48, s51, v87, v48\n\t" \
-    "s_set_gpr_idx_idx s52\n\t" \
-    "v_fma_f32 v0, s53, v88, v0\n\t" \
-    "v_fma_f32 v16, s53, v89, v16\n\t" \
-    "v_fma_f32 v32, s53, v90, v32\n\t" \
-    "v_fma_f32 v48, s53, v91, v48\n\t" \
-    "s_set_gpr_idx_idx s54\n\t" \
-    "v_fma_f32 v0, s55, v92, v0\n\t" \
-    "v_fma_f32 v16, s55, v93, v16\n\t" \
-    "v_fma_f32 v32, s55, v94, v32\n\t" \
-    "v_fma_f32 v48, s55, v95, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_sub_u32 s73, s73, 8\n\t" \
-    "s_add_u32 s74, s74, 64\n\t" \
-    "s_cmp_eq_u32 s73, 0\n\t" \
-    "s_cbranch_scc1 .Lend_%=\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    ".Lbb_%=:\n\t" \
-    "s_cmp_lt_u32 s73, 8\n\t" \
-    "s_cbranch_scc1 .Ltb_%=\n\t" \
-    "s_cmp_ge_u32 s73, 16\n\t" \
-    "s_cbranch_scc0 .Lpb_%=\n\t" \
-    "s_setprio 3\n\t" \
-    "s_branch .Lqb_%=\n\t" \
-    ".Lpb_%=:\n\t" \
-    "s_setprio 2\n\t" \
-    ".Lqb_%=:\n\t" \
-    "s_lshr_b32 s72, s56, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
-    "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s72, s58, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
-    "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s72, s60, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s72, s62, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "s_lshr_b32 s72, s64, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s72, s66, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s72, s68, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s72, s70, 8\n\t" \
-    "v_add_u32_e32 v103, s72, %[base]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_cmp_le_u32 s73, 8\n\t" \
-    "s_cbranch_scc1 .Lnb_%=\n\t" \
-    "s_load_dwordx16 s[40:55], %[lp], s74\n\t" \
-    ".Lnb_%=:\n\t" \
-    "s_set_gpr_idx_on s56, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s57, v64, v0\n\t" \
-    "v_fma_f32 v16, s57, v65, v16\n\t" \
-    "v_fma_f32 v32, s57, v66, v32\n\t" \
-    "v_fma_f32 v48, s57, v67, v48\n\t" \
-    "s_set_gpr_idx_idx s58\n\t" \
-    "v_fma_f32 v0, s59, v68, v0\n\t" \
-    "v_fma_f32 v16, s59, v69, v16\n\t" \
-    "v_fma_f32 v32, s59, v70, v32\n\t" \
-    "v_fma_f32 v48, s59, v71, v48\n\t" \
-    "s_set_gpr_idx_idx s60\n\t" \
-    "v_fma_f32 v0, s61, v72, v0\n\t" \
-    "v_fma_f32 v16, s61, v73, v16\n\t" \
-    "v_fma_f32 v32, s61, v74, v32\n\t" \
-    "v_fma_f32 v48, s61, v75, v48\n\t" \
-    "s_set_gpr_idx_idx s62\n\t" \
-    "v_fma_f32 v0, s63, v76, v0\n\t" \
-    "v_fma_f32 v16, s63, v77, v16\n\t" \
-    "v_fma_f32 v32, s63, v78, v32\n\t" \
-    "v_fma_f32 v48, s63, v79, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_set_gpr_idx_on s64, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s65, v80, v0\n\t" \
-    "v_fma_f32 v16, s65, v81, v16\n\t" \
-    "v_fma_f32 v32, s65, v82, v32\n\t" \
-    "v_fma_f32 v48, s65, v83, v48\n\t" \
-    "s_set_gpr_idx_idx s66\n\t" \
-    "v_fma_f32 v0, s67, v84, v0\n\t" \
-    "v_fma_f32 v16, s67, v85, v16\n\t" \
-    "v_fma_f32 v32, s67, v86, v32\n\t" \
-    "v_fma_f32 v48, s67, v87, v48\n\t" \
-    "s_set_gpr_idx_idx s68\n\t" \
-    "v_fma_f32 v0, s69, v88, v0\n\t" \
-    "v_fma_f32 v16, s69, v89, v16\n\t" \
-    "v_fma_f32 v32, s69, v90, v32\n\t" \
-    "v_fma_f32 v48, s69, v91, v48\n\t" \
-    "s_set_gpr_idx_idx s70\n\t" \
-    "v_fma_f32 v0, s71, v92, v0\n\t" \
-    "v_fma_f32 v16, s71, v93, v16\n\t" \
-    "v_fma_f32 v32, s71, v94, v32\n\t" \
-    "v_fma_f32 v48, s71, v95, v48\n\t" \
-    "s_set_gpr_idx_off\n\t" \
-    "s_sub_u32 s73, s73, 8\n\t" \
-    "s_add_u32 s74, s74, 64\n\t" \
-    "s_cmp_eq_u32 s73, 0\n\t" \
-    "s_cbranch_scc1 .Lend_%=\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_branch .Lba_%=\n\t" \
-    ".Lta_%=:\n\t" \
-    "s_lshr_b32 s72, s40, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
-    "ds_read_b128 v[64:67], v96\n\t" \
-    "s_cmp_gt_u32 s73, 1\n\t" \
-    "s_cbranch_scc0 .Lgwa_%=\n\t" \
-    "s_lshr_b32 s72, s42, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
-    "ds_read_b128 v[68:71], v97\n\t" \
-    "s_cmp_gt_u32 s73, 2\n\t" \
-    "s_cbranch_scc0 .Lgwa_%=\n\t" \
-    "s_lshr_b32 s72, s44, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "s_cmp_gt_u32 s73, 3\n\t" \
-    "s_cbranch_scc0 .Lgwa_%=\n\t" \
-    "s_lshr_b32 s72, s46, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "s_cmp_gt_u32 s73, 4\n\t" \
-    "s_cbranch_scc0 .Lgwa_%=\n\t" \
-    "s_lshr_b32 s72, s48, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "s_cmp_gt_u32 s73, 5\n\t" \
-    "s_cbranch_scc0 .Lgwa_%=\n\t" \
-    "s_lshr_b32 s72, s50, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "s_cmp_gt_u32 s73, 6\n\t" \
-    "s_cbranch_scc0 .Lgwa_%=\n\t" \
-    "s_lshr_b32 s72, s52, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    ".Lgwa_%=:\n\t" \
-    "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s40, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s41, v64, v0\n\t" \
-    "v_fma_f32 v16, s41, v65, v16\n\t" \
-    "v_fma_f32 v32, s41, v66, v32\n\t" \
-    "v_fma_f32 v48, s41, v67, v48\n\t" \
-    "s_cmp_gt_u32 s73, 1\n\t" \
-    "s_cbranch_scc0 .Lgea_%=\n\t" \
-    "s_set_gpr_idx_idx s42\n\t" \
-    "v_fma_f32 v0, s43, v68, v0\n\t" \
-    "v_fma_f32 v16, s43, v69, v16\n\t" \
-    "v_fma_f32 v32, s43, v70, v32\n\t" \
-    "v_fma_f32 v48, s43, v71, v48\n\t" \
-    "s_cmp_gt_u32 s73, 2\n\t" \
-    "s_cbranch_scc0 .Lgea_%=\n\t" \
-    "s_set_gpr_idx_idx s44\n\t" \
-    "v_fma_f32 v0, s45, v72, v0\n\t" \
-    "v_fma_f32 v16, s45, v73, v16\n\t" \
-    "v_fma_f32 v32, s45, v74, v32\n\t" \
-    "v_fma_f32 v48, s45, v75, v48\n\t" \
-    "s_cmp_gt_u32 s73, 3\n\t" \
-    "s_cbranch_scc0 .Lgea_%=\n\t" \
-    "s_set_gpr_idx_idx s46\n\t" \
-    "v_fma_f32 v0, s47, v76, v0\n\t" \
-    "v_fma_f32 v16, s47, v77, v16\n\t" \
-    "v_fma_f32 v32, s47, v78, v32\n\t" \
-    "v_fma_f32 v48, s47, v79, v48\n\t" \
-    "s_cmp_gt_u32 s73, 4\n\t" \
-    "s_cbranch_scc0 .Lgea_%=\n\t" \
+    "v_fma_f32 v0, s47, v84, v0\n\t" \
+    "v_fma_f32 v16, s47, v85, v16\n\t" \
+    "v_fma_f32 v32, s47, v86, v32\n\t" \
+    "v_fma_f32 v48, s47, v87, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
     "s_set_gpr_idx_idx s48\n\t" \
-    "v_fma_f32 v0, s49, v80, v0\n\t" \
-    "v_fma_f32 v16, s49, v81, v16\n\t" \
-    "v_fma_f32 v32, s49, v82, v32\n\t" \
-    "v_fma_f32 v48, s49, v83, v48\n\t" \
-    "s_cmp_gt_u32 s73, 5\n\t" \
-    "s_cbranch_scc0 .Lgea_%=\n\t" \
+    "v_fma_f32 v0, s49, v88, v0\n\t" \
+    "v_fma_f32 v16, s49, v89, v16\n\t" \
+    "v_fma_f32 v32, s49, v90, v32\n\t" \
+    "v_fma_f32 v48, s49, v91, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
     "s_set_gpr_idx_idx s50\n\t" \
-    "v_fma_f32 v0, s51, v84, v0\n\t" \
-    "v_fma_f32 v16, s51, v85, v16\n\t" \
-    "v_fma_f32 v32, s51, v86, v32\n\t" \
-    "v_fma_f32 v48, s51, v87, v48\n\t" \
-    "s_cmp_gt_u32 s73, 6\n\t" \
-    "s_cbranch_scc0 .Lgea_%=\n\t" \
-    "s_set_gpr_idx_idx s52\n\t" \
-    "v_fma_f32 v0, s53, v88, v0\n\t" \
-    "v_fma_f32 v16, s53, v89, v16\n\t" \
-    "v_fma_f32 v32, s53, v90, v32\n\t" \
-    "v_fma_f32 v48, s53, v91, v48\n\t" \
-    ".Lgea_%=:\n\t" \
+    "v_fma_f32 v0, s51, v92, v0\n\t" \
+    "v_fma_f32 v16, s51, v93, v16\n\t" \
+    "v_fma_f32 v32, s51, v94, v32\n\t" \
+    "v_fma_f32 v48, s51, v95, v48\n\t" \
+    ".Lggp1s0_%=:\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "s_branch .Lend_%=\n\t" \
-    ".Ltb_%=:\n\t" \
-    "s_lshr_b32 s72, s56, 8\n\t" \
-    "v_add_u32_e32 v96, s72, %[base]\n\t" \
-    "ds_read_b128 v[64:67], v96\n\t" \
-    "s_cmp_gt_u32 s73, 1\n\t" \
-    "s_cbranch_scc0 .Lgwb_%=\n\t" \
-    "s_lshr_b32 s72, s58, 8\n\t" \
-    "v_add_u32_e32 v97, s72, %[base]\n\t" \
-    "ds_read_b128 v[68:71], v97\n\t" \
-    "s_cmp_gt_u32 s73, 2\n\t" \
-    "s_cbranch_scc0 .Lgwb_%=\n\t" \
-    "s_lshr_b32 s72, s60, 8\n\t" \
-    "v_add_u32_e32 v98, s72, %[base]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "s_cmp_gt_u32 s73, 3\n\t" \
-    "s_cbranch_scc0 .Lgwb_%=\n\t" \
-    "s_lshr_b32 s72, s62, 8\n\t" \
-    "v_add_u32_e32 v99, s72, %[base]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "s_cmp_gt_u32 s73, 4\n\t" \
-    "s_cbranch_scc0 .Lgwb_%=\n\t" \
-    "s_lshr_b32 s72, s64, 8\n\t" \
-    "v_add_u32_e32 v100, s72, %[base]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "s_cmp_gt_u32 s73, 5\n\t" \
-    "s_cbranch_scc0 .Lgwb_%=\n\t" \
-    "s_lshr_b32 s72, s66, 8\n\t" \
-    "v_add_u32_e32 v101, s72, %[base]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "s_cmp_gt_u32 s73, 6\n\t" \
-    "s_cbranch_scc0 .Lgwb_%=\n\t" \
-    "s_lshr_b32 s72, s68, 8\n\t" \
-    "v_add_u32_e32 v102, s72, %[base]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    ".Lgwb_%=:\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz00_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "s_set_gpr_idx_on s56, gpr_idx(SRC2,DST)\n\t" \
-    "v_fma_f32 v0, s57, v64, v0\n\t" \
-    "v_fma_f32 v16, s57, v65, v16\n\t" \
-    "v_fma_f32 v32, s57, v66, v32\n\t" \
-    "v_fma_f32 v48, s57, v67, v48\n\t" \
-    "s_cmp_gt_u32 s73, 1\n\t" \
-    "s_cbranch_scc0 .Lgeb_%=\n\t" \
-    "s_set_gpr_idx_idx s58\n\t" \
-    "v_fma_f32 v0, s59, v68, v0\n\t" \
-    "v_fma_f32 v16, s59, v69, v16\n\t" \
-    "v_fma_f32 v32, s59, v70, v32\n\t" \
-    "v_fma_f32 v48, s59, v71, v48\n\t" \
-    "s_cmp_gt_u32 s73, 2\n\t" \
-    "s_cbranch_scc0 .Lgeb_%=\n\t" \
-    "s_set_gpr_idx_idx s60\n\t" \
-    "v_fma_f32 v0, s61, v72, v0\n\t" \
-    "v_fma_f32 v16, s61, v73, v16\n\t" \
-    "v_fma_f32 v32, s61, v74, v32\n\t" \
-    "v_fma_f32 v48, s61, v75, v48\n\t" \
-    "s_cmp_gt_u32 s73, 3\n\t" \
-    "s_cbranch_scc0 .Lgeb_%=\n\t" \
-    "s_set_gpr_idx_idx s62\n\t" \
-    "v_fma_f32 v0, s63, v76, v0\n\t" \
-    "v_fma_f32 v16, s63, v77, v16\n\t" \
-    "v_fma_f32 v32, s63, v78, v32\n\t" \
-    "v_fma_f32 v48, s63, v79, v48\n\t" \
-    "s_cmp_gt_u32 s73, 4\n\t" \
-    "s_cbranch_scc0 .Lgeb_%=\n\t" \
-    "s_set_gpr_idx_idx s64\n\t" \
-    "v_fma_f32 v0, s65, v80, v0\n\t" \
-    "v_fma_f32 v16, s65, v81, v16\n\t" \
-    "v_fma_f32 v32, s65, v82, v32\n\t" \
-    "v_fma_f32 v48, s65, v83, v48\n\t" \
-    "s_cmp_gt_u32 s73, 5\n\t" \
-    "s_cbranch_scc0 .Lgeb_%=\n\t" \
-    "s_set_gpr_idx_idx s66\n\t" \
-    "v_fma_f32 v0, s67, v84, v0\n\t" \
-    "v_fma_f32 v16, s67, v85, v16\n\t" \
-    "v_fma_f32 v32, s67, v86, v32\n\t" \
-    "v_fma_f32 v48, s67, v87, v48\n\t" \
-    "s_cmp_gt_u32 s73, 6\n\t" \
-    "s_cbranch_scc0 .Lgeb_%=\n\t" \
-    "s_set_gpr_idx_idx s68\n\t" \
-    "v_fma_f32 v0, s69, v88, v0\n\t" \
-    "v_fma_f32 v16, s69, v89, v16\n\t" \
-    "v_fma_f32 v32, s69, v90, v32\n\t" \
-    "v_fma_f32 v48, s69, v91, v48\n\t" \
-    ".Lgeb_%=:\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v64, v0\n\t" \
+    "v_fma_f32 v16, s37, v65, v16\n\t" \
+    "v_fma_f32 v32, s37, v66, v32\n\t" \
+    "v_fma_f32 v48, s37, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v68, v0\n\t" \
+    "v_fma_f32 v16, s39, v69, v16\n\t" \
+    "v_fma_f32 v32, s39, v70, v32\n\t" \
+    "v_fma_f32 v48, s39, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v72, v0\n\t" \
+    "v_fma_f32 v16, s41, v73, v16\n\t" \
+    "v_fma_f32 v32, s41, v74, v32\n\t" \
+    "v_fma_f32 v48, s41, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v76, v0\n\t" \
+    "v_fma_f32 v16, s43, v77, v16\n\t" \
+    "v_fma_f32 v32, s43, v78, v32\n\t" \
+    "v_fma_f32 v48, s43, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    ".Lend_%=:\n\t" \
-    "s_mov_b32 m0, s75\n\t"
-#define GCOO_PANEL_CHUNK_CLOBBERS \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf10_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lq10_%=\n\t" \
+    "s_lshr_b32 s24, s52, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s54, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s56, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s58, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v80, v0\n\t" \
+    "v_fma_f32 v16, s45, v81, v16\n\t" \
+    "v_fma_f32 v32, s45, v82, v32\n\t" \
+    "v_fma_f32 v48, s45, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v84, v0\n\t" \
+    "v_fma_f32 v16, s47, v85, v16\n\t" \
+    "v_fma_f32 v32, s47, v86, v32\n\t" \
+    "v_fma_f32 v48, s47, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v88, v0\n\t" \
+    "v_fma_f32 v16, s49, v89, v16\n\t" \
+    "v_fma_f32 v32, s49, v90, v32\n\t" \
+    "v_fma_f32 v48, s49, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v92, v0\n\t" \
+    "v_fma_f32 v16, s51, v93, v16\n\t" \
+    "v_fma_f32 v32, s51, v94, v32\n\t" \
+    "v_fma_f32 v48, s51, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf20_%=\n\t" \
+    ".Lq10_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lz10_%=\n\t" \
+    "s_sub_u32 s27, s25, 8\n\t" \
+    "s_lshr_b32 s24, s52, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "s_lshr_b32 s24, s54, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "s_lshr_b32 s24, s56, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "s_lshr_b32 s24, s58, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    ".Lp2r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v80, v0\n\t" \
+    "v_fma_f32 v16, s45, v81, v16\n\t" \
+    "v_fma_f32 v32, s45, v82, v32\n\t" \
+    "v_fma_f32 v48, s45, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v84, v0\n\t" \
+    "v_fma_f32 v16, s47, v85, v16\n\t" \
+    "v_fma_f32 v32, s47, v86, v32\n\t" \
+    "v_fma_f32 v48, s47, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v88, v0\n\t" \
+    "v_fma_f32 v16, s49, v89, v16\n\t" \
+    "v_fma_f32 v32, s49, v90, v32\n\t" \
+    "v_fma_f32 v48, s49, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v92, v0\n\t" \
+    "v_fma_f32 v16, s51, v93, v16\n\t" \
+    "v_fma_f32 v32, s51, v94, v32\n\t" \
+    "v_fma_f32 v48, s51, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v64, v0\n\t" \
+    "v_fma_f32 v16, s53, v65, v16\n\t" \
+    "v_fma_f32 v32, s53, v66, v32\n\t" \
+    "v_fma_f32 v48, s53, v67, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v68, v0\n\t" \
+    "v_fma_f32 v16, s55, v69, v16\n\t" \
+    "v_fma_f32 v32, s55, v70, v32\n\t" \
+    "v_fma_f32 v48, s55, v71, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v72, v0\n\t" \
+    "v_fma_f32 v16, s57, v73, v16\n\t" \
+    "v_fma_f32 v32, s57, v74, v32\n\t" \
+    "v_fma_f32 v48, s57, v75, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v76, v0\n\t" \
+    "v_fma_f32 v16, s59, v77, v16\n\t" \
+    "v_fma_f32 v32, s59, v78, v32\n\t" \
+    "v_fma_f32 v48, s59, v79, v48\n\t" \
+    ".Lggp2s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz10_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v80, v0\n\t" \
+    "v_fma_f32 v16, s45, v81, v16\n\t" \
+    "v_fma_f32 v32, s45, v82, v32\n\t" \
+    "v_fma_f32 v48, s45, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v84, v0\n\t" \
+    "v_fma_f32 v16, s47, v85, v16\n\t" \
+    "v_fma_f32 v32, s47, v86, v32\n\t" \
+    "v_fma_f32 v48, s47, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v88, v0\n\t" \
+    "v_fma_f32 v16, s49, v89, v16\n\t" \
+    "v_fma_f32 v32, s49, v90, v32\n\t" \
+    "v_fma_f32 v48, s49, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v92, v0\n\t" \
+    "v_fma_f32 v16, s51, v93, v16\n\t" \
+    "v_fma_f32 v32, s51, v94, v32\n\t" \
+    "v_fma_f32 v48, s51, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf20_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lq20_%=\n\t" \
+    "s_lshr_b32 s24, s60, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s62, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s64, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s66, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v64, v0\n\t" \
+    "v_fma_f32 v16, s53, v65, v16\n\t" \
+    "v_fma_f32 v32, s53, v66, v32\n\t" \
+    "v_fma_f32 v48, s53, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v68, v0\n\t" \
+    "v_fma_f32 v16, s55, v69, v16\n\t" \
+    "v_fma_f32 v32, s55, v70, v32\n\t" \
+    "v_fma_f32 v48, s55, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v72, v0\n\t" \
+    "v_fma_f32 v16, s57, v73, v16\n\t" \
+    "v_fma_f32 v32, s57, v74, v32\n\t" \
+    "v_fma_f32 v48, s57, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v76, v0\n\t" \
+    "v_fma_f32 v16, s59, v77, v16\n\t" \
+    "v_fma_f32 v32, s59, v78, v32\n\t" \
+    "v_fma_f32 v48, s59, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf30_%=\n\t" \
+    ".Lq20_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lz20_%=\n\t" \
+    "s_sub_u32 s27, s25, 12\n\t" \
+    "s_lshr_b32 s24, s60, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "s_lshr_b32 s24, s62, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "s_lshr_b32 s24, s64, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "s_lshr_b32 s24, s66, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    ".Lp3r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v64, v0\n\t" \
+    "v_fma_f32 v16, s53, v65, v16\n\t" \
+    "v_fma_f32 v32, s53, v66, v32\n\t" \
+    "v_fma_f32 v48, s53, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v68, v0\n\t" \
+    "v_fma_f32 v16, s55, v69, v16\n\t" \
+    "v_fma_f32 v32, s55, v70, v32\n\t" \
+    "v_fma_f32 v48, s55, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v72, v0\n\t" \
+    "v_fma_f32 v16, s57, v73, v16\n\t" \
+    "v_fma_f32 v32, s57, v74, v32\n\t" \
+    "v_fma_f32 v48, s57, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v76, v0\n\t" \
+    "v_fma_f32 v16, s59, v77, v16\n\t" \
+    "v_fma_f32 v32, s59, v78, v32\n\t" \
+    "v_fma_f32 v48, s59, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v80, v0\n\t" \
+    "v_fma_f32 v16, s61, v81, v16\n\t" \
+    "v_fma_f32 v32, s61, v82, v32\n\t" \
+    "v_fma_f32 v48, s61, v83, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v84, v0\n\t" \
+    "v_fma_f32 v16, s63, v85, v16\n\t" \
+    "v_fma_f32 v32, s63, v86, v32\n\t" \
+    "v_fma_f32 v48, s63, v87, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v88, v0\n\t" \
+    "v_fma_f32 v16, s65, v89, v16\n\t" \
+    "v_fma_f32 v32, s65, v90, v32\n\t" \
+    "v_fma_f32 v48, s65, v91, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v92, v0\n\t" \
+    "v_fma_f32 v16, s67, v93, v16\n\t" \
+    "v_fma_f32 v32, s67, v94, v32\n\t" \
+    "v_fma_f32 v48, s67, v95, v48\n\t" \
+    ".Lggp3s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz20_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v64, v0\n\t" \
+    "v_fma_f32 v16, s53, v65, v16\n\t" \
+    "v_fma_f32 v32, s53, v66, v32\n\t" \
+    "v_fma_f32 v48, s53, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v68, v0\n\t" \
+    "v_fma_f32 v16, s55, v69, v16\n\t" \
+    "v_fma_f32 v32, s55, v70, v32\n\t" \
+    "v_fma_f32 v48, s55, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v72, v0\n\t" \
+    "v_fma_f32 v16, s57, v73, v16\n\t" \
+    "v_fma_f32 v32, s57, v74, v32\n\t" \
+    "v_fma_f32 v48, s57, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v76, v0\n\t" \
+    "v_fma_f32 v16, s59, v77, v16\n\t" \
+    "v_fma_f32 v32, s59, v78, v32\n\t" \
+    "v_fma_f32 v48, s59, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf30_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v80, v0\n\t" \
+    "v_fma_f32 v16, s61, v81, v16\n\t" \
+    "v_fma_f32 v32, s61, v82, v32\n\t" \
+    "v_fma_f32 v48, s61, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v84, v0\n\t" \
+    "v_fma_f32 v16, s63, v85, v16\n\t" \
+    "v_fma_f32 v32, s63, v86, v32\n\t" \
+    "v_fma_f32 v48, s63, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v88, v0\n\t" \
+    "v_fma_f32 v16, s65, v89, v16\n\t" \
+    "v_fma_f32 v32, s65, v90, v32\n\t" \
+    "v_fma_f32 v48, s65, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v92, v0\n\t" \
+    "v_fma_f32 v16, s67, v93, v16\n\t" \
+    "v_fma_f32 v32, s67, v94, v32\n\t" \
+    "v_fma_f32 v48, s67, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_cmp_le_u32 s25, 16\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_sub_u32 s25, s25, 16\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
+    "s_add_u32 s26, s26, 0x40\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lba0_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lta0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpa0_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqa0_%=\n\t" \
+    ".Lpa0_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqa0_%=:\n\t" \
+    "s_lshr_b32 s24, s36, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s38, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s40, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s42, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_lshr_b32 s24, s44, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s46, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s48, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s50, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lna0_%=\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s26\n\t" \
+    ".Lna0_%=:\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v64, v0\n\t" \
+    "v_fma_f32 v16, s37, v65, v16\n\t" \
+    "v_fma_f32 v32, s37, v66, v32\n\t" \
+    "v_fma_f32 v48, s37, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v68, v0\n\t" \
+    "v_fma_f32 v16, s39, v69, v16\n\t" \
+    "v_fma_f32 v32, s39, v70, v32\n\t" \
+    "v_fma_f32 v48, s39, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v72, v0\n\t" \
+    "v_fma_f32 v16, s41, v73, v16\n\t" \
+    "v_fma_f32 v32, s41, v74, v32\n\t" \
+    "v_fma_f32 v48, s41, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v76, v0\n\t" \
+    "v_fma_f32 v16, s43, v77, v16\n\t" \
+    "v_fma_f32 v32, s43, v78, v32\n\t" \
+    "v_fma_f32 v48, s43, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v80, v0\n\t" \
+    "v_fma_f32 v16, s45, v81, v16\n\t" \
+    "v_fma_f32 v32, s45, v82, v32\n\t" \
+    "v_fma_f32 v48, s45, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v84, v0\n\t" \
+    "v_fma_f32 v16, s47, v85, v16\n\t" \
+    "v_fma_f32 v32, s47, v86, v32\n\t" \
+    "v_fma_f32 v48, s47, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v88, v0\n\t" \
+    "v_fma_f32 v16, s49, v89, v16\n\t" \
+    "v_fma_f32 v32, s49, v90, v32\n\t" \
+    "v_fma_f32 v48, s49, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v92, v0\n\t" \
+    "v_fma_f32 v16, s51, v93, v16\n\t" \
+    "v_fma_f32 v32, s51, v94, v32\n\t" \
+    "v_fma_f32 v48, s51, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lbb0_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Ltb0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpb0_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqb0_%=\n\t" \
+    ".Lpb0_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqb0_%=:\n\t" \
+    "s_lshr_b32 s24, s52, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s54, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s56, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s58, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_lshr_b32 s24, s60, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s62, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s64, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s66, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lnb0_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
+    ".Lnb0_%=:\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v64, v0\n\t" \
+    "v_fma_f32 v16, s53, v65, v16\n\t" \
+    "v_fma_f32 v32, s53, v66, v32\n\t" \
+    "v_fma_f32 v48, s53, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v68, v0\n\t" \
+    "v_fma_f32 v16, s55, v69, v16\n\t" \
+    "v_fma_f32 v32, s55, v70, v32\n\t" \
+    "v_fma_f32 v48, s55, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v72, v0\n\t" \
+    "v_fma_f32 v16, s57, v73, v16\n\t" \
+    "v_fma_f32 v32, s57, v74, v32\n\t" \
+    "v_fma_f32 v48, s57, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v76, v0\n\t" \
+    "v_fma_f32 v16, s59, v77, v16\n\t" \
+    "v_fma_f32 v32, s59, v78, v32\n\t" \
+    "v_fma_f32 v48, s59, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v80, v0\n\t" \
+    "v_fma_f32 v16, s61, v81, v16\n\t" \
+    "v_fma_f32 v32, s61, v82, v32\n\t" \
+    "v_fma_f32 v48, s61, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v84, v0\n\t" \
+    "v_fma_f32 v16, s63, v85, v16\n\t" \
+    "v_fma_f32 v32, s63, v86, v32\n\t" \
+    "v_fma_f32 v48, s63, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v88, v0\n\t" \
+    "v_fma_f32 v16, s65, v89, v16\n\t" \
+    "v_fma_f32 v32, s65, v90, v32\n\t" \
+    "v_fma_f32 v48, s65, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v92, v0\n\t" \
+    "v_fma_f32 v16, s67, v93, v16\n\t" \
+    "v_fma_f32 v32, s67, v94, v32\n\t" \
+    "v_fma_f32 v48, s67, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_branch .Lba0_%=\n\t" \
+    ".Lta0_%=:\n\t" \
+    "s_lshr_b32 s24, s36, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "s_lshr_b32 s24, s38, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "s_lshr_b32 s24, s40, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "s_lshr_b32 s24, s42, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "s_lshr_b32 s24, s44, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "s_lshr_b32 s24, s46, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "s_lshr_b32 s24, s48, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    ".Lgwa0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v64, v0\n\t" \
+    "v_fma_f32 v16, s37, v65, v16\n\t" \
+    "v_fma_f32 v32, s37, v66, v32\n\t" \
+    "v_fma_f32 v48, s37, v67, v48\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v68, v0\n\t" \
+    "v_fma_f32 v16, s39, v69, v16\n\t" \
+    "v_fma_f32 v32, s39, v70, v32\n\t" \
+    "v_fma_f32 v48, s39, v71, v48\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v72, v0\n\t" \
+    "v_fma_f32 v16, s41, v73, v16\n\t" \
+    "v_fma_f32 v32, s41, v74, v32\n\t" \
+    "v_fma_f32 v48, s41, v75, v48\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v76, v0\n\t" \
+    "v_fma_f32 v16, s43, v77, v16\n\t" \
+    "v_fma_f32 v32, s43, v78, v32\n\t" \
+    "v_fma_f32 v48, s43, v79, v48\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s44\n\t" \
+    "v_fma_f32 v0, s45, v80, v0\n\t" \
+    "v_fma_f32 v16, s45, v81, v16\n\t" \
+    "v_fma_f32 v32, s45, v82, v32\n\t" \
+    "v_fma_f32 v48, s45, v83, v48\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v84, v0\n\t" \
+    "v_fma_f32 v16, s47, v85, v16\n\t" \
+    "v_fma_f32 v32, s47, v86, v32\n\t" \
+    "v_fma_f32 v48, s47, v87, v48\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v88, v0\n\t" \
+    "v_fma_f32 v16, s49, v89, v16\n\t" \
+    "v_fma_f32 v32, s49, v90, v32\n\t" \
+    "v_fma_f32 v48, s49, v91, v48\n\t" \
+    ".Lgea0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Ltb0_%=:\n\t" \
+    "s_lshr_b32 s24, s52, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "s_lshr_b32 s24, s54, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "s_lshr_b32 s24, s56, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "s_lshr_b32 s24, s58, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "s_lshr_b32 s24, s60, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "s_lshr_b32 s24, s62, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "s_lshr_b32 s24, s64, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    ".Lgwb0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v64, v0\n\t" \
+    "v_fma_f32 v16, s53, v65, v16\n\t" \
+    "v_fma_f32 v32, s53, v66, v32\n\t" \
+    "v_fma_f32 v48, s53, v67, v48\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v68, v0\n\t" \
+    "v_fma_f32 v16, s55, v69, v16\n\t" \
+    "v_fma_f32 v32, s55, v70, v32\n\t" \
+    "v_fma_f32 v48, s55, v71, v48\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v72, v0\n\t" \
+    "v_fma_f32 v16, s57, v73, v16\n\t" \
+    "v_fma_f32 v32, s57, v74, v32\n\t" \
+    "v_fma_f32 v48, s57, v75, v48\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v76, v0\n\t" \
+    "v_fma_f32 v16, s59, v77, v16\n\t" \
+    "v_fma_f32 v32, s59, v78, v32\n\t" \
+    "v_fma_f32 v48, s59, v79, v48\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s60\n\t" \
+    "v_fma_f32 v0, s61, v80, v0\n\t" \
+    "v_fma_f32 v16, s61, v81, v16\n\t" \
+    "v_fma_f32 v32, s61, v82, v32\n\t" \
+    "v_fma_f32 v48, s61, v83, v48\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v84, v0\n\t" \
+    "v_fma_f32 v16, s63, v85, v16\n\t" \
+    "v_fma_f32 v32, s63, v86, v32\n\t" \
+    "v_fma_f32 v48, s63, v87, v48\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v88, v0\n\t" \
+    "v_fma_f32 v16, s65, v89, v16\n\t" \
+    "v_fma_f32 v32, s65, v90, v32\n\t" \
+    "v_fma_f32 v48, s65, v91, v48\n\t" \
+    ".Lgeb0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lce0_%=:\n\t" \
+    "s_setprio 0\n\t" \
+    "s_add_u32 %[sli], %[sli], 1\n\t" \
+    "s_cmp_lt_u32 %[sli], %[liend]\n\t" \
+    "s_cbranch_scc0 .Lx_%=\n\t" \
+    ".Lc1_%=:\n\t" \
+    "s_mov_b32 s25, s30\n\t" \
+    "s_add_u32 s26, s29, 0x80\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lw1_%=\n\t" \
+    "s_waitcnt vmcnt(0)\n\t" \
+    ".Lw1_%=:\n\t" \
+    "s_barrier\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_add_u32 s24, %[sli], 1\n\t" \
+    "s_cmp_lt_u32 s24, %[liend]\n\t" \
+    "s_cbranch_scc0 .Lnp1_%=\n\t" \
+    "v_readlane_b32 s29, %[vpbo], s24\n\t" \
+    "v_readlane_b32 s30, %[vpn], s24\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lnp1_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lnp1_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s24\n\t" \
+    ".Lnp1_%=:\n\t" \
+    "s_cmp_le_u32 %[sso], %[lim]\n\t" \
+    "s_cbranch_scc0 .Lsg1_%=\n\t" \
+    "s_mov_b32 m0, %[sdst]\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsd1_%=\n\t" \
+    ".Lsg1_%=:\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lsd1_%=\n\t" \
+    "s_mov_b32 m0, %[sdst]\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    ".Lsd1_%=:\n\t" \
+    "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lpr21_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lpr1_%=\n\t" \
+    ".Lpr21_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lpr1_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 4\n\t" \
+    "s_cbranch_scc1 .Lf01_%=\n\t" \
+    "s_mov_b32 s27, s25\n\t" \
+    "s_lshr_b32 s24, s68, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "s_lshr_b32 s24, s70, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "s_lshr_b32 s24, s72, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "s_lshr_b32 s24, s74, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    ".Lp0r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v64, v0\n\t" \
+    "v_fma_f32 v16, s69, v65, v16\n\t" \
+    "v_fma_f32 v32, s69, v66, v32\n\t" \
+    "v_fma_f32 v48, s69, v67, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v68, v0\n\t" \
+    "v_fma_f32 v16, s71, v69, v16\n\t" \
+    "v_fma_f32 v32, s71, v70, v32\n\t" \
+    "v_fma_f32 v48, s71, v71, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v72, v0\n\t" \
+    "v_fma_f32 v16, s73, v73, v16\n\t" \
+    "v_fma_f32 v32, s73, v74, v32\n\t" \
+    "v_fma_f32 v48, s73, v75, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v76, v0\n\t" \
+    "v_fma_f32 v16, s75, v77, v16\n\t" \
+    "v_fma_f32 v32, s75, v78, v32\n\t" \
+    "v_fma_f32 v48, s75, v79, v48\n\t" \
+    ".Lggp0s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf01_%=:\n\t" \
+    "s_lshr_b32 s24, s68, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s70, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s72, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s74, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_cmp_ge_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lq01_%=\n\t" \
+    "s_lshr_b32 s24, s76, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s78, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s80, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s82, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v64, v0\n\t" \
+    "v_fma_f32 v16, s69, v65, v16\n\t" \
+    "v_fma_f32 v32, s69, v66, v32\n\t" \
+    "v_fma_f32 v48, s69, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v68, v0\n\t" \
+    "v_fma_f32 v16, s71, v69, v16\n\t" \
+    "v_fma_f32 v32, s71, v70, v32\n\t" \
+    "v_fma_f32 v48, s71, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v72, v0\n\t" \
+    "v_fma_f32 v16, s73, v73, v16\n\t" \
+    "v_fma_f32 v32, s73, v74, v32\n\t" \
+    "v_fma_f32 v48, s73, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v76, v0\n\t" \
+    "v_fma_f32 v16, s75, v77, v16\n\t" \
+    "v_fma_f32 v32, s75, v78, v32\n\t" \
+    "v_fma_f32 v48, s75, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf11_%=\n\t" \
+    ".Lq01_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lz01_%=\n\t" \
+    "s_sub_u32 s27, s25, 4\n\t" \
+    "s_lshr_b32 s24, s76, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "s_lshr_b32 s24, s78, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "s_lshr_b32 s24, s80, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "s_lshr_b32 s24, s82, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    ".Lp1r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v64, v0\n\t" \
+    "v_fma_f32 v16, s69, v65, v16\n\t" \
+    "v_fma_f32 v32, s69, v66, v32\n\t" \
+    "v_fma_f32 v48, s69, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v68, v0\n\t" \
+    "v_fma_f32 v16, s71, v69, v16\n\t" \
+    "v_fma_f32 v32, s71, v70, v32\n\t" \
+    "v_fma_f32 v48, s71, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v72, v0\n\t" \
+    "v_fma_f32 v16, s73, v73, v16\n\t" \
+    "v_fma_f32 v32, s73, v74, v32\n\t" \
+    "v_fma_f32 v48, s73, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v76, v0\n\t" \
+    "v_fma_f32 v16, s75, v77, v16\n\t" \
+    "v_fma_f32 v32, s75, v78, v32\n\t" \
+    "v_fma_f32 v48, s75, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v80, v0\n\t" \
+    "v_fma_f32 v16, s77, v81, v16\n\t" \
+    "v_fma_f32 v32, s77, v82, v32\n\t" \
+    "v_fma_f32 v48, s77, v83, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v84, v0\n\t" \
+    "v_fma_f32 v16, s79, v85, v16\n\t" \
+    "v_fma_f32 v32, s79, v86, v32\n\t" \
+    "v_fma_f32 v48, s79, v87, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v88, v0\n\t" \
+    "v_fma_f32 v16, s81, v89, v16\n\t" \
+    "v_fma_f32 v32, s81, v90, v32\n\t" \
+    "v_fma_f32 v48, s81, v91, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v92, v0\n\t" \
+    "v_fma_f32 v16, s83, v93, v16\n\t" \
+    "v_fma_f32 v32, s83, v94, v32\n\t" \
+    "v_fma_f32 v48, s83, v95, v48\n\t" \
+    ".Lggp1s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz01_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v64, v0\n\t" \
+    "v_fma_f32 v16, s69, v65, v16\n\t" \
+    "v_fma_f32 v32, s69, v66, v32\n\t" \
+    "v_fma_f32 v48, s69, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v68, v0\n\t" \
+    "v_fma_f32 v16, s71, v69, v16\n\t" \
+    "v_fma_f32 v32, s71, v70, v32\n\t" \
+    "v_fma_f32 v48, s71, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v72, v0\n\t" \
+    "v_fma_f32 v16, s73, v73, v16\n\t" \
+    "v_fma_f32 v32, s73, v74, v32\n\t" \
+    "v_fma_f32 v48, s73, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v76, v0\n\t" \
+    "v_fma_f32 v16, s75, v77, v16\n\t" \
+    "v_fma_f32 v32, s75, v78, v32\n\t" \
+    "v_fma_f32 v48, s75, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf11_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lq11_%=\n\t" \
+    "s_lshr_b32 s24, s84, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s86, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s88, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s90, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v80, v0\n\t" \
+    "v_fma_f32 v16, s77, v81, v16\n\t" \
+    "v_fma_f32 v32, s77, v82, v32\n\t" \
+    "v_fma_f32 v48, s77, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v84, v0\n\t" \
+    "v_fma_f32 v16, s79, v85, v16\n\t" \
+    "v_fma_f32 v32, s79, v86, v32\n\t" \
+    "v_fma_f32 v48, s79, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v88, v0\n\t" \
+    "v_fma_f32 v16, s81, v89, v16\n\t" \
+    "v_fma_f32 v32, s81, v90, v32\n\t" \
+    "v_fma_f32 v48, s81, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v92, v0\n\t" \
+    "v_fma_f32 v16, s83, v93, v16\n\t" \
+    "v_fma_f32 v32, s83, v94, v32\n\t" \
+    "v_fma_f32 v48, s83, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf21_%=\n\t" \
+    ".Lq11_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lz11_%=\n\t" \
+    "s_sub_u32 s27, s25, 8\n\t" \
+    "s_lshr_b32 s24, s84, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "s_lshr_b32 s24, s86, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "s_lshr_b32 s24, s88, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "s_lshr_b32 s24, s90, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    ".Lp2r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v80, v0\n\t" \
+    "v_fma_f32 v16, s77, v81, v16\n\t" \
+    "v_fma_f32 v32, s77, v82, v32\n\t" \
+    "v_fma_f32 v48, s77, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v84, v0\n\t" \
+    "v_fma_f32 v16, s79, v85, v16\n\t" \
+    "v_fma_f32 v32, s79, v86, v32\n\t" \
+    "v_fma_f32 v48, s79, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v88, v0\n\t" \
+    "v_fma_f32 v16, s81, v89, v16\n\t" \
+    "v_fma_f32 v32, s81, v90, v32\n\t" \
+    "v_fma_f32 v48, s81, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v92, v0\n\t" \
+    "v_fma_f32 v16, s83, v93, v16\n\t" \
+    "v_fma_f32 v32, s83, v94, v32\n\t" \
+    "v_fma_f32 v48, s83, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v64, v0\n\t" \
+    "v_fma_f32 v16, s85, v65, v16\n\t" \
+    "v_fma_f32 v32, s85, v66, v32\n\t" \
+    "v_fma_f32 v48, s85, v67, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v68, v0\n\t" \
+    "v_fma_f32 v16, s87, v69, v16\n\t" \
+    "v_fma_f32 v32, s87, v70, v32\n\t" \
+    "v_fma_f32 v48, s87, v71, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v72, v0\n\t" \
+    "v_fma_f32 v16, s89, v73, v16\n\t" \
+    "v_fma_f32 v32, s89, v74, v32\n\t" \
+    "v_fma_f32 v48, s89, v75, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v76, v0\n\t" \
+    "v_fma_f32 v16, s91, v77, v16\n\t" \
+    "v_fma_f32 v32, s91, v78, v32\n\t" \
+    "v_fma_f32 v48, s91, v79, v48\n\t" \
+    ".Lggp2s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz11_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v80, v0\n\t" \
+    "v_fma_f32 v16, s77, v81, v16\n\t" \
+    "v_fma_f32 v32, s77, v82, v32\n\t" \
+    "v_fma_f32 v48, s77, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v84, v0\n\t" \
+    "v_fma_f32 v16, s79, v85, v16\n\t" \
+    "v_fma_f32 v32, s79, v86, v32\n\t" \
+    "v_fma_f32 v48, s79, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v88, v0\n\t" \
+    "v_fma_f32 v16, s81, v89, v16\n\t" \
+    "v_fma_f32 v32, s81, v90, v32\n\t" \
+    "v_fma_f32 v48, s81, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v92, v0\n\t" \
+    "v_fma_f32 v16, s83, v93, v16\n\t" \
+    "v_fma_f32 v32, s83, v94, v32\n\t" \
+    "v_fma_f32 v48, s83, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf21_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lq21_%=\n\t" \
+    "s_lshr_b32 s24, s92, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s94, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s96, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s98, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v64, v0\n\t" \
+    "v_fma_f32 v16, s85, v65, v16\n\t" \
+    "v_fma_f32 v32, s85, v66, v32\n\t" \
+    "v_fma_f32 v48, s85, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v68, v0\n\t" \
+    "v_fma_f32 v16, s87, v69, v16\n\t" \
+    "v_fma_f32 v32, s87, v70, v32\n\t" \
+    "v_fma_f32 v48, s87, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v72, v0\n\t" \
+    "v_fma_f32 v16, s89, v73, v16\n\t" \
+    "v_fma_f32 v32, s89, v74, v32\n\t" \
+    "v_fma_f32 v48, s89, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v76, v0\n\t" \
+    "v_fma_f32 v16, s91, v77, v16\n\t" \
+    "v_fma_f32 v32, s91, v78, v32\n\t" \
+    "v_fma_f32 v48, s91, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf31_%=\n\t" \
+    ".Lq21_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lz21_%=\n\t" \
+    "s_sub_u32 s27, s25, 12\n\t" \
+    "s_lshr_b32 s24, s92, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "s_lshr_b32 s24, s94, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "s_lshr_b32 s24, s96, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "s_lshr_b32 s24, s98, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    ".Lp3r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v64, v0\n\t" \
+    "v_fma_f32 v16, s85, v65, v16\n\t" \
+    "v_fma_f32 v32, s85, v66, v32\n\t" \
+    "v_fma_f32 v48, s85, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v68, v0\n\t" \
+    "v_fma_f32 v16, s87, v69, v16\n\t" \
+    "v_fma_f32 v32, s87, v70, v32\n\t" \
+    "v_fma_f32 v48, s87, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v72, v0\n\t" \
+    "v_fma_f32 v16, s89, v73, v16\n\t" \
+    "v_fma_f32 v32, s89, v74, v32\n\t" \
+    "v_fma_f32 v48, s89, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v76, v0\n\t" \
+    "v_fma_f32 v16, s91, v77, v16\n\t" \
+    "v_fma_f32 v32, s91, v78, v32\n\t" \
+    "v_fma_f32 v48, s91, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v80, v0\n\t" \
+    "v_fma_f32 v16, s93, v81, v16\n\t" \
+    "v_fma_f32 v32, s93, v82, v32\n\t" \
+    "v_fma_f32 v48, s93, v83, v48\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v84, v0\n\t" \
+    "v_fma_f32 v16, s95, v85, v16\n\t" \
+    "v_fma_f32 v32, s95, v86, v32\n\t" \
+    "v_fma_f32 v48, s95, v87, v48\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v88, v0\n\t" \
+    "v_fma_f32 v16, s97, v89, v16\n\t" \
+    "v_fma_f32 v32, s97, v90, v32\n\t" \
+    "v_fma_f32 v48, s97, v91, v48\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v92, v0\n\t" \
+    "v_fma_f32 v16, s99, v93, v16\n\t" \
+    "v_fma_f32 v32, s99, v94, v32\n\t" \
+    "v_fma_f32 v48, s99, v95, v48\n\t" \
+    ".Lggp3s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz21_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v64, v0\n\t" \
+    "v_fma_f32 v16, s85, v65, v16\n\t" \
+    "v_fma_f32 v32, s85, v66, v32\n\t" \
+    "v_fma_f32 v48, s85, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v68, v0\n\t" \
+    "v_fma_f32 v16, s87, v69, v16\n\t" \
+    "v_fma_f32 v32, s87, v70, v32\n\t" \
+    "v_fma_f32 v48, s87, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v72, v0\n\t" \
+    "v_fma_f32 v16, s89, v73, v16\n\t" \
+    "v_fma_f32 v32, s89, v74, v32\n\t" \
+    "v_fma_f32 v48, s89, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v76, v0\n\t" \
+    "v_fma_f32 v16, s91, v77, v16\n\t" \
+    "v_fma_f32 v32, s91, v78, v32\n\t" \
+    "v_fma_f32 v48, s91, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf31_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v80, v0\n\t" \
+    "v_fma_f32 v16, s93, v81, v16\n\t" \
+    "v_fma_f32 v32, s93, v82, v32\n\t" \
+    "v_fma_f32 v48, s93, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v84, v0\n\t" \
+    "v_fma_f32 v16, s95, v85, v16\n\t" \
+    "v_fma_f32 v32, s95, v86, v32\n\t" \
+    "v_fma_f32 v48, s95, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v88, v0\n\t" \
+    "v_fma_f32 v16, s97, v89, v16\n\t" \
+    "v_fma_f32 v32, s97, v90, v32\n\t" \
+    "v_fma_f32 v48, s97, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v92, v0\n\t" \
+    "v_fma_f32 v16, s99, v93, v16\n\t" \
+    "v_fma_f32 v32, s99, v94, v32\n\t" \
+    "v_fma_f32 v48, s99, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_cmp_le_u32 s25, 16\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_sub_u32 s25, s25, 16\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
+    "s_add_u32 s26, s26, 0x40\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lba1_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lta1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpa1_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqa1_%=\n\t" \
+    ".Lpa1_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqa1_%=:\n\t" \
+    "s_lshr_b32 s24, s68, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s70, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s72, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s74, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_lshr_b32 s24, s76, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s78, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s80, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s82, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lna1_%=\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s26\n\t" \
+    ".Lna1_%=:\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v64, v0\n\t" \
+    "v_fma_f32 v16, s69, v65, v16\n\t" \
+    "v_fma_f32 v32, s69, v66, v32\n\t" \
+    "v_fma_f32 v48, s69, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v68, v0\n\t" \
+    "v_fma_f32 v16, s71, v69, v16\n\t" \
+    "v_fma_f32 v32, s71, v70, v32\n\t" \
+    "v_fma_f32 v48, s71, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v72, v0\n\t" \
+    "v_fma_f32 v16, s73, v73, v16\n\t" \
+    "v_fma_f32 v32, s73, v74, v32\n\t" \
+    "v_fma_f32 v48, s73, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v76, v0\n\t" \
+    "v_fma_f32 v16, s75, v77, v16\n\t" \
+    "v_fma_f32 v32, s75, v78, v32\n\t" \
+    "v_fma_f32 v48, s75, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v80, v0\n\t" \
+    "v_fma_f32 v16, s77, v81, v16\n\t" \
+    "v_fma_f32 v32, s77, v82, v32\n\t" \
+    "v_fma_f32 v48, s77, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v84, v0\n\t" \
+    "v_fma_f32 v16, s79, v85, v16\n\t" \
+    "v_fma_f32 v32, s79, v86, v32\n\t" \
+    "v_fma_f32 v48, s79, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v88, v0\n\t" \
+    "v_fma_f32 v16, s81, v89, v16\n\t" \
+    "v_fma_f32 v32, s81, v90, v32\n\t" \
+    "v_fma_f32 v48, s81, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v92, v0\n\t" \
+    "v_fma_f32 v16, s83, v93, v16\n\t" \
+    "v_fma_f32 v32, s83, v94, v32\n\t" \
+    "v_fma_f32 v48, s83, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lbb1_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Ltb1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpb1_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqb1_%=\n\t" \
+    ".Lpb1_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqb1_%=:\n\t" \
+    "s_lshr_b32 s24, s84, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_lshr_b32 s24, s86, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_lshr_b32 s24, s88, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_lshr_b32 s24, s90, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_lshr_b32 s24, s92, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_lshr_b32 s24, s94, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_lshr_b32 s24, s96, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    "s_lshr_b32 s24, s98, 8\n\t" \
+    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[92:95], v103\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lnb1_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
+    ".Lnb1_%=:\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v64, v0\n\t" \
+    "v_fma_f32 v16, s85, v65, v16\n\t" \
+    "v_fma_f32 v32, s85, v66, v32\n\t" \
+    "v_fma_f32 v48, s85, v67, v48\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v68, v0\n\t" \
+    "v_fma_f32 v16, s87, v69, v16\n\t" \
+    "v_fma_f32 v32, s87, v70, v32\n\t" \
+    "v_fma_f32 v48, s87, v71, v48\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v72, v0\n\t" \
+    "v_fma_f32 v16, s89, v73, v16\n\t" \
+    "v_fma_f32 v32, s89, v74, v32\n\t" \
+    "v_fma_f32 v48, s89, v75, v48\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v76, v0\n\t" \
+    "v_fma_f32 v16, s91, v77, v16\n\t" \
+    "v_fma_f32 v32, s91, v78, v32\n\t" \
+    "v_fma_f32 v48, s91, v79, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v80, v0\n\t" \
+    "v_fma_f32 v16, s93, v81, v16\n\t" \
+    "v_fma_f32 v32, s93, v82, v32\n\t" \
+    "v_fma_f32 v48, s93, v83, v48\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v84, v0\n\t" \
+    "v_fma_f32 v16, s95, v85, v16\n\t" \
+    "v_fma_f32 v32, s95, v86, v32\n\t" \
+    "v_fma_f32 v48, s95, v87, v48\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v88, v0\n\t" \
+    "v_fma_f32 v16, s97, v89, v16\n\t" \
+    "v_fma_f32 v32, s97, v90, v32\n\t" \
+    "v_fma_f32 v48, s97, v91, v48\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v92, v0\n\t" \
+    "v_fma_f32 v16, s99, v93, v16\n\t" \
+    "v_fma_f32 v32, s99, v94, v32\n\t" \
+    "v_fma_f32 v48, s99, v95, v48\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_branch .Lba1_%=\n\t" \
+    ".Lta1_%=:\n\t" \
+    "s_lshr_b32 s24, s68, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "s_lshr_b32 s24, s70, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "s_lshr_b32 s24, s72, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "s_lshr_b32 s24, s74, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "s_lshr_b32 s24, s76, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "s_lshr_b32 s24, s78, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "s_lshr_b32 s24, s80, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    ".Lgwa1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v64, v0\n\t" \
+    "v_fma_f32 v16, s69, v65, v16\n\t" \
+    "v_fma_f32 v32, s69, v66, v32\n\t" \
+    "v_fma_f32 v48, s69, v67, v48\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v68, v0\n\t" \
+    "v_fma_f32 v16, s71, v69, v16\n\t" \
+    "v_fma_f32 v32, s71, v70, v32\n\t" \
+    "v_fma_f32 v48, s71, v71, v48\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v72, v0\n\t" \
+    "v_fma_f32 v16, s73, v73, v16\n\t" \
+    "v_fma_f32 v32, s73, v74, v32\n\t" \
+    "v_fma_f32 v48, s73, v75, v48\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v76, v0\n\t" \
+    "v_fma_f32 v16, s75, v77, v16\n\t" \
+    "v_fma_f32 v32, s75, v78, v32\n\t" \
+    "v_fma_f32 v48, s75, v79, v48\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s76\n\t" \
+    "v_fma_f32 v0, s77, v80, v0\n\t" \
+    "v_fma_f32 v16, s77, v81, v16\n\t" \
+    "v_fma_f32 v32, s77, v82, v32\n\t" \
+    "v_fma_f32 v48, s77, v83, v48\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v84, v0\n\t" \
+    "v_fma_f32 v16, s79, v85, v16\n\t" \
+    "v_fma_f32 v32, s79, v86, v32\n\t" \
+    "v_fma_f32 v48, s79, v87, v48\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v88, v0\n\t" \
+    "v_fma_f32 v16, s81, v89, v16\n\t" \
+    "v_fma_f32 v32, s81, v90, v32\n\t" \
+    "v_fma_f32 v48, s81, v91, v48\n\t" \
+    ".Lgea1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Ltb1_%=:\n\t" \
+    "s_lshr_b32 s24, s84, 8\n\t" \
+    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[64:67], v96\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "s_lshr_b32 s24, s86, 8\n\t" \
+    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[68:71], v97\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "s_lshr_b32 s24, s88, 8\n\t" \
+    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[72:75], v98\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "s_lshr_b32 s24, s90, 8\n\t" \
+    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[76:79], v99\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "s_lshr_b32 s24, s92, 8\n\t" \
+    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[80:83], v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "s_lshr_b32 s24, s94, 8\n\t" \
+    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[84:87], v101\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "s_lshr_b32 s24, s96, 8\n\t" \
+    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "ds_read_b128 v[88:91], v102\n\t" \
+    ".Lgwb1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v64, v0\n\t" \
+    "v_fma_f32 v16, s85, v65, v16\n\t" \
+    "v_fma_f32 v32, s85, v66, v32\n\t" \
+    "v_fma_f32 v48, s85, v67, v48\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v68, v0\n\t" \
+    "v_fma_f32 v16, s87, v69, v16\n\t" \
+    "v_fma_f32 v32, s87, v70, v32\n\t" \
+    "v_fma_f32 v48, s87, v71, v48\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v72, v0\n\t" \
+    "v_fma_f32 v16, s89, v73, v16\n\t" \
+    "v_fma_f32 v32, s89, v74, v32\n\t" \
+    "v_fma_f32 v48, s89, v75, v48\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v76, v0\n\t" \
+    "v_fma_f32 v16, s91, v77, v16\n\t" \
+    "v_fma_f32 v32, s91, v78, v32\n\t" \
+    "v_fma_f32 v48, s91, v79, v48\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s92\n\t" \
+    "v_fma_f32 v0, s93, v80, v0\n\t" \
+    "v_fma_f32 v16, s93, v81, v16\n\t" \
+    "v_fma_f32 v32, s93, v82, v32\n\t" \
+    "v_fma_f32 v48, s93, v83, v48\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v84, v0\n\t" \
+    "v_fma_f32 v16, s95, v85, v16\n\t" \
+    "v_fma_f32 v32, s95, v86, v32\n\t" \
+    "v_fma_f32 v48, s95, v87, v48\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v88, v0\n\t" \
+    "v_fma_f32 v16, s97, v89, v16\n\t" \
+    "v_fma_f32 v32, s97, v90, v32\n\t" \
+    "v_fma_f32 v48, s97, v91, v48\n\t" \
+    ".Lgeb1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lce1_%=:\n\t" \
+    "s_setprio 0\n\t" \
+    "s_add_u32 %[sli], %[sli], 1\n\t" \
+    "s_cmp_lt_u32 %[sli], %[liend]\n\t" \
+    "s_cbranch_scc0 .Lx_%=\n\t" \
+    "s_branch .Lc0_%=\n\t" \
+    ".Lx_%=:\n\t" \
+    "s_mov_b32 m0, s28\n\t"
+#define GCOO_PANEL_SEGMENT_CLOBBERS \
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", \
     "v76", "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", \
     "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", \
-    "v100", "v101", "v102", "v103", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", \
-    "s48", "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", \
-    "s60", "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", \
-    "s72", "s73", "s74", "s75", "s76", "scc", "memory"
+    "v100", "v101", "v102", "v103", "s24", "s25", "s26", "s27", "s28", "s29", "s30", "s36", \
+    "s37", "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", \
+    "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", \
+    "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", \
+    "s73", "s74", "s75", "s76", "s77", "s78", "s79", "s80", "s81", "s82", "s83", "s84", \
+    "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95", "s96", \
+    "s97", "s98", "s99", "scc", "memory"

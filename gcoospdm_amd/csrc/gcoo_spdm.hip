@@ -838,96 +838,123 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     };
     load_window();
     stage(0, 0);
-    int pb = __builtin_amdgcn_readlane(vpb, 0), pe = __builtin_amdgcn_readlane(vpe, 0);
-    for (int c = 0; c < nC; ++c) {
-        if constexpr (E::bytes != 4) {
+    if constexpr (E::bytes == 4) {
+        // FP32: the chunk loop in asm (gcoo_panel_asm.h), one block per
+        // segment of consecutive chunks of a 64-chunk window.  Per chunk: the
+        // barrier, the scalar loads of the next chunk's first 16 entries
+        // (landing while this chunk runs), the staging of the next chunk,
+        // and the entry loop.  The L2 touch wave ends a segment before each
+        // chunk at which it touches, and touches between segments.
+        static_assert(LPW == 15 && RPL == 1 && CHUNKB == 75 * 1024,
+                      "the asm block stages 15 one-row loads into 75 KB buffers");
+        typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+        const u32x4 rs = {(unsigned)reinterpret_cast<uintptr_t>(D),
+                          (unsigned)(reinterpret_cast<uintptr_t>(D) >> 32) & 0xffffu,
+                          (unsigned)dBytes, (unsigned)kBufferFlags};
+        const unsigned vb0 =
+            (unsigned)(size_t)((const __attribute__((address_space(3))) unsigned char *)ldsLane);
+        const unsigned vb1 = vb0 + CHUNKB;
+        const unsigned sdst = __builtin_amdgcn_readfirstlane((unsigned)(size_t)ldsBase);
+        // staging: row offsets below kb are loaded, and when the wave's last
+        // row of a chunk is below K too (offset <= lim) without tests; the
+        // other waves have kb = 0 and stage nothing
+        const unsigned urow = (unsigned)rowBytes;
+        const unsigned kb = __builtin_amdgcn_readfirstlane(wave < SW ? dBytes : 0u);
+        const unsigned lim = __builtin_amdgcn_readfirstlane(
+            (wave < SW && dBytes >= (unsigned)LPW * urow) ? dBytes - (unsigned)(LPW - 1) * urow - urow : 0u);
+        const unsigned tkrowb = (unsigned)TK * urow;
+        unsigned sso = __builtin_amdgcn_readfirstlane((unsigned)(TK + wave * LPW) * urow);
+        int c = 0;
+        for (;;) {
+            // the window as list byte offsets from its first chunk's entries
+            const int pb0 = __builtin_amdgcn_readlane(vpb, 0);
+            const uint2 *pbase = plist + pb0;
+            const bool inw = cw + lane < nC;
+            const unsigned vpbo = inw ? (unsigned)(vpb - pb0) * 8u : 0u;
+            const unsigned vpn = inw ? (unsigned)(vpe - vpb) : 0u;
+            const int wend = nC < cw + 64 ? nC : cw + 64;
+            while (c < wend) {
+                int cend = wend;
+                if (wave == TW) {
+                    if (c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
+                    int t = (panel - (c + 1)) % nPanels;
+                    t = c + 1 + (t < 0 ? t + nPanels : t);  // next chunk that is touched at
+                    if (t < cend && t + kPanelPrefetch < nC) cend = t;
+                }
+                unsigned sli = __builtin_amdgcn_readfirstlane((unsigned)(c - cw));
+                const unsigned liend = __builtin_amdgcn_readfirstlane((unsigned)(cend - cw));
+                const unsigned par = __builtin_amdgcn_readfirstlane((unsigned)c & 1u);
+                asm volatile(GCOO_PANEL_SEGMENT_ASM
+                             : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2),
+                               "+{v[48:63]}"(a3), [sli] "+s"(sli), [sso] "+s"(sso)
+                             : [pbase] "s"(pbase), [vpbo] "v"(vpbo), [vpn] "v"(vpn),
+                               [liend] "s"(liend), [par] "s"(par), [vb0] "v"(vb0), [vb1] "v"(vb1),
+                               [sdst] "s"(sdst), [lim] "s"(lim), [kb] "s"(kb), [rowb] "s"(urow),
+                               [tkrowb] "s"(tkrowb), [voff] "v"(pvoff), [rsrc] "s"(rs)
+                             : GCOO_PANEL_SEGMENT_CLOBBERS);
+                c = cend;
+            }
+            if (c >= nC) break;
+            cw = c;
+            load_window();
+        }
+    } else {
+        // BF16: the chunk loop in C++, the same order per chunk
+        int pb = __builtin_amdgcn_readlane(vpb, 0), pe = __builtin_amdgcn_readlane(vpe, 0);
+        for (int c = 0; c < nC; ++c) {
             // chunk c has landed in LDS for every wave, and every wave is
             // done with chunk c - 1, whose buffer the next stage overwrites
             // (a bare barrier: a fence here would wait for wave 15's touches
-            // too).  The FP32 path does this inside its asm block.
+            // too)
             if (wave < SW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             asm volatile("s_barrier" ::: "memory");
             if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
-        }
-        if (wave == TW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
-        const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
-        if (c + 1 == cw + 64) {
-            cw += 64;
-            load_window();
-        }
-        const int li = c + 1 < nC ? c + 1 - cw : 0;
-        const int pbn = __builtin_amdgcn_readlane(vpb, li);
-        const int pen = __builtin_amdgcn_readlane(vpe, li);
-        if constexpr (E::bytes == 4) {
-            // The chunk in one asm block (gcoo_panel_asm.h): the first batch's
-            // entry load issued before the barrier, the barrier, the staging
-            // of chunk c + 1, and the entry loop (next batch's entries loaded
-            // under this batch's steps, one guarded round for the last 1..7,
-            // priority from the entries left).
-            static_assert(LPW == 15 && RPL == 1, "the asm block stages 15 one-row loads");
-            typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-            const u32x4 rs = {(unsigned)reinterpret_cast<uintptr_t>(D),
-                              (unsigned)(reinterpret_cast<uintptr_t>(D) >> 32) & 0xffffu,
-                              (unsigned)dBytes, (unsigned)kBufferFlags};
-            const uint2 *lp = plist + pb;
-            const int nn = pe - pb;
-            const unsigned base =
-                (unsigned)(size_t)((const __attribute__((address_space(3))) unsigned char *)sb);
-            const int k1 = (c + 1) * TK + wave * LPW;  // this wave's first row of chunk c + 1
-            int nld = 0;
-            if (wave < SW && c + 1 < nC) nld = K - k1 < 0 ? 0 : (K - k1 < LPW ? K - k1 : LPW);
-            nld = __builtin_amdgcn_readfirstlane(nld);
-            const int stg = __builtin_amdgcn_readfirstlane(wave < SW ? 1 : 0);
-            const unsigned dst = __builtin_amdgcn_readfirstlane(
-                (unsigned)(size_t)(ldsBase + (((c + 1) & 1) ? CHUNKB : 0)));
-            const int soff1 = __builtin_amdgcn_readfirstlane(k1 * rowBytes);
-            asm volatile(GCOO_PANEL_CHUNK_ASM
-                         : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
-                         : [lp] "s"(lp), [n] "s"(nn), [base] "v"(base), [stg] "s"(stg),
-                           [nld] "s"(nld), [soff] "s"(soff1), [dst] "s"(dst), [voff] "v"(pvoff),
-                           [rsrc] "s"(rs), [rowb] "s"(rowBytes)
-                         : GCOO_PANEL_CHUNK_CLOBBERS);
+            if (wave == TW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
+            const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
+            if (c + 1 == cw + 64) {
+                cw += 64;
+                load_window();
+            }
+            const int li = c + 1 < nC ? c + 1 - cw : 0;
+            const int pbn = __builtin_amdgcn_readlane(vpb, li);
+            const int pen = __builtin_amdgcn_readlane(vpe, li);
+            // B entries: list entry x = (LDS byte offset << 8) | row slot
+            auto run = [&](auto bn, int p) {
+                constexpr int B = decltype(bn)::value;
+                uint2 m[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u) m[u] = plist[p + u];
+                float d[B][VEC];
+#pragma unroll
+                for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + (m[u].x >> 8));
+                const int m0 = panel_m0_save();
+                int u = 0;
+#pragma unroll
+                for (; u + 4 <= B; u += 4) {
+                    const unsigned q[4] = {m[u].x, m[u + 1].x, m[u + 2].x, m[u + 3].x};
+                    const float x[4] = {E::val((int)m[u].y), E::val((int)m[u + 1].y),
+                                        E::val((int)m[u + 2].y), E::val((int)m[u + 3].y)};
+                    panel_fma4(a0, a1, a2, a3, q, x, d[u], d[u + 1], d[u + 2], d[u + 3]);
+                }
+#pragma unroll
+                for (; u < B; ++u)
+                    panel_fma1(a0, a1, a2, a3, m[u].x, E::val((int)m[u].y), d[u]);
+                panel_m0_restore(m0);
+            };
+            int p = pb;
+            for (; p + 8 <= pe; p += 8) {
+                const int rem = pe - p;
+                if (rem >= 16) __builtin_amdgcn_s_setprio(3);
+                else __builtin_amdgcn_s_setprio(2);
+                run(std::integral_constant<int, 8>{}, p);
+            }
+            if (p + 4 <= pe) { run(std::integral_constant<int, 4>{}, p); p += 4; }
+            if (p + 2 <= pe) { run(std::integral_constant<int, 2>{}, p); p += 2; }
+            if (p < pe) run(std::integral_constant<int, 1>{}, p);
             __builtin_amdgcn_s_setprio(0);
             pb = pbn;
             pe = pen;
-            continue;
         }
-        // B entries: list entry x = (LDS byte offset << 8) | row slot
-        auto run = [&](auto bn, int p) {
-            constexpr int B = decltype(bn)::value;
-            uint2 m[B];
-#pragma unroll
-            for (int u = 0; u < B; ++u) m[u] = plist[p + u];
-            float d[B][VEC];
-#pragma unroll
-            for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + (m[u].x >> 8));
-            const int m0 = panel_m0_save();
-            int u = 0;
-#pragma unroll
-            for (; u + 4 <= B; u += 4) {
-                const unsigned q[4] = {m[u].x, m[u + 1].x, m[u + 2].x, m[u + 3].x};
-                const float x[4] = {E::val((int)m[u].y), E::val((int)m[u + 1].y),
-                                    E::val((int)m[u + 2].y), E::val((int)m[u + 3].y)};
-                panel_fma4(a0, a1, a2, a3, q, x, d[u], d[u + 1], d[u + 2], d[u + 3]);
-            }
-#pragma unroll
-            for (; u < B; ++u)
-                panel_fma1(a0, a1, a2, a3, m[u].x, E::val((int)m[u].y), d[u]);
-            panel_m0_restore(m0);
-        };
-        int p = pb;
-        for (; p + 8 <= pe; p += 8) {
-            const int rem = pe - p;
-            if (rem >= 16) __builtin_amdgcn_s_setprio(3);
-            else __builtin_amdgcn_s_setprio(2);
-            run(std::integral_constant<int, 8>{}, p);
-        }
-        if (p + 4 <= pe) { run(std::integral_constant<int, 4>{}, p); p += 4; }
-        if (p + 2 <= pe) { run(std::integral_constant<int, 2>{}, p); p += 2; }
-        if (p < pe) run(std::integral_constant<int, 1>{}, p);
-        __builtin_amdgcn_s_setprio(0);
-        pb = pbn;
-        pe = pen;
     }
     // no LDS write may land after the workgroup's LDS is released
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");

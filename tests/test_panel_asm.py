@@ -1,33 +1,57 @@
 """CPU check of the FP32 panel multiply's generated asm block
-(gcoospdm_amd/csrc/gcoo_panel_asm.h).
+(gcoospdm_amd/csrc/gcoo_panel_asm.h): a segment of consecutive chunks.
 
 A small interpreter runs the block's instructions for one lane: scalar and
-vector registers, SCC, M0 and the VGPR-index mode, with list entries in
-memory and LDS reads recorded symbolically.  Every fmaf step is logged as
-(accumulator register written, value SGPR contents, D word read).  For each
-entry count, the log must be exactly the list's entries in order, 4 steps
-each (one per column component), each on accumulator v[16c + slot] with the
-entry's value and the D row at the entry's LDS offset -- the per-row chains
-of the multiply.  The staging loads are logged too and checked against the
-requested count.
+vector registers, SCC, M0 and the VGPR-index mode, with the wave's list
+entries in memory and LDS reads recorded symbolically.  Every fmaf step is
+logged as (accumulator register written, value SGPR contents, D word read),
+tagged with the chunk (barrier) it ran under.  For each chunk the log must be
+exactly that chunk's entries in list order, 4 steps each (one per column
+component), each on accumulator v[16c + slot] with the entry's value and the
+D row at the entry's LDS offset in the chunk's buffer -- the per-row chains
+of the multiply.  The staging loads are logged and checked against the rows
+below K, and the block's final state (segment end, staging offset, M0)
+against what the kernel carries to the next segment.
+
+The interpreter also tracks the loads in flight under the LGKM counter: LDS
+reads complete in order, scalar loads in any order, so after
+s_waitcnt lgkmcnt(n) an LDS read is known complete only with n later LDS
+reads issued, and a scalar load only for n = 0.  Reading or overwriting a
+register that a load in flight may still write fails the test, and so does
+leaving a load in flight at the block's end.
 """
 import os
+import random
 import re
 
 import pytest
 
 HDR = os.path.join(os.path.dirname(__file__), "..", "gcoospdm_amd", "csrc", "gcoo_panel_asm.h")
+CHUNKB = 75 * 1024
+ROWB = 32000
 
 
 def _instructions():
     text = open(HDR).read()
-    body = text.split("#define GCOO_PANEL_CHUNK_ASM", 1)[1].split("#define GCOO_PANEL_CHUNK_CLOBBERS", 1)[0]
+    body = text.split("#define GCOO_PANEL_SEGMENT_ASM", 1)[1].split(
+        "#define GCOO_PANEL_SEGMENT_CLOBBERS", 1)[0]
     lines = re.findall(r'"([^"]*)\\n\\t"', body)
     return [ln.strip() for ln in lines if ln.strip()]
 
 
+def _regs(tok):
+    """Register numbers named by an operand like s5, v[64:67], s[36:51]."""
+    m = re.fullmatch(r"([sv])\[(\d+):(\d+)\]", tok)
+    if m:
+        return m.group(1), list(range(int(m.group(2)), int(m.group(3)) + 1))
+    m = re.fullmatch(r"([sv])(\d+)", tok)
+    if m:
+        return m.group(1), [int(m.group(2))]
+    return None, []
+
+
 class Lane:
-    def __init__(self, prog, entries, n, ops):
+    def __init__(self, prog, entries, ops, vlanes):
         self.prog = prog
         self.labels = {ln[:-1]: i for i, ln in enumerate(prog) if ln.endswith(":")}
         self.s = {}
@@ -36,12 +60,27 @@ class Lane:
         self.m0 = 0x1234
         self.idx_on = False
         self.idx = 0
-        self.entries = entries
-        self.ops = ops  # operand values
+        self.entries = entries     # the list from %[pbase], 8 bytes each
+        self.ops = dict(ops)       # operand values (in/out ones updated)
+        self.vlanes = vlanes       # per-lane values of the window VGPRs
         self.fmas = []
         self.stage = []
-        self.s["n"] = n
+        self.barriers = 0
+        self.inflight = []         # ("smem"|"lds", set of ("s"|"v", reg))
 
+    # -- loads in flight --
+    def _check(self, kind, regs, what):
+        for _, dest in self.inflight:
+            for r in regs:
+                assert (kind, r) not in dest, f"{what} {kind}{r} while a load to it is in flight"
+
+    def wait_lgkm(self, n):
+        lds = [op for op in self.inflight if op[0] == "lds"]
+        keep = set(map(id, lds[len(lds) - n:])) if n else set()
+        self.inflight = [op for op in self.inflight
+                         if (op[0] == "smem" and n > 0) or (op[0] == "lds" and id(op) in keep)]
+
+    # -- operands --
     def sval(self, tok):
         tok = tok.strip()
         if tok == "m0":
@@ -54,16 +93,29 @@ class Lane:
             return int(tok)
         m = re.fullmatch(r"s(\d+)", tok)
         assert m, tok
-        return self.s[int(m.group(1))]
+        r = int(m.group(1))
+        self._check("s", [r], "read")
+        return self.s[r]
 
     def setS(self, tok, val):
         tok = tok.strip()
+        val &= 0xFFFFFFFF
         if tok == "m0":
-            self.m0 = val & 0xFFFFFFFF
+            self.m0 = val
             return
-        self.s[int(re.fullmatch(r"s(\d+)", tok).group(1))] = val & 0xFFFFFFFF
+        if tok.startswith("%["):
+            self.ops[tok[2:-1]] = val
+            return
+        r = int(re.fullmatch(r"s(\d+)", tok).group(1))
+        self._check("s", [r], "write")
+        self.s[r] = val
 
-    def run(self, limit=200000):
+    def vread(self, tok):
+        r = int(tok[1:])
+        self._check("v", [r], "read")
+        return self.v[r]
+
+    def run(self, limit=400000):
         pc = 0
         steps = 0
         while pc < len(self.prog):
@@ -75,11 +127,16 @@ class Lane:
                 continue
             op, _, rest = ln.partition(" ")
             args = [a.strip() for a in rest.split(",")] if rest else []
-            if op in ("s_waitcnt", "s_barrier", "s_setprio", "s_nop"):
+            if op in ("s_setprio", "s_nop"):
                 continue
-            if op == "s_mov_b32":
-                self.setS(args[0], self.sval(args[1]))
-            elif op == "s_movk_i32":
+            if op == "s_barrier":
+                self.barriers += 1
+            elif op == "s_waitcnt":
+                m = re.fullmatch(r"(vmcnt|lgkmcnt)\((\d+)\)", rest.strip())
+                assert m, ln
+                if m.group(1) == "lgkmcnt":
+                    self.wait_lgkm(int(m.group(2)))
+            elif op in ("s_mov_b32", "s_movk_i32"):
                 self.setS(args[0], self.sval(args[1]))
             elif op in ("s_add_u32", "s_sub_u32"):
                 a, b = self.sval(args[1]), self.sval(args[2])
@@ -100,15 +157,17 @@ class Lane:
             elif op == "s_branch":
                 pc = self.labels[args[0]]
             elif op == "s_load_dwordx16":
-                lo, hi = map(int, re.fullmatch(r"s\[(\d+):(\d+)\]", args[0]).groups())
-                assert hi - lo == 15
+                kind, regs = _regs(args[0])
+                assert kind == "s" and len(regs) == 16 and args[1] == "%[pbase]"
                 byte_off = self.sval(args[2])
                 assert byte_off % 8 == 0
+                self._check("s", regs, "load into")
                 first = byte_off // 8
                 for u in range(8):
                     x, val = self.entries[first + u]  # entries past the list are padding
-                    self.s[lo + 2 * u] = x
-                    self.s[lo + 2 * u + 1] = val
+                    self.s[regs[2 * u]] = x
+                    self.s[regs[2 * u + 1]] = val
+                self.inflight.append(("smem", {("s", r) for r in regs}))
             elif op == "s_set_gpr_idx_on":
                 assert not self.idx_on
                 self.idx_on = True
@@ -119,29 +178,38 @@ class Lane:
             elif op == "s_set_gpr_idx_off":
                 self.idx_on = False
             elif op == "v_readlane_b32":
-                raise AssertionError("unexpected")
+                assert not self.idx_on
+                lane = self.sval(args[2])
+                assert 0 <= lane < 64
+                self.setS(args[0], self.vlanes[args[1][2:-1]][lane])
             elif op == "v_add_u32_e32":
                 assert not self.idx_on
                 d = int(args[0][1:])
-                self.v[d] = ("addr", self.sval(args[1]) + self.ops["base"])
+                self._check("v", [d], "write")
+                self.v[d] = ("addr", self.sval(args[1]) + self.sval(args[2]))
             elif op == "ds_read_b128":
-                lo, hi = map(int, re.fullmatch(r"v\[(\d+):(\d+)\]", args[0]).groups())
-                kind, addr = self.v[int(args[1][1:])]
-                assert kind == "addr"
+                kind, regs = _regs(args[0])
+                akind, addr = self.vread(args[1])
+                assert akind == "addr"
+                self._check("v", regs, "load into")
                 for c in range(4):
-                    self.v[lo + c] = ("d", addr, c)
+                    self.v[regs[c]] = ("d", addr, c)
+                self.inflight.append(("lds", {("v", r) for r in regs}))
             elif op == "buffer_load_dwordx4":
-                self.stage.append((self.m0, self.sval(args[2].split()[0])))
+                assert args[0] == "%[voff]" and args[1] == "%[rsrc]"
+                self.stage.append((self.barriers, self.m0, self.sval(args[2].split()[0])))
             elif op == "v_fma_f32":
                 assert self.idx_on, "fma outside index mode"
                 d = int(args[0][1:]) + self.idx
                 s2 = int(args[3][1:]) + self.idx
                 assert d == s2
-                src = self.v[int(args[2][1:])]
-                self.fmas.append((d, self.sval(args[1]), src))
+                src = self.vread(args[2])
+                self._check("v", [d], "accumulate into")
+                self.fmas.append((self.barriers, d, self.sval(args[1]), src))
             else:
                 raise AssertionError("unhandled " + ln)
         assert not self.idx_on
+        assert not self.inflight, "loads in flight at the block's end"
         return self
 
 
@@ -155,41 +223,105 @@ def _prog():
     return PROG
 
 
-def _run(n, nld=0, stg=0, seed=0):
-    import random
-    rng = random.Random(seed * 1000 + n)
-    base = 0x100
-    # the wave's entries: rows in order, k ascending inside a row
-    ents = []
-    rows = sorted(rng.randrange(16) for _ in range(n))
-    for slot in rows:
-        k = rng.randrange(75)
-        ents.append(((k * 1024) << 8 | slot, rng.getrandbits(32)))
-    pad = [((rng.randrange(75) * 1024) << 8 | rng.randrange(16), rng.getrandbits(32)) for _ in range(24)]
-    ops = {"n": n, "lp": 0, "base": base, "stg": stg, "nld": nld, "soff": 5000, "dst": 0x9000,
-           "voff": 0, "rsrc": 0, "rowb": 32000}
-    lane = Lane(_prog(), ents + pad, n, ops).run()
-    return lane, ents, base
+def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0):
+    """Run one segment: chunks c0 .. c0 + len(counts) - 1 of the window at
+    cw, with counts[i] entries in chunk c0 + i (the window's other chunks get
+    random counts).  Returns the lane, the per-chunk entries and the inputs."""
+    rng = random.Random(seed)
+    nchunks = c0 - cw + len(counts)
+    assert nchunks <= 64
+    per = [rng.randrange(20) for _ in range(c0 - cw)] + list(counts)
+    per += [rng.randrange(20) for _ in range(64 - len(per))]
+    entries, vpbo, vpn, chunks = [], [], [], []
+    for n in per:
+        vpbo.append(len(entries) * 8)
+        vpn.append(n)
+        rows = sorted(rng.randrange(16) for _ in range(n))
+        ch = [((rng.randrange(75) * 1024) << 8 | slot, rng.getrandbits(32)) for slot in rows]
+        chunks.append(ch)
+        entries += ch
+    entries += [((rng.randrange(75) * 1024) << 8 | rng.randrange(16), rng.getrandbits(32))
+                for _ in range(24)]
+    wave = 2
+    kb = K * ROWB if stager else 0
+    lim = max(kb - 15 * ROWB, 0) if stager else 0
+    sso = ((c0 + 1) * 75 + wave * 15) * ROWB
+    ops = {"pbase": 0, "sli": c0 - cw, "liend": c0 - cw + len(counts), "par": c0 & 1,
+           "vb0": 0x40, "vb1": 0x40 + CHUNKB, "sdst": wave * 15 * 1024, "sso": sso,
+           "lim": lim, "kb": kb, "rowb": ROWB, "tkrowb": 75 * ROWB}
+    lane = Lane(_prog(), entries, ops, {"vpbo": vpbo, "vpn": vpn}).run()
+    return lane, chunks[c0 - cw:c0 - cw + len(counts)], ops
+
+
+def _check_steps(lane, chunks, c0, ops):
+    for i, ch in enumerate(chunks):
+        base = ops["vb0"] if (c0 + i) % 2 == 0 else ops["vb1"]
+        expect = []
+        for x, val in ch:
+            slot, off = x & 0xFF, x >> 8
+            for c in range(4):
+                expect.append((16 * c + slot, val, ("d", base + off, c)))
+        got = [f[1:] for f in lane.fmas if f[0] == i + 1]
+        assert len(got) == len(expect), f"chunk {c0 + i}: {len(got) // 4} entries"
+        # the 4 steps of one entry may be in any component order, but entries
+        # (and so every row's chain) must be in list order
+        for j in range(0, len(expect), 4):
+            assert sorted(got[j:j + 4]) == sorted(expect[j:j + 4]), f"chunk {c0 + i} entry {j // 4}"
 
 
 @pytest.mark.parametrize("n", list(range(0, 49)))
-def test_entry_steps_follow_the_list(n):
-    lane, ents, base = _run(n)
-    expect = []
-    for x, val in ents:
-        slot, off = x & 0xFF, x >> 8
-        for c in range(4):
-            expect.append((16 * c + slot, val, ("d", base + off, c)))
-    got = lane.fmas
-    # the steps of one entry may be in any component order, but entries and
-    # their per-row chains must be in list order
-    assert len(got) == len(expect)
-    for i in range(0, len(expect), 4):
-        assert sorted(got[i:i + 4]) == sorted(expect[i:i + 4]), f"entry {i // 4} of {n}"
+@pytest.mark.parametrize("c0", [0, 1])
+def test_one_chunk_steps_follow_the_list(n, c0):
+    lane, chunks, ops = _segment([n], c0=c0, cw=0, seed=n)
+    assert lane.barriers == 1
+    _check_steps(lane, chunks, c0, ops)
 
 
-@pytest.mark.parametrize("nld", [0, 1, 7, 15])
-def test_staging_loads(nld):
-    lane, _, _ = _run(5, nld=nld, stg=1)
-    assert lane.stage == [(0x9000 + 1024 * i, 5000 + 32000 * i) for i in range(nld)]
+@pytest.mark.parametrize("seed", range(12))
+def test_segment_steps_follow_the_list(seed):
+    rng = random.Random(100 + seed)
+    cw = rng.choice([0, 64, 128])
+    c0 = cw + rng.randrange(8)
+    counts = [rng.choice([0, 1, 3, 4, 5, 8, 9, 12, 15, 16, 17, 23, 24, 31, 40])
+              for _ in range(rng.randrange(1, 14))]
+    lane, chunks, ops = _segment(counts, c0=c0, cw=cw, seed=seed)
+    assert lane.barriers == len(counts)
+    _check_steps(lane, chunks, c0, ops)
+    assert lane.ops["sli"] == c0 - cw + len(counts)
     assert lane.m0 == 0x1234  # restored
+
+
+@pytest.mark.parametrize("K", [8000, 300, 161, 75 * 7 + 33])
+def test_staging_loads(K):
+    counts = [5, 0, 17, 2, 9, 30]
+    c0 = 0
+    lane, _, ops = _segment(counts, c0=c0, K=K, seed=K)
+    wave = 2
+    want = []
+    for i in range(len(counts)):
+        c1 = c0 + i + 1                      # the chunk this barrier's staging loads
+        k0 = c1 * 75 + wave * 15
+        dst = ops["sdst"] + (c1 % 2) * CHUNKB
+        for r in range(15):
+            if k0 + r < K:
+                want.append((i + 1, dst + 1024 * r, (k0 + r) * ROWB))
+    assert lane.stage == want
+    assert lane.ops["sso"] == ops["sso"] + len(counts) * 75 * ROWB
+
+
+def test_non_staging_wave_loads_nothing():
+    lane, chunks, ops = _segment([3, 20, 0, 8], c0=5, cw=0, stager=False, seed=7)
+    assert lane.stage == []
+    _check_steps(lane, chunks, 5, ops)
+
+
+def test_interpreter_catches_an_early_read():
+    # the hazard model itself: a wait that is one read short must fail
+    prog = [ln.replace("lgkmcnt(4)", "lgkmcnt(5)") for ln in _prog()]
+    lane_cls = Lane
+    rng = random.Random(3)
+    ents = [((rng.randrange(75) * 1024) << 8 | i % 16, i) for i in range(40)]
+    ops = {"pbase": 0, "sli": 0, "liend": 1, "par": 0, "vb0": 0, "vb1": CHUNKB, "sdst": 0,
+           "sso": 75 * ROWB, "lim": 0, "kb": 0, "rowb": ROWB, "tkrowb": 75 * ROWB}
+    with pytest.raises(AssertionError, match="in flight"):
+        lane_cls(prog, ents, ops, {"vpbo": [0] * 64, "vpn": [12] + [0] * 63}).run()
