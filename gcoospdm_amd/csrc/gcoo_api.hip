@@ -95,6 +95,11 @@ int read_status(gcoo_handle h, hipStream_t s)
     GCOO_HIP_TRY(hipMemcpyAsync(h->h_status, h->d_status, sizeof(ConvStatus),
                                 hipMemcpyDeviceToHost, s));
     GCOO_HIP_TRY(hipStreamSynchronize(s));
+    if (h->h_status->stall) {
+        // clear the single-pass abort word for the next conversion
+        if (h->lb_abort) (void)hipMemset(h->lb_abort, 0, sizeof(unsigned));
+        return set_error(GCOO_ERR_HIP, "the conversion's scan stopped waiting for a group");
+    }
     if (h->h_status->overflow)
         return set_error(GCOO_ERR_OVERFLOW, "nnz %lld does not fit in int32",
                          (long long)h->h_status->total);
@@ -112,10 +117,13 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
     h->converted = false;
     h->seg16_valid = false;
     invalidate_panel_lists(h);
-    int rc = launch_count_and_scan(h, S, s);
+    // one pass over S where the single-pass kernel applies, else count + scan
+    // then scatter
+    const bool fused = fused_conversion_ok(h, S);
+    int rc = fused ? launch_convert_fused(h, S, s) : launch_count_and_scan(h, S, s);
     if (rc) return rc;
     if (h->cap > 0) {
-        if ((rc = launch_scatter(h, S, s))) return rc;
+        if (!fused && (rc = launch_scatter(h, S, s))) return rc;
         if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
     }
     if ((rc = read_status(h, s))) return rc;
@@ -125,9 +133,9 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
         if ((rc = ensure_capacity(h, need))) return rc;
         h->nnz = total;  // the multiply's automatic variant choice reads it
         h->maxRow = h->h_status->maxRow;
-        if ((rc = launch_scatter(h, S, s))) return rc;
+        if ((rc = fused ? launch_convert_fused(h, S, s) : launch_scatter(h, S, s))) return rc;
         if (D && (rc = launch_spdm_f32(h, D, N, C, nullptr, s))) return rc;
-        GCOO_HIP_TRY(hipStreamSynchronize(s));
+        if ((rc = read_status(h, s))) return rc;  // synchronises
     }
     h->nnz = total;
     h->maxRow = h->h_status->maxRow;
@@ -327,8 +335,12 @@ int gcoo_create(int M, int K, int cpg, gcoo_handle *out)
         gcoo_destroy(h);
         return set_error(GCOO_ERR_ALLOC, "cannot allocate GCOO index arrays");
     }
-    h->h_status->total = 0;
-    h->h_status->overflow = 0;
+    if (hipMemset(h->d_status, 0, sizeof(ConvStatus)) != hipSuccess) {
+        (void)hipGetLastError();
+        gcoo_destroy(h);
+        return set_error(GCOO_ERR_HIP, "cannot clear the status word");
+    }
+    *h->h_status = ConvStatus{};
     *out = h;
     return GCOO_OK;
 }
@@ -348,6 +360,8 @@ int gcoo_destroy(gcoo_handle h)
     if (h->rowptr) (void)hipFree(h->rowptr);
     if (h->rowcnt) (void)hipFree(h->rowcnt);
     if (h->mask) (void)hipFree(h->mask);
+    if (h->lb_flags) (void)hipFree(h->lb_flags);
+    if (h->lb_abort) (void)hipFree(h->lb_abort);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->coo_tmp) (void)hipFree(h->coo_tmp);

@@ -20,6 +20,12 @@
 // All three are HBM-streaming integer/byte work: no MFMA, coalesced 16-byte
 // loads along rows of S.
 //
+// On the 16-byte path with cpg 4 (the reference's default) one kernel does
+// all three in a single pass over S (convert_fused_kernel below: counts, a
+// decoupled look-back scan across groups, and the scatter from LDS).  The
+// three kernels remain for cpg 1 and 2, for K % 4 != 0 or an unaligned S,
+// and when the look-back state cannot be allocated.
+//
 // On the 16-byte path (K % 4 == 0, S 16-byte aligned) the count pass also
 // writes a nonzero mask of S -- per row and 256-column block, the four wave
 // ballots of `v != 0` over the block's 16-byte pieces -- and the scatter takes
@@ -199,6 +205,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
         st->total = carry;
         st->overflow = carry > (long long)INT_MAX ? 1 : 0;
         st->maxRow = m;
+        st->stall = 0;
         if (clear_invalid) st->invalid = 0;
     }
     __syncthreads();
@@ -438,6 +445,381 @@ __global__ __launch_bounds__(kConvThreads) void scatter_mask_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Single-pass conversion (16-byte path, cpg 4): count, scan and scatter in one
+// kernel that reads S once.  Workgroup g takes group g: each XCD dispatches
+// its workgroups in index order, so every group a workgroup waits for is
+// running or done.  (Device-scope atomics on one address serialize, so
+// neither a ticket counter nor a finish counter is used.)
+//   1. Wave j streams row j of the group in 256-column blocks, kFusedDepth
+//      blocks of loads in flight.  Per block the four ballots of `v != 0` are the mask
+//      words, kept in LDS with the row's running count at the block's start;
+//      the row's entries (column, value bits) are compacted into an LDS list
+//      in column order (rank = running count + set bits in lower lanes).
+//   2. Decoupled look-back (one wave): the group publishes its entry count
+//      and longest row, then reads up to 64 predecessors' flags at a time --
+//      their own values, or values over everything up to them -- until it
+//      reaches one of the latter, and publishes its own inclusive sum and
+//      maximum; the last group's are the total and the longest row.  A flag
+//      is one 64-bit word (state, epoch, value), two per group; the epoch
+//      makes clearing between conversions unnecessary.  A wait that does not
+//      end within a bounded number of polls raises an abort flag that every
+//      waiting group sees, and the status word reports the stall.
+//   3. Thread t takes entries t, t + 256, ... of the group's LDS lists in row
+//      order, so the row segments are a straight copy; an entry's place in
+//      (k, r) order is its rank in its row plus, for every other row, that
+//      row's entries left of it (at its column too for a lower row), read off
+//      the block counts and mask words.  A group with a row longer than its
+//      LDS list takes the block-parallel form: each wave takes blocks w,
+//      w + 4, ...; a lane's nibble of each row is bit l of the block's mask
+//      words, one wave-wide scan of the packed per-row counts places every
+//      entry as in scatter_kernel, and values come from S.
+// Rows, ranks and both orders are those of the count / scan / scatter kernels.
+// ---------------------------------------------------------------------------
+constexpr int kFusedThreads = 256;
+constexpr int kFusedCpg = 4;
+constexpr int kFusedLdsBytes = 40 * 1024;  // dynamic LDS per workgroup (4 per CU)
+constexpr int kFusedDepth = 4;             // 256-column blocks of loads in flight per wave
+constexpr unsigned long long kLbValue = (1ull << 40) - 1;
+constexpr unsigned long long kLbTotal = 1ull << 62;   // the group's own total
+constexpr unsigned long long kLbPrefix = 2ull << 62;  // total including all before
+constexpr unsigned kLbEpochMask = (1u << 22) - 1;
+constexpr int kLbSpinLimit = 1 << 20;
+
+__device__ __forceinline__ unsigned long long lb_word(unsigned long long state, unsigned epoch,
+                                                      long long v)
+{
+    return state | ((unsigned long long)epoch << 40) | ((unsigned long long)v & kLbValue);
+}
+
+// set bits of b in the lanes below this one
+__device__ __forceinline__ int lanes_below(unsigned long long b)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(b >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)b, 0u));
+}
+
+__device__ __forceinline__ long long wave_sum64(long long v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// flags: [nGroup][2] (entry count, longest row); abort: zero unless a
+// conversion stalled (the host clears it then).
+__global__ __launch_bounds__(kFusedThreads) void convert_fused_kernel(
+    const float *__restrict__ S, int M, int K, int KB, int nGroup, int lcap, unsigned epoch,
+    unsigned long long *__restrict__ flags, unsigned *__restrict__ abort,
+    int *__restrict__ groupIndex, int *__restrict__ nnzPerGroup, int *__restrict__ rowptr,
+    ConvStatus *__restrict__ st, long long cap, float *__restrict__ vals,
+    int *__restrict__ rows, int *__restrict__ cols, int2 *__restrict__ seg)
+{
+    constexpr int CPG = kFusedCpg;
+    extern __shared__ unsigned long long fused_lds[];
+    unsigned long long *smask = fused_lds;                                   // [CPG][KB][4]
+    int2 *slist = reinterpret_cast<int2 *>(smask + (size_t)CPG * KB * 4);  // [CPG][lcap]
+    int *sbpre = reinterpret_cast<int *>(slist + (size_t)CPG * lcap);      // [CPG][KB]
+    __shared__ int sCnt[CPG];
+    __shared__ long long sBase;
+
+    const int tid = threadIdx.x, lane = __lane_id();
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = blockIdx.x;
+    if (g == 0 && tid == 0) st->stall = 0;
+    const int r0 = g * CPG;
+    const int nrows = min(CPG, M - r0);
+    const int K4 = K >> 2;
+
+    // 1. count, mask and compact row `wave`
+    if (wave < nrows) {
+        const f32x4 *rowp = reinterpret_cast<const f32x4 *>(S + (size_t)(r0 + wave) * K);
+        unsigned long long *mrow = smask + (size_t)wave * KB * 4;
+        int2 *lrow = slist + (size_t)wave * lcap;
+        int *brow = sbpre + (size_t)wave * KB;
+        int cnt = 0;
+        // loads of blocks kb0 .. kb0 + kFusedDepth - 1 (the index clamped in
+        // range, so that every load is issued; values past K are zeroed when
+        // processed), the next set in flight while one is processed
+        auto load = [&](int kb0, f32x4 (&v)[kFusedDepth]) {
+#pragma unroll
+            for (int u = 0; u < kFusedDepth; ++u) {
+                const int i = min((kb0 + u) * 64 + lane, K4 - 1);
+                v[u] = __builtin_nontemporal_load(rowp + i);
+            }
+        };
+        auto process = [&](int kb0, f32x4 (&v)[kFusedDepth]) {
+#pragma unroll
+            for (int u = 0; u < kFusedDepth; ++u) {
+                const int kb = kb0 + u;
+                if (kb >= KB) break;
+                if (kb * 64 + lane >= K4) v[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+                const unsigned long long b0 = __ballot(nz(v[u].x)), b1 = __ballot(nz(v[u].y));
+                const unsigned long long b2 = __ballot(nz(v[u].z)), b3 = __ballot(nz(v[u].w));
+                if (lane == 0) brow[kb] = cnt;
+                if (lane < 4) mrow[kb * 4 + lane] = lane == 0 ? b0 : lane == 1 ? b1 : lane == 2 ? b2 : b3;
+                int rank = cnt + lanes_below(b0) + lanes_below(b1) + lanes_below(b2) + lanes_below(b3);
+                const int col = 4 * (kb * 64 + lane);
+                const float e[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (nz(e[q])) {
+                        if (rank < lcap) lrow[rank] = make_int2(col + q, __float_as_int(e[q]));
+                        ++rank;
+                    }
+                }
+                cnt += __popcll(b0) + __popcll(b1) + __popcll(b2) + __popcll(b3);
+            }
+        };
+        // two sets of registers in turn: a set's next loads go out as soon as
+        // it has been processed, so one set is always in flight
+        constexpr int D = kFusedDepth;
+        f32x4 va[D], vb[D];
+        load(0, va);
+        for (int kb0 = 0; kb0 < KB; kb0 += 2 * D) {
+            if (kb0 + D < KB) load(kb0 + D, vb);
+            process(kb0, va);
+            if (kb0 + 2 * D < KB) load(kb0 + 2 * D, va);
+            if (kb0 + D < KB) process(kb0 + D, vb);
+        }
+        if (lane == 0) sCnt[wave] = cnt;
+    } else if (lane == 0) {
+        sCnt[wave] = 0;
+    }
+    __syncthreads();
+    long long agg = 0;
+    int maxr = 0;
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) {
+        agg += sCnt[j];
+        maxr = max(maxr, sCnt[j]);
+    }
+
+    // 2. look-back for the group's base (and the longest row so far)
+    if (wave == 0) {
+        long long excl = 0;
+        int emax = 0;
+        unsigned long long *fs = flags + 2 * (size_t)g;
+        if (g == 0) {
+            if (lane == 0) {
+                __hip_atomic_store(&fs[0], lb_word(kLbPrefix, epoch, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&fs[1], lb_word(kLbPrefix, epoch, maxr), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else {
+            if (lane == 0) {
+                __hip_atomic_store(&fs[0], lb_word(kLbTotal, epoch, agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&fs[1], lb_word(kLbTotal, epoch, maxr), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
+            int p = g - 1;
+            int spins = 0;
+            for (;;) {
+                const int q = p - lane;
+                unsigned long long f0 = lb_word(kLbPrefix, epoch, 0);  // before group 0
+                unsigned long long f1 = f0;
+                if (q >= 0) {
+                    f0 = __hip_atomic_load(&flags[2 * (size_t)q], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    f1 = __hip_atomic_load(&flags[2 * (size_t)q + 1], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                }
+                const bool v0 = (unsigned)((f0 >> 40) & kLbEpochMask) == epoch && (f0 >> 62) != 0;
+                const bool v1 = (unsigned)((f1 >> 40) & kLbEpochMask) == epoch && (f1 >> 62) != 0;
+                const bool valid = v0 && v1;
+                const bool pre0 = v0 && (f0 >> 62) == 2, pre1 = v1 && (f1 >> 62) == 2;
+                // the nearest predecessor with any word already inclusive; it
+                // must have both (the two stores of a group land apart)
+                const unsigned long long anyPre = __ballot(pre0 || pre1);
+                const unsigned long long bothPre = __ballot(pre0 && pre1);
+                const unsigned long long bad = __ballot(!valid);
+                const int fp = anyPre ? __ffsll((long long)anyPre) - 1 : 64;
+                const unsigned long long need = fp >= 63 ? ~0ull : (2ull << fp) - 1;
+                const bool ready = !(bad & need) && (fp == 64 || ((bothPre >> fp) & 1ull));
+                if (!ready) {
+                    bool give_up = ++spins > kLbSpinLimit;
+                    if (!give_up && (spins & 63) == 0)
+                        give_up = __hip_atomic_load(abort, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT) != 0;
+                    if (__builtin_amdgcn_readfirstlane(give_up ? 1 : 0)) {
+                        if (lane == 0) {
+                            atomicExch(abort, 1u);
+                            st->stall = 1;
+                        }
+                        excl = 0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                const bool use = lane <= fp;
+                excl += wave_sum64(use ? (long long)(f0 & kLbValue) : 0);
+                int m = use ? (int)(f1 & kLbValue) : 0;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) m = max(m, __shfl_xor(m, d, 64));
+                emax = max(emax, m);
+                if (fp < 64) break;
+                p -= 64;
+            }
+            if (lane == 0) {
+                __hip_atomic_store(&fs[0], lb_word(kLbPrefix, epoch, excl + agg), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&fs[1], lb_word(kLbPrefix, epoch, max(emax, maxr)),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (lane == 0) {
+            sBase = excl;
+            groupIndex[g] = (int)excl;
+            nnzPerGroup[g] = (int)agg;
+            long long r = excl;
+            for (int j = 0; j < nrows; ++j) {
+                rowptr[r0 + j] = (int)r;
+                r += sCnt[j];
+            }
+            if (g == nGroup - 1) {
+                const long long total = excl + agg;
+                rowptr[M] = (int)total;
+                groupIndex[nGroup] = (int)total;
+                nnzPerGroup[nGroup] = 0;
+                st->total = total;
+                st->overflow = total > (long long)INT_MAX ? 1 : 0;
+                st->maxRow = max(emax, maxr);
+                st->invalid = 0;
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. scatter (only when the group's entries fit the arrays)
+    const long long gbase = sBase;
+    bool anyBig = false;
+#pragma unroll
+    for (int j = 0; j < CPG; ++j) anyBig |= sCnt[j] > lcap;
+    if (agg > 0 && gbase + agg <= cap && !anyBig) {
+        // entry-parallel: thread t takes entries t, t + 256, ... of the
+        // group's rows in row order (the LDS lists, back to back), so the
+        // row segments are a straight copy; the entry's place in (k, r)
+        // order is its rank in its row plus, for every other row, that row's
+        // entries left of its column (at its column too for a lower row):
+        // the row's count before the block plus the mask bits before it
+        int rowend[CPG];
+        int acc = 0;
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) {
+            acc += sCnt[j];
+            rowend[j] = acc;
+        }
+        for (int e = tid; e < (int)agg; e += kFusedThreads) {
+            int j = 0;
+#pragma unroll
+            for (int i = 0; i < CPG - 1; ++i) j += e >= rowend[i] ? 1 : 0;
+            const int t = e - (j ? rowend[j - 1] : 0);
+            const int2 en = slist[(size_t)j * lcap + t];
+            const int c = en.x;
+            const int kb = c >> 8, l = (c >> 2) & 63, q = c & 3;
+            const unsigned long long below = l ? ~0ull >> (64 - l) : 0ull;
+            int pos = t;
+#pragma unroll
+            for (int i = 0; i < CPG; ++i) {
+                if (i == j || i >= nrows) continue;
+                const unsigned long long *mp = smask + ((size_t)i * KB + kb) * 4;
+                int n = sbpre[i * KB + kb];
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    const unsigned long long m = mp[w];
+                    n += __popcll(m & below);
+                    if (w < q || (w == q && i < j)) n += (int)((m >> l) & 1ull);
+                }
+                pos += n;
+            }
+            vals[gbase + pos] = __int_as_float(en.y);
+            rows[gbase + pos] = r0 + j;
+            cols[gbase + pos] = c;
+            seg[gbase + e] = en;
+        }
+    } else if (agg > 0 && gbase + agg <= cap) {
+        // a row longer than its LDS list: block-parallel, values from S
+        int rowoff[CPG];
+        bool has[CPG], big[CPG];
+        int off = 0;
+#pragma unroll
+        for (int j = 0; j < CPG; ++j) {
+            rowoff[j] = off;
+            off += sCnt[j];
+            big[j] = sCnt[j] > lcap;
+            has[j] = j < nrows;
+        }
+        const f32x4 *grow = reinterpret_cast<const f32x4 *>(S + (size_t)r0 * K);
+        for (int kb = wave; kb < KB; kb += kFusedThreads / 64) {
+            unsigned nib[CPG];
+            int bpre[CPG];
+            unsigned long long packed = 0;
+            int gpre = 0;
+#pragma unroll
+            for (int j = 0; j < CPG; ++j) {
+                nib[j] = 0;
+                bpre[j] = 0;
+                if (has[j]) {
+                    const unsigned long long *mp = smask + ((size_t)j * KB + kb) * 4;
+                    nib[j] = (unsigned)((mp[0] >> lane) & 1ull) | (unsigned)((mp[1] >> lane) & 1ull) << 1 |
+                             (unsigned)((mp[2] >> lane) & 1ull) << 2 | (unsigned)((mp[3] >> lane) & 1ull) << 3;
+                    bpre[j] = sbpre[j * KB + kb];
+                }
+                gpre += bpre[j];
+                packed |= (unsigned long long)__popc(nib[j]) << (16 * j);
+            }
+            unsigned long long incl = packed;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const unsigned long long t = __shfl_up(incl, d, 64);
+                if (lane >= d) incl += t;
+            }
+            if (!packed) continue;
+            const unsigned long long ex = incl - packed;
+            int rank[CPG];
+            long long posg = gbase + gpre;
+#pragma unroll
+            for (int j = 0; j < CPG; ++j) {
+                const int e = (int)((ex >> (16 * j)) & 0xFFFFu);
+                rank[j] = bpre[j] + e;
+                posg += e;
+            }
+            const int i = kb * 64 + lane;
+            const int kcol = 4 * i;
+            float sv[CPG][4];
+#pragma unroll
+            for (int j = 0; j < CPG; ++j) {
+                if (big[j] && nib[j]) {
+                    const f32x4 x = __builtin_nontemporal_load(grow + (size_t)j * K4 + i);
+                    sv[j][0] = x.x;
+                    sv[j][1] = x.y;
+                    sv[j][2] = x.z;
+                    sv[j][3] = x.w;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int j = 0; j < CPG; ++j) {
+                    if ((nib[j] >> q) & 1u) {
+                        const int bits = big[j] ? __float_as_int(sv[j][q])
+                                                : slist[(size_t)j * lcap + rank[j]].y;
+                        vals[posg] = __int_as_float(bits);
+                        rows[posg] = r0 + j;
+                        cols[posg] = kcol + q;
+                        ++posg;
+                        seg[gbase + rowoff[j] + rank[j]] = make_int2(kcol + q, bits);
+                        ++rank[j];
+                    }
+                }
+            }
+        }
+    }
+}
+
 // 256-column blocks of the mask.
 int mask_blocks(const gcoo_matrix *h) { return (h->K + 255) / 256; }
 
@@ -513,6 +895,44 @@ bool use_vec4(const gcoo_matrix *h, const float *S)
 }
 
 }  // namespace
+
+bool fused_conversion_ok(gcoo_matrix *h, const float *S)
+{
+    if (h->cpg != kFusedCpg || h->nGroup == 0 || !use_vec4(h, S)) return false;
+    const int KB = mask_blocks(h);
+    const long long fixed = 144ll * KB;  // masks + block counts
+    if (fixed + 32ll * 256 > kFusedLdsBytes) return false;
+    if (!h->lb_flags) {
+        if (hipMalloc(&h->lb_flags, 2 * sizeof(unsigned long long) * (size_t)h->nGroup) != hipSuccess ||
+            hipMalloc(&h->lb_abort, sizeof(unsigned)) != hipSuccess ||
+            hipMemset(h->lb_flags, 0, 2 * sizeof(unsigned long long) * (size_t)h->nGroup) != hipSuccess ||
+            hipMemset(h->lb_abort, 0, sizeof(unsigned)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (h->lb_flags) (void)hipFree(h->lb_flags);
+            if (h->lb_abort) (void)hipFree(h->lb_abort);
+            h->lb_flags = nullptr;
+            h->lb_abort = nullptr;
+            return false;
+        }
+    }
+    return true;
+}
+
+int launch_convert_fused(gcoo_matrix *h, const float *S, hipStream_t s)
+{
+    const int KB = mask_blocks(h);
+    const int lcap = (int)((kFusedLdsBytes - 144ll * KB) / 32) & ~63;
+    const size_t lds = 144 * (size_t)KB + 32 * (size_t)lcap;
+    h->lb_epoch = (h->lb_epoch + 1) & kLbEpochMask;
+    if (h->lb_epoch == 0) h->lb_epoch = 1;
+    h->mask_valid = false;
+    hipLaunchKernelGGL(convert_fused_kernel, dim3(h->nGroup), dim3(kFusedThreads), lds, s, S, h->M,
+                       h->K, KB, h->nGroup, lcap, h->lb_epoch, h->lb_flags, h->lb_abort,
+                       h->groupIndex(), h->nnzPerGroup(), h->rowptr, h->d_status,
+                       (long long)h->cap, h->vals, h->rows(), h->cols(), h->seg);
+    GCOO_CHECK_LAUNCH("convert_fused_kernel");
+    return GCOO_OK;
+}
 
 int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s)
 {

@@ -44,7 +44,7 @@ struct ConvStatus {
     int overflow;
     int invalid;
     int maxRow;  // most entries in one row
-    int pad;
+    int stall;   // the single-pass conversion gave up waiting for a predecessor
 };
 
 }  // namespace gcoo
@@ -100,6 +100,13 @@ struct gcoo_matrix {
     mutable int plist_tk = 0;                 // chunk length they were built for
     mutable int plist_rowb = 0;               // bytes of a staged D row they encode
 
+    // single-pass conversion: per-group look-back flags (two (state, epoch,
+    // value) words per group, never cleared: each conversion has its own
+    // epoch) and the abort word (zero unless a conversion stalled)
+    unsigned long long *lb_flags = nullptr;  // [2 * nGroup]
+    unsigned *lb_abort = nullptr;            // [1]
+    unsigned lb_epoch = 0;
+
     // multiply-kernel configuration (gcoo_set_spdm_config)
     gcoo_spdm_config spdm_cfg = {0, 0, 0, 0, 0};
 
@@ -121,6 +128,13 @@ namespace gcoo {
 // ---- conversion (gcoo_convert.hip) ----
 // Enqueue count + scan into h->idx and h->d_status.
 int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s);
+// Count, scan and scatter in one kernel that reads S once (16-byte path,
+// cpg 4); fused_conversion_ok says whether it applies (and allocates its
+// look-back state on first use).  Groups whose entries do not fit the
+// capacity skip their writes; the status word carries the total as after
+// launch_count_and_scan.
+bool fused_conversion_ok(gcoo_matrix *h, const float *S);
+int launch_convert_fused(gcoo_matrix *h, const float *S, hipStream_t s);
 // Enqueue the row-pointer scan alone (rowcnt -> rowptr, groupIndex, ...).
 int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid);
 // Enqueue the scatter.  Writes nothing when d_status reports overflow.

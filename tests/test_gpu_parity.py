@@ -39,6 +39,7 @@ EDGE_SHAPES = [
     (1, 17, 1, 0.4),
     (64, 1000, 300, 0.02),
     (257, 515, 129, 0.1),
+    (8, 4000, 16, 0.5),   # rows longer than the single-pass conversion's LDS list
 ]
 
 
