@@ -35,6 +35,7 @@
 // time.  Same entries, same order: only where the values come from changes.
 #include "gcoo_internal.h"
 
+#include <algorithm>
 #include <climits>
 
 namespace gcoo {
@@ -921,7 +922,14 @@ bool fused_conversion_ok(gcoo_matrix *h, const float *S)
 int launch_convert_fused(gcoo_matrix *h, const float *S, hipStream_t s)
 {
     const int KB = mask_blocks(h);
-    const int lcap = (int)((kFusedLdsBytes - 144ll * KB) / 32) & ~63;
+    // LDS list length per row: what the budget allows, or -- once the handle
+    // has converted before -- its last longest row plus an eighth, so that
+    // more workgroups fit on a CU (a longer row only takes the slower form)
+    int lcap = (int)((kFusedLdsBytes - 144ll * KB) / 32) & ~63;
+    if (h->maxRow > 0) {
+        const long long want = ((long long)h->maxRow + h->maxRow / 8 + 64 + 63) & ~63ll;
+        lcap = (int)std::min<long long>(lcap, std::max<long long>(256, want));
+    }
     const size_t lds = 144 * (size_t)KB + 32 * (size_t)lcap;
     h->lb_epoch = (h->lb_epoch + 1) & kLbEpochMask;
     if (h->lb_epoch == 0) h->lb_epoch = 1;
