@@ -164,17 +164,12 @@ constexpr int kPanelRows = 256;
 constexpr int kPanelRowsPerWave = 16;
 // Enqueue (on s) the panel lists of h's entries from seg (h->seg or
 // h->seg16) for k chunks of tk rows staged at rowb bytes per D row, unless
-// they are current.  Kernels exit early when guard is non-null and
+// they are current.  Wave w of panel p has nC + 1 list offsets at
+// poff[(p * 16 + w) * (nC + 1)]: its chunk c entries are
+// plist[off[c] .. off[c + 1]), ordered by (row, k).  Kernels exit early when guard is non-null and
 // guard->total > h->cap.
 int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
                        const ConvStatus *guard, hipStream_t s);
-// Ints between the offset arrays of consecutive panels: nC * kPanelRows + 1
-// offsets, padded to a multiple of 4 so that every panel's array is 16-byte
-// aligned.
-__host__ __device__ inline size_t panel_stride(int nC)
-{
-    return ((size_t)nC * kPanelRows + 1 + 3) & ~(size_t)3;
-}
 // Forget the panel lists (after a conversion or a capacity change).
 inline void invalidate_panel_lists(const gcoo_matrix *h) { h->plist_src = nullptr; }
 

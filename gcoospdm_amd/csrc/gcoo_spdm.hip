@@ -818,7 +818,8 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     };
 
     f32x16 a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
-    const int *op = poff + (size_t)panel * panel_stride(nC) + wave * RPW;
+    // this wave's chunk list starts (nC + 1 of them, the last its end)
+    const int *op = poff + ((size_t)panel * (kPanelRows / RPW) + wave) * ((size_t)nC + 1);
     const unsigned char *ldsLane = reinterpret_cast<const unsigned char *>(sD) + lane * LB;
 
     // This wave's list bounds of chunks [cw, cw + 64), lane i holding chunk
@@ -831,8 +832,8 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         vpb = 0;
         vpe = 0;
         if (cw + lane < nC) {
-            vpb = op[(size_t)(cw + lane) * kPanelRows];
-            vpe = op[(size_t)(cw + lane) * kPanelRows + RPW];
+            vpb = op[cw + lane];
+            vpe = op[cw + lane + 1];
         }
         asm volatile("" : "+v"(vpb), "+v"(vpe));  // the wait for the loads lands here
     };

@@ -222,6 +222,7 @@ def test_kernel_variants_bit_exact(dev, variant, vec, M, K, N, dens):
     (5, 3, 8, 0.5),            # one partial panel, one partial chunk, one partial tile
     (1000, 5000, 256, 0.005),  # most (row, chunk) lists empty
     (256, 10000, 512, 0.01),   # 134 chunks: three 64-chunk windows
+    (300, 80000, 256, 0.001),  # 1067 chunks: list counts beyond LDS
     (64, 2000, 1030, 0.05),    # N * 4 not a multiple of 16: row-segment fallback
 ])
 def test_panel_variant_shapes(dev, M, K, N, dens):
