@@ -655,7 +655,7 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPanelPrefetch = 6;  // chunks between the L2 touch and the staging
-constexpr double kPanelMinDensity = 0.0045;  // automatic choice of variant 6 (FP32) ...
+constexpr double kPanelMinDensity = 0.003;  // automatic choice of variant 6 (FP32) ...
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
 // a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform row slot
