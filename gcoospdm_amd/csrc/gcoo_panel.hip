@@ -28,7 +28,7 @@
 namespace gcoo {
 namespace {
 
-constexpr int kListThreads = 256;
+constexpr int kListThreads = 1024;  // 16 waves: one row of the 16 each
 constexpr int kListLdsInts = 16384;  // 64 KB of counts in LDS
 
 // For every active lane, the index of the first entry of the run (same
@@ -43,8 +43,8 @@ __device__ __forceinline__ int run_start(bool first, int i0, int lane, int &carr
     return s;
 }
 
-template <bool LDS>
-__global__ __launch_bounds__(kListThreads) void panel_wave_lists_kernel(
+template <bool LDS, int NT>
+__global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int rowb,
     int nC, int *__restrict__ poff, int *__restrict__ gcnt, uint2 *__restrict__ plist,
     const ConvStatus *__restrict__ guard, long long cap)
@@ -63,10 +63,10 @@ __global__ __launch_bounds__(kListThreads) void panel_wave_lists_kernel(
     const int R1 = (int)(r0 + RPW < M ? r0 + RPW : M);
     const int base = rowptr[R0];
 
-    for (int i = tid; i < L; i += kListThreads) cnt[i] = 0;
+    for (int i = tid; i < L; i += NT) cnt[i] = 0;
     __syncthreads();
     // 1. run lengths per (chunk, row)
-    for (int rr = wave; rr < R1 - R0; rr += kListThreads / 64) {
+    for (int rr = wave; rr < R1 - R0; rr += NT / 64) {
         const int b = rowptr[R0 + rr], e = rowptr[R0 + rr + 1];
         int carry = b;
         for (int i0 = b; i0 < e; i0 += 64) {
@@ -83,8 +83,8 @@ __global__ __launch_bounds__(kListThreads) void panel_wave_lists_kernel(
     // 2. exclusive scan in (chunk, row) order: thread t takes a slice of
     // `per` counts
     {
-        __shared__ int wsum[kListThreads / 64];
-        const int per = (L + kListThreads - 1) / kListThreads;
+        __shared__ int wsum[NT / 64];
+        const int per = (L + NT - 1) / NT;
         const int a0 = tid * per;
         int sum = 0;
         for (int i = 0; i < per; ++i)
@@ -104,10 +104,10 @@ __global__ __launch_bounds__(kListThreads) void panel_wave_lists_kernel(
         }
     }
     __syncthreads();
-    for (int c = tid; c < nC; c += kListThreads) op[c] = cnt[c * RPW];
+    for (int c = tid; c < nC; c += NT) op[c] = cnt[c * RPW];
     if (tid == 0) op[nC] = rowptr[R1];
     // 3. entries to their slots
-    for (int rr = wave; rr < R1 - R0; rr += kListThreads / 64) {
+    for (int rr = wave; rr < R1 - R0; rr += NT / 64) {
         const int b = rowptr[R0 + rr], e = rowptr[R0 + rr + 1];
         int carry = b;
         for (int i0 = b; i0 < e; i0 += 64) {
@@ -161,11 +161,11 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
     if (nW > 0) {
         int *gcnt = h->poff + nW * ((size_t)nC + 1);
         if (lds)
-            hipLaunchKernelGGL(panel_wave_lists_kernel<true>, dim3((unsigned)nW),
+            hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
                                rowb, nC, h->poff, gcnt, h->plist, guard, (long long)h->cap);
         else
-            hipLaunchKernelGGL(panel_wave_lists_kernel<false>, dim3((unsigned)nW),
+            hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC,
                                h->poff, gcnt, h->plist, guard, (long long)h->cap);
         GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
