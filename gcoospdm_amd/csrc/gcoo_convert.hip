@@ -899,7 +899,9 @@ bool use_vec4(const gcoo_matrix *h, const float *S)
 
 bool fused_conversion_ok(gcoo_matrix *h, const float *S)
 {
-    if (h->cpg != kFusedCpg || h->nGroup == 0 || !use_vec4(h, S)) return false;
+    // the first loads of a row are issued before any test of K (their index
+    // clamped to the row's last 16 bytes): the kernel needs K >= 4
+    if (h->cpg != kFusedCpg || h->nGroup == 0 || h->K == 0 || !use_vec4(h, S)) return false;
     const int KB = mask_blocks(h);
     const long long fixed = 144ll * KB;  // masks + block counts
     if (fixed + 32ll * 256 > kFusedLdsBytes) return false;

@@ -40,6 +40,7 @@ EDGE_SHAPES = [
     (64, 1000, 300, 0.02),
     (257, 515, 129, 0.1),
     (8, 4000, 16, 0.5),   # rows longer than the single-pass conversion's LDS list
+    (6, 0, 5, 0.5),       # K = 0: no entries, C all +0.0
 ]
 
 
