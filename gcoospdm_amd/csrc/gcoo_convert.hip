@@ -7,8 +7,8 @@
 //                  group's cpg rows, per-row wave64 shuffle + LDS reductions;
 //   scan_kernel    one 1024-thread workgroup: exclusive scan of the per-row
 //                  counts into row pointers (the reference scans per group on
-//                  a single thread), from which groupIndex / nnzPerGroup are
-//                  read off; 64-bit total and overflow flag in a status word;
+//                  a single thread), groupIndex / nnzPerGroup in the same
+//                  pass; 64-bit total and overflow flag in a status word;
 //   scatter_kernel one workgroup per group, k-chunks of 1024 columns, one
 //                  64-bit wave64 prefix sum carrying the cpg per-row counts in
 //                  16-bit fields; writes every entry twice:
@@ -44,7 +44,7 @@ namespace {
 constexpr int kConvThreads = 256;
 constexpr int kConvWaves = kConvThreads / 64;
 constexpr int kScanThreads = 1024;
-constexpr int kScanItems = 8;
+constexpr int kScanItems = 16;  // rows per thread per pass
 
 // `v != 0.0` exactly as the reference tests it (src/gcoo_spdm_kernels.cu:17):
 // true for NaN and denormals, false for +0.0 and -0.0.
@@ -145,54 +145,113 @@ __global__ __launch_bounds__(kConvThreads) void count_mask_kernel(
     }
 }
 
-// rowptr[0..M] = exclusive scan of rowcnt[0..M) (rowptr[M] = total); then
+// rowptr[0..M] = exclusive scan of rowcnt[0..M) (rowptr[M] = total),
 // groupIndex[g] = rowptr[min(g*cpg, M)] for g <= nGroup and
 // nnzPerGroup[g] = groupIndex[g+1] - groupIndex[g], nnzPerGroup[nGroup] = 0
 // (the reference's scan of the group counts, src/gcoo_spdm_kernels.cu:160-168,
-// and the memset of the shared allocation, src/ShGCOOSpDM.cu:37).
+// and the memset of the shared allocation, src/ShGCOOSpDM.cu:37).  With
+// `longList` it also lists the rows of more than `longMin` entries in
+// ascending order and reports their number in st->longRows.
+// One workgroup: thread t takes kScanItems consecutive rows per pass (four
+// 16-byte loads and stores), and a group's offsets come from the thread
+// holding its first row (CPG divides kScanItems, so its end is there too).
+template <int CPG>
 __global__ __launch_bounds__(kScanThreads) void scan_kernel(
-    const int *__restrict__ rowcnt, int *__restrict__ rowptr, int M, int cpg, int nGroup,
+    const int *__restrict__ rowcnt, int *__restrict__ rowptr, int M, int nGroup,
     int *__restrict__ groupIndex, int *__restrict__ nnzPerGroup,
-    ConvStatus *__restrict__ st, int clear_invalid)
+    ConvStatus *__restrict__ st, int clear_invalid, int *__restrict__ longList, int longMin)
 {
+    static_assert(kScanItems % 4 == 0 && kScanItems % CPG == 0, "rows per thread");
     __shared__ long long wtot[kScanThreads / 64];
+    __shared__ int wlong[kScanThreads / 64];
     __shared__ int wmax[kScanThreads / 64];
     const int tid = threadIdx.x, lane = __lane_id(), wave = tid >> 6;
     long long carry = 0;
-    int vmax = 0;
+    int lcarry = 0, vmax = 0;
     for (int base = 0; base < M; base += kScanThreads * kScanItems) {
+        const int r0 = base + tid * kScanItems;
+        const bool whole = r0 + kScanItems <= M;
         int v[kScanItems];
+        if (whole) {
+            const int4 *p = reinterpret_cast<const int4 *>(rowcnt + r0);
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q) {
+                const int4 x = p[q];
+                v[4 * q] = x.x;
+                v[4 * q + 1] = x.y;
+                v[4 * q + 2] = x.z;
+                v[4 * q + 3] = x.w;
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kScanItems; ++i) v[i] = r0 + i < M ? rowcnt[r0 + i] : 0;
+        }
         long long tsum = 0;
+        int tlong = 0;
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) {
-            int idx = base + tid * kScanItems + i;
-            v[i] = idx < M ? rowcnt[idx] : 0;
             tsum += v[i];
+            tlong += v[i] > longMin ? 1 : 0;
             vmax = max(vmax, v[i]);
         }
         long long incl = tsum;
+        int lincl = tlong;
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            long long t = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += t;
+            const long long t = __shfl_up(incl, d, 64);
+            const int lt = __shfl_up(lincl, d, 64);
+            if (lane >= d) {
+                incl += t;
+                lincl += lt;
+            }
         }
-        if (lane == 63) wtot[wave] = incl;
+        if (lane == 63) {
+            wtot[wave] = incl;
+            wlong[wave] = lincl;
+        }
         __syncthreads();
         long long before = 0, tile = 0;
+        int lbefore = 0, ltile = 0;
 #pragma unroll
         for (int w = 0; w < kScanThreads / 64; ++w) {
-            long long x = wtot[w];
+            const long long x = wtot[w];
+            const int lx = wlong[w];
             before += (w < wave) ? x : 0;
+            lbefore += (w < wave) ? lx : 0;
             tile += x;
+            ltile += lx;
         }
         long long run = carry + before + incl - tsum;
+        int lrun = lcarry + lbefore + lincl - tlong;
+        int out[kScanItems];
 #pragma unroll
         for (int i = 0; i < kScanItems; ++i) {
-            int idx = base + tid * kScanItems + i;
-            if (idx < M) rowptr[idx] = (int)run;
+            out[i] = (int)run;
+            if (longList != nullptr && v[i] > longMin) longList[lrun++] = r0 + i;
             run += v[i];
         }
+        const int endv = (int)run;  // rowptr at r0 + kScanItems (or at M)
+        if (whole) {
+            int4 *p = reinterpret_cast<int4 *>(rowptr + r0);
+#pragma unroll
+            for (int q = 0; q < kScanItems / 4; ++q)
+                p[q] = make_int4(out[4 * q], out[4 * q + 1], out[4 * q + 2], out[4 * q + 3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < kScanItems; ++i)
+                if (r0 + i < M) rowptr[r0 + i] = out[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kScanItems; i += CPG) {
+            if (r0 + i < M) {
+                const int g = (r0 + i) / CPG;
+                const int e = i + CPG < kScanItems ? out[i + CPG] : endv;
+                groupIndex[g] = out[i];
+                nnzPerGroup[g] = e - out[i];
+            }
+        }
         carry += tile;
+        lcarry += ltile;
         __syncthreads();
     }
 #pragma unroll
@@ -203,23 +262,14 @@ __global__ __launch_bounds__(kScanThreads) void scan_kernel(
         int m = 0;
         for (int w = 0; w < kScanThreads / 64; ++w) m = max(m, wmax[w]);
         rowptr[M] = (int)carry;
+        groupIndex[nGroup] = (int)carry;
+        nnzPerGroup[nGroup] = 0;
         st->total = carry;
         st->overflow = carry > (long long)INT_MAX ? 1 : 0;
         st->maxRow = m;
         st->stall = 0;
+        st->longRows = longList != nullptr ? lcarry : 0;
         if (clear_invalid) st->invalid = 0;
-    }
-    __syncthreads();
-    for (int g = tid; g <= nGroup; g += kScanThreads) {
-        const long long r = (long long)g * cpg;
-        const int a = rowptr[r < M ? r : M];
-        groupIndex[g] = a;
-        if (g < nGroup) {
-            const long long r1 = r + cpg;
-            nnzPerGroup[g] = rowptr[r1 < M ? r1 : M] - a;
-        } else {
-            nnzPerGroup[g] = 0;
-        }
     }
 }
 
@@ -961,11 +1011,28 @@ int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s)
     return launch_row_scan(h, s, true);
 }
 
-int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid)
+int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid, int *longList,
+                    int longMin)
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(kScanThreads), 0, s, h->rowcnt, h->rowptr,
-                       h->M, h->cpg, h->nGroup, h->groupIndex(), h->nnzPerGroup(),
-                       h->d_status, clear_invalid ? 1 : 0);
+    const int ci = clear_invalid ? 1 : 0;
+    switch (h->cpg) {
+    case 1:
+        hipLaunchKernelGGL(scan_kernel<1>, dim3(1), dim3(kScanThreads), 0, s, h->rowcnt,
+                           h->rowptr, h->M, h->nGroup, h->groupIndex(), h->nnzPerGroup(),
+                           h->d_status, ci, longList, longMin);
+        break;
+    case 2:
+        hipLaunchKernelGGL(scan_kernel<2>, dim3(1), dim3(kScanThreads), 0, s, h->rowcnt,
+                           h->rowptr, h->M, h->nGroup, h->groupIndex(), h->nnzPerGroup(),
+                           h->d_status, ci, longList, longMin);
+        break;
+    case 4:
+        hipLaunchKernelGGL(scan_kernel<4>, dim3(1), dim3(kScanThreads), 0, s, h->rowcnt,
+                           h->rowptr, h->M, h->nGroup, h->groupIndex(), h->nnzPerGroup(),
+                           h->d_status, ci, longList, longMin);
+        break;
+    default: return set_error(GCOO_ERR_INVALID, "cpg %d not supported", h->cpg);
+    }
     GCOO_CHECK_LAUNCH("scan_kernel");
     return GCOO_OK;
 }

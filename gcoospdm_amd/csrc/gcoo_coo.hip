@@ -8,15 +8,19 @@
 // because only integer counts and slot reservations are atomic and the final
 // order is fixed by unique sort keys):
 //   1. coo_count_kernel    per-row counts of entries with val != 0.0 (integer
-//                          atomics), range checks of rows / cols;
+//                          atomics, one per run of equal rows in a wave),
+//                          range checks of rows / cols;
 //   2. scan_kernel         row pointers, groupIndex, nnzPerGroup, total
-//                          (shared with the dense path);
+//                          (shared with the dense path), and the list of rows
+//                          of more than 64 entries;
 //   3. coo_place_kernel    each entry into its row's segment at an atomically
-//                          reserved slot (unordered within the row);
+//                          reserved slot (one reservation per run of equal
+//                          rows in a wave; unordered within the row);
 //   4. sort_rows_*         every row segment sorted by k: a wave64 bitonic
-//                          network in registers for rows of <= 64 entries, an
-//                          LDS bitonic network for <= 16384, the same network
-//                          in global memory beyond;
+//                          network in registers for rows of <= 64 entries;
+//                          for the listed longer rows one workgroup each, an
+//                          LDS bitonic network for <= 16384 entries, the same
+//                          network in global memory beyond;
 //   5. gcoo_from_rows      each entry's position in its group's (k, r) order
 //                          = its rank in its row + binary searches in the
 //                          group's other rows; duplicate (r, k) are rejected.
@@ -28,35 +32,74 @@ namespace gcoo {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kWaveSortMax = 64;  // rows up to this length are sorted by one wave
 
+// Runs of equal rows among a wave's 64 consecutive entries (COO input sorted
+// by row gives long runs): whether this lane starts a run, the run's length
+// (valid in its first lane) and the lane that starts this lane's run.
+__device__ __forceinline__ void row_run(int r, int lane, bool &head, int &len, int &headLane)
+{
+    const int rp = __shfl_up(r, 1, 64);
+    head = lane == 0 || rp != r;
+    const unsigned long long hm = __ballot(head);
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1;  // lanes <= this one
+    const unsigned long long above = hm & ~upto;
+    len = (above ? __ffsll((long long)above) - 1 : 64) - lane;
+    headLane = 63 - __clzll(hm & upto);  // lane 0 always starts a run
+}
+
+// Grid-stride over whole waves: entry b + lane.  Entries outside the matrix
+// set the invalid flag; entries whose value is 0.0 are not counted.  The
+// first lane of a run of equal rows adds the run's length.
 __global__ __launch_bounds__(kThreads) void coo_count_kernel(
     const int *__restrict__ rows, const int *__restrict__ cols, const float *__restrict__ vals,
     long long nnz, int M, int K, int *__restrict__ rowcnt, ConvStatus *__restrict__ st)
 {
+    const int lane = __lane_id();
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride) {
-        const int r = rows[i], k = cols[i];
-        if (r < 0 || r >= M || k < 0 || k >= K) {
-            st->invalid = 1;
-            continue;
+    for (long long b = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b < nnz; b += stride) {
+        const long long i = b + lane;
+        int r = -1;
+        if (i < nnz) {
+            const int ri = rows[i], k = cols[i];
+            if (ri < 0 || ri >= M || k < 0 || k >= K) st->invalid = 1;
+            else if (vals[i] != 0.0f) r = ri;
         }
-        if (vals[i] != 0.0f) atomicAdd(&rowcnt[r], 1);
+        bool head;
+        int len, hl;
+        row_run(r, lane, head, len, hl);
+        if (head && r >= 0) atomicAdd(&rowcnt[r], len);
     }
 }
 
+// Each counted entry into its row's segment: the first lane of a run of
+// equal rows reserves the run's slots with one atomic, the run's lanes take
+// consecutive slots.
 __global__ __launch_bounds__(kThreads) void coo_place_kernel(
     const int *__restrict__ rows, const int *__restrict__ cols, const float *__restrict__ vals,
     long long nnz, int M, int K, const int *__restrict__ rowptr, int *__restrict__ cursor,
     const ConvStatus *__restrict__ st, long long cap, int2 *__restrict__ seg)
 {
     if (st->total > cap || st->overflow || st->invalid) return;
+    const int lane = __lane_id();
     const long long stride = (long long)gridDim.x * blockDim.x;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nnz; i += stride) {
-        const int r = rows[i], k = cols[i];
-        const float v = vals[i];
-        if (r < 0 || r >= M || k < 0 || k >= K || !(v != 0.0f)) continue;
-        const int slot = atomicAdd(&cursor[r], 1);
-        seg[rowptr[r] + slot] = make_int2(k, __float_as_int(v));
+    for (long long b = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63); b < nnz; b += stride) {
+        const long long i = b + lane;
+        int r = -1, k = 0;
+        float v = 0.0f;
+        if (i < nnz) {
+            const int ri = rows[i];
+            k = cols[i];
+            v = vals[i];
+            if (ri >= 0 && ri < M && k >= 0 && k < K && v != 0.0f) r = ri;
+        }
+        bool head;
+        int len, hl;
+        row_run(r, lane, head, len, hl);
+        int slot = 0;
+        if (head && r >= 0) slot = atomicAdd(&cursor[r], len);
+        slot = __shfl(slot, hl, 64) + (lane - hl);
+        if (r >= 0) seg[rowptr[r] + slot] = make_int2(k, __float_as_int(v));
     }
 }
 
@@ -70,7 +113,7 @@ __global__ __launch_bounds__(kThreads) void sort_rows_wave_kernel(
     const int r = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
     if (r >= M) return;
     const int p0 = rowptr[r], n = rowptr[r + 1] - p0;
-    if (n <= 1 || n > 64) return;
+    if (n <= 1 || n > kWaveSortMax) return;
     const int lane = __lane_id();
     int2 e = lane < n ? seg[p0 + lane] : make_int2(INT_MAX, 0);
     for (int size = 2; size <= 64; size <<= 1) {
@@ -127,17 +170,20 @@ struct GlobalAccess {
 
 constexpr int kSortLds = 16384;  // entries sorted in LDS (128 KB)
 
-// One workgroup per row with more than 64 entries.
+// One workgroup per listed row (more than 64 entries): its segment sorted in
+// LDS when it has at most CAP entries, in global memory otherwise.  CAP is
+// chosen from the longest row, so short lists leave room for more
+// workgroups per CU.
+template <int CAP>
 __global__ __launch_bounds__(1024) void sort_rows_block_kernel(
-    const int *__restrict__ rowptr, int M, const ConvStatus *__restrict__ st, long long cap,
-    int2 *__restrict__ seg)
+    const int *__restrict__ rowptr, const int *__restrict__ longList,
+    const ConvStatus *__restrict__ st, long long cap, int2 *__restrict__ seg)
 {
     if (st->total > cap || st->overflow || st->invalid) return;
-    const int r = blockIdx.x;
+    __shared__ int2 sm[CAP];
+    const int r = longList[blockIdx.x];
     const int p0 = rowptr[r], n = rowptr[r + 1] - p0;
-    if (n <= 64) return;
-    if (n <= kSortLds) {
-        __shared__ int2 sm[kSortLds];
+    if (n <= CAP) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = seg[p0 + i];
         __syncthreads();
         bitonic_sort(LdsAccess{sm}, n);
@@ -209,7 +255,20 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols, const float *v
                            cols, vals, (long long)nnz, h->M, h->K, h->rowcnt, h->d_status);
         GCOO_CHECK_LAUNCH("coo_count_kernel");
     }
-    int rc = launch_row_scan(h, s, false);
+    const size_t listBytes = sizeof(int) * ((size_t)h->M + 1);
+    if (h->coo_tmp_bytes < listBytes) {
+        if (h->coo_tmp) GCOO_HIP_TRY(hipFree(h->coo_tmp));
+        h->coo_tmp = nullptr;
+        h->coo_tmp_bytes = 0;
+        if (hipMalloc(&h->coo_tmp, listBytes) != hipSuccess) {
+            (void)hipGetLastError();
+            h->coo_tmp = nullptr;
+            return set_error(GCOO_ERR_ALLOC, "cannot allocate the long-row list");
+        }
+        h->coo_tmp_bytes = listBytes;
+    }
+    int *longList = static_cast<int *>(h->coo_tmp);
+    int rc = launch_row_scan(h, s, false, longList, kWaveSortMax);
     if (rc) return rc;
     GCOO_HIP_TRY(hipMemcpyAsync(h->h_status, h->d_status, sizeof(ConvStatus),
                                 hipMemcpyDeviceToHost, s));
@@ -231,9 +290,21 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols, const float *v
         hipLaunchKernelGGL(sort_rows_wave_kernel, dim3(rowBlocks), dim3(kThreads), 0, s,
                            h->rowptr, h->M, h->d_status, (long long)h->cap, h->seg);
         GCOO_CHECK_LAUNCH("sort_rows_wave_kernel");
-        hipLaunchKernelGGL(sort_rows_block_kernel, dim3((unsigned)h->M), dim3(1024), 0, s,
-                           h->rowptr, h->M, h->d_status, (long long)h->cap, h->seg);
-        GCOO_CHECK_LAUNCH("sort_rows_block_kernel");
+        const int nLong = h->h_status->longRows;
+        if (nLong > 0) {
+            const dim3 grid((unsigned)nLong), block(1024);
+            const int longest = h->h_status->maxRow;
+            if (longest <= 1024)
+                hipLaunchKernelGGL(sort_rows_block_kernel<1024>, grid, block, 0, s, h->rowptr,
+                                   longList, h->d_status, (long long)h->cap, h->seg);
+            else if (longest <= 4096)
+                hipLaunchKernelGGL(sort_rows_block_kernel<4096>, grid, block, 0, s, h->rowptr,
+                                   longList, h->d_status, (long long)h->cap, h->seg);
+            else
+                hipLaunchKernelGGL(sort_rows_block_kernel<kSortLds>, grid, block, 0, s,
+                                   h->rowptr, longList, h->d_status, (long long)h->cap, h->seg);
+            GCOO_CHECK_LAUNCH("sort_rows_block_kernel");
+        }
         hipLaunchKernelGGL(gcoo_from_rows_kernel, dim3(rowBlocks), dim3(kThreads), 0, s,
                            h->rowptr, h->groupIndex(), h->M, h->cpg, h->seg, h->d_status,
                            (long long)h->cap, h->vals, h->rows(), h->cols());

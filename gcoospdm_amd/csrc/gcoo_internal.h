@@ -45,6 +45,7 @@ struct ConvStatus {
     int invalid;
     int maxRow;  // most entries in one row
     int stall;   // the single-pass conversion gave up waiting for a predecessor
+    int longRows;  // rows listed by the row scan as longer than its threshold
 };
 
 }  // namespace gcoo
@@ -113,7 +114,7 @@ struct gcoo_matrix {
     gcoo::ConvStatus *d_status = nullptr;  // device
     gcoo::ConvStatus *h_status = nullptr;  // pinned host mirror
 
-    // scratch for the COO path (keys / permutation), grown on demand
+    // scratch for the COO path (the rows to sort by workgroup), grown on demand
     void *coo_tmp = nullptr;
     size_t coo_tmp_bytes = 0;
 
@@ -135,8 +136,10 @@ int launch_count_and_scan(gcoo_matrix *h, const float *S, hipStream_t s);
 // launch_count_and_scan.
 bool fused_conversion_ok(gcoo_matrix *h, const float *S);
 int launch_convert_fused(gcoo_matrix *h, const float *S, hipStream_t s);
-// Enqueue the row-pointer scan alone (rowcnt -> rowptr, groupIndex, ...).
-int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid);
+// Enqueue the row-pointer scan alone (rowcnt -> rowptr, groupIndex, ...);
+// with longList (M ints) also the rows of more than longMin entries.
+int launch_row_scan(gcoo_matrix *h, hipStream_t s, bool clear_invalid, int *longList = nullptr,
+                    int longMin = 0);
 // Enqueue the scatter.  Writes nothing when d_status reports overflow.
 int launch_scatter(gcoo_matrix *h, const float *S, hipStream_t s);
 // Grow entry capacity to at least need (discarding contents).

@@ -41,6 +41,8 @@ EDGE_SHAPES = [
     (257, 515, 129, 0.1),
     (8, 4000, 16, 0.5),   # rows longer than the single-pass conversion's LDS list
     (6, 0, 5, 0.5),       # K = 0: no entries, C all +0.0
+    (0, 8, 3, 0.5),       # M = 0: no groups
+    (5, 8, 0, 0.5),       # N = 0: nothing to write
 ]
 
 
@@ -264,7 +266,9 @@ def test_row_segments_match_gcoo(dev):
     assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(oracle.convert(S, 4), D))
 
 
-def _coo_of(S, rng, extra_zeros=0):
+def _coo_of(S, rng, extra_zeros=0, row_sorted=False):
+    """COO arrays of S in random order, or (row_sorted) sorted by row with
+    the columns of a row in random order."""
     r, k = np.nonzero(S != 0.0)
     v = S[r, k]
     if extra_zeros:   # explicit zeros in the COO input are dropped
@@ -278,21 +282,25 @@ def _coo_of(S, rng, extra_zeros=0):
         k = np.concatenate([k, zk])
         v = np.concatenate([v, np.zeros(zr.size, np.float32)])
     perm = rng.permutation(r.size)
+    if row_sorted:
+        perm = perm[np.argsort(r[perm], kind="stable")]
     return (r[perm].astype(np.int32), k[perm].astype(np.int32), v[perm].astype(np.float32))
 
 
+@pytest.mark.parametrize("row_sorted", [False, True])
 @pytest.mark.parametrize("cpg", [1, 2, 4])
 @pytest.mark.parametrize("M,K,dens", [(37, 50, 0.1), (200, 3000, 0.05), (9, 20000, 0.5),
-                                      (64, 64, 0.0)])
-def test_coo_conversion_matches_dense(dev, M, K, dens, cpg):
-    """Shuffled COO input (with explicit zeros) gives the dense path's
-    arrays bit for bit; rows of > 64 and > 16384 entries take the LDS and
-    global-memory sorts."""
+                                      (5, 40000, 0.5), (64, 64, 0.0)])
+def test_coo_conversion_matches_dense(dev, M, K, dens, cpg, row_sorted):
+    """COO input (with explicit zeros), shuffled or sorted by row, gives the
+    dense path's arrays bit for bit; rows of > 64, > 8192 and > 16384
+    entries take the LDS sort (beyond 64 KB of LDS from 8192) and the
+    global-memory sort."""
     import torch
-    print(f"coo M={M} K={K} dens={dens} cpg={cpg}", flush=True)
+    print(f"coo M={M} K={K} dens={dens} cpg={cpg} row_sorted={row_sorted}", flush=True)
     rng = np.random.default_rng(M * 31 + K + cpg)
     S = H.rng_sparse(rng, M, K, dens)
-    r, k, v = _coo_of(S, rng, extra_zeros=17)
+    r, k, v = _coo_of(S, rng, extra_zeros=17, row_sorted=row_sorted)
     g = oracle.convert(S, cpg)
     m = G.GCOOMatrix(M, K, cpg).convert_coo(to_dev(r, dev), to_dev(k, dev), to_dev(v, dev))
     check_arrays(m, g)
