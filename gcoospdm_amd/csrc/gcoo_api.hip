@@ -365,6 +365,7 @@ int gcoo_destroy(gcoo_handle h)
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->coo_tmp) (void)hipFree(h->coo_tmp);
+    if (h->coo_rowof) (void)hipFree(h->coo_rowof);
     delete h;
     return GCOO_OK;
 }

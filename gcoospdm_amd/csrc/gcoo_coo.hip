@@ -18,12 +18,14 @@
 //                          rows in a wave; unordered within the row);
 //   4. sort_rows_*         every row segment sorted by k: a wave64 bitonic
 //                          network in registers for rows of <= 64 entries;
-//                          for the listed longer rows one workgroup each, an
-//                          LDS bitonic network for <= 16384 entries, the same
-//                          network in global memory beyond;
-//   5. gcoo_from_rows      each entry's position in its group's (k, r) order
-//                          = its rank in its row + binary searches in the
-//                          group's other rows; duplicate (r, k) are rejected.
+//                          the rows the scan lists as longer, one workgroup
+//                          each sized to the row (64, 256 or 1024 threads),
+//                          in LDS up to 16384 entries and in global memory
+//                          beyond;
+//   5. gcoo_from_rows      one thread per entry: its position in its group's
+//                          (k, r) order = its rank in its row + binary
+//                          searches in the group's other rows; duplicate
+//                          (r, k) are rejected.
 #include "gcoo_internal.h"
 
 #include <climits>
@@ -72,13 +74,14 @@ __global__ __launch_bounds__(kThreads) void coo_count_kernel(
     }
 }
 
-// Each counted entry into its row's segment: the first lane of a run of
-// equal rows reserves the run's slots with one atomic, the run's lanes take
-// consecutive slots.
+// Each counted entry into its row's segment (and its row into `rowof` at
+// the same place): the first lane of a run of equal rows reserves the run's
+// slots with one atomic, the run's lanes take consecutive slots.
 __global__ __launch_bounds__(kThreads) void coo_place_kernel(
     const int *__restrict__ rows, const int *__restrict__ cols, const float *__restrict__ vals,
     long long nnz, int M, int K, const int *__restrict__ rowptr, int *__restrict__ cursor,
-    const ConvStatus *__restrict__ st, long long cap, int2 *__restrict__ seg)
+    const ConvStatus *__restrict__ st, long long cap, int2 *__restrict__ seg,
+    int *__restrict__ rowof)
 {
     if (st->total > cap || st->overflow || st->invalid) return;
     const int lane = __lane_id();
@@ -99,7 +102,11 @@ __global__ __launch_bounds__(kThreads) void coo_place_kernel(
         int slot = 0;
         if (head && r >= 0) slot = atomicAdd(&cursor[r], len);
         slot = __shfl(slot, hl, 64) + (lane - hl);
-        if (r >= 0) seg[rowptr[r] + slot] = make_int2(k, __float_as_int(v));
+        if (r >= 0) {
+            const int pos = rowptr[r] + slot;
+            seg[pos] = make_int2(k, __float_as_int(v));
+            rowof[pos] = r;
+        }
     }
 }
 
@@ -170,19 +177,21 @@ struct GlobalAccess {
 
 constexpr int kSortLds = 16384;  // entries sorted in LDS (128 KB)
 
-// One workgroup per listed row (more than 64 entries): its segment sorted in
-// LDS when it has at most CAP entries, in global memory otherwise.  CAP is
-// chosen from the longest row, so short lists leave room for more
-// workgroups per CU.
-template <int CAP>
-__global__ __launch_bounds__(1024) void sort_rows_block_kernel(
-    const int *__restrict__ rowptr, const int *__restrict__ longList,
+// One workgroup of NT threads per listed row of minN < n <= maxN entries:
+// its segment sorted in LDS when it has at most CAP entries, in global memory
+// otherwise.  Three size classes (NT = 64, 256, 1024) give each thread about
+// one compare-exchange per stage; a workgroup whose row is in another class
+// exits at once.
+template <int CAP, int NT>
+__global__ __launch_bounds__(NT) void sort_rows_block_kernel(
+    const int *__restrict__ rowptr, const int *__restrict__ longList, int minN, int maxN,
     const ConvStatus *__restrict__ st, long long cap, int2 *__restrict__ seg)
 {
     if (st->total > cap || st->overflow || st->invalid) return;
     __shared__ int2 sm[CAP];
     const int r = longList[blockIdx.x];
     const int p0 = rowptr[r], n = rowptr[r + 1] - p0;
+    if (n <= minN || n > maxN) return;
     if (n <= CAP) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) sm[i] = seg[p0 + i];
         __syncthreads();
@@ -204,35 +213,37 @@ __device__ __forceinline__ int lower_bound_k(const int2 *__restrict__ s, int n, 
     return lo;
 }
 
-// One wave per row: entry i of row r (sorted by k) goes to
+// One thread per entry e of the row segments: entry i of row r (sorted by
+// k) goes to
 //   groupIndex[g] + i + sum over the group's other rows r' of
 //   #{k' <= k} for r' < r  and  #{k' < k} for r' > r,
 // i.e. its place in the group's (k, r) order.
 __global__ __launch_bounds__(kThreads) void gcoo_from_rows_kernel(
-    const int *__restrict__ rowptr, const int *__restrict__ groupIndex, int M, int cpg,
-    const int2 *__restrict__ seg, ConvStatus *__restrict__ st, long long cap,
-    float *__restrict__ vals, int *__restrict__ rows, int *__restrict__ cols)
+    const int *__restrict__ rowptr, const int *__restrict__ groupIndex,
+    const int *__restrict__ rowof, int total, int M, int cpg, const int2 *__restrict__ seg,
+    ConvStatus *__restrict__ st, long long cap, float *__restrict__ vals,
+    int *__restrict__ rows, int *__restrict__ cols)
 {
     if (st->total > cap || st->overflow || st->invalid) return;
-    const int r = blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
-    if (r >= M) return;
+    const int e = blockIdx.x * kThreads + threadIdx.x;
+    if (e >= total) return;
+    const int r = rowof[e];
+    const int p0 = rowptr[r];
+    const int i = e - p0;
+    const int2 x = seg[e];
+    if (i > 0 && seg[e - 1].x == x.x) st->invalid = 2;  // duplicate (r, k)
     const int g = r / cpg;
     const int rg0 = g * cpg;
     const int rg1 = min(rg0 + cpg, M);
-    const int p0 = rowptr[r], n = rowptr[r + 1] - p0;
-    for (int i = __lane_id(); i < n; i += 64) {
-        const int2 e = seg[p0 + i];
-        if (i > 0 && seg[p0 + i - 1].x == e.x) st->invalid = 2;  // duplicate (r, k)
-        int pos = groupIndex[g] + i;
-        for (int r2 = rg0; r2 < rg1; ++r2) {
-            if (r2 == r) continue;
-            const int q0 = rowptr[r2], qn = rowptr[r2 + 1] - q0;
-            pos += lower_bound_k(seg + q0, qn, r2 < r ? e.x + 1 : e.x);
-        }
-        vals[pos] = __int_as_float(e.y);
-        rows[pos] = r;
-        cols[pos] = e.x;
+    int pos = groupIndex[g] + i;
+    for (int r2 = rg0; r2 < rg1; ++r2) {
+        if (r2 == r) continue;
+        const int q0 = rowptr[r2], qn = rowptr[r2 + 1] - q0;
+        pos += lower_bound_k(seg + q0, qn, r2 < r ? x.x + 1 : x.x);
     }
+    vals[pos] = __int_as_float(x.y);
+    rows[pos] = r;
+    cols[pos] = x.x;
 }
 
 unsigned grid_for(long long n)
@@ -280,34 +291,53 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols, const float *v
                          (long long)h->h_status->total);
     const int64_t total = h->h_status->total;
     if ((rc = ensure_capacity(h, total + h->h_status->maxRow + kSegSlack))) return rc;
+    if ((size_t)total > h->coo_rowof_elems) {
+        if (h->coo_rowof) GCOO_HIP_TRY(hipFree(h->coo_rowof));
+        h->coo_rowof = nullptr;
+        h->coo_rowof_elems = 0;
+        if (hipMalloc(&h->coo_rowof, sizeof(int) * (size_t)total) != hipSuccess) {
+            (void)hipGetLastError();
+            h->coo_rowof = nullptr;
+            return set_error(GCOO_ERR_ALLOC, "cannot allocate the entry rows");
+        }
+        h->coo_rowof_elems = (size_t)total;
+    }
     if (total > 0) {
         GCOO_HIP_TRY(hipMemsetAsync(h->rowcnt, 0, sizeof(int) * ((size_t)h->M + 1), s));
         hipLaunchKernelGGL(coo_place_kernel, dim3(grid_for(nnz)), dim3(kThreads), 0, s, rows,
                            cols, vals, (long long)nnz, h->M, h->K, h->rowptr, h->rowcnt,
-                           h->d_status, (long long)h->cap, h->seg);
+                           h->d_status, (long long)h->cap, h->seg, h->coo_rowof);
         GCOO_CHECK_LAUNCH("coo_place_kernel");
         const unsigned rowBlocks = (unsigned)((h->M + kThreads / 64 - 1) / (kThreads / 64));
         hipLaunchKernelGGL(sort_rows_wave_kernel, dim3(rowBlocks), dim3(kThreads), 0, s,
                            h->rowptr, h->M, h->d_status, (long long)h->cap, h->seg);
         GCOO_CHECK_LAUNCH("sort_rows_wave_kernel");
         const int nLong = h->h_status->longRows;
+        const int longest = h->h_status->maxRow;
         if (nLong > 0) {
-            const dim3 grid((unsigned)nLong), block(1024);
-            const int longest = h->h_status->maxRow;
-            if (longest <= 1024)
-                hipLaunchKernelGGL(sort_rows_block_kernel<1024>, grid, block, 0, s, h->rowptr,
-                                   longList, h->d_status, (long long)h->cap, h->seg);
-            else if (longest <= 4096)
-                hipLaunchKernelGGL(sort_rows_block_kernel<4096>, grid, block, 0, s, h->rowptr,
-                                   longList, h->d_status, (long long)h->cap, h->seg);
-            else
-                hipLaunchKernelGGL(sort_rows_block_kernel<kSortLds>, grid, block, 0, s,
-                                   h->rowptr, longList, h->d_status, (long long)h->cap, h->seg);
+            const dim3 grid((unsigned)nLong);
+            hipLaunchKernelGGL((sort_rows_block_kernel<128, 64>), grid, dim3(64), 0, s,
+                               h->rowptr, longList, kWaveSortMax, 128, h->d_status,
+                               (long long)h->cap, h->seg);
+            if (longest > 128)
+                hipLaunchKernelGGL((sort_rows_block_kernel<512, 256>), grid, dim3(256), 0, s,
+                                   h->rowptr, longList, 128, 512, h->d_status,
+                                   (long long)h->cap, h->seg);
+            if (longest > 512 && longest <= 4096)
+                hipLaunchKernelGGL((sort_rows_block_kernel<4096, 1024>), grid, dim3(1024), 0, s,
+                                   h->rowptr, longList, 512, INT_MAX, h->d_status,
+                                   (long long)h->cap, h->seg);
+            else if (longest > 4096)
+                hipLaunchKernelGGL((sort_rows_block_kernel<kSortLds, 1024>), grid, dim3(1024), 0,
+                                   s, h->rowptr, longList, 512, INT_MAX, h->d_status,
+                                   (long long)h->cap, h->seg);
             GCOO_CHECK_LAUNCH("sort_rows_block_kernel");
         }
-        hipLaunchKernelGGL(gcoo_from_rows_kernel, dim3(rowBlocks), dim3(kThreads), 0, s,
-                           h->rowptr, h->groupIndex(), h->M, h->cpg, h->seg, h->d_status,
-                           (long long)h->cap, h->vals, h->rows(), h->cols());
+        hipLaunchKernelGGL(gcoo_from_rows_kernel,
+                           dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                           s, h->rowptr, h->groupIndex(), h->coo_rowof, (int)total, h->M, h->cpg,
+                           h->seg, h->d_status, (long long)h->cap, h->vals, h->rows(),
+                           h->cols());
         GCOO_CHECK_LAUNCH("gcoo_from_rows_kernel");
     }
     GCOO_HIP_TRY(hipMemcpyAsync(h->h_status, h->d_status, sizeof(ConvStatus),

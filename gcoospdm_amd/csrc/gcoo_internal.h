@@ -114,9 +114,12 @@ struct gcoo_matrix {
     gcoo::ConvStatus *d_status = nullptr;  // device
     gcoo::ConvStatus *h_status = nullptr;  // pinned host mirror
 
-    // scratch for the COO path (the rows to sort by workgroup), grown on demand
+    // scratch for the COO path, grown on demand: the rows the scan lists as
+    // long, and the row of every entry of the row segments
     void *coo_tmp = nullptr;
     size_t coo_tmp_bytes = 0;
+    int *coo_rowof = nullptr;
+    size_t coo_rowof_elems = 0;
 
     int *rows() const { return rowcol; }
     int *cols() const { return rowcol + cap; }
