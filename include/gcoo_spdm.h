@@ -176,7 +176,7 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *   waves      waves per workgroup (1..16).
  *   chunk      k-chunk length of variant 3; long-row threshold of variant 4.
  *   unroll     entries whose D loads are in flight per wave in variant 2
- *              (8, 16 or 32).
+ *              (8, 16 or 32; 0 means 16).
  * All variants give bit-identical C. */
 typedef struct gcoo_spdm_config {
     int variant;
