@@ -14,6 +14,8 @@
 // is graph-capturable (no memset, no host synchronisation, no allocation).
 #include "gcoo_internal.h"
 
+#include <type_traits>
+
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -81,6 +83,12 @@ __device__ __forceinline__ void fmac_v(float &acc, float a, float d)
     asm("v_fmac_f32 %0, %1, %2" : "+v"(acc) : "v"(a), "v"(d));
 }
 
+// the same step with a wave-uniform multiplier held in a scalar register
+__device__ __forceinline__ void fmac_s(float &acc, float a, float d)
+{
+    asm("v_fmac_f32 %0, %1, %2" : "+v"(acc) : "s"(a), "v"(d));
+}
+
 // One row stream of a work item: lane (sub, lc) walks row r's segment and
 // accumulates columns j .. j+vec-1.
 template <int VEC>
@@ -128,6 +136,56 @@ __device__ __forceinline__ void run_row(const JobDesc &jd, __amdgpu_buffer_rsrc_
     }
 }
 
+// One row per wave (64-lane streams): the row is wave-uniform, so its entries
+// are read through the constant address space (scalar loads; the segments
+// are not written during the batch), and rounds of R entries (16, or 8 at 8
+// columns per lane, so the D values take at most 64 registers) issue all
+// their D loads before the first FMA, then rounds of 8, then one guarded
+// round of the last 1..7 -- the same chain order, so the same bits.
+template <int VEC>
+__device__ __forceinline__ void run_row_wave(const JobDesc &jd, __amdgpu_buffer_rsrc_t rsrc,
+                                             int r, int j, bool active)
+{
+    typedef __attribute__((address_space(4))) const int cint;
+    float acc[VEC];
+#pragma unroll
+    for (int v = 0; v < VEC; ++v) acc[v] = 0.0f;
+    const bool valid = r < jd.M;
+    int p = __builtin_amdgcn_readfirstlane(valid ? jd.rowptr[r] : 0);
+    const int pend = __builtin_amdgcn_readfirstlane(valid ? jd.rowptr[r + 1] : 0);
+    const unsigned rowBytes = (unsigned)jd.N * (unsigned)jd.bytes;
+    const unsigned voffc = active ? (unsigned)j * (unsigned)jd.bytes : jd.dBytes;
+    const bool bf = jd.bytes == 2;
+    cint *sg = (cint *)(jd.seg);  // (k, value bits) pairs; a C-style cast may change the address space
+    auto round = [&](auto bc, int q, int n) {
+        constexpr int B = decltype(bc)::value;
+        float a[B];
+        float d[B][8];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (u < n) {
+                const int ek = sg[2 * (q + u)], ev = sg[2 * (q + u) + 1];
+                a[u] = bf ? __int_as_float(ev << 16) : __int_as_float(ev);
+                load_elems(d[u], rsrc, (int)(voffc + (unsigned)ek * rowBytes), jd.bytes, VEC);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+            if (u < n)
+#pragma unroll
+                for (int v = 0; v < VEC; ++v) fmac_s(acc[v], a[u], d[u][v]);
+    };
+    constexpr int R = VEC >= 8 ? 8 : 16;
+    for (; p + R <= pend; p += R) round(std::integral_constant<int, R>{}, p, R);
+    for (; p + 8 <= pend; p += 8) round(std::integral_constant<int, 8>{}, p, 8);
+    if (p < pend) round(std::integral_constant<int, 8>{}, p, pend - p);
+    if (valid && active) {
+        float *dst = jd.C + (size_t)r * jd.N + j;
+#pragma unroll
+        for (int v = 0; v < VEC; ++v) __builtin_nontemporal_store(acc[v], dst + v);
+    }
+}
+
 __global__ __launch_bounds__(64) void batch_reset_kernel(unsigned *counter) { *counter = 0; }
 
 __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
@@ -161,6 +219,15 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
         const int r0 = rb * jd.rows;
         for (int q = 0; q < jd.rows; q += streams) {
             const int r = r0 + q + sub;
+            if (streams == 1) {
+                switch (jd.vec) {
+                case 1: run_row_wave<1>(jd, rsrc, r, j, active); break;
+                case 2: run_row_wave<2>(jd, rsrc, r, j, active); break;
+                case 4: run_row_wave<4>(jd, rsrc, r, j, active); break;
+                default: run_row_wave<8>(jd, rsrc, r, j, active); break;
+                }
+                continue;
+            }
             switch (jd.vec) {
             case 1: run_row<1>(jd, rsrc, r, j, active); break;
             case 2: run_row<2>(jd, rsrc, r, j, active); break;
