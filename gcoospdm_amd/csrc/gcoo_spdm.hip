@@ -289,10 +289,10 @@ __device__ __forceinline__ void lds_load(float (&d)[VEC], const unsigned char *p
 }
 
 // ---------------------------------------------------------------------------
-// Variant 2: row segments.  One wave per (group, column tile) as above, but
-// each of the group's rows is walked over its own k-sorted segment
-// (rowptr / seg, written by the conversion scatter) with one accumulator set:
-// no per-entry row dispatch.  D is read with buffer loads: the entry's row
+// Variant 2: row segments.  One wave per (rows, column tile): the rows are
+// one row or a group's cpg rows (`nGroup` units of `cpg` rows here), each
+// walked over its own k-sorted segment (rowptr / seg, written by the
+// conversion scatter) with one accumulator set: no per-entry row dispatch.  D is read with buffer loads: the entry's row
 // offset k*N*bytes is a scalar offset, the lane's column offset a vector
 // offset, and lanes past N read zeros from the hardware range check.
 // ---------------------------------------------------------------------------
@@ -1142,7 +1142,20 @@ template <typename E, int VEC, int U>
 int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s, int skipLen)
 {
-    const int nGroupBlocks = (h->nGroup + kSpdmWaves - 1) / kSpdmWaves;
+    // A wave walks `rpw` consecutive rows: one row, or the group's cpg rows
+    // (the segments are per row, so any split gives the same bits).  One row
+    // per wave shortens each wave's serial chain when rows are long enough
+    // to keep its loads busy: at n = 8000, 370 -> 347 us at s = 0.997 and
+    // 591 -> 564 us (BF16) at s = 0.99, 23 -> 18 us at 800^2, s = 0.9; with
+    // 8 entries per row (s = 0.999) the group per wave is faster (150
+    // against 161 us).  Automatic: one row once rows average 12 entries
+    // (the handle's last conversion); config chunk 1 forces one row, another
+    // nonzero chunk the group.
+    const gcoo_spdm_config &cfg = h->spdm_cfg;
+    const bool longRows = h->M > 0 && h->nnz >= 12ll * h->M;
+    const int rpw = (cfg.chunk == 1 || (cfg.chunk == 0 && longRows)) ? 1 : h->cpg;
+    const int nUnits = (h->M + rpw - 1) / rpw;
+    const int nGroupBlocks = (nUnits + kSpdmWaves - 1) / kSpdmWaves;
     const int tileCols = 64 * VEC;
     const int nColTiles = (N + tileCols - 1) / tileCols;
     const long long nblocks = (long long)nGroupBlocks * nColTiles;
@@ -1150,7 +1163,7 @@ int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N,
         return set_error(GCOO_ERR_OVERFLOW, "grid of %lld workgroups too large", nblocks);
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
     hipLaunchKernelGGL((spdm_rowseg_kernel<E, VEC, U>), dim3((unsigned)nblocks),
-                       dim3(kSpdmThreads), 0, s, h->rowptr, seg, h->nGroup, h->cpg, h->M, D,
+                       dim3(kSpdmThreads), 0, s, h->rowptr, seg, nUnits, rpw, h->M, D,
                        (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap, skipLen);
     GCOO_CHECK_LAUNCH("spdm_rowseg_kernel");
     return GCOO_OK;

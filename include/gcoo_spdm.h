@@ -174,7 +174,9 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
  *              D per lane; a wave covers 64*vec columns.
  *   waves      waves per workgroup (1..16).
- *   chunk      k-chunk length of variant 3; long-row threshold of variant 4.
+ *   chunk      k-chunk length of variant 3; long-row threshold of variant 4;
+ *              rows per wave of variant 2 (1: one row, another nonzero value:
+ *              the group's cpg rows, 0: one row once rows average 12 entries).
  *   unroll     entries whose D loads are in flight per wave in variant 2
  *              (8, 16 or 32; 0 means 16).
  * All variants give bit-identical C. */
