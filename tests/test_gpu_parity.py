@@ -218,6 +218,32 @@ def test_kernel_variants_bit_exact(dev, variant, vec, M, K, N, dens):
     assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
 
 
+@pytest.mark.parametrize("rows_per_wave", [1, 4])
+@pytest.mark.parametrize("unroll", [8, 16, 32])
+@pytest.mark.parametrize("M,K,N,dens", [(37, 200, 70, 0.5), (130, 3000, 130, 0.02),
+                                        (9, 5000, 64, 0.3)])
+def test_row_segment_rounds_bit_exact(dev, rows_per_wave, unroll, M, K, N, dens):
+    """Variant 2 with one row or a 4-row group per wave and 8, 16 or 32
+    entries per round (then rounds of 8, then the guarded last 1..7), FP32
+    and BF16: rows of every length modulo the round sizes."""
+    import torch
+    print(f"rowseg rpw={rows_per_wave} unroll={unroll} M={M} K={K} N={N}", flush=True)
+    rng = np.random.default_rng(M * 3 + K + N + unroll)
+    S = H.rng_sparse(rng, M, K, dens)
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    g = oracle.convert(S, 4)
+    m = G.GCOOMatrix(M, K, 4).convert_dense(to_dev(S, dev))
+    m.set_config(variant=2, chunk=rows_per_wave, unroll=unroll)
+    C = torch.full((M, N), float("nan"), device=dev)
+    m.spdm(to_dev(D, dev), C)
+    assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+    D16 = G.to_bf16(to_dev(D, dev))
+    C.fill_(float("nan"))
+    m.spdm_bf16(D16, C)
+    ref16 = oracle.spdm_bf16(g, oracle.to_bf16(g["vals"]), oracle.to_bf16(D))
+    assert H.bits_equal(C.cpu().numpy(), ref16)
+
+
 @pytest.mark.parametrize("M,K,N,dens", [
     (300, 1000, 512, 0.02),    # 2 panels (one partial), 16 chunks
     (257, 129, 260, 0.3),      # dense chunks: the 8-entry batches and every tail size
