@@ -95,3 +95,13 @@ def test_main_driver_flag_validation():
     r = subprocess.run([MAIN, "-algo=gcoospdm", "-file=x.mtx"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus 2 exits with status 2 and a message,
+    before anything touches a device."""
+    r = subprocess.run(["python3", os.path.join(ROOT, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
+    assert r.stdout == ""
