@@ -348,6 +348,29 @@ def profile_kernel(n, s, seed, iters):
     print(f"profiled spdm n={n} s={s} nnz={m.nnz} x{iters}; copy 256MiB x{iters}", flush=True)
 
 
+def profile_coo(iters):
+    """A few COO -> GCOO conversions of the 65536^2 Zipf matrix and FP32
+    multiplies by a 65536 x 256 D, for a rocprofv3 kernel trace (no JSON)."""
+    import numpy as np
+    import torch
+    import gcoospdm_amd as G
+    from gcoospdm_amd import workloads as W
+    dev = torch.device("cuda:0")
+    M = K = 65536
+    N = 256
+    rows, cols, vals = W.zipf_coo(M, K, seed=11)
+    r, c, v = (torch.from_numpy(x).to(dev) for x in (rows, cols, vals))
+    m = G.GCOOMatrix(M, K, 4)
+    D = torch.from_numpy(np.random.default_rng(12).standard_normal((K, N)).astype(np.float32)).to(dev)
+    C = torch.empty((M, N), device=dev)
+    for _ in range(iters):
+        m.convert_coo(r, c, v)
+    for _ in range(iters):
+        m.spdm(D, C, sync=False)
+    torch.cuda.synchronize()
+    print(f"profiled coo conversion zipf 65536^2 nnz={m.nnz} x{iters}; spdm N={N} x{iters}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -361,6 +384,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--profile-kernel", action="store_true")
+    ap.add_argument("--profile-coo", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -371,6 +395,9 @@ def main():
         sys.exit(2)
     if args.profile_kernel:
         profile_kernel(args.n, args.sparsity, args.seed, 10)
+        return
+    if args.profile_coo:
+        profile_coo(6)
         return
 
     import torch
