@@ -73,6 +73,7 @@ _SIGNATURES = [
     ("gcoo_spdm_f32_async", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_spdm_f32", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_multiply_by", _I, [_P, _P, _P, _I, _I, _P, _P]),
+    ("gcoo_test_set_conversion_abort", _I, [_P, _P]),
     ("gcoo_set_spdm_config", _I, [_P, _P]),
     ("gcoo_get_spdm_config", _I, [_P, _P]),
     ("gcoo_f32_to_bf16_async", _I, [_P, _P, _I64, _P]),
@@ -97,6 +98,7 @@ _SIGNATURES = [
     ("gcoo_server_release", _I, [_P, _U64]),
     ("gcoo_server_stats", _I, [_P, _P]),
     ("gcoo_server_reset_stats", _I, [_P]),
+    ("gcoo_test_server_inject_failure", _I, [_P, _I]),
 ]
 
 _lib: Optional[ctypes.CDLL] = None

@@ -87,18 +87,21 @@ namespace {
 
 inline hipStream_t as_stream(gcoo_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Internal result of read_status: the single-pass conversion stalled.
+constexpr int kStalled = -1;
+
 // Reads the conversion status word back (one small D2H copy plus a stream
 // synchronisation: the replacement of the reference's nnz readback,
-// src/ShGCOOSpDM.cu:83-90).
+// src/ShGCOOSpDM.cu:83-90).  Returns kStalled after a stalled single-pass
+// conversion, having queued the clearing of its abort word.
 int read_status(gcoo_handle h, hipStream_t s)
 {
     GCOO_HIP_TRY(hipMemcpyAsync(h->h_status, h->d_status, sizeof(ConvStatus),
                                 hipMemcpyDeviceToHost, s));
     GCOO_HIP_TRY(hipStreamSynchronize(s));
     if (h->h_status->stall) {
-        // clear the single-pass abort word for the next conversion
-        if (h->lb_abort) (void)hipMemset(h->lb_abort, 0, sizeof(unsigned));
-        return set_error(GCOO_ERR_HIP, "the conversion's scan stopped waiting for a group");
+        GCOO_HIP_TRY(hipMemsetAsync(&h->d_status->stall, 0, sizeof(int), s));
+        return kStalled;
     }
     if (h->h_status->overflow)
         return set_error(GCOO_ERR_OVERFLOW, "nnz %lld does not fit in int32",
@@ -110,23 +113,30 @@ int read_status(gcoo_handle h, hipStream_t s)
 // total exceeds it, grow and scatter again.  On return the conversion is
 // complete and h->nnz is set.  When D is given, the multiply is enqueued
 // right behind the scatter (guarded by the status word) so that a conversion
-// that fits the existing capacity costs a single synchronisation.
-int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
-                               int N, float *C, hipStream_t s)
+// that fits the existing capacity costs a single synchronisation.  When the
+// single-pass kernel's look-back gives up (st->stall), the kernels queued
+// behind it skip their work and the conversion runs again on the count /
+// scan / scatter kernels, whose launches always complete (the reference's
+// three launches, src/ShGCOOSpDM.cu:44-111).
+int convert_impl(gcoo_handle h, const float *S, const float *D, int N, float *C,
+                 hipStream_t s, bool allow_fused)
 {
     h->converted = false;
     h->seg16_valid = false;
+    ++h->generation;
     invalidate_panel_lists(h);
     // one pass over S where the single-pass kernel applies, else count + scan
     // then scatter
-    const bool fused = fused_conversion_ok(h, S);
+    const bool fused = allow_fused && fused_conversion_ok(h, S);
     int rc = fused ? launch_convert_fused(h, S, s) : launch_count_and_scan(h, S, s);
     if (rc) return rc;
     if (h->cap > 0) {
         if (!fused && (rc = launch_scatter(h, S, s))) return rc;
         if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
     }
-    if ((rc = read_status(h, s))) return rc;
+    rc = read_status(h, s);
+    if (rc == kStalled) return convert_impl(h, S, D, N, C, s, false);
+    if (rc) return rc;
     const int64_t total = h->h_status->total;
     const int64_t need = total + h->h_status->maxRow + kSegSlack;
     if (need > h->cap || h->cap == 0) {
@@ -134,13 +144,21 @@ int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D,
         h->nnz = total;  // the multiply's automatic variant choice reads it
         h->maxRow = h->h_status->maxRow;
         if ((rc = fused ? launch_convert_fused(h, S, s) : launch_scatter(h, S, s))) return rc;
-        if (D && (rc = launch_spdm_f32(h, D, N, C, nullptr, s))) return rc;
-        if ((rc = read_status(h, s))) return rc;  // synchronises
+        if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
+        rc = read_status(h, s);  // synchronises
+        if (rc == kStalled) return convert_impl(h, S, D, N, C, s, false);
+        if (rc) return rc;
     }
     h->nnz = total;
     h->maxRow = h->h_status->maxRow;
     h->converted = true;
     return GCOO_OK;
+}
+
+int convert_and_maybe_multiply(gcoo_handle h, const float *S, const float *D, int N, float *C,
+                               hipStream_t s)
+{
+    return convert_impl(h, S, D, N, C, s, true);
 }
 
 int check_spdm_args(gcoo_handle h, const float *D, int K, int N, float *C)
@@ -361,7 +379,6 @@ int gcoo_destroy(gcoo_handle h)
     if (h->rowcnt) (void)hipFree(h->rowcnt);
     if (h->mask) (void)hipFree(h->mask);
     if (h->lb_flags) (void)hipFree(h->lb_flags);
-    if (h->lb_abort) (void)hipFree(h->lb_abort);
     if (h->d_status) (void)hipFree(h->d_status);
     if (h->h_status) (void)hipHostFree(h->h_status);
     if (h->coo_tmp) (void)hipFree(h->coo_tmp);
@@ -386,6 +403,7 @@ int gcoo_convert_coo(gcoo_handle h, const int *rows, const int *cols, const floa
     if (nnz > INT32_MAX) return set_error(GCOO_ERR_OVERFLOW, "nnz %lld > int32", (long long)nnz);
     h->converted = false;
     h->seg16_valid = false;
+    ++h->generation;
     invalidate_panel_lists(h);
     int rc = convert_coo(h, rows, cols, vals, nnz, as_stream(s));
     if (rc) return rc;
@@ -481,6 +499,15 @@ int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg)
 {
     if (!h || !cfg) return set_error(GCOO_ERR_INVALID, "null argument");
     *cfg = h->spdm_cfg;
+    return GCOO_OK;
+}
+
+int gcoo_test_set_conversion_abort(gcoo_handle h, gcoo_stream s)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    GCOO_HIP_TRY(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(&h->d_status->stall), 1, 1,
+                                   as_stream(s)));
+    GCOO_HIP_TRY(hipStreamSynchronize(as_stream(s)));
     return GCOO_OK;
 }
 

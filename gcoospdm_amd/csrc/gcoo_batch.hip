@@ -247,6 +247,12 @@ struct gcoo_batch {
     std::vector<JobDesc> jobs;
     std::vector<int> itemStart;  // nJobs + 1
     std::vector<gcoo_handle> handles;
+    // each handle's conversion generation when the job table was committed:
+    // a run refuses to start once one moved (the table would point at
+    // re-converted or freed entry arrays)
+    std::vector<uint64_t> gens;
+    // resident workgroups of the persistent kernel, queried once at creation
+    int cus = 256, perCu = 4;
     JobDesc *d_jobs = nullptr;
     int *d_itemStart = nullptr;
     unsigned *d_counter = nullptr;
@@ -277,6 +283,20 @@ void destroy_graph(gcoo_batch *b)
     b->graph = nullptr;
 }
 
+// GCOO_OK when every job's handle still holds the conversion the job table
+// was committed with.
+int check_generations(const gcoo_batch *b)
+{
+    for (size_t i = 0; i < b->handles.size(); ++i) {
+        const gcoo_handle h = b->handles[i];
+        if (i >= b->gens.size() || !h->converted || h->generation != b->gens[i])
+            return set_error(GCOO_ERR_STATE,
+                             "job %zu: its matrix was converted again after the batch was "
+                             "committed; commit (and capture) the batch again", i);
+    }
+    return GCOO_OK;
+}
+
 int enqueue_run(gcoo_batch *b, hipStream_t s)
 {
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
@@ -302,6 +322,19 @@ int gcoo_batch_create(gcoo_batch_t *out)
         delete b;
         return set_error(GCOO_ERR_ALLOC, "batch counter");
     }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        b->cus < 1) {
+        (void)hipGetLastError();
+        b->cus = 256;
+    }
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->perCu, batch_persistent_kernel,
+                                                     kBatchThreads, 0) != hipSuccess ||
+        b->perCu < 1) {
+        (void)hipGetLastError();
+        b->perCu = 4;
+    }
     b->itemStart.push_back(0);
     *out = b;
     return GCOO_OK;
@@ -326,6 +359,7 @@ int gcoo_batch_clear(gcoo_batch_t b)
     destroy_graph(b);
     b->jobs.clear();
     b->handles.clear();
+    b->gens.clear();
     b->itemStart.assign(1, 0);
     b->committed = false;
     return GCOO_OK;
@@ -351,8 +385,8 @@ int gcoo_batch_add(gcoo_batch_t b, gcoo_handle h, const void *D, int dtype, int 
     const uintptr_t al = reinterpret_cast<uintptr_t>(D);
     while (vec > 1 && (N % vec != 0 || (al & (uintptr_t)(vec * bytes - 1)) != 0)) vec >>= 1;
     JobDesc jd{};
-    jd.rowptr = h->rowptr;
-    jd.seg = dtype == 0 ? h->seg : nullptr;  // BF16 segments bound at commit
+    jd.rowptr = h->rowptr;  // entry arrays bound at commit
+    jd.seg = nullptr;
     jd.D = D;
     jd.C = C;
     jd.M = h->M;
@@ -397,13 +431,21 @@ int gcoo::batch_commit_async(gcoo_batch_t b, gcoo_stream stream)
 {
     if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    // BF16 jobs need their handles' BF16 segments
+    // bind every job to its handle's current entry arrays (BF16 jobs to the
+    // BF16 segments, derived here when stale) and record the generation
+    b->gens.resize(b->jobs.size());
     for (size_t i = 0; i < b->jobs.size(); ++i) {
-        if (b->jobs[i].bytes != 2) continue;
         gcoo_handle h = b->handles[i];
-        int rc = ensure_seg16(h, s);
-        if (rc) return rc;
-        b->jobs[i].seg = h->seg16;
+        if (!h->converted) return set_error(GCOO_ERR_STATE, "job %zu: matrix not converted", i);
+        if (b->jobs[i].bytes == 2) {
+            int rc = ensure_seg16(h, s);
+            if (rc) return rc;
+            b->jobs[i].seg = h->seg16;
+        } else {
+            b->jobs[i].seg = h->seg;
+        }
+        b->jobs[i].rowptr = h->rowptr;
+        b->gens[i] = h->generation;
     }
     const size_t n = b->jobs.size();
     if (n > b->capJobs) {
@@ -435,19 +477,9 @@ int gcoo::batch_commit_async(gcoo_batch_t b, gcoo_stream stream)
         GCOO_HIP_TRY(hipMemcpyAsync(b->d_itemStart, b->h_itemStart, sizeof(int) * (n + 1),
                                     hipMemcpyHostToDevice, s));
     }
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     const long long items = b->itemStart.back();
     // persistent: as many workgroups as can be resident at once
-    int perCu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, batch_persistent_kernel,
-                                                     kBatchThreads, 0) != hipSuccess ||
-        perCu < 1) {
-        (void)hipGetLastError();
-        perCu = 4;
-    }
-    long long blocks = (long long)cus * perCu;
+    long long blocks = (long long)b->cus * b->perCu;
     const long long needed = (items + kBatchWaves - 1) / kBatchWaves;
     if (blocks > needed) blocks = needed;
     if (blocks < 1) blocks = 1;
@@ -463,6 +495,7 @@ int gcoo_batch_run_async(gcoo_batch_t b, gcoo_stream stream)
 {
     if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
     if (!b->committed) return set_error(GCOO_ERR_STATE, "batch not committed");
+    if (int rc = check_generations(b)) return rc;
     if (b->jobs.empty() || b->itemStart.back() == 0) return GCOO_OK;
     return enqueue_run(b, reinterpret_cast<hipStream_t>(stream));
 }
@@ -473,6 +506,7 @@ int gcoo_batch_graph_capture(gcoo_batch_t b, gcoo_stream stream)
     if (!b->committed) return set_error(GCOO_ERR_STATE, "batch not committed");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (!s) return set_error(GCOO_ERR_INVALID, "graph capture needs a non-null stream");
+    if (int rc = check_generations(b)) return rc;
     destroy_graph(b);
     GCOO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = GCOO_OK;
@@ -497,6 +531,7 @@ int gcoo_batch_graph_launch(gcoo_batch_t b, gcoo_stream stream)
 {
     if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
     if (!b->graphExec) return set_error(GCOO_ERR_STATE, "no captured graph");
+    if (int rc = check_generations(b)) return rc;
     GCOO_HIP_TRY(hipGraphLaunch(b->graphExec, reinterpret_cast<hipStream_t>(stream)));
     return GCOO_OK;
 }

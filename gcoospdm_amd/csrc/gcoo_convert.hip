@@ -514,8 +514,9 @@ __global__ __launch_bounds__(kConvThreads) void scatter_mask_kernel(
 //      maximum; the last group's are the total and the longest row.  A flag
 //      is one 64-bit word (state, epoch, value), two per group; the epoch
 //      makes clearing between conversions unnecessary.  A wait that does not
-//      end within a bounded number of polls raises an abort flag that every
-//      waiting group sees, and the status word reports the stall.
+//      end within a bounded number of polls raises the abort word (the
+//      status word's stall field) that every waiting group sees; the host
+//      then converts again with the count / scan / scatter kernels.
 //   3. Thread t takes entries t, t + 256, ... of the group's LDS lists in row
 //      order, so the row segments are a straight copy; an entry's place in
 //      (k, r) order is its rank in its row plus, for every other row, that
@@ -557,12 +558,13 @@ __device__ __forceinline__ long long wave_sum64(long long v)
     return v;
 }
 
-// flags: [nGroup][2] (entry count, longest row); abort: zero unless a
-// conversion stalled (the host clears it then).
+// flags: [nGroup][2] (entry count, longest row).  st->stall is the abort
+// word: zero unless a group's look-back gave up (or the host set it); a
+// group that gives up sets it, so that the others stop waiting too, and the
+// host then clears it and converts again with the three-kernel path.
 __global__ __launch_bounds__(kFusedThreads) void convert_fused_kernel(
     const float *__restrict__ S, int M, int K, int KB, int nGroup, int lcap, unsigned epoch,
-    unsigned long long *__restrict__ flags, unsigned *__restrict__ abort,
-    int *__restrict__ groupIndex, int *__restrict__ nnzPerGroup, int *__restrict__ rowptr,
+    unsigned long long *__restrict__ flags, int *__restrict__ groupIndex, int *__restrict__ nnzPerGroup, int *__restrict__ rowptr,
     ConvStatus *__restrict__ st, long long cap, float *__restrict__ vals,
     int *__restrict__ rows, int *__restrict__ cols, int2 *__restrict__ seg)
 {
@@ -577,7 +579,7 @@ __global__ __launch_bounds__(kFusedThreads) void convert_fused_kernel(
     const int tid = threadIdx.x, lane = __lane_id();
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int g = blockIdx.x;
-    if (g == 0 && tid == 0) st->stall = 0;
+    unsigned *abort = reinterpret_cast<unsigned *>(&st->stall);
     const int r0 = g * CPG;
     const int nrows = min(CPG, M - r0);
     const int K4 = K >> 2;
@@ -691,14 +693,11 @@ __global__ __launch_bounds__(kFusedThreads) void convert_fused_kernel(
                 const bool ready = !(bad & need) && (fp == 64 || ((bothPre >> fp) & 1ull));
                 if (!ready) {
                     bool give_up = ++spins > kLbSpinLimit;
-                    if (!give_up && (spins & 63) == 0)
+                    if (!give_up && (spins & 63) == 1)
                         give_up = __hip_atomic_load(abort, __ATOMIC_RELAXED,
                                                     __HIP_MEMORY_SCOPE_AGENT) != 0;
                     if (__builtin_amdgcn_readfirstlane(give_up ? 1 : 0)) {
-                        if (lane == 0) {
-                            atomicExch(abort, 1u);
-                            st->stall = 1;
-                        }
+                        if (lane == 0) atomicExch(abort, 1u);
                         excl = 0;
                         break;
                     }
@@ -957,14 +956,10 @@ bool fused_conversion_ok(gcoo_matrix *h, const float *S)
     if (fixed + 32ll * 256 > kFusedLdsBytes) return false;
     if (!h->lb_flags) {
         if (hipMalloc(&h->lb_flags, 2 * sizeof(unsigned long long) * (size_t)h->nGroup) != hipSuccess ||
-            hipMalloc(&h->lb_abort, sizeof(unsigned)) != hipSuccess ||
-            hipMemset(h->lb_flags, 0, 2 * sizeof(unsigned long long) * (size_t)h->nGroup) != hipSuccess ||
-            hipMemset(h->lb_abort, 0, sizeof(unsigned)) != hipSuccess) {
+            hipMemset(h->lb_flags, 0, 2 * sizeof(unsigned long long) * (size_t)h->nGroup) != hipSuccess) {
             (void)hipGetLastError();
             if (h->lb_flags) (void)hipFree(h->lb_flags);
-            if (h->lb_abort) (void)hipFree(h->lb_abort);
             h->lb_flags = nullptr;
-            h->lb_abort = nullptr;
             return false;
         }
     }
@@ -987,8 +982,7 @@ int launch_convert_fused(gcoo_matrix *h, const float *S, hipStream_t s)
     if (h->lb_epoch == 0) h->lb_epoch = 1;
     h->mask_valid = false;
     hipLaunchKernelGGL(convert_fused_kernel, dim3(h->nGroup), dim3(kFusedThreads), lds, s, S, h->M,
-                       h->K, KB, h->nGroup, lcap, h->lb_epoch, h->lb_flags, h->lb_abort,
-                       h->groupIndex(), h->nnzPerGroup(), h->rowptr, h->d_status,
+                       h->K, KB, h->nGroup, lcap, h->lb_epoch, h->lb_flags, h->groupIndex(), h->nnzPerGroup(), h->rowptr, h->d_status,
                        (long long)h->cap, h->vals, h->rows(), h->cols(), h->seg);
     GCOO_CHECK_LAUNCH("convert_fused_kernel");
     return GCOO_OK;

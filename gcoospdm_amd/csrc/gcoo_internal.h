@@ -44,9 +44,22 @@ struct ConvStatus {
     int overflow;
     int invalid;
     int maxRow;  // most entries in one row
-    int stall;   // the single-pass conversion gave up waiting for a predecessor
+    // Abort word of the single-pass conversion: set by a group whose bounded
+    // look-back wait gave up (the offsets of that conversion are then not
+    // consistent), read by the other groups' waits, cleared only by the host
+    // (which then converts again with the count / scan / scatter kernels).
+    int stall;
     int longRows;  // rows listed by the row scan as longer than its threshold
 };
+
+// True when a kernel queued behind a conversion must leave the entry arrays
+// alone: the entries (plus `extra`) outgrew the capacity, or the single-pass
+// conversion stalled.  A null guard never skips.
+__device__ __forceinline__ bool guard_skip(const ConvStatus *g, long long cap,
+                                           long long extra = 0)
+{
+    return g != nullptr && (g->total + extra > cap || g->stall != 0);
+}
 
 }  // namespace gcoo
 
@@ -59,6 +72,9 @@ struct gcoo_matrix {
     int64_t nnz = 0;
     bool converted = false;
     int device = 0;
+    // Bumped by every conversion (and so by every reallocation of the entry
+    // arrays): a batch records it per job and refuses to run once it moved.
+    uint64_t generation = 0;
 
     float *vals = nullptr;     // [cap]
     int *rowcol = nullptr;     // [2 * cap]: rows at 0, cols at cap
@@ -103,9 +119,8 @@ struct gcoo_matrix {
 
     // single-pass conversion: per-group look-back flags (two (state, epoch,
     // value) words per group, never cleared: each conversion has its own
-    // epoch) and the abort word (zero unless a conversion stalled)
+    // epoch); its abort word is d_status->stall
     unsigned long long *lb_flags = nullptr;  // [2 * nGroup]
-    unsigned *lb_abort = nullptr;            // [1]
     unsigned lb_epoch = 0;
 
     // multiply-kernel configuration (gcoo_set_spdm_config)

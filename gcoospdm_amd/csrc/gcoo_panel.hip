@@ -49,7 +49,7 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     int nC, int *__restrict__ poff, int *__restrict__ gcnt, uint2 *__restrict__ plist,
     const ConvStatus *__restrict__ guard, long long cap)
 {
-    if (guard != nullptr && guard->total > cap) return;
+    if (guard_skip(guard, cap)) return;
     constexpr int RPW = kPanelRowsPerWave;
     extern __shared__ int lcnt[];
     const int pw = blockIdx.x;  // panel * 16 + wave

@@ -11,9 +11,12 @@
 //   * whenever one of `slots` streams is idle, launches every admitted, not
 //     yet running request (up to `max_batch_jobs`) as one persistent batch on
 //     it -- requests that arrive while other batches are on the GPU go into
-//     the next batch on another stream, so their work overlaps on the device;
-//   * retires a batch when its completion event fires, timestamps its
-//     requests (arrival -> completion latency) and wakes their waiters.
+//     the next batch on another stream, so their work overlaps on the device.
+//     With max_batch_jobs == 1 it is a plain FIFO: one stream, each request
+//     one launch of the plain multiply, up to kPlainDepth launches queued.
+// A waiter thread per stream blocks on its oldest launch's completion event,
+// then timestamps that launch's requests (arrival -> completion latency),
+// wakes their waiters and the scheduler.
 // Results stay in the arena until the client releases the request.  Every
 // result is the plain kernels' fmaf chain, bit-identical alone or batched.
 #include "gcoo_internal.h"
@@ -24,6 +27,7 @@
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -42,21 +46,38 @@ struct Operand {
     int rows, vec;
 };
 
+enum ReqState { kQueued = 0, kAdmitted = 1, kRunning = 2, kDone = 3, kFailed = 4 };
+
 struct Request {
     uint64_t id;
     int op;
     size_t offset = 0, bytes = 0;
+    bool allocated = false;  // holds arena bytes [offset, offset + bytes)
     Clock::time_point arrival, done;
-    int state = 0;  // 0 queued, 1 admitted, 2 running, 3 done, 4 released
+    int state = kQueued;
+    int err = GCOO_OK;       // error code of a failed request
 };
 
+// One launch in flight on a slot's stream: its completion event and requests.
+struct Launch {
+    hipEvent_t event = nullptr;
+    std::vector<uint64_t> reqs;
+};
+
+// A stream with its launches in flight (oldest first).  Batched mode keeps
+// one launch per slot in flight (the slot's batch object holds its job
+// table); plain mode keeps up to kPlainDepth on the one slot.  A waiter
+// thread per slot blocks on the oldest launch's event and retires it.
 struct Slot {
     hipStream_t stream = nullptr;
-    hipEvent_t event = nullptr;
     gcoo_batch_t batch = nullptr;
-    std::vector<uint64_t> reqs;
-    bool busy = false;
+    std::vector<hipEvent_t> freeEvents;
+    std::deque<Launch> inflight;
+    std::condition_variable cv;  // a launch was queued, or the server stops
+    std::thread waiter;
 };
+
+constexpr int kPlainDepth = 64;
 
 // First-fit allocator over [0, size) with 256-byte granularity.
 class Arena {
@@ -118,20 +139,24 @@ using namespace gcoo;
 struct gcoo_server {
     gcoo_server_config cfg{};
     int device = 0;
+    bool plain = false;  // max_batch_jobs == 1: one request per plain launch
+    int depth = 1;       // launches in flight per slot
     char *arenaBase = nullptr;
     Arena arena;
     std::vector<Operand> ops;
     std::unordered_map<uint64_t, Request> reqs;
     std::deque<uint64_t> queue;     // waiting for admission (FIFO)
     std::deque<uint64_t> admitted;  // output allocated, not yet launched
-    std::vector<Slot> slots;
+    std::vector<std::unique_ptr<Slot>> slots;
+    int injectFailure = 0;  // testing hook: the next launch fails with this code
     uint64_t nextId = 1;
     std::mutex mu;
     std::condition_variable cvWork, cvDone;
     std::thread worker;
     bool stop = false;
     // bumped by every event that can make admission or launch possible
-    // (submit, release, stop); the idle scheduler sleeps until it changes
+    // (submit, release, a retired launch, stop); the scheduler sleeps until
+    // it changes
     uint64_t wakeups = 0;
     int workerError = 0;
     char workerMsg[256] = "";
@@ -150,6 +175,28 @@ size_t out_bytes(const gcoo_server *s, int op)
     return (size_t)o.h->M * (size_t)o.N * sizeof(float);
 }
 
+// Called with the lock held: record the first failure and fail every request
+// not yet launched, so that their waiters return the error; nothing is
+// launched afterwards.
+void fail_pending(gcoo_server *s, int code, const char *msg)
+{
+    if (!s->workerError) {
+        s->workerError = code;
+        snprintf(s->workerMsg, sizeof(s->workerMsg), "%s", msg);
+    }
+    for (std::deque<uint64_t> *q : {&s->queue, &s->admitted}) {
+        for (uint64_t id : *q) {
+            Request &r = s->reqs[id];
+            r.state = kFailed;
+            r.err = s->workerError;
+        }
+        q->clear();
+    }
+    ++s->wakeups;
+    s->cvDone.notify_all();
+    s->cvWork.notify_all();
+}
+
 // Called with the lock held: admit queued requests in order while they fit.
 void admit(gcoo_server *s)
 {
@@ -158,35 +205,75 @@ void admit(gcoo_server *s)
         size_t off;
         if (!s->arena.alloc(r.bytes, &off)) break;
         r.offset = off;
-        r.state = 1;
+        r.allocated = true;
+        r.state = kAdmitted;
         s->admitted.push_back(r.id);
         s->queue.pop_front();
     }
 }
 
-// Called with the lock held: launch admitted requests on an idle slot.
+// Called with the lock held: launch admitted requests on a slot with room --
+// in batched mode all of them (up to max_batch_jobs) as one persistent
+// batch, in plain mode the oldest one through the plain multiply.
 int launch(gcoo_server *s, Slot &slot)
 {
-    int rc = gcoo_batch_clear(slot.batch);
-    if (rc) return rc;
-    slot.reqs.clear();
-    while (!s->admitted.empty() && (int)slot.reqs.size() < s->cfg.max_batch_jobs) {
+    Launch L;
+    int rc = GCOO_OK;
+    if (s->injectFailure) {
+        // testing hook: fail the oldest admitted request as a failed launch
+        // would, without enqueuing anything
+        rc = set_error(s->injectFailure, "injected launch failure");
+        s->injectFailure = 0;
+        L.reqs.push_back(s->admitted.front());
+        s->admitted.pop_front();
+    } else if (s->plain) {
         Request &r = s->reqs[s->admitted.front()];
         const Operand &o = s->ops[r.op];
         float *C = reinterpret_cast<float *>(s->arenaBase + r.offset);
-        rc = gcoo_batch_add(slot.batch, o.h, o.D, o.dtype, o.N, C, o.rows, o.vec, nullptr);
-        if (rc) return rc;
-        r.state = 2;
-        slot.reqs.push_back(r.id);
+        rc = o.dtype == 0
+                 ? gcoo_spdm_f32_async(o.h, static_cast<const float *>(o.D), o.h->K, o.N, C,
+                                       slot.stream)
+                 : gcoo_spdm_bf16_async(o.h, static_cast<const uint16_t *>(o.D), o.h->K, o.N, C,
+                                        slot.stream);
+        L.reqs.push_back(r.id);
         s->admitted.pop_front();
+    } else {
+        if ((rc = gcoo_batch_clear(slot.batch))) return rc;
+        while (!s->admitted.empty() && (int)L.reqs.size() < s->cfg.max_batch_jobs) {
+            Request &r = s->reqs[s->admitted.front()];
+            const Operand &o = s->ops[r.op];
+            float *C = reinterpret_cast<float *>(s->arenaBase + r.offset);
+            L.reqs.push_back(r.id);
+            s->admitted.pop_front();
+            if ((rc = gcoo_batch_add(slot.batch, o.h, o.D, o.dtype, o.N, C, o.rows, o.vec,
+                                     nullptr)))
+                break;
+        }
+        if (!rc) rc = batch_commit_async(slot.batch, slot.stream);
+        if (!rc) rc = gcoo_batch_run_async(slot.batch, slot.stream);
     }
-    if (slot.reqs.empty()) return GCOO_OK;
-    if ((rc = batch_commit_async(slot.batch, slot.stream))) return rc;
-    if ((rc = gcoo_batch_run_async(slot.batch, slot.stream))) return rc;
-    GCOO_HIP_TRY(hipEventRecord(slot.event, slot.stream));
-    slot.busy = true;
+    if (!rc) {
+        L.event = slot.freeEvents.back();
+        if (hipEventRecord(L.event, slot.stream) != hipSuccess) {
+            (void)hipGetLastError();
+            rc = set_error(GCOO_ERR_HIP, "hipEventRecord failed");
+        }
+    }
+    if (rc) {
+        // the launch's requests fail with the error of the call that failed
+        for (uint64_t id : L.reqs) {
+            Request &r = s->reqs[id];
+            r.state = kFailed;
+            r.err = rc;
+        }
+        return rc;
+    }
+    slot.freeEvents.pop_back();
+    for (uint64_t id : L.reqs) s->reqs[id].state = kRunning;
     ++s->batches;
-    s->jobsRun += slot.reqs.size();
+    s->jobsRun += L.reqs.size();
+    slot.inflight.push_back(std::move(L));
+    slot.cv.notify_one();
     return GCOO_OK;
 }
 
@@ -195,62 +282,68 @@ void worker_main(gcoo_server *s)
     (void)hipSetDevice(s->device);
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
-        bool progressed = false;
-        // retire finished batches
-        for (Slot &slot : s->slots) {
-            if (!slot.busy) continue;
-            const hipError_t q = hipEventQuery(slot.event);
-            if (q == hipErrorNotReady) continue;
-            if (q != hipSuccess) {
-                s->workerError = GCOO_ERR_HIP;
-                snprintf(s->workerMsg, sizeof(s->workerMsg), "batch failed: %s",
-                         hipGetErrorString(q));
+        const uint64_t seen = s->wakeups;
+        if (!s->workerError) {
+            admit(s);
+            for (auto &sp : s->slots) {
+                Slot &slot = *sp;
+                while (!s->workerError && !s->admitted.empty() &&
+                       (int)slot.inflight.size() < s->depth) {
+                    const int rc = launch(s, slot);
+                    if (rc) {
+                        char msg[256];
+                        snprintf(msg, sizeof(msg), "launch failed: %s", gcoo_last_error());
+                        fail_pending(s, rc, msg);
+                    }
+                }
             }
-            const auto now = Clock::now();
-            for (uint64_t id : slot.reqs) {
+        }
+        if (s->stop) break;
+        s->cvWork.wait(lk, [&] { return s->stop || s->wakeups != seen; });
+    }
+}
+
+// Blocks on the oldest launch's event of its slot (without the lock), then
+// retires it: timestamps its requests, or fails them when the launch failed.
+void waiter_main(gcoo_server *s, Slot *slot)
+{
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (true) {
+        slot->cv.wait(lk, [&] { return s->stop || !slot->inflight.empty(); });
+        if (slot->inflight.empty()) break;  // stopping, nothing left in flight
+        const hipEvent_t ev = slot->inflight.front().event;
+        lk.unlock();
+        const hipError_t e = hipEventSynchronize(ev);
+        const auto now = Clock::now();
+        lk.lock();
+        Launch L = std::move(slot->inflight.front());
+        slot->inflight.pop_front();
+        slot->freeEvents.push_back(L.event);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            char msg[256];
+            snprintf(msg, sizeof(msg), "batch failed: %s", hipGetErrorString(e));
+            for (uint64_t id : L.reqs) {
+                Request &r = s->reqs[id];
+                r.state = kFailed;
+                r.err = GCOO_ERR_HIP;
+            }
+            fail_pending(s, GCOO_ERR_HIP, msg);
+        } else {
+            for (uint64_t id : L.reqs) {
                 Request &r = s->reqs[id];
                 r.done = now;
-                r.state = 3;
-                s->latMs.push_back(std::chrono::duration<double, std::milli>(now - r.arrival).count());
+                r.state = kDone;
+                s->latMs.push_back(
+                    std::chrono::duration<double, std::milli>(now - r.arrival).count());
             }
             s->lastDone = now;
-            slot.reqs.clear();
-            slot.busy = false;
-            progressed = true;
-            s->cvDone.notify_all();
         }
-        admit(s);
-        for (Slot &slot : s->slots) {
-            if (slot.busy || s->admitted.empty()) continue;
-            const int rc = launch(s, slot);
-            if (rc) {
-                s->workerError = rc;
-                snprintf(s->workerMsg, sizeof(s->workerMsg), "launch failed: %s",
-                         gcoo_last_error());
-                s->stop = true;
-            }
-            progressed = true;
-        }
-        const bool anyBusy = std::any_of(s->slots.begin(), s->slots.end(),
-                                         [](const Slot &x) { return x.busy; });
-        if (s->stop && !anyBusy) break;
-        if (!progressed) {
-            if (anyBusy) {
-                // poll completion events without holding the lock
-                lk.unlock();
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-                lk.lock();
-            } else {
-                // Nothing runs and nothing could be admitted or launched:
-                // wait for a submit, a release or stop.  (Waiting on "queue
-                // non-empty" instead would spin with the lock held while a
-                // queued request waits for budget.)
-                const uint64_t seen = s->wakeups;
-                s->cvWork.wait(lk, [&] { return s->stop || s->wakeups != seen; });
-            }
-        }
+        ++s->wakeups;
+        s->cvWork.notify_one();
+        s->cvDone.notify_all();
     }
-    s->cvDone.notify_all();
 }
 
 double percentile(std::vector<double> v, double p)
@@ -276,6 +369,10 @@ int gcoo_server_create(const gcoo_server_config *cfg, gcoo_server_t *out)
     auto *s = new (std::nothrow) gcoo_server;
     if (!s) return set_error(GCOO_ERR_ALLOC, "server");
     s->cfg = *cfg;
+    // plain FIFO: one stream, requests launched one by one in arrival order
+    s->plain = cfg->max_batch_jobs == 1;
+    s->depth = s->plain ? kPlainDepth : 1;
+    const int nSlots = s->plain ? 1 : cfg->slots;
     (void)hipGetDevice(&s->device);
     if (hipMalloc(&s->arenaBase, cfg->hbm_budget_bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -284,16 +381,24 @@ int gcoo_server_create(const gcoo_server_config *cfg, gcoo_server_t *out)
                          cfg->hbm_budget_bytes);
     }
     s->arena.reset(cfg->hbm_budget_bytes);
-    s->slots.resize(cfg->slots);
-    for (Slot &slot : s->slots) {
-        if (hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipEventCreateWithFlags(&slot.event, hipEventDisableTiming) != hipSuccess ||
-            gcoo_batch_create(&slot.batch) != GCOO_OK) {
+    for (int i = 0; i < nSlots; ++i) {
+        s->slots.emplace_back(new (std::nothrow) Slot);
+        Slot *slot = s->slots.back().get();
+        bool ok = slot != nullptr &&
+                  hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking) == hipSuccess &&
+                  (s->plain || gcoo_batch_create(&slot->batch) == GCOO_OK);
+        for (int e = 0; ok && e < s->depth; ++e) {
+            hipEvent_t ev = nullptr;
+            ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+            if (ok) slot->freeEvents.push_back(ev);
+        }
+        if (!ok) {
             (void)hipGetLastError();
             gcoo_server_destroy(s);
             return set_error(GCOO_ERR_HIP, "cannot create server streams");
         }
     }
+    for (auto &slot : s->slots) slot->waiter = std::thread(waiter_main, s, slot.get());
     s->worker = std::thread(worker_main, s);
     *out = s;
     return GCOO_OK;
@@ -305,16 +410,29 @@ int gcoo_server_destroy(gcoo_server_t s)
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
-        s->queue.clear();
         ++s->wakeups;
     }
     s->cvWork.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    for (Slot &slot : s->slots) {
-        if (slot.stream) (void)hipStreamSynchronize(slot.stream);
-        if (slot.batch) gcoo_batch_destroy(slot.batch);
-        if (slot.event) (void)hipEventDestroy(slot.event);
-        if (slot.stream) (void)hipStreamDestroy(slot.stream);
+    // the waiters retire what is still in flight, then exit
+    for (auto &slot : s->slots) {
+        if (!slot) continue;
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            slot->cv.notify_all();
+        }
+        if (slot->waiter.joinable()) slot->waiter.join();
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->cvDone.notify_all();
+    }
+    for (auto &slot : s->slots) {
+        if (!slot) continue;
+        if (slot->stream) (void)hipStreamSynchronize(slot->stream);
+        if (slot->batch) gcoo_batch_destroy(slot->batch);
+        for (hipEvent_t ev : slot->freeEvents) (void)hipEventDestroy(ev);
+        if (slot->stream) (void)hipStreamDestroy(slot->stream);
     }
     if (s->arenaBase) (void)hipFree(s->arenaBase);
     delete s;
@@ -374,13 +492,26 @@ int gcoo_server_wait(gcoo_server_t s, uint64_t req_id, double *latency_ms)
 {
     if (!s) return set_error(GCOO_ERR_INVALID, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
+    // the request is looked up again after every wake-up: submits from other
+    // threads may rehash the map while this one waits
+    auto state = [&]() -> int {
+        auto it = s->reqs.find(req_id);
+        return it == s->reqs.end() ? -1 : it->second.state;
+    };
+    if (state() < 0)
+        return set_error(GCOO_ERR_INVALID, "unknown request %llu", (unsigned long long)req_id);
+    s->cvDone.wait(lk, [&] {
+        const int st = state();
+        return st < 0 || st >= kDone || s->stop;
+    });
     auto it = s->reqs.find(req_id);
-    if (it == s->reqs.end()) return set_error(GCOO_ERR_INVALID, "unknown request %llu",
-                                              (unsigned long long)req_id);
-    s->cvDone.wait(lk, [&] { return it->second.state >= 3 || s->workerError || s->stop; });
-    if (it->second.state < 3)
-        return set_error(s->workerError ? s->workerError : GCOO_ERR_STATE, "%s",
-                         s->workerError ? s->workerMsg : "server stopped");
+    if (it == s->reqs.end())
+        return set_error(GCOO_ERR_STATE, "request %llu was released while waiting",
+                         (unsigned long long)req_id);
+    if (it->second.state == kFailed)
+        return set_error(it->second.err, "request %llu failed: %s", (unsigned long long)req_id,
+                         s->workerMsg);
+    if (it->second.state != kDone) return set_error(GCOO_ERR_STATE, "server stopped");
     if (latency_ms)
         *latency_ms = std::chrono::duration<double, std::milli>(it->second.done - it->second.arrival)
                           .count();
@@ -392,7 +523,12 @@ int gcoo_server_result(gcoo_server_t s, uint64_t req_id, const float **C, int *M
     if (!s) return set_error(GCOO_ERR_INVALID, "null server");
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(req_id);
-    if (it == s->reqs.end() || it->second.state != 3)
+    if (it == s->reqs.end()) return set_error(GCOO_ERR_STATE, "unknown request %llu",
+                                              (unsigned long long)req_id);
+    if (it->second.state == kFailed)
+        return set_error(it->second.err, "request %llu failed: %s", (unsigned long long)req_id,
+                         s->workerMsg);
+    if (it->second.state != kDone)
         return set_error(GCOO_ERR_STATE, "request %llu not completed", (unsigned long long)req_id);
     const Operand &o = s->ops[it->second.op];
     if (C) *C = reinterpret_cast<const float *>(s->arenaBase + it->second.offset);
@@ -407,10 +543,11 @@ int gcoo_server_release(gcoo_server_t s, uint64_t req_id)
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->reqs.find(req_id);
-        if (it == s->reqs.end() || it->second.state != 3)
+        if (it == s->reqs.end() ||
+            (it->second.state != kDone && it->second.state != kFailed))
             return set_error(GCOO_ERR_STATE, "request %llu not completed",
                              (unsigned long long)req_id);
-        s->arena.release(it->second.offset, it->second.bytes);
+        if (it->second.allocated) s->arena.release(it->second.offset, it->second.bytes);
         s->reqs.erase(it);
         ++s->wakeups;
     }
@@ -446,6 +583,15 @@ int gcoo_server_reset_stats(gcoo_server_t s)
     s->batches = s->jobsRun = 0;
     s->anyArrival = false;
     s->arena.reset_peak();
+    return GCOO_OK;
+}
+
+int gcoo_test_server_inject_failure(gcoo_server_t s, int code)
+{
+    if (!s) return set_error(GCOO_ERR_INVALID, "null server");
+    if (code <= 0) return set_error(GCOO_ERR_INVALID, "code %d (a GCOO_ERR_* value)", code);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->injectFailure = code;
     return GCOO_OK;
 }
 

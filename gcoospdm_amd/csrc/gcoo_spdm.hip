@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_f32_wave_kernel(
 {
     // When launched right behind an enqueued conversion, skip the work if
     // that conversion overflowed the entry capacity (the host re-runs it).
-    if (guard != nullptr && guard->total > cap) return;
+    if (guard_skip(guard, cap)) return;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ct = L / nGroupBlocks;
     const int gb = L - ct * nGroupBlocks;
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
     int M, const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C,
     int nGroupBlocks, const ConvStatus *__restrict__ guard, long long cap, int skipLen)
 {
-    if (guard != nullptr && guard->total > cap) return;
+    if (guard_skip(guard, cap)) return;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ct = L / nGroupBlocks;
     const int gb = L - ct * nGroupBlocks;
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_grouprows_kernel(
     // a row that runs out keeps reading (unused) entries up to the longest
     // sibling's length past its start: the segments must have nnz + maxRow +
     // kSegSlack entries of capacity (every conversion provides that)
-    if (guard != nullptr && guard->total + guard->maxRow + kSegSlack > cap) return;
+    if (guard != nullptr && guard_skip(guard, cap, (long long)guard->maxRow + kSegSlack)) return;
     const int nQuadBlocks = (nQuads + kSpdmWaves - 1) / kSpdmWaves;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ct = L / nQuadBlocks;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
     static_assert(CHUNK_P % NT == 0, "chunk must split evenly over threads");
     __shared__ f32x4 sD[2][CHUNK_P];
 
-    if (guard != nullptr && guard->total > cap) return;
+    if (guard_skip(guard, cap)) return;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ct = L / nPanels;
     const int panel = L - ct * nPanels;
@@ -757,7 +757,7 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     static_assert(2 * CHUNKB + 256 <= 160 * 1024, "LDS");
     __shared__ f32x4 sD[DUMMY + 16];
 
-    if (guard != nullptr && guard->total > cap) return;
+    if (guard_skip(guard, cap)) return;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ct = L / nPanels;
     const int panel = L - ct * nPanels;
@@ -991,7 +991,7 @@ __global__ __launch_bounds__(256) void longrow_classify_kernel(
 {
     // behind a conversion that overflowed its capacity the segments are
     // stale: list nothing (the host re-runs the multiply)
-    if (guard != nullptr && guard->total > cap) return;
+    if (guard_skip(guard, cap)) return;
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < M && rowptr[r + 1] - rowptr[r] >= minLen) list[atomicAdd(&ctr[0], 1u)] = r;
 }

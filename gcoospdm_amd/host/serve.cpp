@@ -4,11 +4,12 @@
 // both in HBM; M, K log-uniform in [256, 8192], N in {32, 64, 128, 256},
 // sparsity uniform in [0.95, 0.999], FP32 and BF16 forms of D), then drives
 // the library's continuous-batching server (gcoo_server_*):
-//   1. saturated capacity: every job submitted at once;
-//   2. Poisson arrivals at each -loads fraction of that capacity, `-requests`
-//      requests drawn uniformly over the jobs, in two modes:
+//   1. saturated capacity of each mode: every job submitted at once;
+//   2. Poisson arrivals at each -loads fraction of that mode's own capacity,
+//      `-requests` requests drawn uniformly over the jobs, in two modes:
 //        batched  continuous batching (-slots streams, -max_batch per batch)
-//        plain    one request per launch, one at a time, FIFO
+//        plain    a FIFO of plain multiply launches on one stream, one
+//                 request per launch, up to 64 launches in flight
 // and prints one JSON line per (dtype, mode, load) with p50 / p99 latency
 // (submit -> completion), jobs/s and GFLOP/s (2 * nnz * N per job), plus the
 // relative Frobenius error of a sample of results against FP64 products of
@@ -260,33 +261,42 @@ int main(int argc, char **argv)
         const std::vector<int> &opsB = bf16 ? opsBF16B : opsF32B;
         const std::vector<int> &opsP = bf16 ? opsBF16P : opsF32P;
         check(bf16);
-        // 1. saturated capacity: one request per job, all at once
-        std::vector<int> all(opsB.begin(), opsB.end());
-        std::vector<double> zero(all.size(), 0.0);
-        drive(batched, all, gflopOp, zero);  // warm-up
-        RunResult sat = drive(batched, all, gflopOp, zero);
-        const double cap = sat.st.completed / sat.st.wall_s;
-        std::printf("{\"kind\": \"saturated\", \"dtype\": \"%s\", \"mode\": \"batched\", \"jobs\": %llu, "
-                    "\"wall_ms\": %.3f, \"jobs_per_s\": %.1f, \"gflops\": %.1f, \"batches\": %llu, "
-                    "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"peak_bytes\": %zu, \"budget_bytes\": %zu}\n",
-                    bf16 ? "bf16" : "f32", (unsigned long long)sat.st.completed, sat.st.wall_s * 1e3,
-                    cap, sat.gflop / sat.st.wall_s, (unsigned long long)sat.st.batches,
-                    sat.st.p50_ms, sat.st.p99_ms, sat.st.peak_bytes, sat.st.budget_bytes);
-        std::fflush(stdout);
-        // 2. Poisson arrivals
+        // 1. saturated capacity of each mode: one request per job, all at once
+        double cap[2];
+        for (int mode = 0; mode < 2; ++mode) {
+            gcoo_server_t srv = mode == 0 ? batched : plain;
+            const std::vector<int> &ops = mode == 0 ? opsB : opsP;
+            std::vector<int> all(ops.begin(), ops.end());
+            std::vector<double> zero(all.size(), 0.0);
+            drive(srv, all, gflopOp, zero);  // warm-up
+            RunResult sat = drive(srv, all, gflopOp, zero);
+            cap[mode] = sat.st.completed / sat.st.wall_s;
+            std::printf("{\"kind\": \"saturated\", \"dtype\": \"%s\", \"mode\": \"%s\", \"jobs\": %llu, "
+                        "\"wall_ms\": %.3f, \"jobs_per_s\": %.1f, \"gflops\": %.1f, \"batches\": %llu, "
+                        "\"p50_ms\": %.3f, \"p99_ms\": %.3f, \"peak_bytes\": %zu, \"budget_bytes\": %zu}\n",
+                        bf16 ? "bf16" : "f32", mode == 0 ? "batched" : "plain",
+                        (unsigned long long)sat.st.completed, sat.st.wall_s * 1e3, cap[mode],
+                        sat.gflop / sat.st.wall_s, (unsigned long long)sat.st.batches,
+                        sat.st.p50_ms, sat.st.p99_ms, sat.st.peak_bytes, sat.st.budget_bytes);
+            std::fflush(stdout);
+        }
+        // 2. Poisson arrivals, each mode at the given fraction of its own
+        // capacity (the same request sequence, different rates)
         for (double load : loads) {
-            const double lambda = load * cap;
-            std::exponential_distribution<double> gap(lambda);
             std::uniform_int_distribution<int> pick(0, nJobs - 1);
+            std::exponential_distribution<double> unit(1.0);
             std::vector<int> idx(nReq);
-            std::vector<double> gaps(nReq);
+            std::vector<double> unitGaps(nReq);
             for (int r = 0; r < nReq; ++r) {
                 idx[r] = pick(arr);
-                gaps[r] = gap(arr);
+                unitGaps[r] = unit(arr);
             }
             for (int mode = 0; mode < 2; ++mode) {
                 gcoo_server_t srv = mode == 0 ? batched : plain;
                 const std::vector<int> &ops = mode == 0 ? opsB : opsP;
+                const double lambda = load * cap[mode];
+                std::vector<double> gaps(nReq);
+                for (int r = 0; r < nReq; ++r) gaps[r] = unitGaps[r] / lambda;
                 std::vector<int> seq(nReq);
                 for (int r = 0; r < nReq; ++r) seq[r] = ops[idx[r]];
                 const double t0 = now_s();

@@ -195,6 +195,14 @@ int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
 int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,
                      int N, float *C, gcoo_stream s);
 
+/* Testing hook, no counterpart in the reference: raises the abort word of
+ * the single-pass dense conversion, as a group whose look-back wait gave up
+ * does, so that the next gcoo_convert_dense / gcoo_multiply_by takes the
+ * fallback (count, scan and scatter kernels, the reference's three launches
+ * src/ShGCOOSpDM.cu:44-111) and must still return the same arrays.  Every
+ * kernel still exits normally. */
+int gcoo_test_set_conversion_abort(gcoo_handle h, gcoo_stream s);
+
 /* ---- Batches: many independent multiplies in one persistent launch -------
  * No counterpart in the reference (one multiply per launch,
  * src/ShGCOOSpDM.cu:153-167).  A batch holds jobs C_i = S_i * D_i, each S_i
@@ -207,7 +215,11 @@ int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,
  * so it can be captured into a HIP graph and replayed.  Every C element is
  * the same fmaf chain as in gcoo_spdm_f32 / gcoo_spdm_bf16: results are
  * bit-identical to running each job alone.  The batch keeps the pointers it
- * was given; handles and operands must outlive its runs. */
+ * was given; handles and operands must outlive its runs.  Commit binds each
+ * job to its matrix's current conversion: once a job's matrix is converted
+ * again (which may reallocate its arrays), run, capture and graph launch
+ * return GCOO_ERR_STATE, launching nothing, until the batch is committed
+ * (and captured) again. */
 typedef struct gcoo_batch *gcoo_batch_t;
 int gcoo_batch_create(gcoo_batch_t *b);
 int gcoo_batch_destroy(gcoo_batch_t b);
@@ -233,8 +245,18 @@ int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes
  * `slots` streams is idle launches all admitted requests (at most
  * max_batch_jobs) as one persistent batch on it, so requests arriving while
  * other batches run are batched on another stream and overlap on the device.
+ * With max_batch_jobs == 1 the server is a plain FIFO instead: each request
+ * is one launch of the plain multiply (gcoo_spdm_f32_async /
+ * gcoo_spdm_bf16_async) on a single stream, in arrival order, with up to 64
+ * launches in flight (`slots` is not used).  A thread per stream waits on
+ * its launches' completion events, so completion is event-driven.
  * Latency is arrival (submit) to completion.  Results are bit-identical to
- * the same multiply run alone; they stay in the arena until released. */
+ * the same multiply run alone; they stay in the arena until released.
+ * Failures: when a launch cannot be enqueued or a launched batch reports an
+ * error, its requests and every request not yet launched fail with that
+ * error code (gcoo_server_wait and gcoo_server_result return it, release
+ * frees them), and later submits are refused with it.  A registered handle
+ * must not be converted again while the server may run its requests. */
 typedef struct gcoo_server *gcoo_server_t;
 typedef struct gcoo_server_config_s {
     size_t hbm_budget_bytes; /* output arena size: the admission budget */
@@ -261,6 +283,11 @@ int gcoo_server_result(gcoo_server_t s, uint64_t req_id, const float **C, int *M
 int gcoo_server_release(gcoo_server_t s, uint64_t req_id);
 int gcoo_server_stats(gcoo_server_t s, gcoo_server_stats_t *st);
 int gcoo_server_reset_stats(gcoo_server_t s);
+/* Testing hook, no counterpart in the reference: the server's next launch
+ * fails with `code` (a GCOO_ERR_* value) before anything is enqueued, as a
+ * failing launch call would, so that tests can check the failure path
+ * without making the GPU fail. */
+int gcoo_test_server_inject_failure(gcoo_server_t s, int code);
 
 #ifdef __cplusplus
 }

@@ -116,3 +116,96 @@ def test_batch_config4_sample(dev):
             ref = alone(m, D, sp.N, bf16)
             assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), sp
         del jobs
+
+
+def _host_job(sp, dev):
+    """The job's S and D on the host, generated by the same device kernels."""
+    import torch
+    S = torch.empty((sp.M, sp.K), device=dev)
+    G.fill_sparse(S, sp.seed, 1, sp.sparsity)
+    D = torch.empty((sp.K, sp.N), device=dev)
+    G.fill_uniform(D, sp.seed, 0)
+    return S.cpu().numpy(), D.cpu().numpy()
+
+
+@pytest.mark.parametrize("bf16", [False, True])
+def test_batch_config4_512_graph(dev, bf16):
+    """The full batching workload: 512 jobs (M, K log-uniform in [256, 8192],
+    N in {32, 64, 128, 256}, s in [0.95, 0.999]) with tuned tiles, captured
+    into one HIP graph and replayed once.  Every C is bit-equal to the job
+    run alone, and 32 jobs spread over the batch are bit-equal to the
+    oracle's fmaf chain (BF16: the chain over BF16-widened operands), with
+    the relative Frobenius error against FP64 of the FP32 operands within
+    1e-5 (FP32) / 1e-2 (BF16), the check of src/main.cu:128-142."""
+    import torch
+    from gcoospdm_amd import autotune
+    print(f"batch 512 graph bf16={bf16}", flush=True)
+    specs = W.batch_jobs(512, 2024)
+    jobs = [W.materialize(sp, dev, bf16) for sp in specs]
+    b = G.Batch()
+    for (m, D, C), sp in zip(jobs, specs):
+        C.fill_(float("nan"))
+        rows, vec = autotune.batch_tile(sp.N, sp.K * (1 - sp.sparsity), bf16)
+        b.add(m, D, C, rows=rows, vec=vec)
+    b.commit()
+    s = torch.cuda.Stream()
+    b.capture(s)
+    b.replay(s)
+    s.synchronize()
+    assert b.info()["jobs"] == 512
+    for (m, D, C), sp in zip(jobs, specs):
+        ref = alone(m, D, sp.N, bf16)
+        assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), sp
+    tol = 1e-2 if bf16 else 1e-5
+    worst = 0.0
+    for i in np.linspace(0, 511, 32).astype(int):
+        sp = specs[i]
+        S, Dh = _host_job(sp, dev)
+        g = oracle.convert(S, 4)
+        Cn = jobs[i][2].cpu().numpy()
+        if bf16:
+            want = oracle.spdm_bf16(g, oracle.to_bf16(g["vals"]), oracle.to_bf16(Dh),
+                                    threads=oracle.max_threads())
+        else:
+            want = oracle.spdm_f32(g, Dh, threads=oracle.max_threads())
+        assert H.bits_equal(Cn, want), sp
+        Cd, _ = oracle.spdm_fp64(g, Dh, threads=oracle.max_threads())
+        err = oracle.rel_frobenius(Cn, Cd)
+        worst = max(worst, err)
+        assert err <= tol, (sp, err)
+    print(f"batch 512 bf16={bf16}: 32 jobs oracle-exact, worst rel_frobenius {worst:.3e}",
+          flush=True)
+
+
+def test_batch_refuses_after_reconvert(dev):
+    """Converting a job's matrix again after commit (here with growth, so its
+    arrays are reallocated) makes run, capture and replay refuse with
+    GCOO_ERR_STATE, launching nothing; a new commit binds the new arrays."""
+    import torch
+    specs = small_specs(6, 21)
+    jobs = [W.materialize(sp, dev, bf16) for sp, bf16 in zip(specs, [False, True] * 3)]
+    b = G.Batch()
+    for m, D, C in jobs:
+        b.add(m, D, C)
+    b.commit()
+    s = torch.cuda.Stream()
+    b.capture(s)
+    b.replay(s)
+    s.synchronize()
+    # job 1 (BF16) gets a much denser S: its entry arrays are reallocated
+    m, D, C = jobs[1]
+    S = torch.empty((m.M, m.K), device=dev)
+    G.fill_sparse(S, 99, 1, 0.3)
+    m.convert_dense(S)
+    for call in (b.run, lambda: b.capture(s), lambda: b.replay(s)):
+        with pytest.raises(G.GCOOError) as e:
+            call()
+        assert e.value.code == G.GCOO_ERR_STATE
+    for _, _, C in jobs:
+        C.fill_(float("nan"))
+    b.commit()
+    b.run()
+    torch.cuda.synchronize()
+    for (m, D, C), sp in zip(jobs, specs):
+        ref = alone(m, D, sp.N, D.dtype != torch.float32)
+        assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), sp

@@ -133,9 +133,9 @@ def test_config1_800_s09_vs_fp64(dev):
 @pytest.mark.parametrize("variant", [0, 6])
 @pytest.mark.parametrize("sparsity", [0.99, 0.995, 0.999])
 def test_config2_n8000_full_size(dev, sparsity, variant):
-    """n = 8000: conversion bit-exact at full size; the product bit-exact on a
-    sample of row groups spread over the matrix and all columns, plus FP64
-    rel-Frobenius on the same sample."""
+    """n = 8000: conversion bit-exact at full size, and all of C bit-exact
+    against the oracle's fmaf chain plus FP64 rel-Frobenius <= 1e-5 over all
+    of C (the check of src/main.cu:128-142 at this size)."""
     import torch
     n = 8000
     print(f"config2 n={n} s={sparsity} variant={variant}", flush=True)
@@ -150,14 +150,13 @@ def test_config2_n8000_full_size(dev, sparsity, variant):
     D = Dd.cpu().numpy()
     g = oracle.convert(S, 4)
     check_arrays(m, g)
-    groups = np.unique(np.linspace(0, n // 4 - 1, 48).astype(int))
-    rows = np.concatenate([np.arange(4 * q, 4 * q + 4) for q in groups])
-    sub = oracle.convert(S[rows], 4)
-    Csub = oracle.spdm_f32(sub, D, threads=oracle.max_threads())
-    Cn = C.cpu().numpy()[rows]
-    assert H.bits_equal(Cn, Csub)
-    Cd, _ = oracle.spdm_fp64(sub, D, threads=oracle.max_threads())
-    assert oracle.rel_frobenius(Cn, Cd) <= FP32_REL_TOL
+    Cn = C.cpu().numpy()
+    Cref = oracle.spdm_f32(g, D, threads=oracle.max_threads())
+    assert H.bits_equal(Cn, Cref)
+    Cd, _ = oracle.spdm_fp64(g, D, threads=oracle.max_threads())
+    err = oracle.rel_frobenius(Cn, Cd)
+    print(f"config2 s={sparsity} rel_frobenius={err:.3e}", flush=True)
+    assert err <= FP32_REL_TOL
 
 
 @pytest.mark.parametrize("variant", [0, 6])
@@ -381,3 +380,33 @@ def test_main_driver_reference_inputs(dev):
     print(r.stdout)
     assert r.returncode == 0, r.stderr
     assert "Test passed" in r.stdout
+
+
+@pytest.mark.parametrize("M,K,N,dens", [(257, 516, 96, 0.1), (2000, 2048, 256, 0.01)])
+def test_conversion_stall_fallback(dev, M, K, N, dens):
+    """The single-pass conversion's abort word is raised from the host before
+    the launch (as a group whose bounded look-back wait gave up would raise
+    it): every kernel still exits normally, the multiply queued behind the
+    conversion skips, and the library converts again with the count / scan /
+    scatter kernels.  Arrays and C must still equal the oracle's, and the
+    next conversion takes the single-pass kernel again."""
+    import torch
+    print(f"stall fallback M={M} K={K} N={N} dens={dens}", flush=True)
+    rng = np.random.default_rng(M + K)
+    m = G.GCOOMatrix(M, K, 4)
+    for step in range(3):
+        S = H.rng_sparse(rng, M, K, dens * (1 + step))
+        D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        Sd, Dd = to_dev(S, dev), to_dev(D, dev)
+        C = torch.full((M, N), float("nan"), device=dev)
+        if step < 2:
+            rc = G.lib().gcoo_test_set_conversion_abort(m.handle, G._stream())
+            assert rc == 0
+        if step == 1:
+            m.convert_dense(Sd)
+            m.spdm(Dd, C)
+        else:
+            m.multiply_by(Sd, Dd, C)
+        g = oracle.convert(S, 4)
+        check_arrays(m, g)
+        assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D)), step
