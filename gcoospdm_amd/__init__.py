@@ -72,6 +72,7 @@ _SIGNATURES = [
     ("gcoo_get_arrays", _I, [_P, _PP, _PP, _PP, _PP, _PP]),
     ("gcoo_spdm_f32_async", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_spdm_f32", _I, [_P, _P, _I, _I, _P, _P]),
+    ("gcoo_spdm_f32_noalloc_async", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_multiply_by", _I, [_P, _P, _P, _I, _I, _P, _P]),
     ("gcoo_test_set_conversion_abort", _I, [_P, _P]),
     ("gcoo_set_spdm_config", _I, [_P, _P]),

@@ -193,6 +193,26 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
                        const ConvStatus *guard, hipStream_t s);
 // Forget the panel lists (after a conversion or a capacity change).
 inline void invalidate_panel_lists(const gcoo_matrix *h) { h->plist_src = nullptr; }
+// Ints of the offset (and count) area the panel lists of h need for k
+// chunks of tk rows; the lists themselves need h->nnz uint2 entries.
+size_t panel_offset_elems(const gcoo_matrix *h, int tk);
+// Enqueue the build of the panel lists into caller-given buffers (poff of
+// panel_offset_elems ints, plist of h->nnz entries); touches no handle state.
+int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int *poff,
+                      uint2 *plist, const ConvStatus *guard, hipStream_t s);
+
+// Device scratch handed to the multiply by a caller that cannot allocate:
+// the panel lists and the long-row list are carved out of it instead of
+// the handle's own buffers.  base is 256-byte aligned.
+struct ExtScratch {
+    char *base;
+    size_t bytes;
+};
+// FP32 multiply with every scratch buffer taken from xs (rebuilt on every
+// call); returns GCOO_ERR_BUDGET, before enqueuing anything, when the
+// chosen kernel needs more than xs->bytes.
+int launch_spdm_f32_ext(const gcoo_matrix *h, const float *D, int N, float *C,
+                        const ExtScratch *xs, hipStream_t s);
 
 // ---- batches (gcoo_batch.hip) ----
 // gcoo_batch_commit without the final stream synchronisation.

@@ -127,10 +127,20 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
 
 }  // namespace
 
-int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
-                       const ConvStatus *guard, hipStream_t s)
+size_t panel_offset_elems(const gcoo_matrix *h, int tk)
 {
-    if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb) return GCOO_OK;
+    const int nP = (h->M + kPanelRows - 1) / kPanelRows;
+    const int nC = (h->K + tk - 1) / tk;
+    const size_t nW = (size_t)nP * (kPanelRows / kPanelRowsPerWave);
+    const size_t L = (size_t)nC * kPanelRowsPerWave;
+    const bool lds = L <= (size_t)kListLdsInts;
+    // offsets of every wave, then (counts beyond LDS) the count scratch
+    return nW * ((size_t)nC + 1) + (lds ? 0 : nW * L);
+}
+
+int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int *poff,
+                      uint2 *plist, const ConvStatus *guard, hipStream_t s)
+{
     if ((long long)tk * rowb >= (1ll << 24))
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
     const int nP = (h->M + kPanelRows - 1) / kPanelRows;
@@ -138,8 +148,28 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
     const size_t nW = (size_t)nP * (kPanelRows / kPanelRowsPerWave);
     const size_t L = (size_t)nC * kPanelRowsPerWave;
     const bool lds = L <= (size_t)kListLdsInts;
-    // offsets of every wave, then (counts beyond LDS) the count scratch
-    const size_t elems = nW * ((size_t)nC + 1) + (lds ? 0 : nW * L);
+    if (nW > 0) {
+        int *gcnt = poff + nW * ((size_t)nC + 1);
+        if (lds)
+            hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
+                               dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
+                               rowb, nC, poff, gcnt, plist, guard, (long long)h->cap);
+        else
+            hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
+                               dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC,
+                               poff, gcnt, plist, guard, (long long)h->cap);
+        GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
+    }
+    return GCOO_OK;
+}
+
+int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
+                       const ConvStatus *guard, hipStream_t s)
+{
+    if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb) return GCOO_OK;
+    if ((long long)tk * rowb >= (1ll << 24))
+        return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
+    const size_t elems = panel_offset_elems(h, tk);
     if (elems > h->poff_elems) {
         if (h->poff) GCOO_HIP_TRY(hipFree(h->poff));
         h->poff = nullptr;
@@ -158,18 +188,8 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
             return set_error(GCOO_ERR_ALLOC, "cannot allocate the panel lists");
         }
     }
-    if (nW > 0) {
-        int *gcnt = h->poff + nW * ((size_t)nC + 1);
-        if (lds)
-            hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
-                               dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
-                               rowb, nC, h->poff, gcnt, h->plist, guard, (long long)h->cap);
-        else
-            hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
-                               dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC,
-                               h->poff, gcnt, h->plist, guard, (long long)h->cap);
-        GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
-    }
+    int rc = build_panel_lists(h, seg, tk, rowb, h->poff, h->plist, guard, s);
+    if (rc) return rc;
     h->plist_src = seg;
     h->plist_tk = tk;
     h->plist_rowb = rowb;

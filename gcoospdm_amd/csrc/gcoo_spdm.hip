@@ -1119,20 +1119,45 @@ int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, floa
     return GCOO_OK;
 }
 
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
 template <typename E, int TK>
 int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                 const ConvStatus *guard, hipStream_t s)
+                 const ConvStatus *guard, hipStream_t s, const ExtScratch *xs)
 {
     constexpr int ROWB = 256 * E::bytes;  // bytes of a staged D row (256 columns)
-    int rc = ensure_panel_lists(h, seg, TK, ROWB, guard, s);
-    if (rc) return rc;
+    const int *poff = h->poff;
+    const uint2 *plist = h->plist;
+    if (xs) {
+        // lists rebuilt into the caller's scratch: offsets, then the entries
+        // (plus the slack the kernel's batched entry loads may read past a
+        // list's end)
+        const size_t offB = align256(sizeof(int) * panel_offset_elems(h, TK));
+        const size_t listB = sizeof(uint2) * ((size_t)h->nnz + kSegSlack);
+        if (offB + listB > xs->bytes)
+            return set_error(GCOO_ERR_BUDGET,
+                             "panel lists of %lld entries need %zu bytes of scratch, the "
+                             "no-allocation entry point has %zu (variant 2 needs none)",
+                             (long long)h->nnz, offB + listB, xs->bytes);
+        int *po = reinterpret_cast<int *>(xs->base);
+        uint2 *pl = reinterpret_cast<uint2 *>(xs->base + offB);
+        int rc = build_panel_lists(h, seg, TK, ROWB, po, pl, guard, s);
+        if (rc) return rc;
+        poff = po;
+        plist = pl;
+    } else {
+        int rc = ensure_panel_lists(h, seg, TK, ROWB, guard, s);
+        if (rc) return rc;
+        poff = h->poff;
+        plist = h->plist;
+    }
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
     const int nPanels = (h->M + kPanelRows - 1) / kPanelRows;
     const int nC = (h->K + TK - 1) / TK;
     const int tiles = (N + 255) / 256;
     const long long nblocks = (long long)nPanels * tiles;
     hipLaunchKernelGGL((spdm_panel_kernel<E, TK>), dim3((unsigned)nblocks), dim3(1024), 0, s,
-                       h->poff, h->plist, h->M, h->K, nC, D, (unsigned)dBytes, N, C, nPanels,
+                       poff, plist, h->M, h->K, nC, D, (unsigned)dBytes, N, C, nPanels,
                        guard, (long long)h->cap);
     GCOO_CHECK_LAUNCH("spdm_panel_kernel");
     return GCOO_OK;
@@ -1189,9 +1214,19 @@ int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, f
 // kernel.  Scratch (the row list and its counters) lives in the handle.
 template <typename E, int VEC>
 int launch_split(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                 const ConvStatus *guard, hipStream_t s, int minLen)
+                 const ConvStatus *guard, hipStream_t s, int minLen, const ExtScratch *xs)
 {
-    if (!h->longList) {
+    int *longList = h->longList;
+    unsigned *longCtr = h->longCtr;
+    if (xs) {
+        const size_t listB = align256(sizeof(int) * ((size_t)h->M + 1));
+        if (listB + 256 > xs->bytes)
+            return set_error(GCOO_ERR_BUDGET,
+                             "the long-row list of %d rows needs %zu bytes of scratch, the "
+                             "no-allocation entry point has %zu", h->M, listB + 256, xs->bytes);
+        longList = reinterpret_cast<int *>(xs->base);
+        longCtr = reinterpret_cast<unsigned *>(xs->base + listB);
+    } else if (!h->longList) {
         if (hipMalloc(&h->longList, sizeof(int) * ((size_t)h->M + 1)) != hipSuccess ||
             hipMalloc(&h->longCtr, 256) != hipSuccess) {
             (void)hipGetLastError();
@@ -1199,14 +1234,16 @@ int launch_split(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
             h->longList = nullptr;
             return set_error(GCOO_ERR_ALLOC, "long-row scratch");
         }
+        longList = h->longList;
+        longCtr = h->longCtr;
     }
     constexpr int W = 4, T = 32;
     const int colTiles = (N + W * 64 - 1) / (W * 64);
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
-    hipLaunchKernelGGL(longrow_reset_kernel, dim3(1), dim3(64), 0, s, h->longCtr);
+    hipLaunchKernelGGL(longrow_reset_kernel, dim3(1), dim3(64), 0, s, longCtr);
     GCOO_CHECK_LAUNCH("longrow_reset_kernel");
     hipLaunchKernelGGL(longrow_classify_kernel, dim3((h->M + 255) / 256), dim3(256), 0, s,
-                       h->rowptr, h->M, minLen, h->longList, h->longCtr, guard,
+                       h->rowptr, h->M, minLen, longList, longCtr, guard,
                        (long long)h->cap);
     GCOO_CHECK_LAUNCH("longrow_classify_kernel");
     int dev = 0, cus = 256, perCu = 0;
@@ -1218,7 +1255,7 @@ int launch_split(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         perCu = 1;
     }
     hipLaunchKernelGGL((spdm_longrow_kernel<E, W, T>), dim3(cus * perCu), dim3(W * 64), 0, s,
-                       h->rowptr, seg, h->longList, h->longCtr, colTiles, D, (unsigned)dBytes, N,
+                       h->rowptr, seg, longList, longCtr, colTiles, D, (unsigned)dBytes, N,
                        C);
     GCOO_CHECK_LAUNCH("spdm_longrow_kernel");
     return launch_rowseg<E, VEC>(h, seg, D, N, C, guard, s, minLen);
@@ -1268,7 +1305,7 @@ int max_vec(const void *D, int N, const float *C, int elemBytes)
 
 template <typename E>
 int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                 const ConvStatus *guard, hipStream_t s)
+                 const ConvStatus *guard, hipStream_t s, const ExtScratch *xs = nullptr)
 {
     const gcoo_spdm_config &cfg = h->spdm_cfg;
     const int vmax = max_vec(D, N, C, E::bytes);
@@ -1319,8 +1356,8 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
     // 16-byte aligned D) and stores 4 columns per lane (vec 4 allowed)
     if (variant == 6 && (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0 &&
         vmax >= 4) {
-        if constexpr (E::bytes == 4) return launch_panel<E, 75>(h, seg, D, N, C, guard, s);
-        else return launch_panel<E, 150>(h, seg, D, N, C, guard, s);
+        if constexpr (E::bytes == 4) return launch_panel<E, 75>(h, seg, D, N, C, guard, s, xs);
+        else return launch_panel<E, 150>(h, seg, D, N, C, guard, s, xs);
     }
     if (variant == 6) variant = 2;
     if (variant == 5) {
@@ -1345,14 +1382,14 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0;
     if (splitOk && variant == 4) {
         if constexpr (E::bytes == 4) {
-            if (vec >= 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen);
-            if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen);
-            return launch_split<E, 1>(h, seg, D, N, C, guard, s, minLen);
+            if (vec >= 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen, xs);
+            if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen, xs);
+            return launch_split<E, 1>(h, seg, D, N, C, guard, s, minLen, xs);
         } else {
-            if (vec >= 8) return launch_split<E, 8>(h, seg, D, N, C, guard, s, minLen);
-            if (vec == 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen);
-            if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen);
-            return launch_split<E, 1>(h, seg, D, N, C, guard, s, minLen);
+            if (vec >= 8) return launch_split<E, 8>(h, seg, D, N, C, guard, s, minLen, xs);
+            if (vec == 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen, xs);
+            if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen, xs);
+            return launch_split<E, 1>(h, seg, D, N, C, guard, s, minLen, xs);
         }
     }
     if constexpr (E::bytes == 4) {
@@ -1369,8 +1406,23 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
 
 }  // namespace
 
+int launch_spdm_f32_cfg(const gcoo_matrix *h, const float *D, int N, float *C,
+                        const ConvStatus *guard, hipStream_t s, const ExtScratch *xs);
+
+int launch_spdm_f32_ext(const gcoo_matrix *h, const float *D, int N, float *C,
+                        const ExtScratch *xs, hipStream_t s)
+{
+    return launch_spdm_f32_cfg(h, D, N, C, nullptr, s, xs);
+}
+
 int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s)
+{
+    return launch_spdm_f32_cfg(h, D, N, C, guard, s, nullptr);
+}
+
+int launch_spdm_f32_cfg(const gcoo_matrix *h, const float *D, int N, float *C,
+                        const ConvStatus *guard, hipStream_t s, const ExtScratch *xs)
 {
     if (h->nGroup == 0 || N == 0) return GCOO_OK;
     const gcoo_spdm_config &cfg = h->spdm_cfg;
@@ -1388,7 +1440,7 @@ int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
         default: return set_error(GCOO_ERR_INVALID, "cpg %d not supported", h->cpg);
         }
     }
-    return launch_typed<F32>(h, h->seg, D, N, C, guard, s);
+    return launch_typed<F32>(h, h->seg, D, N, C, guard, s, xs);
 }
 
 int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C, hipStream_t s)

@@ -139,6 +139,20 @@ int gcoo_spdm_f32_async(gcoo_handle h, const float *D, int K, int N, float *C,
 int gcoo_spdm_f32(gcoo_handle h, const float *D, int K, int N, float *C,
                   gcoo_stream s);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): gcoo_spdm_f32_async with the same kernels and the same
+ * bits, but calling no hipMalloc / hipFree.  The scratch its kernels need --
+ * the panel lists of the LDS-staged kernel, rebuilt on every call, and the
+ * long-row list and counter -- lives in one 16 MiB device array per GPU,
+ * held by the library until the process ends.  A multiply whose scratch
+ * would exceed 16 MiB (panel lists take 8 bytes per entry, so about 2 M
+ * entries; variant 2 needs none) returns GCOO_ERR_BUDGET and enqueues
+ * nothing.  All calls on one GPU share that array: calls that overlap on a
+ * GPU (other streams, other threads) race, so the caller serialises them. */
+int gcoo_spdm_f32_noalloc_async(gcoo_handle h, const float *D, int K, int N, float *C,
+                                gcoo_stream s);
+
 /* BF16 storage form (no counterpart in the reference, which is FP32 only):
  * S's entry values and D stored as BF16 -- the upper 16 bits of an FP32,
  * rounded to nearest-even by gcoo_f32_to_bf16_async -- widened exactly to
