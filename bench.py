@@ -148,15 +148,39 @@ def cpu_baseline(S_host, D_host, nnz_total, N, budget_s=15.0):
     }
 
 
-def multiply_kernel(nnz, M, K, N):
-    """The FP32 multiply kernel the library chooses for this workload on its
-    own (gcoo_spdm.hip, launch_typed): the panel-staged kernel from 0.7 %
-    density when C has at least 256 tiles of 256 x 256 (N a multiple of 4,
-    aligned operands, as here), else the row-segment kernel."""
-    tiles = ((M + 255) // 256) * ((N + 255) // 256)
-    if nnz >= 0.007 * M * K and tiles >= 256 and N % 4 == 0:
-        return "spdm_panel_kernel<F32, 75>"
-    return "spdm_rowseg_kernel<F32, 2, 8>"
+KERNEL_OF_VARIANT = {
+    1: "spdm_f32_wave_kernel",
+    2: "spdm_rowseg_kernel",
+    3: "spdm_lds_kernel",
+    4: "spdm_longrow_kernel + spdm_rowseg_kernel",
+    5: "spdm_grouprows_kernel",
+    6: "spdm_panel_kernel",
+}
+
+
+def multiply_kernel(m, D, C):
+    """The multiply kernel the library runs for these operands, as it
+    reports it (gcoo_get_spdm_choice)."""
+    variant, vec = m.spdm_choice(D, C)
+    return KERNEL_OF_VARIANT.get(variant, f"variant {variant}"), variant, vec
+
+
+def in_step_multiply_us(m, S, D, C, stream, reps=10):
+    """The multiply's duration right after a conversion of S (as inside a
+    mutiplyBy step, whose conversion streams S through the caches): HIP
+    events around the multiply alone, after each conversion."""
+    import torch
+    tot = 0.0
+    for _ in range(reps):
+        m.convert_dense(S, stream=stream)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        m.spdm(D, C, stream=stream, sync=False)
+        e1.record(stream)
+        e1.synchronize()
+        tot += e0.elapsed_time(e1) * 1e3
+    return tot / reps
 
 
 def traffic_from_profiles():
@@ -435,8 +459,9 @@ def main():
     achieved = bytes_k / t_kern / 1e9
     torch.cuda.synchronize()
     t_conv = event_time(lambda: m.convert_dense(S), 10, stream)
+    t_instep = in_step_multiply_us(m, S, D, C, stream)
 
-    kname = multiply_kernel(nnz, M, K, N)
+    kname, kvariant, kvec = multiply_kernel(m, D, C)
     traffic = None
     tr = traffic_from_profiles()
     if tr and tr.get("workload") == f"square_n{n}_s{s}" and tr.get("kernel") == kname:
@@ -478,10 +503,13 @@ def main():
                    "cpg": 4, "nnz": nnz,
                    "path": "mutiplyBy: dense->GCOO conversion + grouped-COO multiply, FP32",
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": kname + " (FP32 multiply)",
+        "roofline": {"bound": "hbm", "kernel": kname,
+                     "variant": kvariant, "vec": kvec,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": bytes_k, "mean_launch_us": round(t_kern * 1e6, 2),
+                     "in_step_launch_us": round(t_instep, 2),
+                     "in_step_frac": round(bytes_k / (t_instep * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
                      "fp32_tflops": round(2.0 * nnz * N / t_kern / 1e12, 3),
                      "fp32_vector_peak_tflops": FP32_PEAK_TFLOPS},
         "kernel_only_gflops": round(2.0 * nnz * N / t_kern / 1e9, 3),

@@ -170,6 +170,10 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols,
 // guard: when non-null, the kernel exits if guard->total > h->cap.
 int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s);
+// The multiply variant (1..6) and columns per lane a launch of h on these
+// operands runs (dtype 0 FP32, 1 BF16).
+int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const float *C,
+                int *variant, int *vec);
 // BF16 storage form: h->seg16 must hold the BF16 entry values.
 int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
                      hipStream_t s);

@@ -1303,40 +1303,60 @@ int max_vec(const void *D, int N, const float *C, int elemBytes)
     return 1;
 }
 
+// The variant (2..6) and columns per lane that a launch of h on these
+// operands runs: the handle's configuration, or the automatic choice, after
+// the fall-backs each variant's operand requirements impose.
 template <typename E>
-int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                 const ConvStatus *guard, hipStream_t s, const ExtScratch *xs = nullptr)
+void resolve_variant(const gcoo_matrix *h, const void *D, int N, const float *C, int *variantOut,
+                     int *vecOut)
 {
     const gcoo_spdm_config &cfg = h->spdm_cfg;
     const int vmax = max_vec(D, N, C, E::bytes);
     const int vdef = E::bytes == 4 ? 2 : 4;
     int vec = cfg.vec ? cfg.vec : vdef;
     if (vec > vmax) vec = vmax;
-    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
-    if (dBytes >= 0x7fffffffull)
-        return set_error(GCOO_ERR_UNSUPPORTED,
-                         "dense operand of %llu bytes: the buffer-addressed kernels need < 2 GiB",
-                         dBytes);
     int variant = cfg.variant;
     if (variant == 0) {
         // Panel staging costs a fixed K x N read of D per 256-row panel, row
         // segments cost one D-row gather per entry: the panel kernel wins from
-        // about 0.45 % density on (measured at n = 8000, FP32: 527 vs 594 us
-        // at 0.5 %, 509 vs 483 us at 0.4 %).  BF16 gathers move half the
-        // bytes and stay ahead at every density.  The density is the handle's
-        // last completed conversion (a first optimistic launch does not know
-        // the new count yet); every variant gives the same bits.
-        // It also needs a workgroup per CU at least (256 x 256 tiles of C).
+        // about 0.3 % density on (n = 8000, FP32: both take 370 us at
+        // s = 0.997, 389 against 486 us at 0.996, 372 against 274 us at
+        // 0.998).  BF16 gathers move half the bytes and stay ahead at every
+        // density.  The density is the handle's last completed conversion (a
+        // first optimistic launch does not know the new count yet); every
+        // variant gives the same bits.  It also needs a workgroup per CU at
+        // least (256 x 256 tiles of C).
         const double cells = (double)h->M * (double)h->K;
         const bool dense = cells > 0 && (double)h->nnz >= kPanelMinDensity * cells;
         const long long wgs = (long long)((h->M + kPanelRows - 1) / kPanelRows) * ((N + 255) / 256);
         variant = (E::bytes == 4 && dense && wgs >= kPanelMinWorkgroups) ? 6 : 2;
     }
     if (variant == 1) variant = 2;
-    // variant 3 stages 16-byte pieces of D rows: it needs whole pieces per
-    // row (N * bytes % 16 == 0) and a 16-byte aligned D
-    if (variant == 3 && ((N * E::bytes) % 16 != 0 || (reinterpret_cast<uintptr_t>(D) & 15u) != 0))
-        variant = 2;
+    // variants 3, 4 and 6 stage 16-byte pieces of D rows: they need whole
+    // pieces per row (N * bytes % 16 == 0) and a 16-byte aligned D; variant
+    // 6 also stores 4 columns per lane (vec 4 allowed)
+    const bool pieces =
+        (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0;
+    if ((variant == 3 || variant == 4) && !pieces) variant = 2;
+    if (variant == 6 && !(pieces && vmax >= 4)) variant = 2;
+    if (variant == 6) vec = 4;  // 256 columns per workgroup, 4 per lane
+    if (variant == 5) vec = vmax;
+    *variantOut = variant;
+    *vecOut = vec;
+}
+
+template <typename E>
+int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
+                 const ConvStatus *guard, hipStream_t s, const ExtScratch *xs = nullptr)
+{
+    const gcoo_spdm_config &cfg = h->spdm_cfg;
+    const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
+    if (dBytes >= 0x7fffffffull)
+        return set_error(GCOO_ERR_UNSUPPORTED,
+                         "dense operand of %llu bytes: the buffer-addressed kernels need < 2 GiB",
+                         dBytes);
+    int variant = 2, vec = 1;
+    resolve_variant<E>(h, D, N, C, &variant, &vec);
     if (variant == 3) {
         const int waves = cfg.waves ? cfg.waves : 16;
         if constexpr (E::bytes == 4) {
@@ -1352,35 +1372,28 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
             return launch_lds<E, 2, 16, 16, 256>(h, seg, D, N, C, guard, s);
         }
     }
-    // variant 6 stages 16-byte pieces of D rows (N * bytes % 16 == 0, a
-    // 16-byte aligned D) and stores 4 columns per lane (vec 4 allowed)
-    if (variant == 6 && (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0 &&
-        vmax >= 4) {
+    if (variant == 6) {
         if constexpr (E::bytes == 4) return launch_panel<E, 75>(h, seg, D, N, C, guard, s, xs);
         else return launch_panel<E, 150>(h, seg, D, N, C, guard, s, xs);
     }
-    if (variant == 6) variant = 2;
     if (variant == 5) {
         // widest 16-byte-per-lane vector the operands allow
         if constexpr (E::bytes == 4) {
-            if (vmax >= 4) return launch_grouprows<E, 4>(h, seg, D, N, C, guard, s);
-            if (vmax == 2) return launch_grouprows<E, 2>(h, seg, D, N, C, guard, s);
+            if (vec >= 4) return launch_grouprows<E, 4>(h, seg, D, N, C, guard, s);
+            if (vec == 2) return launch_grouprows<E, 2>(h, seg, D, N, C, guard, s);
             return launch_grouprows<E, 1>(h, seg, D, N, C, guard, s);
         } else {
-            if (vmax >= 8) return launch_grouprows<E, 8>(h, seg, D, N, C, guard, s);
-            if (vmax == 4) return launch_grouprows<E, 4>(h, seg, D, N, C, guard, s);
-            if (vmax == 2) return launch_grouprows<E, 2>(h, seg, D, N, C, guard, s);
+            if (vec >= 8) return launch_grouprows<E, 8>(h, seg, D, N, C, guard, s);
+            if (vec == 4) return launch_grouprows<E, 4>(h, seg, D, N, C, guard, s);
+            if (vec == 2) return launch_grouprows<E, 2>(h, seg, D, N, C, guard, s);
             return launch_grouprows<E, 1>(h, seg, D, N, C, guard, s);
         }
     }
     // Long rows through the long-row kernel (variant 4, chosen explicitly or
     // by the autotuner's table; measured slower than variant 2 on the Zipf
-    // workload so far, so never chosen automatically).  The long-row kernel
-    // stages 16-byte pieces of D rows.
-    const int minLen = cfg.chunk > 0 ? cfg.chunk : 256;
-    const bool splitOk =
-        (N * E::bytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0;
-    if (splitOk && variant == 4) {
+    // workload so far, so never chosen automatically).
+    if (variant == 4) {
+        const int minLen = cfg.chunk > 0 ? cfg.chunk : 256;
         if constexpr (E::bytes == 4) {
             if (vec >= 4) return launch_split<E, 4>(h, seg, D, N, C, guard, s, minLen, xs);
             if (vec == 2) return launch_split<E, 2>(h, seg, D, N, C, guard, s, minLen, xs);
@@ -1441,6 +1454,25 @@ int launch_spdm_f32_cfg(const gcoo_matrix *h, const float *D, int N, float *C,
         }
     }
     return launch_typed<F32>(h, h->seg, D, N, C, guard, s, xs);
+}
+
+int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const float *C,
+                int *variant, int *vec)
+{
+    if (dtype == 0) {
+        const unsigned long long dBytes = (unsigned long long)h->K * N * 4;
+        if (h->spdm_cfg.variant == 1 || dBytes >= 0x7fffffffull) {
+            const int vmax = max_vec(D, N, C, 4);
+            const int v = h->spdm_cfg.vec ? h->spdm_cfg.vec : 4;
+            *variant = 1;
+            *vec = v > vmax ? vmax : v;
+            return GCOO_OK;
+        }
+        resolve_variant<F32>(h, D, N, C, variant, vec);
+    } else {
+        resolve_variant<BF16>(h, D, N, C, variant, vec);
+    }
+    return GCOO_OK;
 }
 
 int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C, hipStream_t s)

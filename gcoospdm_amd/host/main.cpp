@@ -16,6 +16,9 @@
 //                      Matrix Market matrix is both the sparse S and the dense
 //                      operand (square matrices only)
 //   -dump=path         write C (row-major float32, no header) after the run
+//   -inputs=prefix     write the generated operands (prefix.D.bin, then
+//                      prefix.S.bin, row-major float32) and exit before any
+//                      device work
 // Output keeps the reference's lines ("nnz cpu:", "A[..] * B[..]",
 // "Algorithm:", "Time:" in microseconds) so its harness can parse it, plus a
 // GFLOPS line (2 * nnz * N / t).
@@ -60,7 +63,7 @@ void usage()
     std::printf("      -algo=gcoospdm|csr|cusparse|cublas\n");
     std::printf("      -kernelonly=0 (Benchmark kernel only)\n");
     std::printf("      -dist=ref|uniform -seed=N -iters=N -cpg=4 -check=0\n");
-    std::printf("      -file=matrix.mtx (S and the dense operand) -dump=C.bin\n");
+    std::printf("      -file=matrix.mtx (S and the dense operand) -dump=C.bin -inputs=prefix\n");
     std::printf("C = B * A with B sparse: nColsB must equal nRowsA\n");
 }
 
@@ -233,6 +236,14 @@ int main(int argc, char **argv)
     }
     std::printf("nnz cpu: %lld, density: %.4f\n", nnz,
                 B.empty() ? 0.0 : nnz * 1.0 / (double)B.size());
+    if (f.has("inputs")) {
+        const std::string prefix = f.get_str("inputs", "");
+        if (!dump_matrix(prefix + ".D.bin", A) || !dump_matrix(prefix + ".S.bin", B)) {
+            std::fprintf(stderr, "cannot write -inputs files\n");
+            return 1;
+        }
+        return 0;
+    }
     Result r;
     try {
         r = bench(A.data(), nRowsA, nColsA, B.data(), nRowsB, nColsB, C.data(), algo, kernelonly,

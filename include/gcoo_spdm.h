@@ -203,6 +203,15 @@ typedef struct gcoo_spdm_config {
 } gcoo_spdm_config;
 int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg);
 int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
+/* The variant (1..6) and columns per lane that a multiply of h with these
+ * operands (dtype 0: gcoo_spdm_f32*, 1: gcoo_spdm_bf16*) runs: the
+ * configuration, or the automatic choice from the last conversion's density
+ * and the shape, after the fall-backs of each variant's operand
+ * requirements.  No counterpart in the reference (one fixed kernel); it
+ * lets measurement tools name the kernel they time.  D and C are only
+ * inspected for alignment. */
+int gcoo_get_spdm_choice(gcoo_handle h, int dtype, const void *D, int N, const float *C,
+                         int *variant, int *vec);
 
 /* Convert S then multiply: ShGCOOSpDM::mutiplyBy (src/ShGCOOSpDM.cu:131-180),
  * without the reference's per-call allocations once capacity is reached. */

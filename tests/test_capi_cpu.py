@@ -105,3 +105,39 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
     assert r.stdout == ""
+
+
+@pytest.mark.skipif(not os.path.exists(MAIN), reason="main not built")
+@pytest.mark.parametrize("seed,sparsity", [(None, 0.7), (7, 0.9), (12345, 0.0)])
+def test_main_ref_inputs_match_reference_randomInit(tmp_path, seed, sparsity):
+    """`main -dist=ref` generates the reference's own inputs (src/main.cu:103-121:
+    the dense operand first with sparsity 0, then the sparse one, both by
+    randomInit, src/utils.cpp:5-19, from glibc's rand()).  Its operands are
+    compared bit for bit with the reference's randomInit compiled from its
+    sources into oracle/_ref, called in the same order from the same seed
+    (glibc's default seed is 1)."""
+    import numpy as np
+    import oracle
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref (the reference's utils.cpp) not built")
+    nRowsA, nColsA, nRowsB = 48, 20, 33
+    prefix = str(tmp_path / "in")
+    cmd = [MAIN, "-algo=gcoospdm", "-dist=ref", f"-nRowsA={nRowsA}", f"-nColsA={nColsA}",
+           f"-nRowsB={nRowsB}", f"-nColsB={nRowsA}", f"-sparsity={sparsity}",
+           f"-inputs={prefix}"]
+    if seed is not None:
+        cmd.append(f"-seed={seed}")
+    out = subprocess.run(cmd, capture_output=True, text=True, timeout=60, check=True).stdout
+    D = np.fromfile(prefix + ".D.bin", np.float32)
+    S = np.fromfile(prefix + ".S.bin", np.float32)
+    R = oracle.ref()
+    R.ref_srand(1 if seed is None else seed)
+    Dr = np.empty(nRowsA * nColsA, np.float32)
+    Sr = np.empty(nRowsB * nRowsA, np.float32)
+    assert R.ref_random_init(Dr, Dr.size, 0.0) == Dr.size
+    nnz = R.ref_random_init(Sr, Sr.size, sparsity)
+    assert np.array_equal(D.view(np.uint32), Dr.view(np.uint32))
+    assert np.array_equal(S.view(np.uint32), Sr.view(np.uint32))
+    assert f"nnz cpu: {nnz}," in out
+    if sparsity > 0:
+        assert 0 < nnz < Sr.size

@@ -325,3 +325,15 @@ def test_interpreter_catches_an_early_read():
            "sso": 75 * ROWB, "lim": 0, "kb": 0, "rowb": ROWB, "tkrowb": 75 * ROWB}
     with pytest.raises(AssertionError, match="in flight"):
         lane_cls(prog, ents, ops, {"vpbo": [0] * 64, "vpn": [12] + [0] * 63}).run()
+
+
+def test_committed_asm_header_is_the_generator_output():
+    """gcoo_panel_asm.h is exactly what gen_panel_asm.py writes."""
+    import subprocess
+    import sys
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "gcoospdm_amd", "csrc")
+    out = subprocess.run([sys.executable, os.path.join(here, "gen_panel_asm.py")],
+                         capture_output=True, text=True, check=True, timeout=120).stdout
+    with open(os.path.join(here, "gcoo_panel_asm.h")) as f:
+        assert out == f.read()
