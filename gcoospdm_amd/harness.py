@@ -7,7 +7,11 @@ tab-separated row per case with the reference's columns:
     N  sparsity  gcoospdm  cusparse  cublas            (synthetic)
     N  sparsity  filename  gcoospdm  cusparse  cublas  (files)
 
-A failed run is written as 10000000.0, as the reference does.  Usage:
+Synthetic rows end with two more columns: the reference's roofline model of
+the GCOO multiply restated for MI355X (perf_model.time_of_s, in ms) and the
+HBM bound of its algorithmic bytes (perf_model.hbm_bound_time), beside the
+measured time.  A failed run is written as 10000000.0, as the reference
+does.  Usage:
 
     python -m gcoospdm_amd.harness --ns 8000 --sparsities 0.99,0.995,0.999
     python -m gcoospdm_amd.harness --files <dir with .mtx files> --kernel-only
@@ -37,7 +41,8 @@ def run_main(args, timeout=1800):
 
 
 def synthetic(ns, sparsities, algos, kernel_only=False, iters=None, dist="ref", out=sys.stdout):
-    out.write("N\tsparsity\t" + "\t".join(algos) + "\n")
+    from . import perf_model
+    out.write("N\tsparsity\t" + "\t".join(algos) + "\tgcoospdm_model\thbm_bound\n")
     for n in ns:
         for s in sparsities:
             ms = []
@@ -49,7 +54,10 @@ def synthetic(ns, sparsities, algos, kernel_only=False, iters=None, dist="ref", 
                 if iters:
                     a.append(f"-iters={iters}")
                 ms.append(run_main(a))
-            out.write(f"{n}\t{s:.4f}\t" + "\t".join(f"{t:.6f}" for t in ms) + "\n")
+            model = perf_model.time_of_s(s, n)[0] * 1e3
+            bound = perf_model.hbm_bound_time(s, n) * 1e3
+            out.write(f"{n}\t{s:.4f}\t" + "\t".join(f"{t:.6f}" for t in ms)
+                      + f"\t{model:.6f}\t{bound:.6f}\n")
             out.flush()
 
 

@@ -141,10 +141,14 @@ def test_harness_tsv(dev, tmp_path):
                       iters=3, dist="uniform", out=buf)
     rows = [ln.split("\t") for ln in buf.getvalue().strip().splitlines()]
     print(buf.getvalue())
-    assert rows[0] == ["N", "sparsity", "gcoospdm", "csr", "cusparse", "cublas"]
+    assert rows[0] == ["N", "sparsity", "gcoospdm", "csr", "cusparse", "cublas",
+                       "gcoospdm_model", "hbm_bound"]
     assert len(rows) == 3
+    from gcoospdm_amd import perf_model
     for r in rows[1:]:
         assert r[0] == "256" and all(0.0 < float(t) < harness.FAILED for t in r[2:])
+        s = float(r[1])
+        assert abs(float(r[6]) - perf_model.time_of_s(s, 256)[0] * 1e3) < 1e-5
     n = 40
     rng = np.random.default_rng(9)
     S = H.rng_sparse(rng, n, n, 0.1)
