@@ -18,9 +18,10 @@
 //      writes the run length (one wave per row, 64 entries at a time);
 //   2. an exclusive scan of the counts in (chunk, row) order;
 //   3. every entry moves to its list slot as
-//      (((k - chunk start) * staged row bytes << 8) | row slot in its wave,
-//      value bits): the LDS byte offset of its D row within the staged chunk
-//      and the accumulator index, ready for the multiply's scalar unit.
+//      ((LDS byte offset of its D row within the staged chunk / 16) << 16 |
+//      row slot in its wave, value bits): the multiply forms the LDS address
+//      with one multiply-add of the high half (hi16 * 16 + lane address) and
+//      takes the accumulator index from the low byte.
 // The counts sit in LDS when nC * 16 ints fit in 64 KB, else in a global
 // scratch area after the offsets.
 #include "gcoo_internal.h"
@@ -119,7 +120,7 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
             const int s = run_start(act && c != cp, i0, lane, carry);
             if (act)
                 plist[cnt[c * RPW + rr] + (i - s)] =
-                    make_uint2(((unsigned)((kv.x - c * tk) * rowb) << 8) | (unsigned)rr,
+                    make_uint2(((unsigned)((kv.x - c * tk) * rowb) >> 4 << 16) | (unsigned)rr,
                                (unsigned)kv.y);
         }
     }
@@ -141,7 +142,7 @@ size_t panel_offset_elems(const gcoo_matrix *h, int tk)
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int *poff,
                       uint2 *plist, const ConvStatus *guard, hipStream_t s)
 {
-    if ((long long)tk * rowb >= (1ll << 24))
+    if ((long long)tk * rowb > (1ll << 20) || rowb % 16 != 0)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
     const int nP = (h->M + kPanelRows - 1) / kPanelRows;
     const int nC = (h->K + tk - 1) / tk;
@@ -167,7 +168,7 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
                        const ConvStatus *guard, hipStream_t s)
 {
     if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb) return GCOO_OK;
-    if ((long long)tk * rowb >= (1ll << 24))
+    if ((long long)tk * rowb > (1ll << 20) || rowb % 16 != 0)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
     const size_t elems = panel_offset_elems(h, tk);
     if (elems > h->poff_elems) {

@@ -190,23 +190,19 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf00_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "s_lshr_b32 s24, s36, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "s_lshr_b32 s24, s38, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "s_lshr_b32 s24, s40, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "s_lshr_b32 s24, s42, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -240,31 +236,23 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce0_%=\n\t" \
     ".Lf00_%=:\n\t" \
-    "s_lshr_b32 s24, s36, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s38, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s40, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s42, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
-    "s_lshr_b32 s24, s44, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s46, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s48, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s50, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -293,23 +281,19 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "s_lshr_b32 s24, s44, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "s_lshr_b32 s24, s46, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "s_lshr_b32 s24, s48, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "s_lshr_b32 s24, s50, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -390,17 +374,13 @@
     ".Lf10_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq10_%=\n\t" \
-    "s_lshr_b32 s24, s52, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s54, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s56, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s58, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
@@ -429,23 +409,19 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz10_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "s_lshr_b32 s24, s52, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "s_lshr_b32 s24, s54, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "s_lshr_b32 s24, s56, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "s_lshr_b32 s24, s58, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -526,17 +502,13 @@
     ".Lf20_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
-    "s_lshr_b32 s24, s60, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s62, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s64, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s66, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -565,23 +537,19 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "s_lshr_b32 s24, s60, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "s_lshr_b32 s24, s62, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "s_lshr_b32 s24, s64, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "s_lshr_b32 s24, s66, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -698,29 +666,21 @@
     ".Lpa0_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa0_%=:\n\t" \
-    "s_lshr_b32 s24, s36, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s38, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s40, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s42, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
-    "s_lshr_b32 s24, s44, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s46, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s48, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s50, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
@@ -784,29 +744,21 @@
     ".Lpb0_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb0_%=:\n\t" \
-    "s_lshr_b32 s24, s52, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s54, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s56, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s58, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
-    "s_lshr_b32 s24, s60, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s62, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s64, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s66, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
@@ -862,38 +814,31 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba0_%=\n\t" \
     ".Lta0_%=:\n\t" \
-    "s_lshr_b32 s24, s36, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "s_lshr_b32 s24, s38, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "s_lshr_b32 s24, s40, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "s_lshr_b32 s24, s42, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "s_lshr_b32 s24, s44, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "s_lshr_b32 s24, s46, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "s_lshr_b32 s24, s48, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -948,38 +893,31 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce0_%=\n\t" \
     ".Ltb0_%=:\n\t" \
-    "s_lshr_b32 s24, s52, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "s_lshr_b32 s24, s54, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "s_lshr_b32 s24, s56, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "s_lshr_b32 s24, s58, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "s_lshr_b32 s24, s60, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "s_lshr_b32 s24, s62, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "s_lshr_b32 s24, s64, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb0]\n\t" \
+    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -1202,23 +1140,19 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "s_lshr_b32 s24, s68, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "s_lshr_b32 s24, s70, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "s_lshr_b32 s24, s72, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "s_lshr_b32 s24, s74, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -1252,31 +1186,23 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
-    "s_lshr_b32 s24, s68, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s70, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s72, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s74, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
-    "s_lshr_b32 s24, s76, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s78, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s80, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s82, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -1305,23 +1231,19 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "s_lshr_b32 s24, s76, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "s_lshr_b32 s24, s78, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "s_lshr_b32 s24, s80, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "s_lshr_b32 s24, s82, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -1402,17 +1324,13 @@
     ".Lf11_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
-    "s_lshr_b32 s24, s84, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s86, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s88, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s90, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
@@ -1441,23 +1359,19 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "s_lshr_b32 s24, s84, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "s_lshr_b32 s24, s86, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "s_lshr_b32 s24, s88, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "s_lshr_b32 s24, s90, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -1538,17 +1452,13 @@
     ".Lf21_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
-    "s_lshr_b32 s24, s92, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s94, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s96, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s98, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -1577,23 +1487,19 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "s_lshr_b32 s24, s92, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "s_lshr_b32 s24, s94, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "s_lshr_b32 s24, s96, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "s_lshr_b32 s24, s98, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -1710,29 +1616,21 @@
     ".Lpa1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
-    "s_lshr_b32 s24, s68, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s70, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s72, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s74, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
-    "s_lshr_b32 s24, s76, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s78, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s80, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s82, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
@@ -1796,29 +1694,21 @@
     ".Lpb1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
-    "s_lshr_b32 s24, s84, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "s_lshr_b32 s24, s86, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "s_lshr_b32 s24, s88, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
-    "s_lshr_b32 s24, s90, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
-    "s_lshr_b32 s24, s92, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
-    "s_lshr_b32 s24, s94, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
-    "s_lshr_b32 s24, s96, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
-    "s_lshr_b32 s24, s98, 8\n\t" \
-    "v_add_u32_e32 v103, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[92:95], v103\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
@@ -1874,38 +1764,31 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
-    "s_lshr_b32 s24, s68, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "s_lshr_b32 s24, s70, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "s_lshr_b32 s24, s72, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "s_lshr_b32 s24, s74, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "s_lshr_b32 s24, s76, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "s_lshr_b32 s24, s78, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "s_lshr_b32 s24, s80, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
@@ -1960,38 +1843,31 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
-    "s_lshr_b32 s24, s84, 8\n\t" \
-    "v_add_u32_e32 v96, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "s_lshr_b32 s24, s86, 8\n\t" \
-    "v_add_u32_e32 v97, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "s_lshr_b32 s24, s88, 8\n\t" \
-    "v_add_u32_e32 v98, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[72:75], v98\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "s_lshr_b32 s24, s90, 8\n\t" \
-    "v_add_u32_e32 v99, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[76:79], v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "s_lshr_b32 s24, s92, 8\n\t" \
-    "v_add_u32_e32 v100, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[80:83], v100\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "s_lshr_b32 s24, s94, 8\n\t" \
-    "v_add_u32_e32 v101, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[84:87], v101\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "s_lshr_b32 s24, s96, 8\n\t" \
-    "v_add_u32_e32 v102, s24, %[vb1]\n\t" \
+    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[88:91], v102\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \

@@ -665,7 +665,7 @@ constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgro
 // in v[16v + q]) and the step runs in VGPR-index mode with source 2 and the
 // destination relative to M0[7:0].  The index instructions copy bits [7:0]
 // of their operand into M0[7:0] and ignore the rest, so callers pass the
-// packed list word ((LDS offset << 8) | q) as it is.  The index mode writes
+// packed list word ((LDS offset / 16) << 16 | q) as it is.  The index mode writes
 // M0, which the buffer-to-LDS loads also use; callers save M0 around a run of
 // steps (panel_m0_save / panel_m0_restore).
 __device__ __forceinline__ void panel_fma1(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
@@ -921,7 +921,7 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             const int li = c + 1 < nC ? c + 1 - cw : 0;
             const int pbn = __builtin_amdgcn_readlane(vpb, li);
             const int pen = __builtin_amdgcn_readlane(vpe, li);
-            // B entries: list entry x = (LDS byte offset << 8) | row slot
+            // B entries: list entry x = (LDS byte offset / 16) << 16 | row slot
             auto run = [&](auto bn, int p) {
                 constexpr int B = decltype(bn)::value;
                 uint2 m[B];
@@ -929,7 +929,7 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
                 for (int u = 0; u < B; ++u) m[u] = plist[p + u];
                 float d[B][VEC];
 #pragma unroll
-                for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + (m[u].x >> 8));
+                for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + ((m[u].x >> 16) << 4));
                 const int m0 = panel_m0_save();
                 int u = 0;
 #pragma unroll

@@ -35,10 +35,12 @@ Per chunk, in order:
     the wait for them, the scalar load of the next batch's 8 entries (run
     under this batch's steps), 8 steps, the wait for the next entries.  Two
     register sets alternate; the last 1..7 entries take one guarded round.
-Entry u is two words in SGPRs: x = (LDS byte offset of its D row << 8) | row
-slot, and the FP32 bits of its value.  A step is acc[slot] = fmaf(value, d,
-acc[slot]) for the 4 columns, in VGPR-index mode (s_set_gpr_idx_on/_idx take
-the slot from bits 7:0 of x).  The wave's issue priority is 3 for a chunk of
+Entry u is two words in SGPRs: x = (LDS byte offset of its D row / 16) << 16
+| row slot, and the FP32 bits of its value.  The LDS address of an entry is
+one vector multiply-add of x's high half (v_mad_u32_u16 with op_sel: hi16 *
+16 + the lane's buffer address), so no scalar instruction goes into it.  A
+step is acc[slot] = fmaf(value, d, acc[slot]) for the 4 columns, in VGPR-index
+mode (s_set_gpr_idx_on/_idx take the slot from bits 7:0 of x).  The wave's issue priority is 3 for a chunk of
 12 or more entries (2 below), 3 / 2 before each batch by the entries left,
 and 0 at the chunk's end.
 
@@ -83,8 +85,7 @@ class Body:
     # ---- batch loop (entries from 16 on) ----
     def reads(self, S, k):
         for u in range(k):
-            e(f"s_lshr_b32 s{ST}, s{S[2*u]}, 8")
-            e(f"v_add_u32_e32 v{AV+u}, s{ST}, {self.base}")
+            e(f"v_mad_u32_u16 v{AV+u}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
             e(f"ds_read_b128 v[{DV+4*u}:{DV+4*u+3}], v{AV+u}")
 
     @staticmethod
@@ -103,14 +104,12 @@ class Body:
 
     def tail(self, S, t):
         p = self.p
-        e(f"s_lshr_b32 s{ST}, s{S[0]}, 8")
-        e(f"v_add_u32_e32 v{AV}, s{ST}, {self.base}")
+        e(f"v_mad_u32_u16 v{AV}, s{S[0]}, 16, {self.base} op_sel:[1,0,0,0]")
         e(f"ds_read_b128 v[{DV}:{DV+3}], v{AV}")
         for u in range(1, 7):
             e(f"s_cmp_gt_u32 s{SREM}, {u}")
             e(f"s_cbranch_scc0 .Lgw{t}{p}_%=")
-            e(f"s_lshr_b32 s{ST}, s{S[2*u]}, 8")
-            e(f"v_add_u32_e32 v{AV+u}, s{ST}, {self.base}")
+            e(f"v_mad_u32_u16 v{AV+u}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
             e(f"ds_read_b128 v[{DV+4*u}:{DV+4*u+3}], v{AV+u}")
         e(f".Lgw{t}{p}_%=:")
         e("s_waitcnt lgkmcnt(0)")
@@ -163,8 +162,7 @@ class Body:
         b = g % 2
         for q in range(4):
             sx, _ = self.group_entry(g, q)
-            e(f"s_lshr_b32 s{ST}, s{sx}, 8")
-            e(f"v_add_u32_e32 v{G_AV[b]+q}, s{ST}, {self.base}")
+            e(f"v_mad_u32_u16 v{G_AV[b]+q}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
             e(f"ds_read_b128 v[{G_D[b]+4*q}:{G_D[b]+4*q+3}], v{G_AV[b]+q}")
 
     def group_entry_steps(self, g, q):
@@ -203,8 +201,7 @@ class Body:
             if q:
                 e(f"s_cmp_gt_u32 s{SCNT}, {q}")
                 e(f"s_cbranch_scc0 .L{tag}{self.p}_%=")
-            e(f"s_lshr_b32 s{ST}, s{sx}, 8")
-            e(f"v_add_u32_e32 v{G_AV[b]+q}, s{ST}, {self.base}")
+            e(f"v_mad_u32_u16 v{G_AV[b]+q}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
             e(f"ds_read_b128 v[{G_D[b]+4*q}:{G_D[b]+4*q+3}], v{G_AV[b]+q}")
         e(f".L{tag}{self.p}_%=:")
 

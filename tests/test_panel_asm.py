@@ -182,11 +182,16 @@ class Lane:
                 lane = self.sval(args[2])
                 assert 0 <= lane < 64
                 self.setS(args[0], self.vlanes[args[1][2:-1]][lane])
-            elif op == "v_add_u32_e32":
+            elif op == "v_mad_u32_u16":
+                # D = S0.u16 * S1.u16 + S2.u32; op_sel:[1,0,0,0] takes S0's
+                # high half
                 assert not self.idx_on
                 d = int(args[0][1:])
+                assert ln.endswith(" op_sel:[1,0,0,0]"), ln
+                src2 = args[3].split()[0]
                 self._check("v", [d], "write")
-                self.v[d] = ("addr", self.sval(args[1]) + self.sval(args[2]))
+                self.v[d] = ("addr", ((self.sval(args[1]) >> 16) & 0xFFFF)
+                             * (self.sval(args[2]) & 0xFFFF) + self.sval(src2))
             elif op == "ds_read_b128":
                 kind, regs = _regs(args[0])
                 akind, addr = self.vread(args[1])
@@ -216,6 +221,12 @@ class Lane:
 PROG = None
 
 
+def word(offset, slot):
+    """A list entry's first word: (LDS byte offset / 16) << 16 | row slot."""
+    assert offset % 16 == 0 and offset // 16 < 65536 and slot < 256
+    return (offset // 16) << 16 | slot
+
+
 def _prog():
     global PROG
     if PROG is None:
@@ -237,10 +248,10 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0):
         vpbo.append(len(entries) * 8)
         vpn.append(n)
         rows = sorted(rng.randrange(16) for _ in range(n))
-        ch = [((rng.randrange(75) * 1024) << 8 | slot, rng.getrandbits(32)) for slot in rows]
+        ch = [(word(rng.randrange(75) * 1024, slot), rng.getrandbits(32)) for slot in rows]
         chunks.append(ch)
         entries += ch
-    entries += [((rng.randrange(75) * 1024) << 8 | rng.randrange(16), rng.getrandbits(32))
+    entries += [(word(rng.randrange(75) * 1024, rng.randrange(16)), rng.getrandbits(32))
                 for _ in range(24)]
     wave = 2
     kb = K * ROWB if stager else 0
@@ -258,7 +269,7 @@ def _check_steps(lane, chunks, c0, ops):
         base = ops["vb0"] if (c0 + i) % 2 == 0 else ops["vb1"]
         expect = []
         for x, val in ch:
-            slot, off = x & 0xFF, x >> 8
+            slot, off = x & 0xFF, (x >> 16) * 16
             for c in range(4):
                 expect.append((16 * c + slot, val, ("d", base + off, c)))
         got = [f[1:] for f in lane.fmas if f[0] == i + 1]
@@ -320,7 +331,7 @@ def test_interpreter_catches_an_early_read():
     prog = [ln.replace("lgkmcnt(4)", "lgkmcnt(5)") for ln in _prog()]
     lane_cls = Lane
     rng = random.Random(3)
-    ents = [((rng.randrange(75) * 1024) << 8 | i % 16, i) for i in range(40)]
+    ents = [(word(rng.randrange(75) * 1024, i % 16), i) for i in range(40)]
     ops = {"pbase": 0, "sli": 0, "liend": 1, "par": 0, "vb0": 0, "vb1": CHUNKB, "sdst": 0,
            "sso": 75 * ROWB, "lim": 0, "kb": 0, "rowb": ROWB, "tkrowb": 75 * ROWB}
     with pytest.raises(AssertionError, match="in flight"):
