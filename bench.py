@@ -158,11 +158,19 @@ KERNEL_OF_VARIANT = {
 }
 
 
+PANEL_ROWS = {16: 16, 12: 28, 8: 42}
+
+
 def multiply_kernel(m, D, C):
     """The multiply kernel the library runs for these operands, as it
-    reports it (gcoo_get_spdm_choice)."""
-    variant, vec = m.spdm_choice(D, C)
-    return KERNEL_OF_VARIANT.get(variant, f"variant {variant}"), variant, vec
+    reports it (gcoo_get_spdm_choice): its name (with the panel kernel's
+    element type, chunk rows, waves and rows per wave, as rocprofv3 lists
+    it), variant and columns per lane."""
+    variant, vec, waves = m.spdm_choice(D, C)
+    name = KERNEL_OF_VARIANT.get(variant, f"variant {variant}")
+    if variant == 6:
+        name += f"<F32, 75, {waves}, {PANEL_ROWS[waves]}>"
+    return name, variant, vec
 
 
 def in_step_multiply_us(m, S, D, C, stream, reps=10):

@@ -116,6 +116,8 @@ struct gcoo_matrix {
     mutable const int2 *plist_src = nullptr;  // segments the lists hold (null: stale)
     mutable int plist_tk = 0;                 // chunk length they were built for
     mutable int plist_rowb = 0;               // bytes of a staged D row they encode
+    mutable int plist_rpw = 0;                // rows per wave they were built for
+    mutable int plist_waves = 0;              // waves per panel they were built for
 
     // single-pass conversion: per-group look-back flags (two (state, epoch,
     // value) words per group, never cleared: each conversion has its own
@@ -170,10 +172,10 @@ int convert_coo(gcoo_matrix *h, const int *rows, const int *cols,
 // guard: when non-null, the kernel exits if guard->total > h->cap.
 int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
                     const ConvStatus *guard, hipStream_t s);
-// The multiply variant (1..6) and columns per lane a launch of h on these
-// operands runs (dtype 0 FP32, 1 BF16).
+// The multiply variant (1..6), columns per lane and (variant 6) waves per
+// workgroup a launch of h on these operands runs (dtype 0 FP32, 1 BF16).
 int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const float *C,
-                int *variant, int *vec);
+                int *variant, int *vec, int *waves);
 // BF16 storage form: h->seg16 must hold the BF16 entry values.
 int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
                      hipStream_t s);
@@ -183,27 +185,27 @@ int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
 int ensure_seg16(gcoo_matrix *h, hipStream_t s);
 
 // ---- panel lists of the LDS-staged multiply (gcoo_panel.hip) ----
-// A panel is kPanelRows consecutive rows; wave w of the multiply owns panel
-// rows [w * kPanelRowsPerWave, (w + 1) * kPanelRowsPerWave).
-constexpr int kPanelRows = 256;
-constexpr int kPanelRowsPerWave = 16;
+// A panel is waves * rpw consecutive rows; wave w of the multiply owns panel
+// rows [w * rpw, (w + 1) * rpw).
 // Enqueue (on s) the panel lists of h's entries from seg (h->seg or
 // h->seg16) for k chunks of tk rows staged at rowb bytes per D row, unless
 // they are current.  Wave w of panel p has nC + 1 list offsets at
-// poff[(p * 16 + w) * (nC + 1)]: its chunk c entries are
-// plist[off[c] .. off[c + 1]), ordered by (row, k).  Kernels exit early when guard is non-null and
-// guard->total > h->cap.
-int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
-                       const ConvStatus *guard, hipStream_t s);
+// poff[(p * waves + w) * (nC + 1)]: its chunk c entries are
+// plist[off[c] .. off[c + 1]), ordered by (row, k).  Kernels exit early when
+// guard_skip(guard, h->cap) holds.
+int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
+                       int waves, const ConvStatus *guard, hipStream_t s);
 // Forget the panel lists (after a conversion or a capacity change).
 inline void invalidate_panel_lists(const gcoo_matrix *h) { h->plist_src = nullptr; }
 // Ints of the offset (and count) area the panel lists of h need for k
-// chunks of tk rows; the lists themselves need h->nnz uint2 entries.
-size_t panel_offset_elems(const gcoo_matrix *h, int tk);
+// chunks of tk rows and panels of waves x rpw rows; the lists themselves
+// need h->nnz uint2 entries.
+size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves);
 // Enqueue the build of the panel lists into caller-given buffers (poff of
 // panel_offset_elems ints, plist of h->nnz entries); touches no handle state.
-int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int *poff,
-                      uint2 *plist, const ConvStatus *guard, hipStream_t s);
+int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
+                      int waves, int *poff, uint2 *plist, const ConvStatus *guard,
+                      hipStream_t s);
 
 // Device scratch handed to the multiply by a caller that cannot allocate:
 // the panel lists and the long-row list are carved out of it instead of

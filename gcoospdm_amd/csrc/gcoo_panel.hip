@@ -1,17 +1,19 @@
 // gcoo_panel.hip -- panel lists for the LDS-staged multiply (variant 6).
 //
-// The LDS-staged multiply gives one workgroup a panel of kPanelRows rows and
-// walks K in chunks of tk rows of D, staged in LDS and shared by all panel
-// rows; wave w of it owns the panel's rows [16 w, 16 w + 16).  For every
+// The LDS-staged multiply gives one workgroup a panel of waves * rpw rows
+// and walks K in chunks of tk rows of D, staged in LDS and shared by all
+// panel rows; wave w of it owns the panel's rows [rpw w, rpw w + rpw).  For
+// every
 // (panel, wave, chunk) it needs that chunk's entries of the wave's rows, row
 // by row with k ascending inside a row, so that every row's fmaf chain stays
 // the reference's (src/gcoo_spdm_kernels.cu:133-142: one chain per output in
 // ascending k).
 //
-// A wave's 16 rows are consecutive, so their entries are one contiguous range
-// of the row segments, seg[rowptr[R0] .. rowptr[R0 + 16]).  The wave's list
-// is that same range reordered by (chunk, row, k); its chunk c list starts at
-// poff[(panel * 16 + wave) * (nC + 1) + c], and element nC holds its end.  No
+// A wave's rpw rows are consecutive, so their entries are one contiguous
+// range of the row segments, seg[rowptr[R0] .. rowptr[R0 + rpw]).  The wave's
+// list is that same range reordered by (chunk, row, k); its chunk c list
+// starts at poff[(panel * waves + wave) * (nC + 1) + c], and element nC holds
+// its end (panel * waves + wave times rpw is the wave's first row).  No
 // scan across waves or panels is needed.  One workgroup per (panel, wave):
 //   1. per (chunk, row) entry counts: a row's entries of one chunk are a
 //      contiguous run of its k-sorted segment, so the run's last entry
@@ -22,7 +24,7 @@
 //      row slot in its wave, value bits): the multiply forms the LDS address
 //      with one multiply-add of the high half (hi16 * 16 + lane address) and
 //      takes the accumulator index from the low byte.
-// The counts sit in LDS when nC * 16 ints fit in 64 KB, else in a global
+// The counts sit in LDS when nC * rpw ints fit in 64 KB, else in a global
 // scratch area after the offsets.
 #include "gcoo_internal.h"
 
@@ -47,13 +49,12 @@ __device__ __forceinline__ int run_start(bool first, int i0, int lane, int &carr
 template <bool LDS, int NT>
 __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int rowb,
-    int nC, int *__restrict__ poff, int *__restrict__ gcnt, uint2 *__restrict__ plist,
+    int nC, int RPW, int *__restrict__ poff, int *__restrict__ gcnt, uint2 *__restrict__ plist,
     const ConvStatus *__restrict__ guard, long long cap)
 {
     if (guard_skip(guard, cap)) return;
-    constexpr int RPW = kPanelRowsPerWave;
     extern __shared__ int lcnt[];
-    const int pw = blockIdx.x;  // panel * 16 + wave
+    const int pw = blockIdx.x;  // panel * waves + wave
     const int tid = threadIdx.x, lane = __lane_id();
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int L = nC * RPW;
@@ -128,49 +129,53 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
 
 }  // namespace
 
-size_t panel_offset_elems(const gcoo_matrix *h, int tk)
+size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves)
 {
-    const int nP = (h->M + kPanelRows - 1) / kPanelRows;
+    const int rows = rpw * waves;
+    const size_t nP = ((size_t)h->M + rows - 1) / rows;
     const int nC = (h->K + tk - 1) / tk;
-    const size_t nW = (size_t)nP * (kPanelRows / kPanelRowsPerWave);
-    const size_t L = (size_t)nC * kPanelRowsPerWave;
+    const size_t nW = nP * waves;
+    const size_t L = (size_t)nC * rpw;
     const bool lds = L <= (size_t)kListLdsInts;
     // offsets of every wave, then (counts beyond LDS) the count scratch
     return nW * ((size_t)nC + 1) + (lds ? 0 : nW * L);
 }
 
-int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int *poff,
-                      uint2 *plist, const ConvStatus *guard, hipStream_t s)
+int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
+                      int waves, int *poff, uint2 *plist, const ConvStatus *guard, hipStream_t s)
 {
     if ((long long)tk * rowb > (1ll << 20) || rowb % 16 != 0)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
-    const int nP = (h->M + kPanelRows - 1) / kPanelRows;
+    if (rpw < 1 || rpw > 255 || waves < 1)
+        return set_error(GCOO_ERR_UNSUPPORTED, "panel of %d waves x %d rows", waves, rpw);
+    const int rows = rpw * waves;
+    const size_t nP = ((size_t)h->M + rows - 1) / rows;
     const int nC = (h->K + tk - 1) / tk;
-    const size_t nW = (size_t)nP * (kPanelRows / kPanelRowsPerWave);
-    const size_t L = (size_t)nC * kPanelRowsPerWave;
+    const size_t nW = nP * waves;
+    const size_t L = (size_t)nC * rpw;
     const bool lds = L <= (size_t)kListLdsInts;
     if (nW > 0) {
         int *gcnt = poff + nW * ((size_t)nC + 1);
         if (lds)
             hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
-                               rowb, nC, poff, gcnt, plist, guard, (long long)h->cap);
+                               rowb, nC, rpw, poff, gcnt, plist, guard, (long long)h->cap);
         else
             hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
-                               dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC,
+                               dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC, rpw,
                                poff, gcnt, plist, guard, (long long)h->cap);
         GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
     }
     return GCOO_OK;
 }
 
-int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
-                       const ConvStatus *guard, hipStream_t s)
+int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
+                       int waves, const ConvStatus *guard, hipStream_t s)
 {
-    if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb) return GCOO_OK;
-    if ((long long)tk * rowb > (1ll << 20) || rowb % 16 != 0)
-        return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
-    const size_t elems = panel_offset_elems(h, tk);
+    if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb &&
+        h->plist_rpw == rpw && h->plist_waves == waves)
+        return GCOO_OK;
+    const size_t elems = panel_offset_elems(h, tk, rpw, waves);
     if (elems > h->poff_elems) {
         if (h->poff) GCOO_HIP_TRY(hipFree(h->poff));
         h->poff = nullptr;
@@ -189,11 +194,13 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb,
             return set_error(GCOO_ERR_ALLOC, "cannot allocate the panel lists");
         }
     }
-    int rc = build_panel_lists(h, seg, tk, rowb, h->poff, h->plist, guard, s);
+    int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, h->poff, h->plist, guard, s);
     if (rc) return rc;
     h->plist_src = seg;
     h->plist_tk = tk;
     h->plist_rowb = rowb;
+    h->plist_rpw = rpw;
+    h->plist_waves = waves;
     return GCOO_OK;
 }
 

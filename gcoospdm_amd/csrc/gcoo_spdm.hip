@@ -615,14 +615,16 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// Variant 6: panel-staged.  One workgroup of 16 waves owns a panel of
-// kPanelRows (256) rows and a 256-column tile of C (4 columns per lane);
-// wave w keeps the accumulators of panel rows [16w, 16w + 16) in VGPRs.  K is
+// Variant 6: panel-staged.  One workgroup of WAVES waves owns a panel of
+// WAVES * RPW rows and a 256-column tile of C (4 columns per lane); wave w
+// keeps the accumulators of panel rows [RPW w, RPW w + RPW) in VGPRs (FP32:
+// 16 waves of 16 rows, 12 of 28 or 8 of 42, so 256 or 336 rows share each
+// staged chunk; BF16: 16 of 16).  K is
 // walked in chunks of TK rows of D: a chunk's tile piece is copied to LDS once,
 // by direct buffer-to-LDS loads issued one chunk ahead into the other of two
 // buffers, and serves every entry of the panel in that chunk -- on average
 // 256 * density entries per staged D row instead of one L2 gather per entry.
-//   * Waves 0..4 stage the chunks (15 wave loads each); wave 15 stages
+//   * Waves 0..4 stage the chunks (15 wave loads each); the last wave stages
 //     nothing and, in one workgroup of each column tile per chunk
 //     (panel == chunk % panels), touches the tile piece of the chunk
 //     kPanelPrefetch chunks ahead so that it is in L2 when the stagers load
@@ -657,7 +659,7 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPanelPrefetch = 6;  // chunks between the L2 touch and the staging
-constexpr double kPanelMinDensity = 0.003;  // automatic choice of variant 6 (FP32) ...
+constexpr double kPanelMinDensity = 0.0025;  // automatic choice of variant 6 (FP32) ...
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
 // a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform row slot
@@ -730,22 +732,27 @@ __device__ __forceinline__ void panel_m0_restore(int m0)
     asm volatile("s_mov_b32 m0, %0" : : "s"(m0) : "memory");
 }
 
-template <typename E, int TK>
-__global__ __launch_bounds__(1024) void spdm_panel_kernel(
+typedef float f32x8 __attribute__((ext_vector_type(8)));
+
+template <typename E, int TK, int WAVES, int RPW>
+__global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
     const int *__restrict__ poff, const uint2 *__restrict__ plist, int M, int K, int nC,
     const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C, int nPanels,
     const ConvStatus *__restrict__ guard, long long cap)
 {
     constexpr int VEC = 4;
-    constexpr int RPW = kPanelRowsPerWave;
-    static_assert(RPW == 16 && kPanelRows == 256, "16 waves of 16 rows");
+    constexpr int PR = WAVES * RPW;           // panel rows
+    static_assert(E::bytes == 4 ? (RPW == 16 || RPW == 28 || RPW == 42) && RPW % 2 == 0
+                                : (RPW == 16 && WAVES == 16),
+                  "FP32: the generated asm blocks; BF16: 16 waves of 16 rows");
     constexpr int TN = 64 * VEC;              // columns per tile
     constexpr int LB = VEC * E::bytes;        // bytes of a D row per lane
     constexpr int ROWB = 64 * LB;             // bytes of a staged D row
     constexpr int CHUNKB = TK * ROWB;         // bytes of a staged chunk
     constexpr int NLOAD = CHUNKB / 1024;      // 1 KB wave loads per chunk
     constexpr int SW = 5;                     // staging waves
-    constexpr int TW = 15;                    // L2 touch wave
+    constexpr int TW = WAVES - 1;             // L2 touch wave
+    static_assert(TW >= SW, "the touch wave stages nothing");
     static_assert(CHUNKB % 1024 == 0 && NLOAD % SW == 0, "chunk loads must split over the stagers");
     constexpr int LPW = NLOAD / SW;           // wave loads per staging wave per chunk
     constexpr int PPR = ROWB / 16;            // 16-byte pieces per staged D row
@@ -801,7 +808,7 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
             }
         }
     };
-    // L2 touch of chunk c by wave 15: 4 bytes of every 128-byte line of the
+    // L2 touch of chunk c by the last wave: 4 bytes of every 128-byte line of the
     // tile piece (lane: row lane / LINES, line lane % LINES).
     const int frow = lane / LINES;
     const int fbyte = col0 * E::bytes + (lane % LINES) * 128;
@@ -819,9 +826,14 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
         }
     };
 
+    // FP32: 4 * RPW accumulators pinned to v0.. in f32x8 blocks (component c
+    // of row slot q in register RPW c + q); BF16: four f32x16 (RPW = 16)
+    f32x8 acc[E::bytes == 4 ? RPW / 2 : 1];
+#pragma unroll
+    for (int i = 0; i < (E::bytes == 4 ? RPW / 2 : 1); ++i) acc[i] = 0.0f;
     f32x16 a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
     // this wave's chunk list starts (nC + 1 of them, the last its end)
-    const int *op = poff + ((size_t)panel * (kPanelRows / RPW) + wave) * ((size_t)nC + 1);
+    const int *op = poff + ((size_t)panel * WAVES + wave) * ((size_t)nC + 1);
     const unsigned char *ldsLane = reinterpret_cast<const unsigned char *>(sD) + lane * LB;
 
     // This wave's list bounds of chunks [cw, cw + 64), lane i holding chunk
@@ -887,14 +899,18 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
                 unsigned sli = __builtin_amdgcn_readfirstlane((unsigned)(c - cw));
                 const unsigned liend = __builtin_amdgcn_readfirstlane((unsigned)(cend - cw));
                 const unsigned par = __builtin_amdgcn_readfirstlane((unsigned)c & 1u);
-                asm volatile(GCOO_PANEL_SEGMENT_ASM
-                             : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2),
-                               "+{v[48:63]}"(a3), [sli] "+s"(sli), [sso] "+s"(sso)
-                             : [pbase] "s"(pbase), [vpbo] "v"(vpbo), [vpn] "v"(vpn),
-                               [liend] "s"(liend), [par] "s"(par), [vb0] "v"(vb0), [vb1] "v"(vb1),
-                               [sdst] "s"(sdst), [lim] "s"(lim), [kb] "s"(kb), [rowb] "s"(urow),
-                               [tkrowb] "s"(tkrowb), [voff] "v"(pvoff), [rsrc] "s"(rs)
-                             : GCOO_PANEL_SEGMENT_CLOBBERS);
+#define GCOO_PANEL_SEGMENT(R)                                                              \
+    asm volatile(GCOO_PANEL_SEGMENT_ASM_R##R                                               \
+                 : GCOO_PANEL_ACC_OPERANDS_R##R(acc), [sli] "+s"(sli), [sso] "+s"(sso)     \
+                 : [pbase] "s"(pbase), [vpbo] "v"(vpbo), [vpn] "v"(vpn), [liend] "s"(liend), \
+                   [par] "s"(par), [vb0] "v"(vb0), [vb1] "v"(vb1), [sdst] "s"(sdst),      \
+                   [lim] "s"(lim), [kb] "s"(kb), [rowb] "s"(urow), [tkrowb] "s"(tkrowb),  \
+                   [voff] "v"(pvoff), [rsrc] "s"(rs)                                       \
+                 : GCOO_PANEL_SEGMENT_CLOBBERS_R##R)
+                if constexpr (RPW == 16) GCOO_PANEL_SEGMENT(16);
+                else if constexpr (RPW == 28) GCOO_PANEL_SEGMENT(28);
+                else GCOO_PANEL_SEGMENT(42);
+#undef GCOO_PANEL_SEGMENT
                 c = cend;
             }
             if (c >= nC) break;
@@ -966,8 +982,17 @@ __global__ __launch_bounds__(1024) void spdm_panel_kernel(
     if (j < N) {
 #pragma unroll
         for (int q = 0; q < RPW; ++q) {
-            const int r = panel * kPanelRows + wave * RPW + q;
-            const float out[VEC] = {a0[q], a1[q], a2[q], a3[q]};
+            const int r = panel * PR + wave * RPW + q;
+            float out[VEC];
+            if constexpr (E::bytes == 4) {
+#pragma unroll
+                for (int c = 0; c < VEC; ++c) out[c] = acc[(RPW * c + q) / 8][(RPW * c + q) % 8];
+            } else {
+                out[0] = a0[q];
+                out[1] = a1[q];
+                out[2] = a2[q];
+                out[3] = a3[q];
+            }
             if (r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
         }
     }
@@ -1121,18 +1146,19 @@ int launch_lds(const gcoo_matrix *h, const int2 *seg, const void *D, int N, floa
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-template <typename E, int TK>
+template <typename E, int TK, int WAVES, int RPW>
 int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
                  const ConvStatus *guard, hipStream_t s, const ExtScratch *xs)
 {
     constexpr int ROWB = 256 * E::bytes;  // bytes of a staged D row (256 columns)
+    constexpr int PR = WAVES * RPW;       // panel rows
     const int *poff = h->poff;
     const uint2 *plist = h->plist;
     if (xs) {
         // lists rebuilt into the caller's scratch: offsets, then the entries
         // (plus the slack the kernel's batched entry loads may read past a
         // list's end)
-        const size_t offB = align256(sizeof(int) * panel_offset_elems(h, TK));
+        const size_t offB = align256(sizeof(int) * panel_offset_elems(h, TK, RPW, WAVES));
         const size_t listB = sizeof(uint2) * ((size_t)h->nnz + kSegSlack);
         if (offB + listB > xs->bytes)
             return set_error(GCOO_ERR_BUDGET,
@@ -1141,22 +1167,23 @@ int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
                              (long long)h->nnz, offB + listB, xs->bytes);
         int *po = reinterpret_cast<int *>(xs->base);
         uint2 *pl = reinterpret_cast<uint2 *>(xs->base + offB);
-        int rc = build_panel_lists(h, seg, TK, ROWB, po, pl, guard, s);
+        int rc = build_panel_lists(h, seg, TK, ROWB, RPW, WAVES, po, pl, guard, s);
         if (rc) return rc;
         poff = po;
         plist = pl;
     } else {
-        int rc = ensure_panel_lists(h, seg, TK, ROWB, guard, s);
+        int rc = ensure_panel_lists(h, seg, TK, ROWB, RPW, WAVES, guard, s);
         if (rc) return rc;
         poff = h->poff;
         plist = h->plist;
     }
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
-    const int nPanels = (h->M + kPanelRows - 1) / kPanelRows;
+    const int nPanels = (h->M + PR - 1) / PR;
     const int nC = (h->K + TK - 1) / TK;
     const int tiles = (N + 255) / 256;
     const long long nblocks = (long long)nPanels * tiles;
-    hipLaunchKernelGGL((spdm_panel_kernel<E, TK>), dim3((unsigned)nblocks), dim3(1024), 0, s,
+    hipLaunchKernelGGL((spdm_panel_kernel<E, TK, WAVES, RPW>), dim3((unsigned)nblocks),
+                       dim3(WAVES * 64), 0, s,
                        poff, plist, h->M, h->K, nC, D, (unsigned)dBytes, N, C, nPanels,
                        guard, (long long)h->cap);
     GCOO_CHECK_LAUNCH("spdm_panel_kernel");
@@ -1303,6 +1330,24 @@ int max_vec(const void *D, int N, const float *C, int elemBytes)
     return 1;
 }
 
+// Waves of the panel kernel's workgroup: the configured `waves` (FP32: 16
+// waves of 16 rows, 12 of 28 or 8 of 42; BF16: 16 of 16), or automatically
+// 16 x 16 from kPanelTallDensity on and 12 x 28 below it.  The taller panel
+// shares each staged chunk among 336 rows instead of 256, a quarter less
+// staging, but has fewer waves to hide the entry loop's latencies (n = 8000,
+// FP32: 457 against 467 us at 1 % density, 425 against 407 us at 0.8 %, 392
+// against 339 us at 0.5 %, 370 against 311 us at 0.3 %).
+constexpr double kPanelTallDensity = 0.009;
+template <typename E>
+int panel_waves(const gcoo_matrix *h, int cfgWaves)
+{
+    if constexpr (E::bytes != 4) return 16;
+    if (cfgWaves == 16 || cfgWaves == 12 || cfgWaves == 8) return cfgWaves;
+    const double cells = (double)h->M * (double)h->K;
+    return cells > 0 && (double)h->nnz >= kPanelTallDensity * cells ? 16 : 12;
+}
+inline int panel_rows_of(int waves) { return waves == 12 ? 12 * 28 : waves == 8 ? 8 * 42 : 256; }
+
 // The variant (2..6) and columns per lane that a launch of h on these
 // operands runs: the handle's configuration, or the automatic choice, after
 // the fall-backs each variant's operand requirements impose.
@@ -1317,18 +1362,18 @@ void resolve_variant(const gcoo_matrix *h, const void *D, int N, const float *C,
     if (vec > vmax) vec = vmax;
     int variant = cfg.variant;
     if (variant == 0) {
-        // Panel staging costs a fixed K x N read of D per 256-row panel, row
+        // Panel staging costs a fixed K x N read of D per panel of rows, row
         // segments cost one D-row gather per entry: the panel kernel wins from
-        // about 0.3 % density on (n = 8000, FP32: both take 370 us at
-        // s = 0.997, 389 against 486 us at 0.996, 372 against 274 us at
-        // 0.998).  BF16 gathers move half the bytes and stay ahead at every
-        // density.  The density is the handle's last completed conversion (a
+        // about 0.25 % density on (n = 8000, FP32, 336-row panels: 311
+        // against 351 us at s = 0.997, 316 against 248 us at 0.998).  BF16
+        // gathers move half the bytes and stay ahead at every density.  The density is the handle's last completed conversion (a
         // first optimistic launch does not know the new count yet); every
         // variant gives the same bits.  It also needs a workgroup per CU at
         // least (256 x 256 tiles of C).
         const double cells = (double)h->M * (double)h->K;
         const bool dense = cells > 0 && (double)h->nnz >= kPanelMinDensity * cells;
-        const long long wgs = (long long)((h->M + kPanelRows - 1) / kPanelRows) * ((N + 255) / 256);
+        const int pr = panel_rows_of(panel_waves<E>(h, cfg.waves));
+        const long long wgs = (long long)((h->M + pr - 1) / pr) * ((N + 255) / 256);
         variant = (E::bytes == 4 && dense && wgs >= kPanelMinWorkgroups) ? 6 : 2;
     }
     if (variant == 1) variant = 2;
@@ -1373,8 +1418,15 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         }
     }
     if (variant == 6) {
-        if constexpr (E::bytes == 4) return launch_panel<E, 75>(h, seg, D, N, C, guard, s, xs);
-        else return launch_panel<E, 150>(h, seg, D, N, C, guard, s, xs);
+        if constexpr (E::bytes == 4) {
+            switch (panel_waves<E>(h, cfg.waves)) {
+            case 12: return launch_panel<E, 75, 12, 28>(h, seg, D, N, C, guard, s, xs);
+            case 8: return launch_panel<E, 75, 8, 42>(h, seg, D, N, C, guard, s, xs);
+            default: return launch_panel<E, 75, 16, 16>(h, seg, D, N, C, guard, s, xs);
+            }
+        } else {
+            return launch_panel<E, 150, 16, 16>(h, seg, D, N, C, guard, s, xs);
+        }
     }
     if (variant == 5) {
         // widest 16-byte-per-lane vector the operands allow
@@ -1457,8 +1509,9 @@ int launch_spdm_f32_cfg(const gcoo_matrix *h, const float *D, int N, float *C,
 }
 
 int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const float *C,
-                int *variant, int *vec)
+                int *variant, int *vec, int *waves)
 {
+    *waves = 0;
     if (dtype == 0) {
         const unsigned long long dBytes = (unsigned long long)h->K * N * 4;
         if (h->spdm_cfg.variant == 1 || dBytes >= 0x7fffffffull) {
@@ -1469,8 +1522,10 @@ int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const flo
             return GCOO_OK;
         }
         resolve_variant<F32>(h, D, N, C, variant, vec);
+        if (*variant == 6) *waves = panel_waves<F32>(h, h->spdm_cfg.waves);
     } else {
         resolve_variant<BF16>(h, D, N, C, variant, vec);
+        if (*variant == 6) *waves = panel_waves<BF16>(h, h->spdm_cfg.waves);
     }
     return GCOO_OK;
 }

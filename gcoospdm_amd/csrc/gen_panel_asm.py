@@ -1,7 +1,10 @@
 #!/usr/bin/env python3
 """Writes gcoo_panel_asm.h: the chunk loop of the FP32 panel multiply
 (spdm_panel_kernel, gcoo_spdm.hip) for one wave, as one inline-asm body that
-runs a segment of consecutive chunks.
+runs a segment of consecutive chunks, for each supported number R of rows
+per wave (ROWS below): macros GCOO_PANEL_SEGMENT_ASM_R<R>, its clobbers
+GCOO_PANEL_SEGMENT_CLOBBERS_R<R>, and GCOO_PANEL_ACC_OPERANDS_R<R>(a), the
+operands pinning the accumulators (an array of 4 R / 8 f32x8) to v0..v(4R-1).
 
     python3 gen_panel_asm.py > gcoo_panel_asm.h
 
@@ -51,9 +54,9 @@ lane's LDS byte address in buffer 0 / 1; %[sdst] SGPR, the wave's staging
 destination in buffer 0; %[sso] (in/out: D byte offset of the wave's first
 staging row of the next chunk, advanced by %[tkrowb] per chunk), %[lim],
 %[kb], %[rowb], %[tkrowb] SGPRs; %[voff] VGPR; %[rsrc] the buffer resource
-of D; the accumulators v0..v63 (component c of row slot q in v[16c + q]).
-Scratch: v64..v103, s24..s30 and s36..s99 (s32..s34 are the stack, frame
-and base pointers).  M0 is saved and restored.  The scalar loads
+of D; the accumulators v0..v(4R-1) (component c of row slot q in
+v[R c + q]).  Scratch: v(4R)..v(4R+39), s24..s30 and s36..s99 (s32..s34 are
+the stack, frame and base pointers).  M0 is saved and restored.  The scalar loads
 read up to 7 entries past the end of a chunk's list, which the list capacity
 covers; no LDS read is issued for an entry past the end.  Every wave runs
 the body once per chunk (one barrier each).  When the block exits, the only
@@ -63,11 +66,24 @@ vmcnt(0) covers.
 ST, SREM, SOFF, SCNT, SM0, PBN, SNN = 24, 25, 26, 27, 28, 29, 30
 BANKS = ((list(range(36, 52)), list(range(52, 68))),
          (list(range(68, 84)), list(range(84, 100))))
-DV, AV = 64, 96
+CHUNKB = 75 * 1024
+ROWS = (16, 28, 42)       # rows per wave of the generated blocks
+R = 16                    # rows per wave of the block being generated
+DV, AV = 64, 96           # D values of the batch loop, their addresses
 G_D = (64, 80)            # D buffers of the rolling phase
 G_AV = (96, 100)          # their address VGPRs
-CHUNKB = 75 * 1024
 L = []
+
+
+def configure(rows):
+    """Register layout for `rows` rows per wave: accumulators v0..v(4 rows - 1),
+    scratch from v(4 rows) on."""
+    global R, DV, AV, G_D, G_AV
+    R = rows
+    base = 4 * rows
+    DV, AV = base, base + 32
+    G_D = (base, base + 16)
+    G_AV = (base + 32, base + 36)
 
 
 def e(x):
@@ -91,7 +107,7 @@ class Body:
     @staticmethod
     def entry(S, u):
         for c in range(4):
-            e(f"v_fma_f32 v{16*c}, s{S[2*u+1]}, v{DV+4*u+c}, v{16*c}")
+            e(f"v_fma_f32 v{R*c}, s{S[2*u+1]}, v{DV+4*u+c}, v{R*c}")
 
     def full_steps(self, S):
         for h in range(2):
@@ -169,7 +185,7 @@ class Body:
         _, sv = self.group_entry(g, q)
         b = g % 2
         for c in range(4):
-            e(f"v_fma_f32 v{16*c}, s{sv}, v{G_D[b]+4*q+c}, v{16*c}")
+            e(f"v_fma_f32 v{R*c}, s{sv}, v{G_D[b]+4*q+c}, v{R*c}")
 
     def group_steps_full(self, g):
         sx, _ = self.group_entry(g, 0)
@@ -336,36 +352,57 @@ class Body:
         e("s_cbranch_scc0 .Lx_%=")
 
 
-body0, body1 = Body(0), Body(1)
-# prologue: the segment's first chunk, its entries loaded before its barrier
-e(f"s_mov_b32 s{SM0}, m0")
-e("s_nop 4")  # %[sli] may have just been written by a vector instruction
-e(f"v_readlane_b32 s{PBN}, %[vpbo], %[sli]")
-e(f"v_readlane_b32 s{SNN}, %[vpn], %[sli]")
-e("s_cmp_eq_u32 %[par], 0")
-e("s_cbranch_scc0 .Lo1_%=")
-Body.load_first16(*BANKS[0], ".Lc0_%=")
-e("s_branch .Lc0_%=")
-e(".Lo1_%=:")
-Body.load_first16(*BANKS[1], ".Lc1_%=")
-e("s_branch .Lc1_%=")
-body0.emit()    # falls through to the odd body
-body1.emit()
-e("s_branch .Lc0_%=")
-e(".Lx_%=:")
-e(f"s_mov_b32 m0, s{SM0}")
+def generate(rows):
+    """The asm lines of the segment block for `rows` rows per wave."""
+    configure(rows)
+    L.clear()
+    body0, body1 = Body(0), Body(1)
+    # prologue: the segment's first chunk, its entries loaded before its barrier
+    e(f"s_mov_b32 s{SM0}, m0")
+    e("s_nop 4")  # %[sli] may have just been written by a vector instruction
+    e(f"v_readlane_b32 s{PBN}, %[vpbo], %[sli]")
+    e(f"v_readlane_b32 s{SNN}, %[vpn], %[sli]")
+    e("s_cmp_eq_u32 %[par], 0")
+    e("s_cbranch_scc0 .Lo1_%=")
+    Body.load_first16(*BANKS[0], ".Lc0_%=")
+    e("s_branch .Lc0_%=")
+    e(".Lo1_%=:")
+    Body.load_first16(*BANKS[1], ".Lc1_%=")
+    e("s_branch .Lc1_%=")
+    body0.emit()    # falls through to the odd body
+    body1.emit()
+    e("s_branch .Lc0_%=")
+    e(".Lx_%=:")
+    e(f"s_mov_b32 m0, s{SM0}")
+    return list(L)
 
-print("// Generated by gen_panel_asm.py (same directory); see its docstring.")
-print("// A segment of chunks of the FP32 panel multiply for one wave: per chunk the")
-print("// barrier, the next chunk's entry loads and staging, and the entry loop.")
-print("#pragma once")
-print("#define GCOO_PANEL_SEGMENT_ASM \\")
-for i, x in enumerate(L):
-    end = " \\" if i + 1 < len(L) else ""
-    print(f'    "{x}\\n\\t"{end}')
-print("#define GCOO_PANEL_SEGMENT_CLOBBERS \\")
-regs = ([f'"v{i}"' for i in range(64, 104)] + [f'"s{i}"' for i in range(24, 31)] +
-        [f'"s{i}"' for i in range(36, 100)] + ['"scc"', '"memory"'])
-for i in range(0, len(regs), 12):
-    end = ", \\" if i + 12 < len(regs) else ""
-    print("    " + ", ".join(regs[i:i+12]) + end)
+
+def main():
+    print("// Generated by gen_panel_asm.py (same directory); see its docstring.")
+    print("// A segment of chunks of the FP32 panel multiply for one wave: per chunk the")
+    print("// barrier, the next chunk's entry loads and staging, and the entry loop.")
+    print("#pragma once")
+    for rows in ROWS:
+        lines = generate(rows)
+        print(f"// {rows} rows per wave: accumulators v0..v{4 * rows - 1}, component c of row slot q")
+        print(f"// in v[{rows} c + q]; scratch v{4 * rows}..v{4 * rows + 39}")
+        print(f"#define GCOO_PANEL_SEGMENT_ASM_R{rows} \\")
+        for i, x in enumerate(lines):
+            end = " \\" if i + 1 < len(lines) else ""
+            print(f'    "{x}\\n\\t"{end}')
+        print(f"#define GCOO_PANEL_SEGMENT_CLOBBERS_R{rows} \\")
+        regs = ([f'"v{i}"' for i in range(4 * rows, 4 * rows + 40)] +
+                [f'"s{i}"' for i in range(24, 31)] +
+                [f'"s{i}"' for i in range(36, 100)] + ['"scc"', '"memory"'])
+        for i in range(0, len(regs), 12):
+            end = ", \\" if i + 12 < len(regs) else ""
+            print("    " + ", ".join(regs[i:i+12]) + end)
+        ops = [f'"+{{v[{8 * i}:{8 * i + 7}]}}"(a[{i}])' for i in range(rows // 2)]
+        print(f"#define GCOO_PANEL_ACC_OPERANDS_R{rows}(a) \\")
+        for i in range(0, len(ops), 4):
+            end = ", \\" if i + 4 < len(ops) else ""
+            print("    " + ", ".join(ops[i:i+4]) + end)
+
+
+if __name__ == "__main__":
+    main()

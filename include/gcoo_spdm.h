@@ -187,7 +187,9 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              16-byte aligned D and C; otherwise variant 2 runs).
  *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
  *              D per lane; a wave covers 64*vec columns.
- *   waves      waves per workgroup (1..16).
+ *   waves      waves per workgroup (1..16) of variant 3; for variant 6
+ *              (FP32) 16, 12 or 8 waves of 16, 28 or 42 rows (0: 16 from
+ *              0.9 % density on, 12 below).
  *   chunk      k-chunk length of variant 3; long-row threshold of variant 4;
  *              rows per wave of variant 2 (1: one row, another nonzero value:
  *              the group's cpg rows, 0: one row once rows average 12 entries).
@@ -203,15 +205,16 @@ typedef struct gcoo_spdm_config {
 } gcoo_spdm_config;
 int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg);
 int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
-/* The variant (1..6) and columns per lane that a multiply of h with these
- * operands (dtype 0: gcoo_spdm_f32*, 1: gcoo_spdm_bf16*) runs: the
+/* The variant (1..6), columns per lane and, for variant 6, waves per
+ * workgroup (16: 16 rows each, 12: 28, 8: 42; else 0) that a multiply of h
+ * with these operands (dtype 0: gcoo_spdm_f32*, 1: gcoo_spdm_bf16*) runs: the
  * configuration, or the automatic choice from the last conversion's density
  * and the shape, after the fall-backs of each variant's operand
  * requirements.  No counterpart in the reference (one fixed kernel); it
  * lets measurement tools name the kernel they time.  D and C are only
  * inspected for alignment. */
 int gcoo_get_spdm_choice(gcoo_handle h, int dtype, const void *D, int N, const float *C,
-                         int *variant, int *vec);
+                         int *variant, int *vec, int *waves);
 
 /* Convert S then multiply: ShGCOOSpDM::mutiplyBy (src/ShGCOOSpDM.cu:131-180),
  * without the reference's per-call allocations once capacity is reached. */

@@ -130,21 +130,21 @@ def test_config1_800_s09_vs_fp64(dev):
     assert err <= FP32_REL_TOL
 
 
-@pytest.mark.parametrize("variant", [0, 6])
+@pytest.mark.parametrize("variant,waves", [(0, 0), (6, 16), (6, 12), (6, 8)])
 @pytest.mark.parametrize("sparsity", [0.99, 0.995, 0.999])
-def test_config2_n8000_full_size(dev, sparsity, variant):
+def test_config2_n8000_full_size(dev, sparsity, variant, waves):
     """n = 8000: conversion bit-exact at full size, and all of C bit-exact
     against the oracle's fmaf chain plus FP64 rel-Frobenius <= 1e-5 over all
     of C (the check of src/main.cu:128-142 at this size)."""
     import torch
     n = 8000
-    print(f"config2 n={n} s={sparsity} variant={variant}", flush=True)
+    print(f"config2 n={n} s={sparsity} variant={variant} waves={waves}", flush=True)
     Sd = torch.empty((n, n), device=dev)
     Dd = torch.empty((n, n), device=dev)
     G.fill_sparse(Sd, 7, 1, sparsity)
     G.fill_uniform(Dd, 7, 0)
     C = torch.empty((n, n), device=dev)
-    m = G.GCOOMatrix(n, n, 4).set_config(variant=variant)
+    m = G.GCOOMatrix(n, n, 4).set_config(variant=variant, waves=waves)
     m.multiply_by(Sd, Dd, C)
     S = Sd.cpu().numpy()
     D = Dd.cpu().numpy()
@@ -253,14 +253,16 @@ def test_row_segment_rounds_bit_exact(dev, rows_per_wave, unroll, M, K, N, dens)
     (300, 80000, 256, 0.001),  # 1067 chunks: list counts beyond LDS
     (64, 2000, 1030, 0.05),    # N * 4 not a multiple of 16: row-segment fallback
 ])
-def test_panel_variant_shapes(dev, M, K, N, dens):
-    """The panel-staged multiply (variant 6) across panel, chunk and tile
-    edges gives the reference's bits, and follows re-conversions of the
-    handle (its panel lists are rebuilt)."""
+@pytest.mark.parametrize("waves", [16, 12, 8])
+def test_panel_variant_shapes(dev, M, K, N, dens, waves):
+    """The panel-staged multiply (variant 6; workgroups of 16 waves x 16
+    rows, 12 x 28 or 8 x 42) across panel, chunk and tile edges gives the
+    reference's bits, and follows re-conversions of the handle (its panel
+    lists are rebuilt)."""
     import torch
-    print(f"panel M={M} K={K} N={N} dens={dens}", flush=True)
+    print(f"panel waves={waves} M={M} K={K} N={N} dens={dens}", flush=True)
     rng = np.random.default_rng(M * 7 + K + N)
-    m = G.GCOOMatrix(M, K, 4).set_config(variant=6)
+    m = G.GCOOMatrix(M, K, 4).set_config(variant=6, waves=waves)
     for d in (dens, dens / 2):
         S = H.rng_sparse(rng, M, K, d)
         S[M // 2, : min(K, 700)] = rng.uniform(-1, 1, min(K, 700))  # a long row

@@ -31,10 +31,13 @@ CHUNKB = 75 * 1024
 ROWB = 32000
 
 
-def _instructions():
+ROWS = (16, 28, 42)
+
+
+def _instructions(rows=16):
     text = open(HDR).read()
-    body = text.split("#define GCOO_PANEL_SEGMENT_ASM", 1)[1].split(
-        "#define GCOO_PANEL_SEGMENT_CLOBBERS", 1)[0]
+    body = text.split(f"#define GCOO_PANEL_SEGMENT_ASM_R{rows} ", 1)[1].split(
+        f"#define GCOO_PANEL_SEGMENT_CLOBBERS_R{rows} ", 1)[0]
     lines = re.findall(r'"([^"]*)\\n\\t"', body)
     return [ln.strip() for ln in lines if ln.strip()]
 
@@ -218,7 +221,7 @@ class Lane:
         return self
 
 
-PROG = None
+PROG = {}
 
 
 def word(offset, slot):
@@ -227,14 +230,13 @@ def word(offset, slot):
     return (offset // 16) << 16 | slot
 
 
-def _prog():
-    global PROG
-    if PROG is None:
-        PROG = _instructions()
-    return PROG
+def _prog(rows=16):
+    if rows not in PROG:
+        PROG[rows] = _instructions(rows)
+    return PROG[rows]
 
 
-def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0):
+def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16):
     """Run one segment: chunks c0 .. c0 + len(counts) - 1 of the window at
     cw, with counts[i] entries in chunk c0 + i (the window's other chunks get
     random counts).  Returns the lane, the per-chunk entries and the inputs."""
@@ -247,11 +249,11 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0):
     for n in per:
         vpbo.append(len(entries) * 8)
         vpn.append(n)
-        rows = sorted(rng.randrange(16) for _ in range(n))
-        ch = [(word(rng.randrange(75) * 1024, slot), rng.getrandbits(32)) for slot in rows]
+        slots = sorted(rng.randrange(rows) for _ in range(n))
+        ch = [(word(rng.randrange(75) * 1024, slot), rng.getrandbits(32)) for slot in slots]
         chunks.append(ch)
         entries += ch
-    entries += [(word(rng.randrange(75) * 1024, rng.randrange(16)), rng.getrandbits(32))
+    entries += [(word(rng.randrange(75) * 1024, rng.randrange(rows)), rng.getrandbits(32))
                 for _ in range(24)]
     wave = 2
     kb = K * ROWB if stager else 0
@@ -260,18 +262,18 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0):
     ops = {"pbase": 0, "sli": c0 - cw, "liend": c0 - cw + len(counts), "par": c0 & 1,
            "vb0": 0x40, "vb1": 0x40 + CHUNKB, "sdst": wave * 15 * 1024, "sso": sso,
            "lim": lim, "kb": kb, "rowb": ROWB, "tkrowb": 75 * ROWB}
-    lane = Lane(_prog(), entries, ops, {"vpbo": vpbo, "vpn": vpn}).run()
+    lane = Lane(_prog(rows), entries, ops, {"vpbo": vpbo, "vpn": vpn}).run()
     return lane, chunks[c0 - cw:c0 - cw + len(counts)], ops
 
 
-def _check_steps(lane, chunks, c0, ops):
+def _check_steps(lane, chunks, c0, ops, rows=16):
     for i, ch in enumerate(chunks):
         base = ops["vb0"] if (c0 + i) % 2 == 0 else ops["vb1"]
         expect = []
         for x, val in ch:
             slot, off = x & 0xFF, (x >> 16) * 16
             for c in range(4):
-                expect.append((16 * c + slot, val, ("d", base + off, c)))
+                expect.append((rows * c + slot, val, ("d", base + off, c)))
         got = [f[1:] for f in lane.fmas if f[0] == i + 1]
         assert len(got) == len(expect), f"chunk {c0 + i}: {len(got) // 4} entries"
         # the 4 steps of one entry may be in any component order, but entries
@@ -280,33 +282,36 @@ def _check_steps(lane, chunks, c0, ops):
             assert sorted(got[j:j + 4]) == sorted(expect[j:j + 4]), f"chunk {c0 + i} entry {j // 4}"
 
 
+@pytest.mark.parametrize("rows", ROWS)
 @pytest.mark.parametrize("n", list(range(0, 49)))
 @pytest.mark.parametrize("c0", [0, 1])
-def test_one_chunk_steps_follow_the_list(n, c0):
-    lane, chunks, ops = _segment([n], c0=c0, cw=0, seed=n)
+def test_one_chunk_steps_follow_the_list(n, c0, rows):
+    lane, chunks, ops = _segment([n], c0=c0, cw=0, seed=n, rows=rows)
     assert lane.barriers == 1
-    _check_steps(lane, chunks, c0, ops)
+    _check_steps(lane, chunks, c0, ops, rows)
 
 
+@pytest.mark.parametrize("rows", ROWS)
 @pytest.mark.parametrize("seed", range(12))
-def test_segment_steps_follow_the_list(seed):
+def test_segment_steps_follow_the_list(seed, rows):
     rng = random.Random(100 + seed)
     cw = rng.choice([0, 64, 128])
     c0 = cw + rng.randrange(8)
     counts = [rng.choice([0, 1, 3, 4, 5, 8, 9, 12, 15, 16, 17, 23, 24, 31, 40])
               for _ in range(rng.randrange(1, 14))]
-    lane, chunks, ops = _segment(counts, c0=c0, cw=cw, seed=seed)
+    lane, chunks, ops = _segment(counts, c0=c0, cw=cw, seed=seed, rows=rows)
     assert lane.barriers == len(counts)
-    _check_steps(lane, chunks, c0, ops)
+    _check_steps(lane, chunks, c0, ops, rows)
     assert lane.ops["sli"] == c0 - cw + len(counts)
     assert lane.m0 == 0x1234  # restored
 
 
+@pytest.mark.parametrize("rows", ROWS)
 @pytest.mark.parametrize("K", [8000, 300, 161, 75 * 7 + 33])
-def test_staging_loads(K):
+def test_staging_loads(K, rows):
     counts = [5, 0, 17, 2, 9, 30]
     c0 = 0
-    lane, _, ops = _segment(counts, c0=c0, K=K, seed=K)
+    lane, _, ops = _segment(counts, c0=c0, K=K, seed=K, rows=rows)
     wave = 2
     want = []
     for i in range(len(counts)):
@@ -320,10 +325,11 @@ def test_staging_loads(K):
     assert lane.ops["sso"] == ops["sso"] + len(counts) * 75 * ROWB
 
 
-def test_non_staging_wave_loads_nothing():
-    lane, chunks, ops = _segment([3, 20, 0, 8], c0=5, cw=0, stager=False, seed=7)
+@pytest.mark.parametrize("rows", ROWS)
+def test_non_staging_wave_loads_nothing(rows):
+    lane, chunks, ops = _segment([3, 20, 0, 8], c0=5, cw=0, stager=False, seed=7, rows=rows)
     assert lane.stage == []
-    _check_steps(lane, chunks, 5, ops)
+    _check_steps(lane, chunks, 5, ops, rows)
 
 
 def test_interpreter_catches_an_early_read():
