@@ -299,13 +299,17 @@ def small_case(G, dev, seed):
 
 def zipf_case(G, dev):
     """65536 x 65536 S with Zipf(1.5) rows (mean 16), built from COO; D is
-    65536 x 256 N(0,1)."""
+    65536 x 256 N(0,1).  The multiply runs with the autotuner's skew-class
+    choice (group width, variant and tile searched on this workload)."""
     import numpy as np
     import torch
+    from gcoospdm_amd import autotune
     from gcoospdm_amd import workloads as W
     M = K = 65536
     N = 256
     rows, cols, vals = W.zipf_coo(M, K, seed=11)
+    counts = np.bincount(rows, minlength=M)
+    per_row, max_row = float(counts.mean()), float(counts.max())
     r, c, v = (torch.from_numpy(x).to(dev) for x in (rows, cols, vals))
     m = G.GCOOMatrix(M, K, 4)
     m.convert_coo(r, c, v)
@@ -320,19 +324,27 @@ def zipf_case(G, dev):
     D16 = G.to_bf16(D)
     C = torch.empty((M, N), device=dev)
     stream = torch.cuda.current_stream()
-    from gcoospdm_amd import autotune
     out = {"workload": "zipf1.5_65536x65536_mean16_N256", "nnz": nnz,
+           "max_row": int(max_row), "skew_class": autotune.skew_key(N, per_row, max_row),
            "coo_convert_us": round(t_conv * 1e6, 1)}
+    handles = {4: m}
     for dt in ("f32", "bf16"):
-        v_, vec = autotune.plain_config(N, nnz / M, dt == "bf16")
-        m.set_config(variant=v_, vec=vec)
-        fn = (lambda: m.spdm(D, C, sync=False)) if dt == "f32" else \
-             (lambda: m.spdm_bf16(D16, C, sync=False))
+        tuned = autotune.skew_config(N, per_row, max_row, dt == "bf16")
+        cfg = dict(cpg=4, variant=0, vec=0, unroll=0, chunk=0)
+        if tuned:
+            cfg.update({k: tuned[k] for k in cfg})
+        if cfg["cpg"] not in handles:
+            handles[cfg["cpg"]] = G.GCOOMatrix(M, K, cfg["cpg"]).convert_coo(r, c, v)
+        h = handles[cfg["cpg"]]
+        h.set_config(variant=cfg["variant"], vec=cfg["vec"], unroll=cfg["unroll"],
+                     chunk=cfg["chunk"])
+        fn = (lambda: h.spdm(D, C, sync=False)) if dt == "f32" else \
+             (lambda: h.spdm_bf16(D16, C, sync=False))
         fn()
         t = event_time(fn, 20, stream)
         by = spdm_bytes(nnz, M, K, N, dt == "bf16")
         out[dt] = {"us": round(t * 1e6, 1), "gflops": round(2 * nnz * N / t / 1e9, 1),
-                   "hbm_frac": round(by / t / 8e12, 4)}
+                   "hbm_frac": round(by / t / 8e12, 4), "tuned": cfg if tuned else None}
     return out
 
 

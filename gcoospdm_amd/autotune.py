@@ -6,10 +6,17 @@ substituting them into templates, recompiling and timing `main`
 are runtime arguments, so the search needs no rebuild: each candidate is
 timed with HIP events on the device and the fastest is written to a table.
 
-Two tables:
+Three tables:
   batch[dtype][class] = {rows, vec}   for gcoo_batch_add, where
       class = (N bucket, entries-per-row bucket) of a job;
-  plain[dtype][class] = {variant, vec} for gcoo_set_spdm_config.
+  plain[dtype][class] = {variant, vec} for gcoo_set_spdm_config;
+  skew[dtype][class]  = {cpg, variant, vec, unroll, chunk} for matrices
+      with skewed rows, class = (N bucket, entries-per-row bucket, skew
+      bucket = longest row over the mean row): the group width (the
+      reference's COLUMN_PER_GROUP, src/constants.h.t:4, grid-searched by
+      src/autotune.py:89-97), the variant and its tile, searched on the
+      skewed workload itself (65536 x 65536, Zipf(1.5) rows of mean 16,
+      N = 256).
 
 Run:  python -m gcoospdm_amd.autotune [--out gcoospdm_amd/tuning.json]
 The table shipped in the package was produced by that command on an MI355X.
@@ -44,6 +51,27 @@ def row_class(per_row: float) -> int:
 
 def class_key(N: int, per_row: float) -> str:
     return f"n{n_class(N)}r{row_class(per_row)}"
+
+
+SKEW_BUCKETS = [4, 32]                    # longest row / mean row: <4, <32, >=32
+
+
+def skew_class(max_row: float, per_row: float) -> int:
+    ratio = max_row / max(per_row, 1e-9)
+    for i, b in enumerate(SKEW_BUCKETS):
+        if ratio < b:
+            return i
+    return len(SKEW_BUCKETS)
+
+
+def skew_key(N: int, per_row: float, max_row: float) -> str:
+    return f"n{n_class(N)}r{row_class(per_row)}s{skew_class(max_row, per_row)}"
+
+
+def skew_config(N: int, per_row: float, max_row: float, bf16: bool, table: dict | None = None):
+    """Tuned {cpg, variant, vec, unroll, chunk} for a skewed matrix, or None."""
+    t = (table if table is not None else load()).get("skew", {}).get("bf16" if bf16 else "f32", {})
+    return t.get(skew_key(N, per_row, max_row))
 
 
 _table_cache = {}
@@ -147,12 +175,64 @@ def tune_plain(bf16: bool, log, device="cuda:0"):
     return out
 
 
+def tune_skew(bf16: bool, log, device="cuda:0", seed=11):
+    """Grid search on the skewed workload: cpg in {1, 2, 4} x variants
+    {2, 4, 6} x their tile parameters; every candidate's C is checked
+    bit-equal to the first one's."""
+    import numpy as np
+    import torch
+    import gcoospdm_amd as G
+    from . import workloads as W
+    M = K = 65536
+    N = 256
+    rows, cols, vals = W.zipf_coo(M, K, seed=seed)
+    counts = np.bincount(rows, minlength=M)
+    per_row, max_row = float(counts.mean()), float(counts.max())
+    r, c, v = (torch.from_numpy(x).to(device) for x in (rows, cols, vals))
+    D = torch.from_numpy(np.random.default_rng(seed + 1).standard_normal((K, N))
+                         .astype(np.float32)).to(device)
+    if bf16:
+        D = G.to_bf16(D)
+    C = torch.empty((M, N), device=device)
+    cands = []
+    vecs = [2, 4, 8] if bf16 else [1, 2, 4]
+    for vec in vecs:
+        for unroll in (8, 16, 32):
+            for chunk in (0, 1):
+                cands.append(dict(variant=2, vec=vec, unroll=unroll, chunk=chunk))
+        for chunk in (64, 256, 1024):
+            cands.append(dict(variant=4, vec=vec, unroll=16, chunk=chunk))
+    cands.append(dict(variant=6, vec=4, unroll=0, chunk=0))
+    best, ref = None, None
+    for cpg in (1, 2, 4):
+        m = G.GCOOMatrix(M, K, cpg).convert_coo(r, c, v)
+        for cfg in cands:
+            m.set_config(**cfg)
+            fn = (lambda: m.spdm_bf16(D, C, sync=False)) if bf16 else \
+                 (lambda: m.spdm(D, C, sync=False))
+            ms = _time(fn, reps=5)
+            if ref is None:
+                ref = C.clone()
+            same = bool(torch.equal(C.view(torch.int32), ref.view(torch.int32)))
+            log(dict(kind="skew", dtype="bf16" if bf16 else "f32", cpg=cpg, ms=round(ms, 4),
+                     same=same, **cfg))
+            if not same:
+                raise RuntimeError(f"skew candidate cpg={cpg} {cfg} changed the bits")
+            if best is None or ms < best[0]:
+                best = (ms, cpg, cfg)
+        m.close()
+    key = skew_key(N, per_row, max_row)
+    return {key: dict(cpg=best[1], ms=round(best[0], 4), **best[2])}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--per-class", type=int, default=24)
     ap.add_argument("--log", default=None)
+    ap.add_argument("--skew-only", action="store_true",
+                    help="rerun only the skew search, keeping the other tables")
     args = ap.parse_args()
     from . import workloads as W
 
@@ -165,7 +245,14 @@ def main():
             logf.write(line + "\n")
 
     t0 = time.time()
-    table = {"device": "MI355X (gfx950)", "batch": {}, "plain": {}}
+    if args.skew_only:
+        table = dict(load(args.out)) if os.path.exists(args.out) else {}
+        table["skew"] = {d: tune_skew(d == "bf16", log) for d in ("f32", "bf16")}
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        print(f"wrote {args.out}", flush=True)
+        return
+    table = {"device": "MI355X (gfx950)", "batch": {}, "plain": {}, "skew": {}}
     specs = W.batch_jobs(args.jobs, 2024)
     for bf16 in (False, True):
         by_class = {}
@@ -178,6 +265,7 @@ def main():
         table["batch"]["bf16" if bf16 else "f32"] = tune_batch(by_class, bf16, log)
         del by_class
         table["plain"]["bf16" if bf16 else "f32"] = tune_plain(bf16, log)
+        table["skew"]["bf16" if bf16 else "f32"] = tune_skew(bf16, log)
     table["seconds"] = round(time.time() - t0, 1)
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
