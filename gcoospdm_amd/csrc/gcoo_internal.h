@@ -189,12 +189,13 @@ int ensure_seg16(gcoo_matrix *h, hipStream_t s);
 // rows [w * rpw, (w + 1) * rpw).
 // Enqueue (on s) the panel lists of h's entries from seg (h->seg or
 // h->seg16) for k chunks of tk rows staged at rowb bytes per D row, unless
-// they are current.  Wave w of panel p has nC + 1 list offsets at
+// they are current (with widen, the list values are BF16 segment values
+// widened to FP32).  Wave w of panel p has nC + 1 list offsets at
 // poff[(p * waves + w) * (nC + 1)]: its chunk c entries are
 // plist[off[c] .. off[c + 1]), ordered by (row, k).  Kernels exit early when
 // guard_skip(guard, h->cap) holds.
 int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
-                       int waves, const ConvStatus *guard, hipStream_t s);
+                       int waves, bool widen, const ConvStatus *guard, hipStream_t s);
 // Forget the panel lists (after a conversion or a capacity change).
 inline void invalidate_panel_lists(const gcoo_matrix *h) { h->plist_src = nullptr; }
 // Ints of the offset (and count) area the panel lists of h need for k
@@ -204,7 +205,7 @@ size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves);
 // Enqueue the build of the panel lists into caller-given buffers (poff of
 // panel_offset_elems ints, plist of h->nnz entries); touches no handle state.
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
-                      int waves, int *poff, uint2 *plist, const ConvStatus *guard,
+                      int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
                       hipStream_t s);
 
 // Device scratch handed to the multiply by a caller that cannot allocate:

@@ -21,7 +21,8 @@
 //   2. an exclusive scan of the counts in (chunk, row) order;
 //   3. every entry moves to its list slot as
 //      ((LDS byte offset of its D row within the staged chunk / 16) << 16 |
-//      row slot in its wave, value bits): the multiply forms the LDS address
+//      row slot in its wave, value bits -- for BF16 segments the value widened
+//      to FP32, BF16 bits << 16): the multiply forms the LDS address
 //      with one multiply-add of the high half (hi16 * 16 + lane address) and
 //      takes the accumulator index from the low byte.
 // The counts sit in LDS when nC * rpw ints fit in 64 KB, else in a global
@@ -49,8 +50,8 @@ __device__ __forceinline__ int run_start(bool first, int i0, int lane, int &carr
 template <bool LDS, int NT>
 __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int rowb,
-    int nC, int RPW, int *__restrict__ poff, int *__restrict__ gcnt, uint2 *__restrict__ plist,
-    const ConvStatus *__restrict__ guard, long long cap)
+    int nC, int RPW, int widen, int *__restrict__ poff, int *__restrict__ gcnt,
+    uint2 *__restrict__ plist, const ConvStatus *__restrict__ guard, long long cap)
 {
     if (guard_skip(guard, cap)) return;
     extern __shared__ int lcnt[];
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
             if (act)
                 plist[cnt[c * RPW + rr] + (i - s)] =
                     make_uint2(((unsigned)((kv.x - c * tk) * rowb) >> 4 << 16) | (unsigned)rr,
-                               (unsigned)kv.y);
+                               widen ? (unsigned)kv.y << 16 : (unsigned)kv.y);
         }
     }
 }
@@ -142,7 +143,8 @@ size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves)
 }
 
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
-                      int waves, int *poff, uint2 *plist, const ConvStatus *guard, hipStream_t s)
+                      int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
+                      hipStream_t s)
 {
     if ((long long)tk * rowb > (1ll << 20) || rowb % 16 != 0)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
@@ -159,18 +161,19 @@ int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, i
         if (lds)
             hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
-                               rowb, nC, rpw, poff, gcnt, plist, guard, (long long)h->cap);
+                               rowb, nC, rpw, widen ? 1 : 0, poff, gcnt, plist, guard,
+                               (long long)h->cap);
         else
             hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC, rpw,
-                               poff, gcnt, plist, guard, (long long)h->cap);
+                               widen ? 1 : 0, poff, gcnt, plist, guard, (long long)h->cap);
         GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
     }
     return GCOO_OK;
 }
 
 int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
-                       int waves, const ConvStatus *guard, hipStream_t s)
+                       int waves, bool widen, const ConvStatus *guard, hipStream_t s)
 {
     if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb &&
         h->plist_rpw == rpw && h->plist_waves == waves)
@@ -194,7 +197,7 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, 
             return set_error(GCOO_ERR_ALLOC, "cannot allocate the panel lists");
         }
     }
-    int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, h->poff, h->plist, guard, s);
+    int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, widen, h->poff, h->plist, guard, s);
     if (rc) return rc;
     h->plist_src = seg;
     h->plist_tk = tk;

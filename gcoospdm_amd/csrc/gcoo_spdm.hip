@@ -629,108 +629,40 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 //     (panel == chunk % panels), touches the tile piece of the chunk
 //     kPanelPrefetch chunks ahead so that it is in L2 when the stagers load
 //     it.  The panels of a tile run in step, so without this every chunk
-//     would be a simultaneous L2 miss of all of them.  Wave 15 never waits
-//     for those loads inside the loop: vector memory completions are counted
-//     in issue order, and a wait would stall on the miss it is there to
-//     absorb.  The hardware issues from the oldest (lowest) waves first, so
+//     would be a simultaneous L2 miss of all of them.  The touch wave never
+//     waits for those loads inside the loop: vector memory completions are
+//     counted in issue order, and a wait would stall on the miss it is there
+//     to absorb.  The hardware issues from the oldest (lowest) waves first, so
 //     the staging sits with them and the younger waves, which finish their
 //     entries last, carry no loads of their own.
 //   * Each wave takes its rows' entries of the chunk from the panel list
-//     (built by gcoo_panel.hip from the row segments, ordered by row then k),
-//     8 at a time from scalar loads: 8 LDS reads in flight, then 8 fmaf steps.
-//     Before each group of 8 a wave sets its issue priority from the entries
-//     it has left in the chunk, so that the wave the barrier waits for is
-//     served first.  For FP32 a chunk is one asm block (gcoo_panel_asm.h):
-//     the first group's scalar load goes out before the chunk barrier (its
-//     latency passes while the wave waits there), then the barrier and the
-//     staging of the next chunk, then the entry loop, in which the next
-//     group's scalar load is issued after this group's LDS wait and runs
-//     under its steps, and the last 1..7 entries take one guarded round
-//     instead of 4/2/1 groups with a load each.
-//   * The step's row is wave-uniform but varies per entry.  The 64
-//     accumulators sit in v0..v63 (component v of row slot q in v[16v + q])
+//     (built by gcoo_panel.hip from the row segments, ordered by row then k)
+//     through scalar loads.  A chunk is one generated asm block
+//     (gcoo_panel_asm.h): the first 16 entries' scalar loads go out before
+//     the chunk barrier (their latency passes while the wave waits there),
+//     then the barrier and the staging of the next chunk, then entries 0..15
+//     in groups of 4 with the next group's LDS reads in flight, then batches
+//     of 8, and the last 1..7 entries in one guarded round.  Before each batch
+//     a wave sets its issue priority from the entries it has left in the
+//     chunk, so that the wave the barrier waits for is served first.
+//   * The step's row is wave-uniform but varies per entry.  The 4 * RPW
+//     accumulators sit in v0.. (component v of row slot q in v[RPW v + q])
 //     and each step is 4 v_fma_f32 in VGPR-index mode, source 2 and
 //     destination relative to M0 = q: acc[q] = fmaf(x, d, acc[q]) with no
-//     moves and no branches.  List entries carry the D row's LDS offset and
-//     the row slot ready for the scalar unit, and four steps share one
-//     index-mode block, so a step costs about four scalar instructions.
+//     moves and no branches.  A list entry's first word is (LDS offset / 16)
+//     << 16 | row slot: one v_mad_u32_u16 of its high half gives the LDS
+//     address, and the index instructions take the slot from its low byte.
+//   * BF16 D is staged as it is (150-row chunks of 512-byte rows, two rows
+//     per wave load), read 8 bytes per lane and widened exactly to FP32
+//     before the steps; the list values are stored widened.
 // Every row's chain is its entries in ascending k, as in all variants.
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPanelPrefetch = 6;  // chunks between the L2 touch and the staging
 constexpr double kPanelMinDensity = 0.0025;  // automatic choice of variant 6 (FP32) ...
+constexpr double kPanelMinDensityBf16 = 0.007;  // ... and for BF16
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
-
-// a[v][q] = fmaf(x, d[v], a[v][q]) for v < 4 and a wave-uniform row slot
-// q < 16: the accumulators are pinned in v0..v63 (component v of row slot q
-// in v[16v + q]) and the step runs in VGPR-index mode with source 2 and the
-// destination relative to M0[7:0].  The index instructions copy bits [7:0]
-// of their operand into M0[7:0] and ignore the rest, so callers pass the
-// packed list word ((LDS offset / 16) << 16 | q) as it is.  The index mode writes
-// M0, which the buffer-to-LDS loads also use; callers save M0 around a run of
-// steps (panel_m0_save / panel_m0_restore).
-__device__ __forceinline__ void panel_fma1(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
-                                           unsigned q, float x, const float (&d)[4])
-{
-    asm volatile(
-        "s_set_gpr_idx_on %4, gpr_idx(SRC2,DST)\n\t"
-        "v_fma_f32 v0, %5, %6, v0\n\t"
-        "v_fma_f32 v16, %5, %7, v16\n\t"
-        "v_fma_f32 v32, %5, %8, v32\n\t"
-        "v_fma_f32 v48, %5, %9, v48\n\t"
-        "s_set_gpr_idx_off"
-        : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
-        : "s"(q), "s"(x), "v"(d[0]), "v"(d[1]), "v"(d[2]), "v"(d[3]));
-}
-
-// Four steps in order, the row index of each later step set with
-// s_set_gpr_idx_idx (index only, mode kept).
-__device__ __forceinline__ void panel_fma4(f32x16 &a0, f32x16 &a1, f32x16 &a2, f32x16 &a3,
-                                           const unsigned (&q)[4], const float (&x)[4],
-                                           const float (&d0)[4], const float (&d1)[4],
-                                           const float (&d2)[4], const float (&d3)[4])
-{
-    asm volatile(
-        "s_set_gpr_idx_on %4, gpr_idx(SRC2,DST)\n\t"
-        "v_fma_f32 v0, %8, %12, v0\n\t"
-        "v_fma_f32 v16, %8, %13, v16\n\t"
-        "v_fma_f32 v32, %8, %14, v32\n\t"
-        "v_fma_f32 v48, %8, %15, v48\n\t"
-        "s_set_gpr_idx_idx %5\n\t"
-        "v_fma_f32 v0, %9, %16, v0\n\t"
-        "v_fma_f32 v16, %9, %17, v16\n\t"
-        "v_fma_f32 v32, %9, %18, v32\n\t"
-        "v_fma_f32 v48, %9, %19, v48\n\t"
-        "s_set_gpr_idx_idx %6\n\t"
-        "v_fma_f32 v0, %10, %20, v0\n\t"
-        "v_fma_f32 v16, %10, %21, v16\n\t"
-        "v_fma_f32 v32, %10, %22, v32\n\t"
-        "v_fma_f32 v48, %10, %23, v48\n\t"
-        "s_set_gpr_idx_idx %7\n\t"
-        "v_fma_f32 v0, %11, %24, v0\n\t"
-        "v_fma_f32 v16, %11, %25, v16\n\t"
-        "v_fma_f32 v32, %11, %26, v32\n\t"
-        "v_fma_f32 v48, %11, %27, v48\n\t"
-        "s_set_gpr_idx_off"
-        : "+{v[0:15]}"(a0), "+{v[16:31]}"(a1), "+{v[32:47]}"(a2), "+{v[48:63]}"(a3)
-        : "s"(q[0]), "s"(q[1]), "s"(q[2]), "s"(q[3]), "s"(x[0]), "s"(x[1]), "s"(x[2]), "s"(x[3]),
-          "v"(d0[0]), "v"(d0[1]), "v"(d0[2]), "v"(d0[3]), "v"(d1[0]), "v"(d1[1]), "v"(d1[2]),
-          "v"(d1[3]), "v"(d2[0]), "v"(d2[1]), "v"(d2[2]), "v"(d2[3]), "v"(d3[0]), "v"(d3[1]),
-          "v"(d3[2]), "v"(d3[3]));
-}
-
-__device__ __forceinline__ int panel_m0_save()
-{
-    int m0;
-    asm volatile("s_mov_b32 %0, m0" : "=s"(m0) : : "memory");
-    return m0;
-}
-
-__device__ __forceinline__ void panel_m0_restore(int m0)
-{
-    asm volatile("s_mov_b32 m0, %0" : : "s"(m0) : "memory");
-}
 
 typedef float f32x8 __attribute__((ext_vector_type(8)));
 
@@ -742,9 +674,9 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
 {
     constexpr int VEC = 4;
     constexpr int PR = WAVES * RPW;           // panel rows
-    static_assert(E::bytes == 4 ? (RPW == 16 || RPW == 28 || RPW == 42) && RPW % 2 == 0
-                                : (RPW == 16 && WAVES == 16),
-                  "FP32: the generated asm blocks; BF16: 16 waves of 16 rows");
+    static_assert(E::bytes == 4 ? (RPW == 16 || RPW == 28 || RPW == 42)
+                                : (RPW == 16 || RPW == 28),
+                  "rows per wave of the generated asm blocks");
     constexpr int TN = 64 * VEC;              // columns per tile
     constexpr int LB = VEC * E::bytes;        // bytes of a D row per lane
     constexpr int ROWB = 64 * LB;             // bytes of a staged D row
@@ -826,12 +758,11 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
         }
     };
 
-    // FP32: 4 * RPW accumulators pinned to v0.. in f32x8 blocks (component c
-    // of row slot q in register RPW c + q); BF16: four f32x16 (RPW = 16)
-    f32x8 acc[E::bytes == 4 ? RPW / 2 : 1];
+    // 4 * RPW accumulators pinned to v0.. in f32x8 blocks (component c of row
+    // slot q in register RPW c + q)
+    f32x8 acc[RPW / 2];
 #pragma unroll
-    for (int i = 0; i < (E::bytes == 4 ? RPW / 2 : 1); ++i) acc[i] = 0.0f;
-    f32x16 a0 = 0.0f, a1 = 0.0f, a2 = 0.0f, a3 = 0.0f;
+    for (int i = 0; i < RPW / 2; ++i) acc[i] = 0.0f;
     // this wave's chunk list starts (nC + 1 of them, the last its end)
     const int *op = poff + ((size_t)panel * WAVES + wave) * ((size_t)nC + 1);
     const unsigned char *ldsLane = reinterpret_cast<const unsigned char *>(sD) + lane * LB;
@@ -853,15 +784,16 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
     };
     load_window();
     stage(0, 0);
-    if constexpr (E::bytes == 4) {
-        // FP32: the chunk loop in asm (gcoo_panel_asm.h), one block per
+    {
+        // The chunk loop in asm (gcoo_panel_asm.h), one block per
         // segment of consecutive chunks of a 64-chunk window.  Per chunk: the
         // barrier, the scalar loads of the next chunk's first 16 entries
         // (landing while this chunk runs), the staging of the next chunk,
         // and the entry loop.  The L2 touch wave ends a segment before each
         // chunk at which it touches, and touches between segments.
-        static_assert(LPW == 15 && RPL == 1 && CHUNKB == 75 * 1024,
-                      "the asm block stages 15 one-row loads into 75 KB buffers");
+        static_assert(LPW == 15 && RPL == (E::bytes == 4 ? 1 : 2) && CHUNKB == 75 * 1024,
+                      "the asm blocks stage 15 loads of one (FP32) or two (BF16) D rows "
+                      "into 75 KB buffers");
         typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
         const u32x4 rs = {(unsigned)reinterpret_cast<uintptr_t>(D),
                           (unsigned)(reinterpret_cast<uintptr_t>(D) >> 32) & 0xffffu,
@@ -872,13 +804,17 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
         const unsigned sdst = __builtin_amdgcn_readfirstlane((unsigned)(size_t)ldsBase);
         // staging: row offsets below kb are loaded, and when the wave's last
         // row of a chunk is below K too (offset <= lim) without tests; the
-        // other waves have kb = 0 and stage nothing
+        // other waves have kb = 0 and stage nothing.  A load advances RPL
+        // rows (uload bytes); BF16 lanes of a load's second row take voffz
+        // when that row is past K.
         const unsigned urow = (unsigned)rowBytes;
+        const unsigned uload = (unsigned)RPL * urow;
         const unsigned kb = __builtin_amdgcn_readfirstlane(wave < SW ? dBytes : 0u);
         const unsigned lim = __builtin_amdgcn_readfirstlane(
-            (wave < SW && dBytes >= (unsigned)LPW * urow) ? dBytes - (unsigned)(LPW - 1) * urow - urow : 0u);
+            (wave < SW && dBytes >= (unsigned)LPW * uload) ? dBytes - (unsigned)LPW * uload : 0u);
         const unsigned tkrowb = (unsigned)TK * urow;
-        unsigned sso = __builtin_amdgcn_readfirstlane((unsigned)(TK + wave * LPW) * urow);
+        unsigned sso = __builtin_amdgcn_readfirstlane((unsigned)(TK + wave * LPW * RPL) * urow);
+        const int pvoffz = (RPL == 1 || prow == 0) ? pvoff : (int)dBytes;
         int c = 0;
         for (;;) {
             // the window as list byte offsets from its first chunk's entries
@@ -899,80 +835,29 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
                 unsigned sli = __builtin_amdgcn_readfirstlane((unsigned)(c - cw));
                 const unsigned liend = __builtin_amdgcn_readfirstlane((unsigned)(cend - cw));
                 const unsigned par = __builtin_amdgcn_readfirstlane((unsigned)c & 1u);
-#define GCOO_PANEL_SEGMENT(R)                                                              \
-    asm volatile(GCOO_PANEL_SEGMENT_ASM_R##R                                               \
-                 : GCOO_PANEL_ACC_OPERANDS_R##R(acc), [sli] "+s"(sli), [sso] "+s"(sso)     \
+#define GCOO_PANEL_SEGMENT(T, N)                                                           \
+    asm volatile(GCOO_PANEL_SEGMENT_ASM_##T##R##N                                          \
+                 : GCOO_PANEL_ACC_OPERANDS_R##N(acc), [sli] "+s"(sli), [sso] "+s"(sso)     \
                  : [pbase] "s"(pbase), [vpbo] "v"(vpbo), [vpn] "v"(vpn), [liend] "s"(liend), \
                    [par] "s"(par), [vb0] "v"(vb0), [vb1] "v"(vb1), [sdst] "s"(sdst),      \
-                   [lim] "s"(lim), [kb] "s"(kb), [rowb] "s"(urow), [tkrowb] "s"(tkrowb),  \
-                   [voff] "v"(pvoff), [rsrc] "s"(rs)                                       \
-                 : GCOO_PANEL_SEGMENT_CLOBBERS_R##R)
-                if constexpr (RPW == 16) GCOO_PANEL_SEGMENT(16);
-                else if constexpr (RPW == 28) GCOO_PANEL_SEGMENT(28);
-                else GCOO_PANEL_SEGMENT(42);
+                   [lim] "s"(lim), [kb] "s"(kb), [rowb] "s"(uload), [row1] "s"(urow),     \
+                   [tkrowb] "s"(tkrowb), [voff] "v"(pvoff), [voffz] "v"(pvoffz),          \
+                   [rsrc] "s"(rs)                                                          \
+                 : GCOO_PANEL_SEGMENT_CLOBBERS_##T##R##N)
+                if constexpr (E::bytes == 4) {
+                    if constexpr (RPW == 16) GCOO_PANEL_SEGMENT(, 16);
+                    else if constexpr (RPW == 28) GCOO_PANEL_SEGMENT(, 28);
+                    else GCOO_PANEL_SEGMENT(, 42);
+                } else {
+                    if constexpr (RPW == 16) GCOO_PANEL_SEGMENT(BF16_, 16);
+                    else GCOO_PANEL_SEGMENT(BF16_, 28);
+                }
 #undef GCOO_PANEL_SEGMENT
                 c = cend;
             }
             if (c >= nC) break;
             cw = c;
             load_window();
-        }
-    } else {
-        // BF16: the chunk loop in C++, the same order per chunk
-        int pb = __builtin_amdgcn_readlane(vpb, 0), pe = __builtin_amdgcn_readlane(vpe, 0);
-        for (int c = 0; c < nC; ++c) {
-            // chunk c has landed in LDS for every wave, and every wave is
-            // done with chunk c - 1, whose buffer the next stage overwrites
-            // (a bare barrier: a fence here would wait for wave 15's touches
-            // too)
-            if (wave < SW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-            asm volatile("s_barrier" ::: "memory");
-            if (c + 1 < nC) stage(c + 1, (c + 1) & 1);
-            if (wave == TW && c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
-            const unsigned char *sb = ldsLane + (c & 1) * CHUNKB;
-            if (c + 1 == cw + 64) {
-                cw += 64;
-                load_window();
-            }
-            const int li = c + 1 < nC ? c + 1 - cw : 0;
-            const int pbn = __builtin_amdgcn_readlane(vpb, li);
-            const int pen = __builtin_amdgcn_readlane(vpe, li);
-            // B entries: list entry x = (LDS byte offset / 16) << 16 | row slot
-            auto run = [&](auto bn, int p) {
-                constexpr int B = decltype(bn)::value;
-                uint2 m[B];
-#pragma unroll
-                for (int u = 0; u < B; ++u) m[u] = plist[p + u];
-                float d[B][VEC];
-#pragma unroll
-                for (int u = 0; u < B; ++u) lds_load<E, VEC>(d[u], sb + ((m[u].x >> 16) << 4));
-                const int m0 = panel_m0_save();
-                int u = 0;
-#pragma unroll
-                for (; u + 4 <= B; u += 4) {
-                    const unsigned q[4] = {m[u].x, m[u + 1].x, m[u + 2].x, m[u + 3].x};
-                    const float x[4] = {E::val((int)m[u].y), E::val((int)m[u + 1].y),
-                                        E::val((int)m[u + 2].y), E::val((int)m[u + 3].y)};
-                    panel_fma4(a0, a1, a2, a3, q, x, d[u], d[u + 1], d[u + 2], d[u + 3]);
-                }
-#pragma unroll
-                for (; u < B; ++u)
-                    panel_fma1(a0, a1, a2, a3, m[u].x, E::val((int)m[u].y), d[u]);
-                panel_m0_restore(m0);
-            };
-            int p = pb;
-            for (; p + 8 <= pe; p += 8) {
-                const int rem = pe - p;
-                if (rem >= 16) __builtin_amdgcn_s_setprio(3);
-                else __builtin_amdgcn_s_setprio(2);
-                run(std::integral_constant<int, 8>{}, p);
-            }
-            if (p + 4 <= pe) { run(std::integral_constant<int, 4>{}, p); p += 4; }
-            if (p + 2 <= pe) { run(std::integral_constant<int, 2>{}, p); p += 2; }
-            if (p < pe) run(std::integral_constant<int, 1>{}, p);
-            __builtin_amdgcn_s_setprio(0);
-            pb = pbn;
-            pe = pen;
         }
     }
     // no LDS write may land after the workgroup's LDS is released
@@ -984,15 +869,8 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
         for (int q = 0; q < RPW; ++q) {
             const int r = panel * PR + wave * RPW + q;
             float out[VEC];
-            if constexpr (E::bytes == 4) {
 #pragma unroll
-                for (int c = 0; c < VEC; ++c) out[c] = acc[(RPW * c + q) / 8][(RPW * c + q) % 8];
-            } else {
-                out[0] = a0[q];
-                out[1] = a1[q];
-                out[2] = a2[q];
-                out[3] = a3[q];
-            }
+            for (int c = 0; c < VEC; ++c) out[c] = acc[(RPW * c + q) / 8][(RPW * c + q) % 8];
             if (r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
         }
     }
@@ -1167,12 +1045,12 @@ int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
                              (long long)h->nnz, offB + listB, xs->bytes);
         int *po = reinterpret_cast<int *>(xs->base);
         uint2 *pl = reinterpret_cast<uint2 *>(xs->base + offB);
-        int rc = build_panel_lists(h, seg, TK, ROWB, RPW, WAVES, po, pl, guard, s);
+        int rc = build_panel_lists(h, seg, TK, ROWB, RPW, WAVES, E::bytes == 2, po, pl, guard, s);
         if (rc) return rc;
         poff = po;
         plist = pl;
     } else {
-        int rc = ensure_panel_lists(h, seg, TK, ROWB, RPW, WAVES, guard, s);
+        int rc = ensure_panel_lists(h, seg, TK, ROWB, RPW, WAVES, E::bytes == 2, guard, s);
         if (rc) return rc;
         poff = h->poff;
         plist = h->plist;
@@ -1341,8 +1219,8 @@ constexpr double kPanelTallDensity = 0.009;
 template <typename E>
 int panel_waves(const gcoo_matrix *h, int cfgWaves)
 {
-    if constexpr (E::bytes != 4) return 16;
-    if (cfgWaves == 16 || cfgWaves == 12 || cfgWaves == 8) return cfgWaves;
+    if (cfgWaves == 16 || cfgWaves == 12 || (cfgWaves == 8 && E::bytes == 4)) return cfgWaves;
+    if constexpr (E::bytes != 4) return 12;  // BF16: 336-row panels at every density
     const double cells = (double)h->M * (double)h->K;
     return cells > 0 && (double)h->nnz >= kPanelTallDensity * cells ? 16 : 12;
 }
@@ -1366,15 +1244,19 @@ void resolve_variant(const gcoo_matrix *h, const void *D, int N, const float *C,
         // segments cost one D-row gather per entry: the panel kernel wins from
         // about 0.25 % density on (n = 8000, FP32, 336-row panels: 311
         // against 351 us at s = 0.997, 316 against 248 us at 0.998).  BF16
-        // gathers move half the bytes and stay ahead at every density.  The density is the handle's last completed conversion (a
+        // gathers move half the bytes and the BF16 panel kernel widens every
+        // D value (4 more vector instructions per entry), so for BF16 it only
+        // wins from about 0.7 % (336-row panels: 492 against 569 us at 1 %,
+        // 419 against 453 us at 0.8 %, 310 against 290 us at 0.5 %).  The density is the handle's last completed conversion (a
         // first optimistic launch does not know the new count yet); every
         // variant gives the same bits.  It also needs a workgroup per CU at
         // least (256 x 256 tiles of C).
         const double cells = (double)h->M * (double)h->K;
-        const bool dense = cells > 0 && (double)h->nnz >= kPanelMinDensity * cells;
+        const double minDensity = E::bytes == 4 ? kPanelMinDensity : kPanelMinDensityBf16;
+        const bool dense = cells > 0 && (double)h->nnz >= minDensity * cells;
         const int pr = panel_rows_of(panel_waves<E>(h, cfg.waves));
         const long long wgs = (long long)((h->M + pr - 1) / pr) * ((N + 255) / 256);
-        variant = (E::bytes == 4 && dense && wgs >= kPanelMinWorkgroups) ? 6 : 2;
+        variant = (dense && wgs >= kPanelMinWorkgroups) ? 6 : 2;
     }
     if (variant == 1) variant = 2;
     // variants 3, 4 and 6 stage 16-byte pieces of D rows: they need whole
@@ -1425,6 +1307,8 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
             default: return launch_panel<E, 75, 16, 16>(h, seg, D, N, C, guard, s, xs);
             }
         } else {
+            if (panel_waves<E>(h, cfg.waves) == 12)
+                return launch_panel<E, 150, 12, 28>(h, seg, D, N, C, guard, s, xs);
             return launch_panel<E, 150, 16, 16>(h, seg, D, N, C, guard, s, xs);
         }
     }

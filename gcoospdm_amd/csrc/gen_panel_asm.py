@@ -4,7 +4,13 @@
 runs a segment of consecutive chunks, for each supported number R of rows
 per wave (ROWS below): macros GCOO_PANEL_SEGMENT_ASM_R<R>, its clobbers
 GCOO_PANEL_SEGMENT_CLOBBERS_R<R>, and GCOO_PANEL_ACC_OPERANDS_R<R>(a), the
-operands pinning the accumulators (an array of 4 R / 8 f32x8) to v0..v(4R-1).
+operands pinning the accumulators (an array of 4 R / 8 f32x8) to v0..v(4R-1);
+and for BF16 D (ROWS_BF16) GCOO_PANEL_SEGMENT_ASM_BF16_R<R> with its
+clobbers.  The BF16 block stages 150-row chunks of 512-byte D rows (two rows
+per 1 KB wave load, %[rowb] the two rows' bytes, %[row1] one row's), reads
+an entry's 4 values with ds_read_b64 and widens them exactly to FP32 (the
+BF16 bits as the high half) before the steps; the list values are already
+widened.
 
     python3 gen_panel_asm.py > gcoo_panel_asm.h
 
@@ -67,19 +73,22 @@ ST, SREM, SOFF, SCNT, SM0, PBN, SNN = 24, 25, 26, 27, 28, 29, 30
 BANKS = ((list(range(36, 52)), list(range(52, 68))),
          (list(range(68, 84)), list(range(84, 100))))
 CHUNKB = 75 * 1024
-ROWS = (16, 28, 42)       # rows per wave of the generated blocks
+ROWS = (16, 28, 42)       # rows per wave of the generated FP32 blocks
+ROWS_BF16 = (16, 28)      # ... and BF16 blocks
 R = 16                    # rows per wave of the block being generated
+BF16 = False              # the block reads BF16 D (8 bytes per lane) and widens it
 DV, AV = 64, 96           # D values of the batch loop, their addresses
 G_D = (64, 80)            # D buffers of the rolling phase
 G_AV = (96, 100)          # their address VGPRs
 L = []
 
 
-def configure(rows):
+def configure(rows, bf16=False):
     """Register layout for `rows` rows per wave: accumulators v0..v(4 rows - 1),
     scratch from v(4 rows) on."""
-    global R, DV, AV, G_D, G_AV
+    global R, DV, AV, G_D, G_AV, BF16
     R = rows
+    BF16 = bf16
     base = 4 * rows
     DV, AV = base, base + 32
     G_D = (base, base + 16)
@@ -98,11 +107,33 @@ class Body:
         self.A, self.B = BANKS[p]
         self.base = f"%[vb{p}]"
 
+    # ---- D reads and their widening ----
+    @staticmethod
+    def dread(d, a):
+        """The LDS read of an entry's 4 D values into v[d..d+3] from the
+        address in v{a} (BF16: two dwords into v[d], v[d+1])."""
+        if BF16:
+            e(f"ds_read_b64 v[{d}:{d+1}], v{a}")
+        else:
+            e(f"ds_read_b128 v[{d}:{d+3}], v{a}")
+
+    @staticmethod
+    def widen(d):
+        """BF16: the 4 values in v[d], v[d+1] (two per dword, low half first)
+        widened to FP32 in v[d..d+3], exactly (the BF16 bits become the FP32
+        high half).  Outside VGPR-index mode, which would offset the
+        destinations."""
+        if BF16:
+            e(f"v_and_b32_e32 v{d+3}, 0xffff0000, v{d+1}")
+            e(f"v_lshlrev_b32_e32 v{d+2}, 16, v{d+1}")
+            e(f"v_and_b32_e32 v{d+1}, 0xffff0000, v{d}")
+            e(f"v_lshlrev_b32_e32 v{d}, 16, v{d}")
+
     # ---- batch loop (entries from 16 on) ----
     def reads(self, S, k):
         for u in range(k):
             e(f"v_mad_u32_u16 v{AV+u}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
-            e(f"ds_read_b128 v[{DV+4*u}:{DV+4*u+3}], v{AV+u}")
+            self.dread(DV + 4 * u, AV + u)
 
     @staticmethod
     def entry(S, u):
@@ -111,6 +142,8 @@ class Body:
 
     def full_steps(self, S):
         for h in range(2):
+            for q in range(4):
+                self.widen(DV + 4 * (4 * h + q))
             e(f"s_set_gpr_idx_on s{S[8*h]}, gpr_idx(SRC2,DST)")
             for q in range(4):
                 if q:
@@ -121,14 +154,21 @@ class Body:
     def tail(self, S, t):
         p = self.p
         e(f"v_mad_u32_u16 v{AV}, s{S[0]}, 16, {self.base} op_sel:[1,0,0,0]")
-        e(f"ds_read_b128 v[{DV}:{DV+3}], v{AV}")
+        self.dread(DV, AV)
         for u in range(1, 7):
             e(f"s_cmp_gt_u32 s{SREM}, {u}")
             e(f"s_cbranch_scc0 .Lgw{t}{p}_%=")
             e(f"v_mad_u32_u16 v{AV+u}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
-            e(f"ds_read_b128 v[{DV+4*u}:{DV+4*u+3}], v{AV+u}")
+            self.dread(DV + 4 * u, AV + u)
         e(f".Lgw{t}{p}_%=:")
         e("s_waitcnt lgkmcnt(0)")
+        if BF16:
+            self.widen(DV)
+            for u in range(1, 7):
+                e(f"s_cmp_gt_u32 s{SREM}, {u}")
+                e(f"s_cbranch_scc0 .Lgv{t}{p}_%=")
+                self.widen(DV + 4 * u)
+            e(f".Lgv{t}{p}_%=:")
         e(f"s_set_gpr_idx_on s{S[0]}, gpr_idx(SRC2,DST)")
         self.entry(S, 0)
         for u in range(1, 7):
@@ -179,7 +219,7 @@ class Body:
         for q in range(4):
             sx, _ = self.group_entry(g, q)
             e(f"v_mad_u32_u16 v{G_AV[b]+q}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
-            e(f"ds_read_b128 v[{G_D[b]+4*q}:{G_D[b]+4*q+3}], v{G_AV[b]+q}")
+            self.dread(G_D[b] + 4 * q, G_AV[b] + q)
 
     def group_entry_steps(self, g, q):
         _, sv = self.group_entry(g, q)
@@ -188,6 +228,8 @@ class Body:
             e(f"v_fma_f32 v{R*c}, s{sv}, v{G_D[b]+4*q+c}, v{R*c}")
 
     def group_steps_full(self, g):
+        for q in range(4):
+            self.widen(G_D[g % 2] + 4 * q)
         sx, _ = self.group_entry(g, 0)
         e(f"s_set_gpr_idx_on s{sx}, gpr_idx(SRC2,DST)")
         for q in range(4):
@@ -198,6 +240,13 @@ class Body:
 
     def group_steps_guarded(self, g, tag):
         # s{SCNT} = valid entries of this group (1..4)
+        if BF16:
+            self.widen(G_D[g % 2])
+            for q in range(1, 4):
+                e(f"s_cmp_gt_u32 s{SCNT}, {q}")
+                e(f"s_cbranch_scc0 .Lgv{tag}{self.p}_%=")
+                self.widen(G_D[g % 2] + 4 * q)
+            e(f".Lgv{tag}{self.p}_%=:")
         sx, _ = self.group_entry(g, 0)
         e(f"s_set_gpr_idx_on s{sx}, gpr_idx(SRC2,DST)")
         self.group_entry_steps(g, 0)
@@ -218,7 +267,7 @@ class Body:
                 e(f"s_cmp_gt_u32 s{SCNT}, {q}")
                 e(f"s_cbranch_scc0 .L{tag}{self.p}_%=")
             e(f"v_mad_u32_u16 v{G_AV[b]+q}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
-            e(f"ds_read_b128 v[{G_D[b]+4*q}:{G_D[b]+4*q+3}], v{G_AV[b]+q}")
+            self.dread(G_D[b] + 4 * q, G_AV[b] + q)
         e(f".L{tag}{self.p}_%=:")
 
     def roll16(self):
@@ -294,7 +343,20 @@ class Body:
         for i in range(15):
             e(f"s_cmp_lt_u32 s{ST}, %[kb]")
             e(f"s_cbranch_scc0 .Lsd{p}_%=")
-            e(f"buffer_load_dwordx4 %[voff], %[rsrc], s{ST} offen lds")
+            if BF16:
+                # a load holds two D rows (lanes 32..63 the second): when
+                # the second is past K those lanes take %[voffz], out of
+                # the buffer's range, and read zeros
+                e(f"s_add_u32 s{SCNT}, s{ST}, %[row1]")
+                e(f"s_cmp_lt_u32 s{SCNT}, %[kb]")
+                e(f"s_cbranch_scc0 .Lsz{i}{p}_%=")
+                e(f"buffer_load_dwordx4 %[voff], %[rsrc], s{ST} offen lds")
+                e(f"s_branch .Lsn{i}{p}_%=")
+                e(f".Lsz{i}{p}_%=:")
+                e(f"buffer_load_dwordx4 %[voffz], %[rsrc], s{ST} offen lds")
+                e(f".Lsn{i}{p}_%=:")
+            else:
+                e(f"buffer_load_dwordx4 %[voff], %[rsrc], s{ST} offen lds")
             e("s_add_u32 m0, m0, 0x400")
             e(f"s_add_u32 s{ST}, s{ST}, %[rowb]")
         e(f".Lsd{p}_%=:")
@@ -352,9 +414,9 @@ class Body:
         e("s_cbranch_scc0 .Lx_%=")
 
 
-def generate(rows):
+def generate(rows, bf16=False):
     """The asm lines of the segment block for `rows` rows per wave."""
-    configure(rows)
+    configure(rows, bf16)
     L.clear()
     body0, body1 = Body(0), Body(1)
     # prologue: the segment's first chunk, its entries loaded before its barrier
@@ -382,21 +444,25 @@ def main():
     print("// A segment of chunks of the FP32 panel multiply for one wave: per chunk the")
     print("// barrier, the next chunk's entry loads and staging, and the entry loop.")
     print("#pragma once")
-    for rows in ROWS:
-        lines = generate(rows)
-        print(f"// {rows} rows per wave: accumulators v0..v{4 * rows - 1}, component c of row slot q")
+    for rows, bf16 in [(r, False) for r in ROWS] + [(r, True) for r in ROWS_BF16]:
+        lines = generate(rows, bf16)
+        tag = f"{'BF16_' if bf16 else ''}R{rows}"
+        print(f"// {'BF16' if bf16 else 'FP32'} D, {rows} rows per wave: accumulators "
+              f"v0..v{4 * rows - 1}, component c of row slot q")
         print(f"// in v[{rows} c + q]; scratch v{4 * rows}..v{4 * rows + 39}")
-        print(f"#define GCOO_PANEL_SEGMENT_ASM_R{rows} \\")
+        print(f"#define GCOO_PANEL_SEGMENT_ASM_{tag} \\")
         for i, x in enumerate(lines):
             end = " \\" if i + 1 < len(lines) else ""
             print(f'    "{x}\\n\\t"{end}')
-        print(f"#define GCOO_PANEL_SEGMENT_CLOBBERS_R{rows} \\")
+        print(f"#define GCOO_PANEL_SEGMENT_CLOBBERS_{tag} \\")
         regs = ([f'"v{i}"' for i in range(4 * rows, 4 * rows + 40)] +
                 [f'"s{i}"' for i in range(24, 31)] +
                 [f'"s{i}"' for i in range(36, 100)] + ['"scc"', '"memory"'])
         for i in range(0, len(regs), 12):
             end = ", \\" if i + 12 < len(regs) else ""
             print("    " + ", ".join(regs[i:i+12]) + end)
+        if bf16:
+            continue
         ops = [f'"+{{v[{8 * i}:{8 * i + 7}]}}"(a[{i}])' for i in range(rows // 2)]
         print(f"#define GCOO_PANEL_ACC_OPERANDS_R{rows}(a) \\")
         for i in range(0, len(ops), 4):

@@ -252,15 +252,18 @@ def test_row_segment_rounds_bit_exact(dev, rows_per_wave, unroll, M, K, N, dens)
     (256, 10000, 512, 0.01),   # 134 chunks: three 64-chunk windows
     (300, 80000, 256, 0.001),  # 1067 chunks: list counts beyond LDS
     (64, 2000, 1030, 0.05),    # N * 4 not a multiple of 16: row-segment fallback
+    (300, 301, 256, 0.05),     # odd K: BF16's last staged row pair straddles K
 ])
-@pytest.mark.parametrize("waves", [16, 12, 8])
-def test_panel_variant_shapes(dev, M, K, N, dens, waves):
+@pytest.mark.parametrize("dtype,waves", [("f32", 16), ("f32", 12), ("f32", 8),
+                                         ("bf16", 16), ("bf16", 12)])
+def test_panel_variant_shapes(dev, M, K, N, dens, dtype, waves):
     """The panel-staged multiply (variant 6; workgroups of 16 waves x 16
-    rows, 12 x 28 or 8 x 42) across panel, chunk and tile edges gives the
-    reference's bits, and follows re-conversions of the handle (its panel
-    lists are rebuilt)."""
+    rows, 12 x 28 or 8 x 42; FP32 D in 75-row chunks, BF16 D in 150-row
+    chunks) across panel, chunk and tile edges gives the reference's bits
+    (BF16: the chain over BF16-widened operands), and follows re-conversions
+    of the handle (its panel lists are rebuilt)."""
     import torch
-    print(f"panel waves={waves} M={M} K={K} N={N} dens={dens}", flush=True)
+    print(f"panel {dtype} waves={waves} M={M} K={K} N={N} dens={dens}", flush=True)
     rng = np.random.default_rng(M * 7 + K + N)
     m = G.GCOOMatrix(M, K, 4).set_config(variant=6, waves=waves)
     for d in (dens, dens / 2):
@@ -270,8 +273,13 @@ def test_panel_variant_shapes(dev, M, K, N, dens, waves):
         g = oracle.convert(S, 4)
         m.convert_dense(to_dev(S, dev))
         C = torch.full((M, N), float("nan"), device=dev)
-        m.spdm(to_dev(D, dev), C)
-        assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+        if dtype == "f32":
+            m.spdm(to_dev(D, dev), C)
+            want = oracle.spdm_f32(g, D)
+        else:
+            m.spdm_bf16(G.to_bf16(to_dev(D, dev)), C)
+            want = oracle.spdm_bf16(g, oracle.to_bf16(g["vals"]), oracle.to_bf16(D))
+        assert H.bits_equal(C.cpu().numpy(), want)
 
 
 def test_row_segments_match_gcoo(dev):

@@ -34,10 +34,14 @@ ROWB = 32000
 ROWS = (16, 28, 42)
 
 
-def _instructions(rows=16):
+ROWS_BF16 = (16, 28)
+
+
+def _instructions(rows=16, bf16=False):
     text = open(HDR).read()
-    body = text.split(f"#define GCOO_PANEL_SEGMENT_ASM_R{rows} ", 1)[1].split(
-        f"#define GCOO_PANEL_SEGMENT_CLOBBERS_R{rows} ", 1)[0]
+    tag = f"{'BF16_' if bf16 else ''}R{rows}"
+    body = text.split(f"#define GCOO_PANEL_SEGMENT_ASM_{tag} ", 1)[1].split(
+        f"#define GCOO_PANEL_SEGMENT_CLOBBERS_{tag} ", 1)[0]
     lines = re.findall(r'"([^"]*)\\n\\t"', body)
     return [ln.strip() for ln in lines if ln.strip()]
 
@@ -203,9 +207,36 @@ class Lane:
                 for c in range(4):
                     self.v[regs[c]] = ("d", addr, c)
                 self.inflight.append(("lds", {("v", r) for r in regs}))
+            elif op == "ds_read_b64":
+                # two dwords of BF16 pairs: dword i holds columns 2i (low
+                # half) and 2i + 1 (high half)
+                kind, regs = _regs(args[0])
+                assert len(regs) == 2
+                akind, addr = self.vread(args[1])
+                assert akind == "addr"
+                self._check("v", regs, "load into")
+                for i in range(2):
+                    self.v[regs[i]] = ("d64", addr, i)
+                self.inflight.append(("lds", {("v", r) for r in regs}))
+            elif op in ("v_and_b32_e32", "v_lshlrev_b32_e32"):
+                # BF16 widening: mask keeps the high half (odd column), a
+                # shift by 16 moves the low half up (even column)
+                assert not self.idx_on, "widening in index mode"
+                d = int(args[0][1:])
+                kind, addr, i = self.vread(args[2])
+                assert kind == "d64", ln
+                if op == "v_and_b32_e32":
+                    assert args[1] == "0xffff0000", ln
+                    col = 2 * i + 1
+                else:
+                    assert args[1] == "16", ln
+                    col = 2 * i
+                self._check("v", [d], "write")
+                self.v[d] = ("d", addr, col)
             elif op == "buffer_load_dwordx4":
-                assert args[0] == "%[voff]" and args[1] == "%[rsrc]"
-                self.stage.append((self.barriers, self.m0, self.sval(args[2].split()[0])))
+                assert args[0] in ("%[voff]", "%[voffz]") and args[1] == "%[rsrc]"
+                self.stage.append((self.barriers, self.m0, self.sval(args[2].split()[0]),
+                                   args[0][2:-1]))
             elif op == "v_fma_f32":
                 assert self.idx_on, "fma outside index mode"
                 d = int(args[0][1:]) + self.idx
@@ -230,17 +261,19 @@ def word(offset, slot):
     return (offset // 16) << 16 | slot
 
 
-def _prog(rows=16):
-    if rows not in PROG:
-        PROG[rows] = _instructions(rows)
-    return PROG[rows]
+def _prog(rows=16, bf16=False):
+    if (rows, bf16) not in PROG:
+        PROG[rows, bf16] = _instructions(rows, bf16)
+    return PROG[rows, bf16]
 
 
-def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16):
+def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16, bf16=False):
     """Run one segment: chunks c0 .. c0 + len(counts) - 1 of the window at
     cw, with counts[i] entries in chunk c0 + i (the window's other chunks get
     random counts).  Returns the lane, the per-chunk entries and the inputs."""
     rng = random.Random(seed)
+    # chunk rows and staged bytes per D row: FP32 75 x 1 KB, BF16 150 x 512 B
+    tk, rowd = (150, 512) if bf16 else (75, 1024)
     nchunks = c0 - cw + len(counts)
     assert nchunks <= 64
     per = [rng.randrange(20) for _ in range(c0 - cw)] + list(counts)
@@ -250,19 +283,21 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16):
         vpbo.append(len(entries) * 8)
         vpn.append(n)
         slots = sorted(rng.randrange(rows) for _ in range(n))
-        ch = [(word(rng.randrange(75) * 1024, slot), rng.getrandbits(32)) for slot in slots]
+        ch = [(word(rng.randrange(tk) * rowd, slot), rng.getrandbits(32)) for slot in slots]
         chunks.append(ch)
         entries += ch
-    entries += [(word(rng.randrange(75) * 1024, rng.randrange(rows)), rng.getrandbits(32))
+    entries += [(word(rng.randrange(tk) * rowd, rng.randrange(rows)), rng.getrandbits(32))
                 for _ in range(24)]
     wave = 2
+    rpl = 2 if bf16 else 1                 # D rows per wave load
+    rowb = rpl * ROWB                      # D bytes a load advances
     kb = K * ROWB if stager else 0
-    lim = max(kb - 15 * ROWB, 0) if stager else 0
-    sso = ((c0 + 1) * 75 + wave * 15) * ROWB
+    lim = max(kb - 15 * rowb, 0) if stager else 0
+    sso = ((c0 + 1) * tk + wave * 15 * rpl) * ROWB
     ops = {"pbase": 0, "sli": c0 - cw, "liend": c0 - cw + len(counts), "par": c0 & 1,
            "vb0": 0x40, "vb1": 0x40 + CHUNKB, "sdst": wave * 15 * 1024, "sso": sso,
-           "lim": lim, "kb": kb, "rowb": ROWB, "tkrowb": 75 * ROWB}
-    lane = Lane(_prog(rows), entries, ops, {"vpbo": vpbo, "vpn": vpn}).run()
+           "lim": lim, "kb": kb, "rowb": rowb, "row1": ROWB, "tkrowb": tk * ROWB}
+    lane = Lane(_prog(rows, bf16), entries, ops, {"vpbo": vpbo, "vpn": vpn}).run()
     return lane, chunks[c0 - cw:c0 - cw + len(counts)], ops
 
 
@@ -320,9 +355,53 @@ def test_staging_loads(K, rows):
         dst = ops["sdst"] + (c1 % 2) * CHUNKB
         for r in range(15):
             if k0 + r < K:
-                want.append((i + 1, dst + 1024 * r, (k0 + r) * ROWB))
+                want.append((i + 1, dst + 1024 * r, (k0 + r) * ROWB, "voff"))
     assert lane.stage == want
     assert lane.ops["sso"] == ops["sso"] + len(counts) * 75 * ROWB
+
+
+@pytest.mark.parametrize("rows", ROWS_BF16)
+@pytest.mark.parametrize("K", [8000, 8001, 300, 301, 161, 150 * 3 + 33])
+def test_staging_loads_bf16(K, rows):
+    """BF16: a wave load holds two D rows; a pair with its first row below K
+    is loaded, and with its second row at K or beyond it takes the
+    out-of-range lane offsets (odd K)."""
+    counts = [5, 0, 17, 2, 9, 30]
+    lane, _, ops = _segment(counts, c0=0, K=K, seed=K, rows=rows, bf16=True)
+    wave = 2
+    want = []
+    for i in range(len(counts)):
+        c1 = i + 1
+        k0 = c1 * 150 + wave * 30
+        dst = ops["sdst"] + (c1 % 2) * CHUNKB
+        for r in range(15):
+            k = k0 + 2 * r
+            if k < K:
+                want.append((i + 1, dst + 1024 * r, k * ROWB, "voff" if k + 1 < K else "voffz"))
+    assert lane.stage == want
+    assert lane.ops["sso"] == ops["sso"] + len(counts) * 150 * ROWB
+
+
+@pytest.mark.parametrize("rows", ROWS_BF16)
+@pytest.mark.parametrize("n", [0, 1, 3, 4, 5, 7, 8, 11, 12, 15, 16, 17, 23, 24, 25, 31, 40, 47])
+def test_bf16_steps_follow_the_list(n, rows):
+    for c0 in (0, 1):
+        lane, chunks, ops = _segment([n], c0=c0, cw=0, seed=n, rows=rows, bf16=True)
+        assert lane.barriers == 1
+        _check_steps(lane, chunks, c0, ops, rows)
+
+
+@pytest.mark.parametrize("rows", ROWS_BF16)
+@pytest.mark.parametrize("seed", range(6))
+def test_bf16_segment_steps_follow_the_list(seed, rows):
+    rng = random.Random(200 + seed)
+    cw = rng.choice([0, 64])
+    c0 = cw + rng.randrange(8)
+    counts = [rng.choice([0, 1, 3, 4, 9, 16, 17, 24, 31, 40]) for _ in range(rng.randrange(1, 10))]
+    lane, chunks, ops = _segment(counts, c0=c0, cw=cw, seed=seed, rows=rows, bf16=True)
+    assert lane.barriers == len(counts)
+    _check_steps(lane, chunks, c0, ops, rows)
+    assert lane.m0 == 0x1234
 
 
 @pytest.mark.parametrize("rows", ROWS)
