@@ -186,27 +186,55 @@ __device__ __forceinline__ void run_row_wave(const JobDesc &jd, __amdgpu_buffer_
     }
 }
 
-__global__ __launch_bounds__(64) void batch_reset_kernel(unsigned *counter) { *counter = 0; }
+// One item queue per XCD: queue x holds the items [start[x], start[x + 1]),
+// a contiguous run of whole jobs (mostly) chosen so that the queues carry
+// about equal work, and its counter sits on a cache line of its own.
+struct BatchQueues {
+    int start[kNumXcd + 1];
+};
+constexpr int kCounterStride = 32;  // unsigned ints between two queue counters
 
+__global__ __launch_bounds__(64) void batch_reset_kernel(unsigned *counter)
+{
+    if (threadIdx.x < kNumXcd) counter[threadIdx.x * kCounterStride] = 0;
+}
+
+// Workgroup b runs on XCD b % kNumXcd (the hardware dispatches workgroups
+// round-robin over the XCDs) and takes items from that XCD's queue first, so
+// a job's D and C tiles are fetched into one XCD's L2 instead of all eight;
+// when its queue is empty it takes items from the others in turn.
 __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
     const JobDesc *__restrict__ jobs, const int *__restrict__ itemStart, int nJobs,
-    int totalItems, unsigned *__restrict__ counter)
+    BatchQueues queues, unsigned *__restrict__ counter)
 {
     const int lane = __lane_id();
-    for (;;) {
+    const int home = blockIdx.x % kNumXcd;
+    // the job of the last item (its item range cached in scalar registers)
+    int lo = 0, jbeg = 0, jend = 0;
+    for (int qi = 0; qi < kNumXcd; ++qi) {
+        const int q = (home + qi) % kNumXcd;
+        const int qbeg = queues.start[q], qend = queues.start[q + 1];
+        if (qbeg >= qend) continue;
+        for (;;) {
         int it = 0;
-        if (lane == 0) it = (int)atomicAdd(counter, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);
-        if (it >= totalItems) return;
-        // job = last index with itemStart[job] <= it (scalar binary search)
-        int lo = 0, hi = nJobs - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (itemStart[mid] <= it) lo = mid;
-            else hi = mid - 1;
+        if (lane == 0) it = (int)atomicAdd(&counter[q * kCounterStride], 1u);
+        it = qbeg + __builtin_amdgcn_readfirstlane(it);
+        if (it >= qend) break;
+        if (it < jbeg || it >= jend) {
+            // job = last index with itemStart[job] <= it (scalar binary
+            // search, only when the item leaves the cached job)
+            lo = 0;
+            int hi = nJobs - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (itemStart[mid] <= it) lo = mid;
+                else hi = mid - 1;
+            }
+            jbeg = itemStart[lo];
+            jend = itemStart[lo + 1];
         }
         const JobDesc jd = jobs[lo];
-        const int local = it - itemStart[lo];
+        const int local = it - jbeg;
         const int rb = local / jd.colTiles;
         const int ct = local - rb * jd.colTiles;
         const int sub = lane / jd.lanes;
@@ -235,6 +263,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
             default: run_row<8>(jd, rsrc, r, j, active); break;
             }
         }
+        }
     }
 }
 
@@ -253,6 +282,7 @@ struct gcoo_batch {
     std::vector<uint64_t> gens;
     // resident workgroups of the persistent kernel, queried once at creation
     int cus = 256, perCu = 4;
+    BatchQueues queues{};          // per-XCD item ranges, set at commit
     JobDesc *d_jobs = nullptr;
     int *d_itemStart = nullptr;
     unsigned *d_counter = nullptr;
@@ -303,7 +333,7 @@ int enqueue_run(gcoo_batch *b, hipStream_t s)
     GCOO_CHECK_LAUNCH("batch_reset_kernel");
     const int nJobs = (int)b->jobs.size();
     hipLaunchKernelGGL(batch_persistent_kernel, dim3(b->gridBlocks), dim3(kBatchThreads), 0, s,
-                       b->d_jobs, b->d_itemStart, nJobs, b->itemStart[nJobs], b->d_counter);
+                       b->d_jobs, b->d_itemStart, nJobs, b->queues, b->d_counter);
     GCOO_CHECK_LAUNCH("batch_persistent_kernel");
     return GCOO_OK;
 }
@@ -317,7 +347,7 @@ int gcoo_batch_create(gcoo_batch_t *out)
     if (!out) return set_error(GCOO_ERR_INVALID, "null batch out");
     auto *b = new (std::nothrow) gcoo_batch;
     if (!b) return set_error(GCOO_ERR_ALLOC, "batch");
-    if (hipMalloc(&b->d_counter, 256) != hipSuccess) {
+    if (hipMalloc(&b->d_counter, sizeof(unsigned) * kCounterStride * kNumXcd) != hipSuccess) {
         (void)hipGetLastError();
         delete b;
         return set_error(GCOO_ERR_ALLOC, "batch counter");
@@ -478,12 +508,43 @@ int gcoo::batch_commit_async(gcoo_batch_t b, gcoo_stream stream)
                                     hipMemcpyHostToDevice, s));
     }
     const long long items = b->itemStart.back();
-    // persistent: as many workgroups as can be resident at once
+    // persistent: as many workgroups as can be resident at once, a multiple
+    // of the XCD count so that every XCD has the same number
     long long blocks = (long long)b->cus * b->perCu;
     const long long needed = (items + kBatchWaves - 1) / kBatchWaves;
     if (blocks > needed) blocks = needed;
-    if (blocks < 1) blocks = 1;
+    blocks = (blocks + kNumXcd - 1) / kNumXcd * kNumXcd;
     b->gridBlocks = (int)blocks;
+    // split the items into kNumXcd contiguous ranges of about equal work: a
+    // job's items share its estimated work (one FMA per entry and column, one
+    // store per output element) equally
+    {
+        double total = 0;
+        std::vector<double> w(b->jobs.size());
+        for (size_t i = 0; i < b->jobs.size(); ++i) {
+            const JobDesc &jd = b->jobs[i];
+            w[i] = (double)b->handles[i]->nnz * jd.N + (double)jd.M * jd.N;
+            total += w[i];
+        }
+        b->queues.start[0] = 0;
+        double acc = 0;
+        int x = 1;
+        for (size_t i = 0; i < b->jobs.size() && x < kNumXcd; ++i) {
+            const int i0 = b->itemStart[i], i1 = b->itemStart[i + 1];
+            const double per = i1 > i0 ? w[i] / (i1 - i0) : 0.0;
+            // place the boundaries that fall inside this job's items
+            while (x < kNumXcd && acc + w[i] >= total * x / kNumXcd) {
+                const double need = total * x / kNumXcd - acc;
+                int cut = per > 0 ? i0 + (int)(need / per + 0.5) : i0;
+                cut = std::min(std::max(cut, i0), i1);
+                b->queues.start[x] = std::max(cut, b->queues.start[x - 1]);
+                ++x;
+            }
+            acc += w[i];
+        }
+        for (; x <= kNumXcd; ++x) b->queues.start[x] = (int)items;
+        b->queues.start[kNumXcd] = (int)items;
+    }
     b->committed = true;
     destroy_graph(b);
     return GCOO_OK;
