@@ -123,6 +123,22 @@ class Lane:
         return self.v[r]
 
     def run(self, limit=400000):
+        for _ in self.execute(limit):
+            pass
+        return self
+
+    # hooks of the multi-wave model (tests/test_panel_asm_waves.py); a single
+    # lane has no other waves and no memory timing
+    def on_dma(self, m0, soff, voff):
+        pass
+
+    def on_lds_read(self, addr, nbytes):
+        pass
+
+    def execute(self, limit=400000):
+        """Run the block; yields ("barrier",) at each s_barrier (before
+        passing it), ("vmcnt", n) at each vector-memory wait, and ("step",)
+        after every other instruction, so a scheduler can interleave waves."""
         pc = 0
         steps = 0
         while pc < len(self.prog):
@@ -137,12 +153,15 @@ class Lane:
             if op in ("s_setprio", "s_nop"):
                 continue
             if op == "s_barrier":
+                yield ("barrier",)
                 self.barriers += 1
             elif op == "s_waitcnt":
                 m = re.fullmatch(r"(vmcnt|lgkmcnt)\((\d+)\)", rest.strip())
                 assert m, ln
                 if m.group(1) == "lgkmcnt":
                     self.wait_lgkm(int(m.group(2)))
+                else:
+                    yield ("vmcnt", int(m.group(2)))
             elif op in ("s_mov_b32", "s_movk_i32"):
                 self.setS(args[0], self.sval(args[1]))
             elif op in ("s_add_u32", "s_sub_u32"):
@@ -204,6 +223,7 @@ class Lane:
                 akind, addr = self.vread(args[1])
                 assert akind == "addr"
                 self._check("v", regs, "load into")
+                self.on_lds_read(addr, 16)
                 for c in range(4):
                     self.v[regs[c]] = ("d", addr, c)
                 self.inflight.append(("lds", {("v", r) for r in regs}))
@@ -215,6 +235,7 @@ class Lane:
                 akind, addr = self.vread(args[1])
                 assert akind == "addr"
                 self._check("v", regs, "load into")
+                self.on_lds_read(addr, 8)
                 for i in range(2):
                     self.v[regs[i]] = ("d64", addr, i)
                 self.inflight.append(("lds", {("v", r) for r in regs}))
@@ -235,8 +256,9 @@ class Lane:
                 self.v[d] = ("d", addr, col)
             elif op == "buffer_load_dwordx4":
                 assert args[0] in ("%[voff]", "%[voffz]") and args[1] == "%[rsrc]"
-                self.stage.append((self.barriers, self.m0, self.sval(args[2].split()[0]),
-                                   args[0][2:-1]))
+                soff = self.sval(args[2].split()[0])
+                self.stage.append((self.barriers, self.m0, soff, args[0][2:-1]))
+                self.on_dma(self.m0, soff, args[0][2:-1])
             elif op == "v_fma_f32":
                 assert self.idx_on, "fma outside index mode"
                 d = int(args[0][1:]) + self.idx
@@ -247,9 +269,9 @@ class Lane:
                 self.fmas.append((self.barriers, d, self.sval(args[1]), src))
             else:
                 raise AssertionError("unhandled " + ln)
+            yield ("step",)
         assert not self.idx_on
         assert not self.inflight, "loads in flight at the block's end"
-        return self
 
 
 PROG = {}
