@@ -394,8 +394,8 @@ def serve_case():
 
 
 def profile_kernel(n, s, seed, iters):
-    """A few launches of the dominant kernel and of a copy kernel of known
-    size, for rocprofv3 counter runs (no JSON)."""
+    """A few launches of the dominant kernel (FP32, then the BF16 form) and
+    of a copy kernel of known size, for rocprofv3 counter runs (no JSON)."""
     import torch
     import gcoospdm_amd as G
     dev = torch.device("cuda:0")
@@ -407,6 +407,9 @@ def profile_kernel(n, s, seed, iters):
     m = G.GCOOMatrix(n, n, 4).convert_dense(S)
     for _ in range(iters):
         m.spdm(D, C, sync=False)
+    D16 = G.to_bf16(D)
+    for _ in range(iters):
+        m.spdm_bf16(D16, C, sync=False)
     X = torch.empty(64 << 20, device=dev)   # 256 MiB
     Y = torch.empty_like(X)
     for _ in range(iters):
