@@ -45,37 +45,53 @@ struct JobDesc {
 __device__ __forceinline__ float lo16(unsigned w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float hi16(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
 
-// Up to 8 widened elements at byte offset voff (per lane) of the buffer.
-__device__ __forceinline__ void load_elems(float (&d)[8], __amdgpu_buffer_rsrc_t r, int voff,
-                                           int bytes, int vec)
+// VEC widened elements of BYTES bytes each at byte offset voff (per lane) +
+// soff (wave-uniform) of the buffer; out-of-range lanes read zeros.  The
+// width is a template parameter, so the entry loops carry no branches on it.
+template <int BYTES, int VEC>
+__device__ __forceinline__ void load_elems(float (&d)[VEC], __amdgpu_buffer_rsrc_t r, int voff,
+                                           int soff)
 {
-    const int B = bytes * vec;
-    if (B == 16) {
-        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-        if (bytes == 4) {
-            d[0] = __uint_as_float(x[0]); d[1] = __uint_as_float(x[1]);
-            d[2] = __uint_as_float(x[2]); d[3] = __uint_as_float(x[3]);
+    constexpr int B = BYTES * VEC;
+    static_assert(B == 2 || B == 4 || B == 8 || B == 16, "at most 16 bytes per lane");
+    if constexpr (B == 16) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+        if constexpr (BYTES == 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d[i] = __uint_as_float(x[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                d[2 * i] = lo16(x[i]);
+                d[2 * i + 1] = hi16(x[i]);
+            }
+        }
+    } else if constexpr (B == 8) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        if constexpr (BYTES == 4) {
+            d[0] = __uint_as_float(x[0]);
+            d[1] = __uint_as_float(x[1]);
         } else {
             d[0] = lo16(x[0]); d[1] = hi16(x[0]); d[2] = lo16(x[1]); d[3] = hi16(x[1]);
-            d[4] = lo16(x[2]); d[5] = hi16(x[2]); d[6] = lo16(x[3]); d[7] = hi16(x[3]);
         }
-    } else if (B == 8) {
-        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
-        if (bytes == 4) {
-            d[0] = __uint_as_float(x[0]); d[1] = __uint_as_float(x[1]);
-        } else {
-            d[0] = lo16(x[0]); d[1] = hi16(x[0]); d[2] = lo16(x[1]); d[3] = hi16(x[1]);
-        }
-    } else if (B == 4) {
-        const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(r, voff, 0, 0);
-        if (bytes == 4) {
+    } else if constexpr (B == 4) {
+        const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        if constexpr (BYTES == 4) {
             d[0] = __uint_as_float(x);
         } else {
-            d[0] = lo16(x); d[1] = hi16(x);
+            d[0] = lo16(x);
+            d[1] = hi16(x);
         }
     } else {
-        d[0] = lo16(__builtin_amdgcn_raw_buffer_load_b16(r, voff, 0, 0));
+        d[0] = lo16(__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
     }
+}
+
+// An entry's value word as FP32 (BF16 segments hold the bits in the low half).
+template <int BYTES>
+__device__ __forceinline__ float entry_val(int w)
+{
+    return BYTES == 2 ? __int_as_float(w << 16) : __int_as_float(w);
 }
 
 __device__ __forceinline__ void fmac_v(float &acc, float a, float d)
@@ -90,8 +106,8 @@ __device__ __forceinline__ void fmac_s(float &acc, float a, float d)
 }
 
 // One row stream of a work item: lane (sub, lc) walks row r's segment and
-// accumulates columns j .. j+vec-1.
-template <int VEC>
+// accumulates columns j .. j+vec-1, U entries' loads in flight at a time.
+template <int BYTES, int VEC>
 __device__ __forceinline__ void run_row(const JobDesc &jd, __amdgpu_buffer_rsrc_t rsrc, int r,
                                         int j, bool active)
 {
@@ -104,25 +120,24 @@ __device__ __forceinline__ void run_row(const JobDesc &jd, __amdgpu_buffer_rsrc_
     const int pend = valid ? jd.rowptr[r + 1] : 0;
     // byte offsets are unsigned: a lane past N starts at dBytes, so every one
     // of its loads is out of range and returns zeros
-    const unsigned rowBytes = (unsigned)jd.N * (unsigned)jd.bytes;
-    const unsigned voffc = active ? (unsigned)j * (unsigned)jd.bytes : jd.dBytes;
-    const bool bf = jd.bytes == 2;
+    const unsigned rowBytes = (unsigned)jd.N * BYTES;
+    const unsigned voffc = active ? (unsigned)j * BYTES : jd.dBytes;
     while (__builtin_amdgcn_ballot_w64(p < pend) != 0) {
         int2 e[U];
 #pragma unroll
         for (int u = 0; u < U; ++u)
             e[u] = (p + u < pend) ? jd.seg[p + u] : make_int2(0, 0);
-        float d[U][8];
+        float d[U][VEC];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const unsigned voff =
                 (p + u < pend) ? voffc + (unsigned)e[u].x * rowBytes : jd.dBytes;
-            load_elems(d[u], rsrc, (int)voff, jd.bytes, VEC);
+            load_elems<BYTES, VEC>(d[u], rsrc, (int)voff, 0);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (p + u < pend) {
-                const float a = bf ? __int_as_float(e[u].y << 16) : __int_as_float(e[u].y);
+                const float a = entry_val<BYTES>(e[u].y);
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) fmac_v(acc[v], a, d[u][v]);
             }
@@ -138,11 +153,12 @@ __device__ __forceinline__ void run_row(const JobDesc &jd, __amdgpu_buffer_rsrc_
 
 // One row per wave (64-lane streams): the row is wave-uniform, so its entries
 // are read through the constant address space (scalar loads; the segments
-// are not written during the batch), and rounds of R entries (16, or 8 at 8
-// columns per lane, so the D values take at most 64 registers) issue all
-// their D loads before the first FMA, then rounds of 8, then one guarded
+// are not written during the batch) and an entry's D row offset is a scalar
+// buffer offset.  Rounds of R entries (16, or 8 at 8 columns per lane, so
+// the D values take at most 64 registers) load all their entries, then issue
+// all their D loads, then run the FMAs; then rounds of 8, then one guarded
 // round of the last 1..7 -- the same chain order, so the same bits.
-template <int VEC>
+template <int BYTES, int VEC>
 __device__ __forceinline__ void run_row_wave(const JobDesc &jd, __amdgpu_buffer_rsrc_t rsrc,
                                              int r, int j, bool active)
 {
@@ -153,36 +169,66 @@ __device__ __forceinline__ void run_row_wave(const JobDesc &jd, __amdgpu_buffer_
     const bool valid = r < jd.M;
     int p = __builtin_amdgcn_readfirstlane(valid ? jd.rowptr[r] : 0);
     const int pend = __builtin_amdgcn_readfirstlane(valid ? jd.rowptr[r + 1] : 0);
-    const unsigned rowBytes = (unsigned)jd.N * (unsigned)jd.bytes;
-    const unsigned voffc = active ? (unsigned)j * (unsigned)jd.bytes : jd.dBytes;
-    const bool bf = jd.bytes == 2;
+    const int rowBytes = jd.N * BYTES;
+    const int voffc = active ? j * BYTES : (int)jd.dBytes;
     cint *sg = (cint *)(jd.seg);  // (k, value bits) pairs; a C-style cast may change the address space
-    auto round = [&](auto bc, int q, int n) {
+    // B entries from p: unguarded (every one exists), so the entry loads merge
+    auto round = [&](auto bc, int q) {
         constexpr int B = decltype(bc)::value;
-        float a[B];
-        float d[B][8];
+        int ek[B], ev[B];
 #pragma unroll
         for (int u = 0; u < B; ++u) {
+            ek[u] = sg[2 * (q + u)];
+            ev[u] = sg[2 * (q + u) + 1];
+        }
+        float d[B][VEC];
+#pragma unroll
+        for (int u = 0; u < B; ++u) load_elems<BYTES, VEC>(d[u], rsrc, voffc, ek[u] * rowBytes);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            const float a = entry_val<BYTES>(ev[u]);
+#pragma unroll
+            for (int v = 0; v < VEC; ++v) fmac_s(acc[v], a, d[u][v]);
+        }
+    };
+    constexpr int R = VEC >= 8 ? 8 : 16;
+    for (; p + R <= pend; p += R) round(std::integral_constant<int, R>{}, p);
+    if constexpr (R > 8)
+        for (; p + 8 <= pend; p += 8) round(std::integral_constant<int, 8>{}, p);
+    if (p < pend) {
+        // the last 1..7 entries
+        const int n = pend - p;
+        float a[7];
+        float d[7][VEC];
+#pragma unroll
+        for (int u = 0; u < 7; ++u) {
             if (u < n) {
-                const int ek = sg[2 * (q + u)], ev = sg[2 * (q + u) + 1];
-                a[u] = bf ? __int_as_float(ev << 16) : __int_as_float(ev);
-                load_elems(d[u], rsrc, (int)(voffc + (unsigned)ek * rowBytes), jd.bytes, VEC);
+                a[u] = entry_val<BYTES>(sg[2 * (p + u) + 1]);
+                load_elems<BYTES, VEC>(d[u], rsrc, voffc, sg[2 * (p + u)] * rowBytes);
             }
         }
 #pragma unroll
-        for (int u = 0; u < B; ++u)
+        for (int u = 0; u < 7; ++u)
             if (u < n)
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) fmac_s(acc[v], a[u], d[u][v]);
-    };
-    constexpr int R = VEC >= 8 ? 8 : 16;
-    for (; p + R <= pend; p += R) round(std::integral_constant<int, R>{}, p, R);
-    for (; p + 8 <= pend; p += 8) round(std::integral_constant<int, 8>{}, p, 8);
-    if (p < pend) round(std::integral_constant<int, 8>{}, p, pend - p);
+    }
     if (valid && active) {
         float *dst = jd.C + (size_t)r * jd.N + j;
 #pragma unroll
         for (int v = 0; v < VEC; ++v) __builtin_nontemporal_store(acc[v], dst + v);
+    }
+}
+
+// The rows of one work item, in streams of jd.lanes lanes.
+template <int BYTES, int VEC>
+__device__ __forceinline__ void run_item(const JobDesc &jd, __amdgpu_buffer_rsrc_t rsrc, int r0,
+                                         int sub, int j, bool active, int streams)
+{
+    for (int q = 0; q < jd.rows; q += streams) {
+        const int r = r0 + q + sub;
+        if (streams == 1) run_row_wave<BYTES, VEC>(jd, rsrc, r, j, active);
+        else run_row<BYTES, VEC>(jd, rsrc, r, j, active);
     }
 }
 
@@ -245,23 +291,15 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<void *>(jd.D), (short)0, (int)jd.dBytes, 0x00020000);
         const int r0 = rb * jd.rows;
-        for (int q = 0; q < jd.rows; q += streams) {
-            const int r = r0 + q + sub;
-            if (streams == 1) {
-                switch (jd.vec) {
-                case 1: run_row_wave<1>(jd, rsrc, r, j, active); break;
-                case 2: run_row_wave<2>(jd, rsrc, r, j, active); break;
-                case 4: run_row_wave<4>(jd, rsrc, r, j, active); break;
-                default: run_row_wave<8>(jd, rsrc, r, j, active); break;
-                }
-                continue;
-            }
-            switch (jd.vec) {
-            case 1: run_row<1>(jd, rsrc, r, j, active); break;
-            case 2: run_row<2>(jd, rsrc, r, j, active); break;
-            case 4: run_row<4>(jd, rsrc, r, j, active); break;
-            default: run_row<8>(jd, rsrc, r, j, active); break;
-            }
+        // one dispatch per item on the element size and width
+        switch (jd.bytes * 16 + jd.vec) {
+        case 4 * 16 + 1: run_item<4, 1>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 4 * 16 + 2: run_item<4, 2>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 4 * 16 + 4: run_item<4, 4>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 2 * 16 + 1: run_item<2, 1>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 2 * 16 + 2: run_item<2, 2>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 2 * 16 + 4: run_item<2, 4>(jd, rsrc, r0, sub, j, active, streams); break;
+        default: run_item<2, 8>(jd, rsrc, r0, sub, j, active, streams); break;
         }
         }
     }
