@@ -101,18 +101,24 @@ def plain_config(N: int, per_row: float, bf16: bool, table: dict | None = None):
     return (e["variant"], e["vec"]) if e else (0, 0)
 
 
-def _time(fn, reps=5):
+def _time(fn, reps=5, trials=3, warm=5):
+    """Median over trials of the mean of reps launches, after warm untimed
+    launches (the clock runs slow for the first launches after an idle gap)."""
     import torch
-    fn()
+    for _ in range(warm):
+        fn()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        fn()
-    e1.record()
-    e1.synchronize()
-    return e0.elapsed_time(e1) / reps
+    ts = []
+    for _ in range(trials):
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1) / reps)
+    return sorted(ts)[len(ts) // 2]
 
 
 def tune_batch(jobs_by_class, bf16: bool, log):
@@ -225,6 +231,19 @@ def tune_skew(bf16: bool, log, device="cuda:0", seed=11):
     return {key: dict(cpg=best[1], ms=round(best[0], 4), **best[2])}
 
 
+def _jobs_by_class(specs, bf16: bool, per_class: int):
+    """Up to per_class materialised jobs of each (N, entries-per-row) class."""
+    from . import workloads as W
+    by_class = {}
+    for sp in specs:
+        key = class_key(sp.N, sp.K * (1.0 - sp.sparsity))
+        lst = by_class.setdefault(key, [])
+        if len(lst) < per_class:
+            m, D, C = W.materialize(sp, bf16=bf16)
+            lst.append((m, D, C, sp.N))
+    return by_class
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=DEFAULT_TABLE)
@@ -233,6 +252,8 @@ def main():
     ap.add_argument("--log", default=None)
     ap.add_argument("--skew-only", action="store_true",
                     help="rerun only the skew search, keeping the other tables")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="rerun only the batch tile search, keeping the other tables")
     args = ap.parse_args()
     from . import workloads as W
 
@@ -252,16 +273,20 @@ def main():
             json.dump(table, f, indent=1, sort_keys=True)
         print(f"wrote {args.out}", flush=True)
         return
-    table = {"device": "MI355X (gfx950)", "batch": {}, "plain": {}, "skew": {}}
     specs = W.batch_jobs(args.jobs, 2024)
+    if args.batch_only:
+        table = dict(load(args.out)) if os.path.exists(args.out) else {}
+        table["batch"] = {}
+        for bf16 in (False, True):
+            table["batch"]["bf16" if bf16 else "f32"] = tune_batch(
+                _jobs_by_class(specs, bf16, args.per_class), bf16, log)
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        print(f"wrote {args.out}", flush=True)
+        return
+    table = {"device": "MI355X (gfx950)", "batch": {}, "plain": {}, "skew": {}}
     for bf16 in (False, True):
-        by_class = {}
-        for sp in specs:
-            key = class_key(sp.N, sp.K * (1.0 - sp.sparsity))
-            lst = by_class.setdefault(key, [])
-            if len(lst) < args.per_class:
-                m, D, C = W.materialize(sp, bf16=bf16)
-                lst.append((m, D, C, sp.N))
+        by_class = _jobs_by_class(specs, bf16, args.per_class)
         table["batch"]["bf16" if bf16 else "f32"] = tune_batch(by_class, bf16, log)
         del by_class
         table["plain"]["bf16" if bf16 else "f32"] = tune_plain(bf16, log)
