@@ -10,9 +10,10 @@ reference's own results time (src/bench.py:9 KERNEL_ONLY=False); `value` is
 `roofline` is the multiply kernel alone (the dominant kernel of the step):
 algorithmic bytes 12*nnz + 8*(nGroup+1) + 4*K*N + 4*M*N per launch over its
 mean duration, measured with HIP events on the stream it is launched on,
-against MI355X's 8 TB/s.  `traffic` is the HBM bytes per launch measured by
-rocprofv3 counters (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel), read
-from profiles/traffic.json when that summary exists.
+against MI355X's 8 TB/s.  `traffic` is the HBM bytes per launch measured in
+the run by two rocprofv3 counter passes over `bench.py --profile-kernel` in a
+child process (FETCH_SIZE, then WRITE_SIZE, each scaled by what it reads for
+a 256 MiB copy); null, with the reason, when rocprofv3 is missing or fails.
 
 `cpu_baseline` times the oracle's C restatement (OpenMP, and on one core) on
 a bounded sample.  Extra objects report the other workloads: the kernel alone
@@ -188,10 +189,12 @@ def multiply_kernel(m, D, C):
     reports it (gcoo_get_spdm_choice): its name (with the panel kernel's
     element type, chunk rows, waves and rows per wave, as rocprofv3 lists
     it), variant and columns per lane."""
+    import torch
     variant, vec, waves = m.spdm_choice(D, C)
     name = KERNEL_OF_VARIANT.get(variant, f"variant {variant}")
     if variant == 6:
-        name += f"<F32, 75, {waves}, {PANEL_ROWS[waves]}>"
+        elem = "F32, 75" if D.dtype == torch.float32 else "BF16, 150"
+        name += f"<{elem}, {waves}, {PANEL_ROWS[waves]}>"
     return name, variant, vec
 
 
@@ -214,12 +217,73 @@ def in_step_multiply_us(m, S, D, C, stream, reps=10):
     return tot / reps
 
 
-def traffic_from_profiles():
-    p = os.path.join(ROOT, "profiles", "traffic.json")
-    if not os.path.exists(p):
-        return None
-    with open(p) as f:
-        return json.load(f)
+def _short_kernel_name(name):
+    return name.replace("gcoo::(anonymous namespace)::", "").replace("void ", "").split("(")[0]
+
+
+def _counter_pass(counter, n, s, seed, timeout_s):
+    """One rocprofv3 counter pass over `bench.py --profile-kernel` in a child
+    process: {short kernel name: [KiB per dispatch, ...]}."""
+    import csv
+    import shutil
+    import signal
+    import tempfile
+    exe = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(exe):
+        raise RuntimeError("rocprofv3 not found")
+    out = tempfile.mkdtemp(prefix="gcoo_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=ROOT)
+    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.join(ROOT, "bench.py"), "--profile-kernel", "--n", str(n),
+           "--sparsity", str(s), "--seed", str(seed)]
+    p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                         stderr=subprocess.STDOUT, text=True, start_new_session=True)
+    try:
+        log, _ = p.communicate(timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        os.killpg(p.pid, signal.SIGKILL)
+        p.communicate()
+        raise RuntimeError(f"{counter} pass exceeded {timeout_s} s")
+    try:
+        if p.returncode != 0 or "profiled spdm" not in log:
+            raise RuntimeError(f"{counter} pass exited {p.returncode}: {log[-300:]}")
+        vals = {}
+        for root, _, files in os.walk(out):
+            for fn in files:
+                if fn.endswith("counter_collection.csv"):
+                    for r in csv.DictReader(open(os.path.join(root, fn))):
+                        vals.setdefault(_short_kernel_name(r["Kernel_Name"]), []).append(
+                            float(r["Counter_Value"]))
+        if not vals:
+            raise RuntimeError(f"{counter} pass wrote no counters")
+        return vals
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+
+
+def measure_traffic(n, s, seed, kernels, timeout_s=180):
+    """HBM bytes per launch of the given kernels, measured now: FETCH_SIZE,
+    then WRITE_SIZE, each in its own rocprofv3 pass, each scaled by what the
+    counter reads for a copy of a 256 MiB tensor (256 MiB read, 256 MiB
+    written; the largest reading of its ten dispatches, which stream
+    through the caches).  Returns {kernel: bytes} plus the two factors, or
+    {"error": ...}."""
+    try:
+        fe = _counter_pass("FETCH_SIZE", n, s, seed, timeout_s)
+        wr = _counter_pass("WRITE_SIZE", n, s, seed, timeout_s)
+        copy = "__amd_rocclr_copyBuffer"
+        ff = 262144.0 / max(fe[copy])
+        wf = 262144.0 / max(wr[copy])
+        out = {"fetch_factor": round(ff, 4), "write_factor": round(wf, 4)}
+        for k in kernels:
+            if k not in fe or k not in wr:
+                raise RuntimeError(f"no counters for {k}")
+            f_b = sum(fe[k]) / len(fe[k]) * 1024 * ff
+            w_b = sum(wr[k]) / len(wr[k]) * 1024 * wf
+            out[k] = {"fetch_bytes": int(f_b), "write_bytes": int(w_b), "bytes": int(f_b + w_b)}
+        return out
+    except Exception as e:  # counters are reported, not required
+        return {"error": str(e)[:400]}
 
 
 def kernel_sweep(G, dev, n, seed):
@@ -455,6 +519,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--profile-kernel", action="store_true")
     ap.add_argument("--profile-coo", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes behind roofline.traffic")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -511,10 +577,16 @@ def main():
     t_instep = in_step_multiply_us(m, S, D, C, stream)
 
     kname, kvariant, kvec = multiply_kernel(m, D, C)
-    traffic = None
-    tr = traffic_from_profiles()
-    if tr and tr.get("workload") == f"square_n{n}_s{s}" and tr.get("kernel") == kname:
-        traffic = tr.get("hbm_bytes_per_launch")
+    D16 = G.to_bf16(D)
+    kname16, _, _ = multiply_kernel(m, D16, C)
+    del D16
+    traffic = traffic16 = None
+    traffic_meas = None
+    if not args.no_traffic:
+        traffic_meas = measure_traffic(n, s, args.seed, (kname, kname16))
+        if kname in traffic_meas:
+            traffic = traffic_meas[kname]["bytes"]
+            traffic16 = traffic_meas[kname16]["bytes"]
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -556,6 +628,10 @@ def main():
                      "variant": kvariant, "vec": kvec,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE passes in this run, "
+                                        "scaled by a 256 MiB copy" if traffic else traffic_meas),
+                     "traffic_over_algorithmic": round(traffic / bytes_k, 3) if traffic else None,
+                     "bf16_kernel": kname16, "bf16_traffic": traffic16,
                      "algorithmic_bytes": bytes_k, "mean_launch_us": round(t_kern * 1e6, 2),
                      "in_step_launch_us": round(t_instep, 2),
                      "in_step_frac": round(bytes_k / (t_instep * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
