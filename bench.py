@@ -442,7 +442,7 @@ def serve_case():
     exe = os.path.join(ROOT, "gcoospdm_amd", "host", "gcoo_serve")
     if not os.path.exists(exe):
         return {"error": "gcoo_serve not built"}
-    r = subprocess.run([exe, "-jobs=512", "-requests=2000", "-loads=0.5,0.8,0.95",
+    r = subprocess.run([exe, "-jobs=512", "-requests=20000", "-loads=0.5,0.8,0.95",
                         "-table=" + os.path.join(ROOT, "gcoospdm_amd", "tuning.json")],
                        capture_output=True, text=True, timeout=600)
     if r.returncode != 0:

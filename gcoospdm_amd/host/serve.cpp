@@ -14,6 +14,7 @@
 // (submit -> completion), jobs/s and GFLOP/s (2 * nnz * N per job), plus the
 // relative Frobenius error of a sample of results against FP64 products of
 // the FP32 operands computed on the host.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -142,7 +143,7 @@ int main(int argc, char **argv)
 {
     Flags f(argc, argv);
     const int nJobs = (int)f.get_int("jobs", 512);
-    const int nReq = (int)f.get_int("requests", 2000);
+    const int nReq = (int)f.get_int("requests", 20000);
     const uint64_t master = (uint64_t)f.get_int("seed", 2024);
     const size_t budget = (size_t)f.get_int("budget_mb", 2048) << 20;
     const int slots = (int)f.get_int("slots", 4);
@@ -269,7 +270,14 @@ int main(int argc, char **argv)
             std::vector<int> all(ops.begin(), ops.end());
             std::vector<double> zero(all.size(), 0.0);
             drive(srv, all, gflopOp, zero);  // warm-up
-            RunResult sat = drive(srv, all, gflopOp, zero);
+            // the median of five drives (one drive takes a few ms, so one
+            // slow batch would move a single reading)
+            std::vector<RunResult> sats;
+            for (int t = 0; t < 5; ++t) sats.push_back(drive(srv, all, gflopOp, zero));
+            std::sort(sats.begin(), sats.end(), [](const RunResult &a, const RunResult &b) {
+                return a.st.wall_s < b.st.wall_s;
+            });
+            const RunResult sat = sats[2];
             cap[mode] = sat.st.completed / sat.st.wall_s;
             std::printf("{\"kind\": \"saturated\", \"dtype\": \"%s\", \"mode\": \"%s\", \"jobs\": %llu, "
                         "\"wall_ms\": %.3f, \"jobs_per_s\": %.1f, \"gflops\": %.1f, \"batches\": %llu, "
@@ -299,6 +307,12 @@ int main(int argc, char **argv)
                 for (int r = 0; r < nReq; ++r) gaps[r] = unitGaps[r] / lambda;
                 std::vector<int> seq(nReq);
                 for (int r = 0; r < nReq; ++r) seq[r] = ops[idx[r]];
+                // one saturated drive right before, so the run starts at the
+                // clock a busy GPU runs at
+                {
+                    std::vector<int> all(ops.begin(), ops.end());
+                    drive(srv, all, gflopOp, std::vector<double>(all.size(), 0.0));
+                }
                 const double t0 = now_s();
                 RunResult rr = drive(srv, seq, gflopOp, gaps);
                 const double wall = now_s() - t0;
