@@ -169,7 +169,7 @@ def _counts(rng, nwaves, nchunks):
             for _ in range(nwaves)]
 
 
-@pytest.mark.parametrize("bf16,rows", [(False, 16), (False, 28), (True, 16)])
+@pytest.mark.parametrize("bf16,rows", [(False, 16), (False, 28), (False, 42), (True, 16), (True, 28)])
 @pytest.mark.parametrize("ooo", [False, True])
 @pytest.mark.parametrize("p_land", [0.05, 0.0005])
 @pytest.mark.parametrize("seed", range(3))
