@@ -98,7 +98,8 @@ class Model:
 
 
 def _simulate(prog, counts, c0=2, bf16=False, ooo=False, seed=0, nwaves=6, rows=16, K=8000,
-              p_land=0.05):
+              p_land=0.05, slow=None):
+    """`slow`: a wave picked 20 times less often than the others when runnable."""
     rng = random.Random(seed)
     model = Model(c0, bf16, ooo, rng)
     tk, rowd = (150, 512) if bf16 else (75, 1024)
@@ -152,6 +153,8 @@ def _simulate(prog, counts, c0=2, bf16=False, ooo=False, seed=0, nwaves=6, rows=
             if all(st in ("done", ("barrier",)) for st in state):
                 continue
             raise AssertionError(f"deadlock: {state}")
+        if slow in run and len(run) > 1 and rng.random() < 0.95:
+            run.remove(slow)
         w = rng.choice(run)
         state[w] = None
         try:
@@ -173,12 +176,14 @@ def _counts(rng, nwaves, nchunks):
 @pytest.mark.parametrize("ooo", [False, True])
 @pytest.mark.parametrize("p_land", [0.05, 0.0005])
 @pytest.mark.parametrize("seed", range(3))
-def test_shipped_schedule_reads_only_landed_chunks(bf16, rows, ooo, p_land, seed):
-    """Staging loads landing quickly or only when waited for."""
+@pytest.mark.parametrize("slow", [None, 0, 5])
+def test_shipped_schedule_reads_only_landed_chunks(bf16, rows, ooo, p_land, seed, slow):
+    """Staging loads landing quickly or only when waited for; all waves at
+    one pace, or a staging wave (0) or the other wave (5) far behind."""
     prog = T._prog(rows, bf16)
     counts = _counts(random.Random(seed), 6, 5)
     model, lanes = _simulate(prog, counts, bf16=bf16, ooo=ooo, seed=seed, rows=rows,
-                             p_land=p_land)
+                             p_land=p_land, slow=slow)
     assert not model.errors, model.errors[:3]
     assert all(ln.barriers == 5 for ln in lanes)
 
@@ -207,7 +212,7 @@ def test_planted_schedule_mistakes_are_caught(name):
     caught = False
     for seed in range(6):
         counts = _counts(random.Random(seed), 6, 5)
-        model, _ = _simulate(prog, counts, seed=seed, p_land=0.0005)
+        model, _ = _simulate(prog, counts, seed=seed, p_land=0.0005, slow=seed % 2 * 5)
         caught |= bool(model.errors)
     assert caught, name
 
