@@ -181,20 +181,17 @@ KERNEL_OF_VARIANT = {
 }
 
 
-PANEL_ROWS = {16: 16, 12: 28, 8: 42}
-
-
 def multiply_kernel(m, D, C):
     """The multiply kernel the library runs for these operands, as it
     reports it (gcoo_get_spdm_choice): its name (with the panel kernel's
     element type, chunk rows, waves and rows per wave, as rocprofv3 lists
     it), variant and columns per lane."""
     import torch
-    variant, vec, waves = m.spdm_choice(D, C)
+    variant, vec, waves, rows = m.spdm_choice(D, C)
     name = KERNEL_OF_VARIANT.get(variant, f"variant {variant}")
     if variant == 6:
         elem = "F32, 75" if D.dtype == torch.float32 else "BF16, 150"
-        name += f"<{elem}, {waves}, {PANEL_ROWS[waves]}>"
+        name += f"<{elem}, {waves}, {rows}>"
     return name, variant, vec
 
 

@@ -78,7 +78,7 @@ _SIGNATURES = [
     ("gcoo_set_spdm_config", _I, [_P, _P]),
     ("gcoo_get_spdm_config", _I, [_P, _P]),
     ("gcoo_get_spdm_choice", _I, [_P, _I, _P, _I, _P, _c.POINTER(_I), _c.POINTER(_I),
-                                  _c.POINTER(_I)]),
+                                  _c.POINTER(_I), _c.POINTER(_I)]),
     ("gcoo_f32_to_bf16_async", _I, [_P, _P, _I64, _P]),
     ("gcoo_spdm_bf16_async", _I, [_P, _P, _I, _I, _P, _P]),
     ("gcoo_spdm_bf16", _I, [_P, _P, _I, _I, _P, _P]),
@@ -292,15 +292,17 @@ class GCOOMatrix:
                     unroll=cfg.unroll)
 
     def spdm_choice(self, D, C):
-        """(variant, vec, waves) of the multiply kernel a launch with D (FP32
-        or BF16 bits) and C would run (gcoo_get_spdm_choice)."""
+        """(variant, vec, waves, rows per wave) of the multiply kernel a launch
+        with D (FP32 or BF16 bits) and C would run (gcoo_get_spdm_choice);
+        waves and rows are 0 unless the variant is 6."""
         import torch
         dtype = 0 if D.dtype == torch.float32 else 1
-        v, vec, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        v, vec, w, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(lib().gcoo_get_spdm_choice(self._h, dtype, _ptr(D), D.shape[1], _ptr(C),
-                                          ctypes.byref(v), ctypes.byref(vec), ctypes.byref(w)),
+                                          ctypes.byref(v), ctypes.byref(vec), ctypes.byref(w),
+                                          ctypes.byref(r)),
                "gcoo_get_spdm_choice")
-        return v.value, vec.value, w.value
+        return v.value, vec.value, w.value, r.value
 
     def multiply_by(self, S, D, C, stream=None):
         """Convert S then C = S * D (the reference's mutiplyBy)."""

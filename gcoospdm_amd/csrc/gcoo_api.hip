@@ -503,12 +503,13 @@ int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg)
 }
 
 int gcoo_get_spdm_choice(gcoo_handle h, int dtype, const void *D, int N, const float *C,
-                         int *variant, int *vec, int *waves)
+                         int *variant, int *vec, int *waves, int *rows)
 {
-    if (!h || !variant || !vec || !waves) return set_error(GCOO_ERR_INVALID, "null argument");
+    if (!h || !variant || !vec || !waves || !rows)
+        return set_error(GCOO_ERR_INVALID, "null argument");
     if (dtype != 0 && dtype != 1) return set_error(GCOO_ERR_INVALID, "dtype %d", dtype);
     if (N < 0) return set_error(GCOO_ERR_INVALID, "N = %d", N);
-    return spdm_choice(h, dtype, D, N, C, variant, vec, waves);
+    return spdm_choice(h, dtype, D, N, C, variant, vec, waves, rows);
 }
 
 int gcoo_test_set_conversion_abort(gcoo_handle h, gcoo_stream s)

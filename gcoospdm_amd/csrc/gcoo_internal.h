@@ -175,7 +175,7 @@ int launch_spdm_f32(const gcoo_matrix *h, const float *D, int N, float *C,
 // The multiply variant (1..6), columns per lane and (variant 6) waves per
 // workgroup a launch of h on these operands runs (dtype 0 FP32, 1 BF16).
 int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const float *C,
-                int *variant, int *vec, int *waves);
+                int *variant, int *vec, int *waves, int *rows);
 // BF16 storage form: h->seg16 must hold the BF16 entry values.
 int launch_spdm_bf16(const gcoo_matrix *h, const uint16_t *D, int N, float *C,
                      hipStream_t s);

@@ -20,7 +20,8 @@
 //      writes the run length (one wave per row, 64 entries at a time);
 //   2. an exclusive scan of the counts in (chunk, row) order;
 //   3. every entry moves to its list slot as
-//      ((LDS byte offset of its D row within the staged chunk / 16) << 16 |
+//      ((LDS byte offset of its D row from the first of the two chunk
+//      buffers, odd chunks staged one chunk further on, / 16) << 16 |
 //      row slot in its wave, value bits -- for BF16 segments the value widened
 //      to FP32, BF16 bits << 16): the multiply forms the LDS address
 //      with one multiply-add of the high half (hi16 * 16 + lane address) and
@@ -122,7 +123,8 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
             const int s = run_start(act && c != cp, i0, lane, carry);
             if (act)
                 plist[cnt[c * RPW + rr] + (i - s)] =
-                    make_uint2(((unsigned)((kv.x - c * tk) * rowb) >> 4 << 16) | (unsigned)rr,
+                    make_uint2(((unsigned)(((c & 1) * tk + kv.x - c * tk) * rowb) >> 4 << 16) |
+                                   (unsigned)rr,
                                widen ? (unsigned)kv.y << 16 : (unsigned)kv.y);
         }
     }
@@ -146,7 +148,8 @@ int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, i
                       int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
                       hipStream_t s)
 {
-    if ((long long)tk * rowb > (1ll << 20) || rowb % 16 != 0)
+    // two chunk buffers of offsets / 16 in 16 bits
+    if (2ll * tk * rowb > (1ll << 20) || rowb % 16 != 0)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
     if (rpw < 1 || rpw > 255 || waves < 1)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel of %d waves x %d rows", waves, rpw);

@@ -3,7 +3,7 @@
 // barrier, the next chunk's entry loads and staging, and the entry loop.
 #pragma once
 // FP32 D, 16 rows per wave: accumulators v0..v63, component c of row slot q
-// in v[16 c + q]; scratch v64..v103
+// in v[16 c + q]; scratch v64..v97
 #define GCOO_PANEL_SEGMENT_ASM_R16 \
     "s_mov_b32 s28, m0\n\t" \
     "s_nop 4\n\t" \
@@ -200,12 +200,12 @@
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -242,20 +242,20 @@
     "ds_read_b128 v[64:67], v96\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
@@ -283,20 +283,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -380,10 +380,10 @@
     "ds_read_b128 v[64:67], v96\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v80, v0\n\t" \
@@ -419,12 +419,12 @@
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
@@ -504,14 +504,14 @@
     ".Lf20_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
@@ -539,20 +539,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -672,18 +672,18 @@
     "ds_read_b128 v[64:67], v96\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
@@ -750,18 +750,18 @@
     "ds_read_b128 v[64:67], v96\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
@@ -824,24 +824,24 @@
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -903,24 +903,24 @@
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -1142,20 +1142,20 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -1188,24 +1188,24 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
@@ -1233,20 +1233,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -1326,14 +1326,14 @@
     ".Lf11_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v80, v0\n\t" \
@@ -1361,20 +1361,20 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
@@ -1454,14 +1454,14 @@
     ".Lf21_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
@@ -1489,20 +1489,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -1618,22 +1618,22 @@
     ".Lpa1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
@@ -1696,22 +1696,22 @@
     ".Lpb1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
-    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[92:95], v103\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
+    "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
@@ -1766,32 +1766,32 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -1845,32 +1845,32 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[64:67], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[68:71], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[72:75], v98\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[72:75], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[76:79], v99\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[76:79], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[80:83], v100\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[80:83], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[84:87], v101\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[88:91], v102\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v96\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -1934,19 +1934,1964 @@
 #define GCOO_PANEL_SEGMENT_CLOBBERS_R16 \
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", \
     "v76", "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", \
-    "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", \
-    "v100", "v101", "v102", "v103", "s24", "s25", "s26", "s27", "s28", "s29", "s30", "s36", \
-    "s37", "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", \
-    "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", \
-    "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", \
-    "s73", "s74", "s75", "s76", "s77", "s78", "s79", "s80", "s81", "s82", "s83", "s84", \
-    "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95", "s96", \
-    "s97", "s98", "s99", "scc", "memory"
+    "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"
 #define GCOO_PANEL_ACC_OPERANDS_R16(a) \
     "+{v[0:7]}"(a[0]), "+{v[8:15]}"(a[1]), "+{v[16:23]}"(a[2]), "+{v[24:31]}"(a[3]), \
     "+{v[32:39]}"(a[4]), "+{v[40:47]}"(a[5]), "+{v[48:55]}"(a[6]), "+{v[56:63]}"(a[7])
+// FP32 D, 21 rows per wave: accumulators v0..v83, component c of row slot q
+// in v[21 c + q]; scratch v84..v117
+#define GCOO_PANEL_SEGMENT_ASM_R21 \
+    "s_mov_b32 s28, m0\n\t" \
+    "s_nop 4\n\t" \
+    "v_readlane_b32 s29, %[vpbo], %[sli]\n\t" \
+    "v_readlane_b32 s30, %[vpn], %[sli]\n\t" \
+    "s_cmp_eq_u32 %[par], 0\n\t" \
+    "s_cbranch_scc0 .Lo1_%=\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lc0_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lc0_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s24\n\t" \
+    "s_branch .Lc0_%=\n\t" \
+    ".Lo1_%=:\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lc1_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lc1_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s24\n\t" \
+    "s_branch .Lc1_%=\n\t" \
+    ".Lc0_%=:\n\t" \
+    "s_mov_b32 s25, s30\n\t" \
+    "s_add_u32 s26, s29, 0x80\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lw0_%=\n\t" \
+    "s_waitcnt vmcnt(0)\n\t" \
+    ".Lw0_%=:\n\t" \
+    "s_barrier\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_add_u32 s24, %[sli], 1\n\t" \
+    "s_cmp_lt_u32 s24, %[liend]\n\t" \
+    "s_cbranch_scc0 .Lnp0_%=\n\t" \
+    "v_readlane_b32 s29, %[vpbo], s24\n\t" \
+    "v_readlane_b32 s30, %[vpn], s24\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lnp0_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lnp0_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s24\n\t" \
+    ".Lnp0_%=:\n\t" \
+    "s_cmp_le_u32 %[sso], %[lim]\n\t" \
+    "s_cbranch_scc0 .Lsg0_%=\n\t" \
+    "s_add_u32 m0, %[sdst], 0x12c00\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsd0_%=\n\t" \
+    ".Lsg0_%=:\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lsd0_%=\n\t" \
+    "s_add_u32 m0, %[sdst], 0x12c00\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    ".Lsd0_%=:\n\t" \
+    "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lpr20_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lpr0_%=\n\t" \
+    ".Lpr20_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lpr0_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 4\n\t" \
+    "s_cbranch_scc1 .Lf00_%=\n\t" \
+    "s_mov_b32 s27, s25\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    ".Lp0r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    ".Lggp0s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf00_%=:\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_cmp_ge_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lq00_%=\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf10_%=\n\t" \
+    ".Lq00_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lz00_%=\n\t" \
+    "s_sub_u32 s27, s25, 4\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    ".Lp1r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    ".Lggp1s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz00_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf10_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lq10_%=\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf20_%=\n\t" \
+    ".Lq10_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lz10_%=\n\t" \
+    "s_sub_u32 s27, s25, 8\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    ".Lp2r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    ".Lggp2s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz10_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf20_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lq20_%=\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf30_%=\n\t" \
+    ".Lq20_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lz20_%=\n\t" \
+    "s_sub_u32 s27, s25, 12\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    ".Lp3r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v112, v0\n\t" \
+    "v_fma_f32 v21, s67, v113, v21\n\t" \
+    "v_fma_f32 v42, s67, v114, v42\n\t" \
+    "v_fma_f32 v63, s67, v115, v63\n\t" \
+    ".Lggp3s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz20_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf30_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v112, v0\n\t" \
+    "v_fma_f32 v21, s67, v113, v21\n\t" \
+    "v_fma_f32 v42, s67, v114, v42\n\t" \
+    "v_fma_f32 v63, s67, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_cmp_le_u32 s25, 16\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_sub_u32 s25, s25, 16\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
+    "s_add_u32 s26, s26, 0x40\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lba0_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lta0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpa0_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqa0_%=\n\t" \
+    ".Lpa0_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqa0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lna0_%=\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s26\n\t" \
+    ".Lna0_%=:\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lbb0_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Ltb0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpb0_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqb0_%=\n\t" \
+    ".Lpb0_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqb0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lnb0_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
+    ".Lnb0_%=:\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v112, v0\n\t" \
+    "v_fma_f32 v21, s67, v113, v21\n\t" \
+    "v_fma_f32 v42, s67, v114, v42\n\t" \
+    "v_fma_f32 v63, s67, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_branch .Lba0_%=\n\t" \
+    ".Lta0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    ".Lgwa0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s44\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    ".Lgea0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Ltb0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    ".Lgwb0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s60\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    ".Lgeb0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lce0_%=:\n\t" \
+    "s_setprio 0\n\t" \
+    "s_add_u32 %[sli], %[sli], 1\n\t" \
+    "s_cmp_lt_u32 %[sli], %[liend]\n\t" \
+    "s_cbranch_scc0 .Lx_%=\n\t" \
+    ".Lc1_%=:\n\t" \
+    "s_mov_b32 s25, s30\n\t" \
+    "s_add_u32 s26, s29, 0x80\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lw1_%=\n\t" \
+    "s_waitcnt vmcnt(0)\n\t" \
+    ".Lw1_%=:\n\t" \
+    "s_barrier\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_add_u32 s24, %[sli], 1\n\t" \
+    "s_cmp_lt_u32 s24, %[liend]\n\t" \
+    "s_cbranch_scc0 .Lnp1_%=\n\t" \
+    "v_readlane_b32 s29, %[vpbo], s24\n\t" \
+    "v_readlane_b32 s30, %[vpn], s24\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lnp1_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lnp1_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s24\n\t" \
+    ".Lnp1_%=:\n\t" \
+    "s_cmp_le_u32 %[sso], %[lim]\n\t" \
+    "s_cbranch_scc0 .Lsg1_%=\n\t" \
+    "s_mov_b32 m0, %[sdst]\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsd1_%=\n\t" \
+    ".Lsg1_%=:\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lsd1_%=\n\t" \
+    "s_mov_b32 m0, %[sdst]\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    ".Lsd1_%=:\n\t" \
+    "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lpr21_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lpr1_%=\n\t" \
+    ".Lpr21_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lpr1_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 4\n\t" \
+    "s_cbranch_scc1 .Lf01_%=\n\t" \
+    "s_mov_b32 s27, s25\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    ".Lp0r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    ".Lggp0s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf01_%=:\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_cmp_ge_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lq01_%=\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf11_%=\n\t" \
+    ".Lq01_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lz01_%=\n\t" \
+    "s_sub_u32 s27, s25, 4\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    ".Lp1r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    ".Lggp1s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz01_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf11_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lq11_%=\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf21_%=\n\t" \
+    ".Lq11_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lz11_%=\n\t" \
+    "s_sub_u32 s27, s25, 8\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    ".Lp2r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    ".Lggp2s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz11_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf21_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lq21_%=\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf31_%=\n\t" \
+    ".Lq21_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lz21_%=\n\t" \
+    "s_sub_u32 s27, s25, 12\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    ".Lp3r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v112, v0\n\t" \
+    "v_fma_f32 v21, s99, v113, v21\n\t" \
+    "v_fma_f32 v42, s99, v114, v42\n\t" \
+    "v_fma_f32 v63, s99, v115, v63\n\t" \
+    ".Lggp3s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz21_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf31_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v112, v0\n\t" \
+    "v_fma_f32 v21, s99, v113, v21\n\t" \
+    "v_fma_f32 v42, s99, v114, v42\n\t" \
+    "v_fma_f32 v63, s99, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_cmp_le_u32 s25, 16\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_sub_u32 s25, s25, 16\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
+    "s_add_u32 s26, s26, 0x40\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lba1_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lta1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpa1_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqa1_%=\n\t" \
+    ".Lpa1_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqa1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lna1_%=\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s26\n\t" \
+    ".Lna1_%=:\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lbb1_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Ltb1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpb1_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqb1_%=\n\t" \
+    ".Lpb1_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqb1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[112:115], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lnb1_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
+    ".Lnb1_%=:\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v112, v0\n\t" \
+    "v_fma_f32 v21, s99, v113, v21\n\t" \
+    "v_fma_f32 v42, s99, v114, v42\n\t" \
+    "v_fma_f32 v63, s99, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_branch .Lba1_%=\n\t" \
+    ".Lta1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    ".Lgwa1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s76\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    ".Lgea1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Ltb1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[84:87], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[88:91], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[92:95], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[96:99], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[100:103], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[104:107], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[108:111], v116\n\t" \
+    ".Lgwb1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s92\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    ".Lgeb1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lce1_%=:\n\t" \
+    "s_setprio 0\n\t" \
+    "s_add_u32 %[sli], %[sli], 1\n\t" \
+    "s_cmp_lt_u32 %[sli], %[liend]\n\t" \
+    "s_cbranch_scc0 .Lx_%=\n\t" \
+    "s_branch .Lc0_%=\n\t" \
+    ".Lx_%=:\n\t" \
+    "s_mov_b32 m0, s28\n\t"
+#define GCOO_PANEL_SEGMENT_CLOBBERS_R21 \
+    "v84", "v85", "v86", "v87", "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", \
+    "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103", "v104", "v105", "v106", "v107", \
+    "v108", "v109", "v110", "v111", "v112", "v113", "v114", "v115", "v116", "v117", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"
+#define GCOO_PANEL_ACC_OPERANDS_R21(a) \
+    "+{v[0:3]}"(a[0]), "+{v[4:7]}"(a[1]), "+{v[8:11]}"(a[2]), "+{v[12:15]}"(a[3]), \
+    "+{v[16:19]}"(a[4]), "+{v[20:23]}"(a[5]), "+{v[24:27]}"(a[6]), "+{v[28:31]}"(a[7]), \
+    "+{v[32:35]}"(a[8]), "+{v[36:39]}"(a[9]), "+{v[40:43]}"(a[10]), "+{v[44:47]}"(a[11]), \
+    "+{v[48:51]}"(a[12]), "+{v[52:55]}"(a[13]), "+{v[56:59]}"(a[14]), "+{v[60:63]}"(a[15]), \
+    "+{v[64:67]}"(a[16]), "+{v[68:71]}"(a[17]), "+{v[72:75]}"(a[18]), "+{v[76:79]}"(a[19]), \
+    "+{v[80:83]}"(a[20])
 // FP32 D, 28 rows per wave: accumulators v0..v111, component c of row slot q
-// in v[28 c + q]; scratch v112..v151
+// in v[28 c + q]; scratch v112..v145
 #define GCOO_PANEL_SEGMENT_ASM_R28 \
     "s_mov_b32 s28, m0\n\t" \
     "s_nop 4\n\t" \
@@ -2143,12 +4088,12 @@
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -2185,20 +4130,20 @@
     "ds_read_b128 v[112:115], v144\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
@@ -2226,20 +4171,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v151, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -2323,10 +4268,10 @@
     "ds_read_b128 v[112:115], v144\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v128, v0\n\t" \
@@ -2362,12 +4307,12 @@
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
@@ -2447,14 +4392,14 @@
     ".Lf20_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
@@ -2482,20 +4427,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v151, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -2615,18 +4560,18 @@
     "ds_read_b128 v[112:115], v144\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
@@ -2693,18 +4638,18 @@
     "ds_read_b128 v[112:115], v144\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
@@ -2767,24 +4712,24 @@
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -2846,24 +4791,24 @@
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -3085,20 +5030,20 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -3131,24 +5076,24 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
@@ -3176,20 +5121,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v151, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -3269,14 +5214,14 @@
     ".Lf11_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v128, v0\n\t" \
@@ -3304,20 +5249,20 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
@@ -3397,14 +5342,14 @@
     ".Lf21_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \
@@ -3432,20 +5377,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v151, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -3561,22 +5506,22 @@
     ".Lpa1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
@@ -3639,22 +5584,22 @@
     ".Lpb1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
-    "v_mad_u32_u16 v151, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[140:143], v151\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
+    "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[140:143], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
@@ -3709,32 +5654,32 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -3788,32 +5733,32 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[112:115], v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[116:119], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[120:123], v146\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[120:123], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[124:127], v147\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[124:127], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[128:131], v148\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[128:131], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[132:135], v149\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[132:135], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[136:139], v150\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[136:139], v144\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -3877,21 +5822,20 @@
 #define GCOO_PANEL_SEGMENT_CLOBBERS_R28 \
     "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", "v122", "v123", \
     "v124", "v125", "v126", "v127", "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135", \
-    "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", "v144", "v145", "v146", "v147", \
-    "v148", "v149", "v150", "v151", "s24", "s25", "s26", "s27", "s28", "s29", "s30", "s36", \
-    "s37", "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", \
-    "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", \
-    "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", \
-    "s73", "s74", "s75", "s76", "s77", "s78", "s79", "s80", "s81", "s82", "s83", "s84", \
-    "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95", "s96", \
-    "s97", "s98", "s99", "scc", "memory"
+    "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", "v144", "v145", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"
 #define GCOO_PANEL_ACC_OPERANDS_R28(a) \
     "+{v[0:7]}"(a[0]), "+{v[8:15]}"(a[1]), "+{v[16:23]}"(a[2]), "+{v[24:31]}"(a[3]), \
     "+{v[32:39]}"(a[4]), "+{v[40:47]}"(a[5]), "+{v[48:55]}"(a[6]), "+{v[56:63]}"(a[7]), \
     "+{v[64:71]}"(a[8]), "+{v[72:79]}"(a[9]), "+{v[80:87]}"(a[10]), "+{v[88:95]}"(a[11]), \
     "+{v[96:103]}"(a[12]), "+{v[104:111]}"(a[13])
 // FP32 D, 42 rows per wave: accumulators v0..v167, component c of row slot q
-// in v[42 c + q]; scratch v168..v207
+// in v[42 c + q]; scratch v168..v201
 #define GCOO_PANEL_SEGMENT_ASM_R42 \
     "s_mov_b32 s28, m0\n\t" \
     "s_nop 4\n\t" \
@@ -4088,12 +6032,12 @@
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v202, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v203, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -4130,20 +6074,20 @@
     "ds_read_b128 v[168:171], v200\n\t" \
     "v_mad_u32_u16 v201, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v200, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
-    "v_mad_u32_u16 v204, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v168, v0\n\t" \
@@ -4171,20 +6115,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v204, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v205, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v206, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v207, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v201, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -4268,10 +6212,10 @@
     "ds_read_b128 v[168:171], v200\n\t" \
     "v_mad_u32_u16 v201, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v200, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v184, v0\n\t" \
@@ -4307,12 +6251,12 @@
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v202, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v203, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
@@ -4392,14 +6336,14 @@
     ".Lf20_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
-    "v_mad_u32_u16 v204, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v168, v0\n\t" \
@@ -4427,20 +6371,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v204, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v205, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v206, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v207, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v201, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -4560,18 +6504,18 @@
     "ds_read_b128 v[168:171], v200\n\t" \
     "v_mad_u32_u16 v201, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
-    "v_mad_u32_u16 v204, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
+    "v_mad_u32_u16 v200, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
@@ -4638,18 +6582,18 @@
     "ds_read_b128 v[168:171], v200\n\t" \
     "v_mad_u32_u16 v201, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
-    "v_mad_u32_u16 v204, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
+    "v_mad_u32_u16 v200, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
@@ -4712,24 +6656,24 @@
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v202, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v203, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v204, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v205, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v206, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
@@ -4791,24 +6735,24 @@
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v202, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v203, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v204, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v205, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v206, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
@@ -5030,20 +6974,20 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "v_mad_u32_u16 v200, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v201, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v202, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v203, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -5076,24 +7020,24 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
-    "v_mad_u32_u16 v200, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
-    "v_mad_u32_u16 v201, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v200, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
-    "v_mad_u32_u16 v204, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v168, v0\n\t" \
@@ -5121,20 +7065,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v204, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v205, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v206, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v207, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v201, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -5214,14 +7158,14 @@
     ".Lf11_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
-    "v_mad_u32_u16 v200, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
-    "v_mad_u32_u16 v201, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v200, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v184, v0\n\t" \
@@ -5249,20 +7193,20 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "v_mad_u32_u16 v200, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v201, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v202, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v203, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
@@ -5342,14 +7286,14 @@
     ".Lf21_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
-    "v_mad_u32_u16 v204, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v168, v0\n\t" \
@@ -5377,20 +7321,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v204, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v205, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v206, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v207, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v201, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -5506,22 +7450,22 @@
     ".Lpa1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
-    "v_mad_u32_u16 v200, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
-    "v_mad_u32_u16 v201, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
-    "v_mad_u32_u16 v204, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
+    "v_mad_u32_u16 v200, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
@@ -5584,22 +7528,22 @@
     ".Lpb1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
-    "v_mad_u32_u16 v200, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
-    "v_mad_u32_u16 v201, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
-    "v_mad_u32_u16 v202, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
-    "v_mad_u32_u16 v203, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
-    "v_mad_u32_u16 v204, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
-    "v_mad_u32_u16 v205, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
-    "v_mad_u32_u16 v206, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
-    "v_mad_u32_u16 v207, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[196:199], v207\n\t" \
+    "v_mad_u32_u16 v200, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
+    "v_mad_u32_u16 v201, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
+    "v_mad_u32_u16 v200, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
+    "v_mad_u32_u16 v201, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
+    "v_mad_u32_u16 v200, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
+    "v_mad_u32_u16 v201, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[196:199], v201\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
@@ -5654,32 +7598,32 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
-    "v_mad_u32_u16 v200, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v201, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v202, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v203, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v204, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v205, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v206, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
@@ -5733,32 +7677,32 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
-    "v_mad_u32_u16 v200, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v200, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[168:171], v200\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v201, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v201, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b128 v[172:175], v201\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v202, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[176:179], v202\n\t" \
+    "v_mad_u32_u16 v200, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[176:179], v200\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v203, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[180:183], v203\n\t" \
+    "v_mad_u32_u16 v201, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[180:183], v201\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v204, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[184:187], v204\n\t" \
+    "v_mad_u32_u16 v200, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[184:187], v200\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v205, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[188:191], v205\n\t" \
+    "v_mad_u32_u16 v201, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[188:191], v201\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v206, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b128 v[192:195], v206\n\t" \
+    "v_mad_u32_u16 v200, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b128 v[192:195], v200\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
@@ -5822,14 +7766,13 @@
 #define GCOO_PANEL_SEGMENT_CLOBBERS_R42 \
     "v168", "v169", "v170", "v171", "v172", "v173", "v174", "v175", "v176", "v177", "v178", "v179", \
     "v180", "v181", "v182", "v183", "v184", "v185", "v186", "v187", "v188", "v189", "v190", "v191", \
-    "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199", "v200", "v201", "v202", "v203", \
-    "v204", "v205", "v206", "v207", "s24", "s25", "s26", "s27", "s28", "s29", "s30", "s36", \
-    "s37", "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", \
-    "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", \
-    "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", \
-    "s73", "s74", "s75", "s76", "s77", "s78", "s79", "s80", "s81", "s82", "s83", "s84", \
-    "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95", "s96", \
-    "s97", "s98", "s99", "scc", "memory"
+    "v192", "v193", "v194", "v195", "v196", "v197", "v198", "v199", "v200", "v201", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"
 #define GCOO_PANEL_ACC_OPERANDS_R42(a) \
     "+{v[0:7]}"(a[0]), "+{v[8:15]}"(a[1]), "+{v[16:23]}"(a[2]), "+{v[24:31]}"(a[3]), \
     "+{v[32:39]}"(a[4]), "+{v[40:47]}"(a[5]), "+{v[48:55]}"(a[6]), "+{v[56:63]}"(a[7]), \
@@ -5838,7 +7781,7 @@
     "+{v[128:135]}"(a[16]), "+{v[136:143]}"(a[17]), "+{v[144:151]}"(a[18]), "+{v[152:159]}"(a[19]), \
     "+{v[160:167]}"(a[20])
 // BF16 D, 16 rows per wave: accumulators v0..v63, component c of row slot q
-// in v[16 c + q]; scratch v64..v103
+// in v[16 c + q]; scratch v64..v97
 #define GCOO_PANEL_SEGMENT_ASM_BF16_R16 \
     "s_mov_b32 s28, m0\n\t" \
     "s_nop 4\n\t" \
@@ -6140,12 +8083,12 @@
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -6205,20 +8148,20 @@
     "ds_read_b64 v[64:65], v96\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
     "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
@@ -6262,20 +8205,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -6414,10 +8357,10 @@
     "ds_read_b64 v[64:65], v96\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
     "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
@@ -6469,12 +8412,12 @@
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
@@ -6609,14 +8552,14 @@
     ".Lf20_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
     "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
@@ -6660,20 +8603,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -6864,18 +8807,18 @@
     "ds_read_b64 v[64:65], v96\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
@@ -6974,18 +8917,18 @@
     "ds_read_b64 v[64:65], v96\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
@@ -7080,24 +9023,24 @@
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v98, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v99, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v100, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v101, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v102, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -7200,24 +9143,24 @@
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v98, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v99, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v100, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v101, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v102, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -7585,20 +9528,20 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -7654,24 +9597,24 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
     "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
@@ -7715,20 +9658,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -7863,14 +9806,14 @@
     ".Lf11_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
     "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
@@ -7914,20 +9857,20 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
@@ -8062,14 +10005,14 @@
     ".Lf21_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
     "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
@@ -8113,20 +10056,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -8313,22 +10256,22 @@
     ".Lpa1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
@@ -8423,22 +10366,22 @@
     ".Lpb1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
-    "v_mad_u32_u16 v103, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v103\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
+    "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
@@ -8525,32 +10468,32 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v97, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v98, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v99, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v100, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v101, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v102, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -8645,32 +10588,32 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
-    "v_mad_u32_u16 v96, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[64:65], v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v97, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[68:69], v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v98, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v98\n\t" \
+    "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[72:73], v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v99, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v99\n\t" \
+    "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[76:77], v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v100, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v100\n\t" \
+    "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[80:81], v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v101, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v101\n\t" \
+    "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v102, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v102\n\t" \
+    "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v96\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
@@ -8775,16 +10718,2960 @@
 #define GCOO_PANEL_SEGMENT_CLOBBERS_BF16_R16 \
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", \
     "v76", "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", \
-    "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", \
-    "v100", "v101", "v102", "v103", "s24", "s25", "s26", "s27", "s28", "s29", "s30", "s36", \
-    "s37", "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", \
-    "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", \
-    "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", \
-    "s73", "s74", "s75", "s76", "s77", "s78", "s79", "s80", "s81", "s82", "s83", "s84", \
-    "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95", "s96", \
-    "s97", "s98", "s99", "scc", "memory"
+    "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"
+// BF16 D, 21 rows per wave: accumulators v0..v83, component c of row slot q
+// in v[21 c + q]; scratch v84..v117
+#define GCOO_PANEL_SEGMENT_ASM_BF16_R21 \
+    "s_mov_b32 s28, m0\n\t" \
+    "s_nop 4\n\t" \
+    "v_readlane_b32 s29, %[vpbo], %[sli]\n\t" \
+    "v_readlane_b32 s30, %[vpn], %[sli]\n\t" \
+    "s_cmp_eq_u32 %[par], 0\n\t" \
+    "s_cbranch_scc0 .Lo1_%=\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lc0_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lc0_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s24\n\t" \
+    "s_branch .Lc0_%=\n\t" \
+    ".Lo1_%=:\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lc1_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lc1_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s24\n\t" \
+    "s_branch .Lc1_%=\n\t" \
+    ".Lc0_%=:\n\t" \
+    "s_mov_b32 s25, s30\n\t" \
+    "s_add_u32 s26, s29, 0x80\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lw0_%=\n\t" \
+    "s_waitcnt vmcnt(0)\n\t" \
+    ".Lw0_%=:\n\t" \
+    "s_barrier\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_add_u32 s24, %[sli], 1\n\t" \
+    "s_cmp_lt_u32 s24, %[liend]\n\t" \
+    "s_cbranch_scc0 .Lnp0_%=\n\t" \
+    "v_readlane_b32 s29, %[vpbo], s24\n\t" \
+    "v_readlane_b32 s30, %[vpn], s24\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lnp0_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lnp0_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s24\n\t" \
+    ".Lnp0_%=:\n\t" \
+    "s_cmp_le_u32 %[sso], %[lim]\n\t" \
+    "s_cbranch_scc0 .Lsg0_%=\n\t" \
+    "s_add_u32 m0, %[sdst], 0x12c00\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsd0_%=\n\t" \
+    ".Lsg0_%=:\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lsd0_%=\n\t" \
+    "s_add_u32 m0, %[sdst], 0x12c00\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz00_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn00_%=\n\t" \
+    ".Lsz00_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn00_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz10_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn10_%=\n\t" \
+    ".Lsz10_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn10_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz20_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn20_%=\n\t" \
+    ".Lsz20_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn20_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz30_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn30_%=\n\t" \
+    ".Lsz30_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn30_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz40_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn40_%=\n\t" \
+    ".Lsz40_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn40_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz50_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn50_%=\n\t" \
+    ".Lsz50_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn50_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz60_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn60_%=\n\t" \
+    ".Lsz60_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn60_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz70_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn70_%=\n\t" \
+    ".Lsz70_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn70_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz80_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn80_%=\n\t" \
+    ".Lsz80_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn80_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz90_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn90_%=\n\t" \
+    ".Lsz90_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn90_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz100_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn100_%=\n\t" \
+    ".Lsz100_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn100_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz110_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn110_%=\n\t" \
+    ".Lsz110_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn110_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz120_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn120_%=\n\t" \
+    ".Lsz120_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn120_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz130_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn130_%=\n\t" \
+    ".Lsz130_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn130_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd0_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz140_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn140_%=\n\t" \
+    ".Lsz140_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn140_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    ".Lsd0_%=:\n\t" \
+    "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lpr20_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lpr0_%=\n\t" \
+    ".Lpr20_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lpr0_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 4\n\t" \
+    "s_cbranch_scc1 .Lf00_%=\n\t" \
+    "s_mov_b32 s27, s25\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp0r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    ".Lp0r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    ".Lgvp0s0_%=:\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp0s0_%=\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    ".Lggp0s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf00_%=:\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_cmp_ge_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lq00_%=\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf10_%=\n\t" \
+    ".Lq00_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lz00_%=\n\t" \
+    "s_sub_u32 s27, s25, 4\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp1r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    ".Lp1r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    ".Lgvp1s0_%=:\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp1s0_%=\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    ".Lggp1s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz00_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf10_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lq10_%=\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf20_%=\n\t" \
+    ".Lq10_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lz10_%=\n\t" \
+    "s_sub_u32 s27, s25, 8\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp2r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    ".Lp2r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    ".Lgvp2s0_%=:\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp2s0_%=\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    ".Lggp2s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz10_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf20_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lq20_%=\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf30_%=\n\t" \
+    ".Lq20_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lz20_%=\n\t" \
+    "s_sub_u32 s27, s25, 12\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp3r0_%=\n\t" \
+    "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    ".Lp3r0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    ".Lgvp3s0_%=:\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp3s0_%=\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v112, v0\n\t" \
+    "v_fma_f32 v21, s67, v113, v21\n\t" \
+    "v_fma_f32 v42, s67, v114, v42\n\t" \
+    "v_fma_f32 v63, s67, v115, v63\n\t" \
+    ".Lggp3s0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lz20_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lf30_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v112, v0\n\t" \
+    "v_fma_f32 v21, s67, v113, v21\n\t" \
+    "v_fma_f32 v42, s67, v114, v42\n\t" \
+    "v_fma_f32 v63, s67, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_cmp_le_u32 s25, 16\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_sub_u32 s25, s25, 16\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
+    "s_add_u32 s26, s26, 0x40\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lba0_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lta0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpa0_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqa0_%=\n\t" \
+    ".Lpa0_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqa0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lna0_%=\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s26\n\t" \
+    ".Lna0_%=:\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s50\n\t" \
+    "v_fma_f32 v0, s51, v112, v0\n\t" \
+    "v_fma_f32 v21, s51, v113, v21\n\t" \
+    "v_fma_f32 v42, s51, v114, v42\n\t" \
+    "v_fma_f32 v63, s51, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lbb0_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Ltb0_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpb0_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqb0_%=\n\t" \
+    ".Lpb0_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqb0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lnb0_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
+    ".Lnb0_%=:\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s66\n\t" \
+    "v_fma_f32 v0, s67, v112, v0\n\t" \
+    "v_fma_f32 v21, s67, v113, v21\n\t" \
+    "v_fma_f32 v42, s67, v114, v42\n\t" \
+    "v_fma_f32 v63, s67, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce0_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_branch .Lba0_%=\n\t" \
+    ".Lta0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwa0_%=\n\t" \
+    "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    ".Lgwa0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgva0_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgva0_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgva0_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgva0_%=\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgva0_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgva0_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    ".Lgva0_%=:\n\t" \
+    "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s37, v84, v0\n\t" \
+    "v_fma_f32 v21, s37, v85, v21\n\t" \
+    "v_fma_f32 v42, s37, v86, v42\n\t" \
+    "v_fma_f32 v63, s37, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s38\n\t" \
+    "v_fma_f32 v0, s39, v88, v0\n\t" \
+    "v_fma_f32 v21, s39, v89, v21\n\t" \
+    "v_fma_f32 v42, s39, v90, v42\n\t" \
+    "v_fma_f32 v63, s39, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s40\n\t" \
+    "v_fma_f32 v0, s41, v92, v0\n\t" \
+    "v_fma_f32 v21, s41, v93, v21\n\t" \
+    "v_fma_f32 v42, s41, v94, v42\n\t" \
+    "v_fma_f32 v63, s41, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s42\n\t" \
+    "v_fma_f32 v0, s43, v96, v0\n\t" \
+    "v_fma_f32 v21, s43, v97, v21\n\t" \
+    "v_fma_f32 v42, s43, v98, v42\n\t" \
+    "v_fma_f32 v63, s43, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s44\n\t" \
+    "v_fma_f32 v0, s45, v100, v0\n\t" \
+    "v_fma_f32 v21, s45, v101, v21\n\t" \
+    "v_fma_f32 v42, s45, v102, v42\n\t" \
+    "v_fma_f32 v63, s45, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s46\n\t" \
+    "v_fma_f32 v0, s47, v104, v0\n\t" \
+    "v_fma_f32 v21, s47, v105, v21\n\t" \
+    "v_fma_f32 v42, s47, v106, v42\n\t" \
+    "v_fma_f32 v63, s47, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgea0_%=\n\t" \
+    "s_set_gpr_idx_idx s48\n\t" \
+    "v_fma_f32 v0, s49, v108, v0\n\t" \
+    "v_fma_f32 v21, s49, v109, v21\n\t" \
+    "v_fma_f32 v42, s49, v110, v42\n\t" \
+    "v_fma_f32 v63, s49, v111, v63\n\t" \
+    ".Lgea0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Ltb0_%=:\n\t" \
+    "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwb0_%=\n\t" \
+    "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    ".Lgwb0_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgvb0_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgvb0_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgvb0_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgvb0_%=\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgvb0_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgvb0_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    ".Lgvb0_%=:\n\t" \
+    "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s53, v84, v0\n\t" \
+    "v_fma_f32 v21, s53, v85, v21\n\t" \
+    "v_fma_f32 v42, s53, v86, v42\n\t" \
+    "v_fma_f32 v63, s53, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s54\n\t" \
+    "v_fma_f32 v0, s55, v88, v0\n\t" \
+    "v_fma_f32 v21, s55, v89, v21\n\t" \
+    "v_fma_f32 v42, s55, v90, v42\n\t" \
+    "v_fma_f32 v63, s55, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s56\n\t" \
+    "v_fma_f32 v0, s57, v92, v0\n\t" \
+    "v_fma_f32 v21, s57, v93, v21\n\t" \
+    "v_fma_f32 v42, s57, v94, v42\n\t" \
+    "v_fma_f32 v63, s57, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s58\n\t" \
+    "v_fma_f32 v0, s59, v96, v0\n\t" \
+    "v_fma_f32 v21, s59, v97, v21\n\t" \
+    "v_fma_f32 v42, s59, v98, v42\n\t" \
+    "v_fma_f32 v63, s59, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s60\n\t" \
+    "v_fma_f32 v0, s61, v100, v0\n\t" \
+    "v_fma_f32 v21, s61, v101, v21\n\t" \
+    "v_fma_f32 v42, s61, v102, v42\n\t" \
+    "v_fma_f32 v63, s61, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s62\n\t" \
+    "v_fma_f32 v0, s63, v104, v0\n\t" \
+    "v_fma_f32 v21, s63, v105, v21\n\t" \
+    "v_fma_f32 v42, s63, v106, v42\n\t" \
+    "v_fma_f32 v63, s63, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgeb0_%=\n\t" \
+    "s_set_gpr_idx_idx s64\n\t" \
+    "v_fma_f32 v0, s65, v108, v0\n\t" \
+    "v_fma_f32 v21, s65, v109, v21\n\t" \
+    "v_fma_f32 v42, s65, v110, v42\n\t" \
+    "v_fma_f32 v63, s65, v111, v63\n\t" \
+    ".Lgeb0_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce0_%=\n\t" \
+    ".Lce0_%=:\n\t" \
+    "s_setprio 0\n\t" \
+    "s_add_u32 %[sli], %[sli], 1\n\t" \
+    "s_cmp_lt_u32 %[sli], %[liend]\n\t" \
+    "s_cbranch_scc0 .Lx_%=\n\t" \
+    ".Lc1_%=:\n\t" \
+    "s_mov_b32 s25, s30\n\t" \
+    "s_add_u32 s26, s29, 0x80\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lw1_%=\n\t" \
+    "s_waitcnt vmcnt(0)\n\t" \
+    ".Lw1_%=:\n\t" \
+    "s_barrier\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_add_u32 s24, %[sli], 1\n\t" \
+    "s_cmp_lt_u32 s24, %[liend]\n\t" \
+    "s_cbranch_scc0 .Lnp1_%=\n\t" \
+    "v_readlane_b32 s29, %[vpbo], s24\n\t" \
+    "v_readlane_b32 s30, %[vpn], s24\n\t" \
+    "s_cmp_eq_u32 s30, 0\n\t" \
+    "s_cbranch_scc1 .Lnp1_%=\n\t" \
+    "s_load_dwordx16 s[36:51], %[pbase], s29\n\t" \
+    "s_cmp_le_u32 s30, 8\n\t" \
+    "s_cbranch_scc1 .Lnp1_%=\n\t" \
+    "s_add_u32 s24, s29, 0x40\n\t" \
+    "s_load_dwordx16 s[52:67], %[pbase], s24\n\t" \
+    ".Lnp1_%=:\n\t" \
+    "s_cmp_le_u32 %[sso], %[lim]\n\t" \
+    "s_cbranch_scc0 .Lsg1_%=\n\t" \
+    "s_mov_b32 m0, %[sdst]\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsd1_%=\n\t" \
+    ".Lsg1_%=:\n\t" \
+    "s_cmp_eq_u32 %[kb], 0\n\t" \
+    "s_cbranch_scc1 .Lsd1_%=\n\t" \
+    "s_mov_b32 m0, %[sdst]\n\t" \
+    "s_mov_b32 s24, %[sso]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz01_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn01_%=\n\t" \
+    ".Lsz01_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn01_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz11_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn11_%=\n\t" \
+    ".Lsz11_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn11_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz21_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn21_%=\n\t" \
+    ".Lsz21_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn21_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz31_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn31_%=\n\t" \
+    ".Lsz31_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn31_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz41_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn41_%=\n\t" \
+    ".Lsz41_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn41_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz51_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn51_%=\n\t" \
+    ".Lsz51_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn51_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz61_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn61_%=\n\t" \
+    ".Lsz61_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn61_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz71_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn71_%=\n\t" \
+    ".Lsz71_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn71_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz81_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn81_%=\n\t" \
+    ".Lsz81_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn81_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz91_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn91_%=\n\t" \
+    ".Lsz91_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn91_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz101_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn101_%=\n\t" \
+    ".Lsz101_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn101_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz111_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn111_%=\n\t" \
+    ".Lsz111_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn111_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz121_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn121_%=\n\t" \
+    ".Lsz121_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn121_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz131_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn131_%=\n\t" \
+    ".Lsz131_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn131_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    "s_cmp_lt_u32 s24, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsd1_%=\n\t" \
+    "s_add_u32 s27, s24, %[row1]\n\t" \
+    "s_cmp_lt_u32 s27, %[kb]\n\t" \
+    "s_cbranch_scc0 .Lsz141_%=\n\t" \
+    "buffer_load_dwordx4 %[voff], %[rsrc], s24 offen lds\n\t" \
+    "s_branch .Lsn141_%=\n\t" \
+    ".Lsz141_%=:\n\t" \
+    "buffer_load_dwordx4 %[voffz], %[rsrc], s24 offen lds\n\t" \
+    ".Lsn141_%=:\n\t" \
+    "s_add_u32 m0, m0, 0x400\n\t" \
+    "s_add_u32 s24, s24, %[rowb]\n\t" \
+    ".Lsd1_%=:\n\t" \
+    "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lpr21_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lpr1_%=\n\t" \
+    ".Lpr21_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lpr1_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 4\n\t" \
+    "s_cbranch_scc1 .Lf01_%=\n\t" \
+    "s_mov_b32 s27, s25\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp0r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    ".Lp0r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    ".Lgvp0s1_%=:\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp0s1_%=\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    ".Lggp0s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf01_%=:\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_cmp_ge_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lq01_%=\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf11_%=\n\t" \
+    ".Lq01_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lz01_%=\n\t" \
+    "s_sub_u32 s27, s25, 4\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp1r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    ".Lp1r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    ".Lgvp1s1_%=:\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp1s1_%=\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    ".Lggp1s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz01_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf11_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lq11_%=\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf21_%=\n\t" \
+    ".Lq11_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 8\n\t" \
+    "s_cbranch_scc0 .Lz11_%=\n\t" \
+    "s_sub_u32 s27, s25, 8\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp2r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    ".Lp2r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    ".Lgvp2s1_%=:\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp2s1_%=\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    ".Lggp2s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz11_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf21_%=:\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lq21_%=\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(4)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lf31_%=\n\t" \
+    ".Lq21_%=:\n\t" \
+    "s_cmp_gt_u32 s25, 12\n\t" \
+    "s_cbranch_scc0 .Lz21_%=\n\t" \
+    "s_sub_u32 s27, s25, 12\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lp3r1_%=\n\t" \
+    "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    ".Lp3r1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    ".Lgvp3s1_%=:\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_cmp_gt_u32 s27, 1\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_cmp_gt_u32 s27, 2\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    "s_cmp_gt_u32 s27, 3\n\t" \
+    "s_cbranch_scc0 .Lggp3s1_%=\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v112, v0\n\t" \
+    "v_fma_f32 v21, s99, v113, v21\n\t" \
+    "v_fma_f32 v42, s99, v114, v42\n\t" \
+    "v_fma_f32 v63, s99, v115, v63\n\t" \
+    ".Lggp3s1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lz21_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lf31_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v112, v0\n\t" \
+    "v_fma_f32 v21, s99, v113, v21\n\t" \
+    "v_fma_f32 v42, s99, v114, v42\n\t" \
+    "v_fma_f32 v63, s99, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_cmp_le_u32 s25, 16\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_sub_u32 s25, s25, 16\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
+    "s_add_u32 s26, s26, 0x40\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lba1_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lta1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpa1_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqa1_%=\n\t" \
+    ".Lpa1_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqa1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lna1_%=\n\t" \
+    "s_load_dwordx16 s[84:99], %[pbase], s26\n\t" \
+    ".Lna1_%=:\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s82\n\t" \
+    "v_fma_f32 v0, s83, v112, v0\n\t" \
+    "v_fma_f32 v21, s83, v113, v21\n\t" \
+    "v_fma_f32 v42, s83, v114, v42\n\t" \
+    "v_fma_f32 v63, s83, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    ".Lbb1_%=:\n\t" \
+    "s_cmp_lt_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Ltb1_%=\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cbranch_scc0 .Lpb1_%=\n\t" \
+    "s_setprio 3\n\t" \
+    "s_branch .Lqb1_%=\n\t" \
+    ".Lpb1_%=:\n\t" \
+    "s_setprio 2\n\t" \
+    ".Lqb1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[112:113], v117\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_cmp_le_u32 s25, 8\n\t" \
+    "s_cbranch_scc1 .Lnb1_%=\n\t" \
+    "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
+    ".Lnb1_%=:\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
+    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    "s_set_gpr_idx_idx s98\n\t" \
+    "v_fma_f32 v0, s99, v112, v0\n\t" \
+    "v_fma_f32 v21, s99, v113, v21\n\t" \
+    "v_fma_f32 v42, s99, v114, v42\n\t" \
+    "v_fma_f32 v63, s99, v115, v63\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_sub_u32 s25, s25, 8\n\t" \
+    "s_add_u32 s26, s26, 64\n\t" \
+    "s_cmp_eq_u32 s25, 0\n\t" \
+    "s_cbranch_scc1 .Lce1_%=\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "s_branch .Lba1_%=\n\t" \
+    ".Lta1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwa1_%=\n\t" \
+    "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    ".Lgwa1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgva1_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgva1_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgva1_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgva1_%=\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgva1_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgva1_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    ".Lgva1_%=:\n\t" \
+    "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s69, v84, v0\n\t" \
+    "v_fma_f32 v21, s69, v85, v21\n\t" \
+    "v_fma_f32 v42, s69, v86, v42\n\t" \
+    "v_fma_f32 v63, s69, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s70\n\t" \
+    "v_fma_f32 v0, s71, v88, v0\n\t" \
+    "v_fma_f32 v21, s71, v89, v21\n\t" \
+    "v_fma_f32 v42, s71, v90, v42\n\t" \
+    "v_fma_f32 v63, s71, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s72\n\t" \
+    "v_fma_f32 v0, s73, v92, v0\n\t" \
+    "v_fma_f32 v21, s73, v93, v21\n\t" \
+    "v_fma_f32 v42, s73, v94, v42\n\t" \
+    "v_fma_f32 v63, s73, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s74\n\t" \
+    "v_fma_f32 v0, s75, v96, v0\n\t" \
+    "v_fma_f32 v21, s75, v97, v21\n\t" \
+    "v_fma_f32 v42, s75, v98, v42\n\t" \
+    "v_fma_f32 v63, s75, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s76\n\t" \
+    "v_fma_f32 v0, s77, v100, v0\n\t" \
+    "v_fma_f32 v21, s77, v101, v21\n\t" \
+    "v_fma_f32 v42, s77, v102, v42\n\t" \
+    "v_fma_f32 v63, s77, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s78\n\t" \
+    "v_fma_f32 v0, s79, v104, v0\n\t" \
+    "v_fma_f32 v21, s79, v105, v21\n\t" \
+    "v_fma_f32 v42, s79, v106, v42\n\t" \
+    "v_fma_f32 v63, s79, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgea1_%=\n\t" \
+    "s_set_gpr_idx_idx s80\n\t" \
+    "v_fma_f32 v0, s81, v108, v0\n\t" \
+    "v_fma_f32 v21, s81, v109, v21\n\t" \
+    "v_fma_f32 v42, s81, v110, v42\n\t" \
+    "v_fma_f32 v63, s81, v111, v63\n\t" \
+    ".Lgea1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Ltb1_%=:\n\t" \
+    "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[84:85], v116\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[88:89], v117\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[92:93], v116\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[96:97], v117\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[100:101], v116\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[104:105], v117\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgwb1_%=\n\t" \
+    "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[108:109], v116\n\t" \
+    ".Lgwb1_%=:\n\t" \
+    "s_waitcnt lgkmcnt(0)\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
+    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgvb1_%=\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
+    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgvb1_%=\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
+    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgvb1_%=\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
+    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgvb1_%=\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
+    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgvb1_%=\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
+    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgvb1_%=\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
+    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    ".Lgvb1_%=:\n\t" \
+    "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
+    "v_fma_f32 v0, s85, v84, v0\n\t" \
+    "v_fma_f32 v21, s85, v85, v21\n\t" \
+    "v_fma_f32 v42, s85, v86, v42\n\t" \
+    "v_fma_f32 v63, s85, v87, v63\n\t" \
+    "s_cmp_gt_u32 s25, 1\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s86\n\t" \
+    "v_fma_f32 v0, s87, v88, v0\n\t" \
+    "v_fma_f32 v21, s87, v89, v21\n\t" \
+    "v_fma_f32 v42, s87, v90, v42\n\t" \
+    "v_fma_f32 v63, s87, v91, v63\n\t" \
+    "s_cmp_gt_u32 s25, 2\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s88\n\t" \
+    "v_fma_f32 v0, s89, v92, v0\n\t" \
+    "v_fma_f32 v21, s89, v93, v21\n\t" \
+    "v_fma_f32 v42, s89, v94, v42\n\t" \
+    "v_fma_f32 v63, s89, v95, v63\n\t" \
+    "s_cmp_gt_u32 s25, 3\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s90\n\t" \
+    "v_fma_f32 v0, s91, v96, v0\n\t" \
+    "v_fma_f32 v21, s91, v97, v21\n\t" \
+    "v_fma_f32 v42, s91, v98, v42\n\t" \
+    "v_fma_f32 v63, s91, v99, v63\n\t" \
+    "s_cmp_gt_u32 s25, 4\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s92\n\t" \
+    "v_fma_f32 v0, s93, v100, v0\n\t" \
+    "v_fma_f32 v21, s93, v101, v21\n\t" \
+    "v_fma_f32 v42, s93, v102, v42\n\t" \
+    "v_fma_f32 v63, s93, v103, v63\n\t" \
+    "s_cmp_gt_u32 s25, 5\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s94\n\t" \
+    "v_fma_f32 v0, s95, v104, v0\n\t" \
+    "v_fma_f32 v21, s95, v105, v21\n\t" \
+    "v_fma_f32 v42, s95, v106, v42\n\t" \
+    "v_fma_f32 v63, s95, v107, v63\n\t" \
+    "s_cmp_gt_u32 s25, 6\n\t" \
+    "s_cbranch_scc0 .Lgeb1_%=\n\t" \
+    "s_set_gpr_idx_idx s96\n\t" \
+    "v_fma_f32 v0, s97, v108, v0\n\t" \
+    "v_fma_f32 v21, s97, v109, v21\n\t" \
+    "v_fma_f32 v42, s97, v110, v42\n\t" \
+    "v_fma_f32 v63, s97, v111, v63\n\t" \
+    ".Lgeb1_%=:\n\t" \
+    "s_set_gpr_idx_off\n\t" \
+    "s_branch .Lce1_%=\n\t" \
+    ".Lce1_%=:\n\t" \
+    "s_setprio 0\n\t" \
+    "s_add_u32 %[sli], %[sli], 1\n\t" \
+    "s_cmp_lt_u32 %[sli], %[liend]\n\t" \
+    "s_cbranch_scc0 .Lx_%=\n\t" \
+    "s_branch .Lc0_%=\n\t" \
+    ".Lx_%=:\n\t" \
+    "s_mov_b32 m0, s28\n\t"
+#define GCOO_PANEL_SEGMENT_CLOBBERS_BF16_R21 \
+    "v84", "v85", "v86", "v87", "v88", "v89", "v90", "v91", "v92", "v93", "v94", "v95", \
+    "v96", "v97", "v98", "v99", "v100", "v101", "v102", "v103", "v104", "v105", "v106", "v107", \
+    "v108", "v109", "v110", "v111", "v112", "v113", "v114", "v115", "v116", "v117", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"
 // BF16 D, 28 rows per wave: accumulators v0..v111, component c of row slot q
-// in v[28 c + q]; scratch v112..v151
+// in v[28 c + q]; scratch v112..v145
 #define GCOO_PANEL_SEGMENT_ASM_BF16_R28 \
     "s_mov_b32 s28, m0\n\t" \
     "s_nop 4\n\t" \
@@ -9086,12 +13973,12 @@
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -9151,20 +14038,20 @@
     "ds_read_b64 v[112:113], v144\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
     "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
@@ -9208,20 +14095,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
-    "v_mad_u32_u16 v151, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -9360,10 +14247,10 @@
     "ds_read_b64 v[112:113], v144\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
     "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
@@ -9415,12 +14302,12 @@
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
@@ -9555,14 +14442,14 @@
     ".Lf20_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
     "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
@@ -9606,20 +14493,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
-    "v_mad_u32_u16 v151, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -9810,18 +14697,18 @@
     "ds_read_b64 v[112:113], v144\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
@@ -9920,18 +14807,18 @@
     "ds_read_b64 v[112:113], v144\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
@@ -10026,24 +14913,24 @@
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v146, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v147, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v148, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v149, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
-    "v_mad_u32_u16 v150, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -10146,24 +15033,24 @@
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v146, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v147, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v148, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v149, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
-    "v_mad_u32_u16 v150, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -10531,20 +15418,20 @@
     "s_cmp_ge_u32 s25, 4\n\t" \
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -10600,24 +15487,24 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
     "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
@@ -10661,20 +15548,20 @@
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
-    "v_mad_u32_u16 v151, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -10809,14 +15696,14 @@
     ".Lf11_%=:\n\t" \
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
     "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
@@ -10860,20 +15747,20 @@
     "s_cmp_gt_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
@@ -11008,14 +15895,14 @@
     ".Lf21_%=:\n\t" \
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(4)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
     "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
@@ -11059,20 +15946,20 @@
     "s_cmp_gt_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
-    "v_mad_u32_u16 v151, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -11259,22 +16146,22 @@
     ".Lpa1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s82, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
@@ -11369,22 +16256,22 @@
     ".Lpb1_%=:\n\t" \
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
-    "v_mad_u32_u16 v151, s98, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v151\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
+    "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[140:141], v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
@@ -11471,32 +16358,32 @@
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s68, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v145, s70, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v146, s72, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v147, s74, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v148, s76, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v149, s78, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
-    "v_mad_u32_u16 v150, s80, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -11591,32 +16478,32 @@
     "s_set_gpr_idx_off\n\t" \
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
-    "v_mad_u32_u16 v144, s84, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[112:113], v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v145, s86, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
+    "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
     "ds_read_b64 v[116:117], v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v146, s88, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v146\n\t" \
+    "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[120:121], v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v147, s90, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v147\n\t" \
+    "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[124:125], v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v148, s92, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v148\n\t" \
+    "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[128:129], v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v149, s94, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v149\n\t" \
+    "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[132:133], v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
-    "v_mad_u32_u16 v150, s96, 16, %[vb1] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v150\n\t" \
+    "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
+    "ds_read_b64 v[136:137], v144\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
@@ -11721,11 +16608,10 @@
 #define GCOO_PANEL_SEGMENT_CLOBBERS_BF16_R28 \
     "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", "v122", "v123", \
     "v124", "v125", "v126", "v127", "v128", "v129", "v130", "v131", "v132", "v133", "v134", "v135", \
-    "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", "v144", "v145", "v146", "v147", \
-    "v148", "v149", "v150", "v151", "s24", "s25", "s26", "s27", "s28", "s29", "s30", "s36", \
-    "s37", "s38", "s39", "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47", "s48", \
-    "s49", "s50", "s51", "s52", "s53", "s54", "s55", "s56", "s57", "s58", "s59", "s60", \
-    "s61", "s62", "s63", "s64", "s65", "s66", "s67", "s68", "s69", "s70", "s71", "s72", \
-    "s73", "s74", "s75", "s76", "s77", "s78", "s79", "s80", "s81", "s82", "s83", "s84", \
-    "s85", "s86", "s87", "s88", "s89", "s90", "s91", "s92", "s93", "s94", "s95", "s96", \
-    "s97", "s98", "s99", "scc", "memory"
+    "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", "v144", "v145", "s24", "s25", \
+    "s26", "s27", "s28", "s29", "s30", "s36", "s37", "s38", "s39", "s40", "s41", "s42", \
+    "s43", "s44", "s45", "s46", "s47", "s48", "s49", "s50", "s51", "s52", "s53", "s54", \
+    "s55", "s56", "s57", "s58", "s59", "s60", "s61", "s62", "s63", "s64", "s65", "s66", \
+    "s67", "s68", "s69", "s70", "s71", "s72", "s73", "s74", "s75", "s76", "s77", "s78", \
+    "s79", "s80", "s81", "s82", "s83", "s84", "s85", "s86", "s87", "s88", "s89", "s90", \
+    "s91", "s92", "s93", "s94", "s95", "s96", "s97", "s98", "s99", "scc", "memory"

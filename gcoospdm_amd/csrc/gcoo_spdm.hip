@@ -661,7 +661,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 constexpr int kPanelPrefetch = 6;  // chunks between the L2 touch and the staging
 constexpr double kPanelMinDensity = 0.0025;  // automatic choice of variant 6 (FP32) ...
-constexpr double kPanelMinDensityBf16 = 0.007;  // ... and for BF16
+constexpr double kPanelMinDensityBf16 = 0.005;  // ... and for BF16
 constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgroup per CU
 
 typedef float f32x8 __attribute__((ext_vector_type(8)));
@@ -674,8 +674,8 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
 {
     constexpr int VEC = 4;
     constexpr int PR = WAVES * RPW;           // panel rows
-    static_assert(E::bytes == 4 ? (RPW == 16 || RPW == 28 || RPW == 42)
-                                : (RPW == 16 || RPW == 28),
+    static_assert(E::bytes == 4 ? (RPW == 16 || RPW == 21 || RPW == 28 || RPW == 42)
+                                : (RPW == 16 || RPW == 21 || RPW == 28),
                   "rows per wave of the generated asm blocks");
     constexpr int TN = 64 * VEC;              // columns per tile
     constexpr int LB = VEC * E::bytes;        // bytes of a D row per lane
@@ -758,11 +758,13 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
         }
     };
 
-    // 4 * RPW accumulators pinned to v0.. in f32x8 blocks (component c of row
-    // slot q in register RPW c + q)
-    f32x8 acc[RPW / 2];
+    // 4 * RPW accumulators pinned to v0.. in f32x8 blocks, f32x4 blocks for
+    // an odd RPW (component c of row slot q in register RPW c + q)
+    constexpr int AW = RPW % 2 == 0 ? 8 : 4;
+    typedef float accblk __attribute__((ext_vector_type(AW)));
+    accblk acc[4 * RPW / AW];
 #pragma unroll
-    for (int i = 0; i < RPW / 2; ++i) acc[i] = 0.0f;
+    for (int i = 0; i < 4 * RPW / AW; ++i) acc[i] = 0.0f;
     // this wave's chunk list starts (nC + 1 of them, the last its end)
     const int *op = poff + ((size_t)panel * WAVES + wave) * ((size_t)nC + 1);
     const unsigned char *ldsLane = reinterpret_cast<const unsigned char *>(sD) + lane * LB;
@@ -800,7 +802,6 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
                           (unsigned)dBytes, (unsigned)kBufferFlags};
         const unsigned vb0 =
             (unsigned)(size_t)((const __attribute__((address_space(3))) unsigned char *)ldsLane);
-        const unsigned vb1 = vb0 + CHUNKB;
         const unsigned sdst = __builtin_amdgcn_readfirstlane((unsigned)(size_t)ldsBase);
         // staging: row offsets below kb are loaded, and when the wave's last
         // row of a chunk is below K too (offset <= lim) without tests; the
@@ -839,17 +840,19 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
     asm volatile(GCOO_PANEL_SEGMENT_ASM_##T##R##N                                          \
                  : GCOO_PANEL_ACC_OPERANDS_R##N(acc), [sli] "+s"(sli), [sso] "+s"(sso)     \
                  : [pbase] "s"(pbase), [vpbo] "v"(vpbo), [vpn] "v"(vpn), [liend] "s"(liend), \
-                   [par] "s"(par), [vb0] "v"(vb0), [vb1] "v"(vb1), [sdst] "s"(sdst),      \
+                   [par] "s"(par), [vb0] "v"(vb0), [sdst] "s"(sdst),                      \
                    [lim] "s"(lim), [kb] "s"(kb), [rowb] "s"(uload), [row1] "s"(urow),     \
                    [tkrowb] "s"(tkrowb), [voff] "v"(pvoff), [voffz] "v"(pvoffz),          \
                    [rsrc] "s"(rs)                                                          \
                  : GCOO_PANEL_SEGMENT_CLOBBERS_##T##R##N)
                 if constexpr (E::bytes == 4) {
                     if constexpr (RPW == 16) GCOO_PANEL_SEGMENT(, 16);
+                    else if constexpr (RPW == 21) GCOO_PANEL_SEGMENT(, 21);
                     else if constexpr (RPW == 28) GCOO_PANEL_SEGMENT(, 28);
                     else GCOO_PANEL_SEGMENT(, 42);
                 } else {
                     if constexpr (RPW == 16) GCOO_PANEL_SEGMENT(BF16_, 16);
+                    else if constexpr (RPW == 21) GCOO_PANEL_SEGMENT(BF16_, 21);
                     else GCOO_PANEL_SEGMENT(BF16_, 28);
                 }
 #undef GCOO_PANEL_SEGMENT
@@ -870,7 +873,7 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
             const int r = panel * PR + wave * RPW + q;
             float out[VEC];
 #pragma unroll
-            for (int c = 0; c < VEC; ++c) out[c] = acc[(RPW * c + q) / 8][(RPW * c + q) % 8];
+            for (int c = 0; c < VEC; ++c) out[c] = acc[(RPW * c + q) / AW][(RPW * c + q) % AW];
             if (r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
         }
     }
@@ -1208,23 +1211,42 @@ int max_vec(const void *D, int N, const float *C, int elemBytes)
     return 1;
 }
 
-// Waves of the panel kernel's workgroup: the configured `waves` (FP32: 16
-// waves of 16 rows, 12 of 28 or 8 of 42; BF16: 16 of 16), or automatically
-// 16 x 16 from kPanelTallDensity on and 12 x 28 below it.  The taller panel
-// shares each staged chunk among 336 rows instead of 256, a quarter less
-// staging, but has fewer waves to hide the entry loop's latencies (n = 8000,
-// FP32: 457 against 467 us at 1 % density, 425 against 407 us at 0.8 %, 392
-// against 339 us at 0.5 %, 370 against 311 us at 0.3 %).
-constexpr double kPanelTallDensity = 0.009;
+// Shape (waves x rows per wave) of the panel kernel's workgroup: the
+// configured one -- `waves` 16, 12 or 8 (8 for FP32 only) with `chunk` rows
+// per wave (16 waves: 16 or 21; 12: 28; 8: 42; 0: the wave count's first) --
+// or automatically 16 x 16 or 16 x 21, whichever the estimate below puts
+// lower.  A workgroup holds a CU's LDS, so the grid runs in rounds of one
+// workgroup per CU; a workgroup's time is about its panel's entries plus its
+// staging of the K x 256 D tile, which does not depend on the panel's rows.
+// Taller panels share each staged chunk among more rows (336 against 256
+// rows: a quarter less staging); 16 x 21 keeps 4 waves per SIMD (128 VGPRs),
+// where 12 x 28 has 3 and hides less of the entry loop's latencies (n =
+// 8000, FP32: 414 / 456 / 472 us for 16 x 21 / 16 x 16 / 12 x 28 at 1 %
+// density, 346 / 408 / 382 at 0.7 %, 319 / 386 / 340 at 0.5 %; BF16 16 x 21
+// / 12 x 28: 460 / 492 us at 1 %).
+struct PanelShape {
+    int waves, rows;
+};
 template <typename E>
-int panel_waves(const gcoo_matrix *h, int cfgWaves)
+PanelShape panel_shape(const gcoo_matrix *h, int N)
 {
-    if (cfgWaves == 16 || cfgWaves == 12 || (cfgWaves == 8 && E::bytes == 4)) return cfgWaves;
-    if constexpr (E::bytes != 4) return 12;  // BF16: 336-row panels at every density
+    const gcoo_spdm_config &cfg = h->spdm_cfg;
+    const int r = cfg.chunk;
+    if (cfg.waves == 16 || (cfg.waves == 0 && (r == 16 || r == 21)))
+        return {16, r == 21 ? 21 : 16};
+    if (cfg.waves == 12 || (cfg.waves == 0 && r == 28)) return {12, 28};
+    if ((cfg.waves == 8 || (cfg.waves == 0 && r == 42)) && E::bytes == 4) return {8, 42};
+    // rounds of workgroups x (entries of the panel, in rows at 1 % density,
+    // plus the staging, counted as 256 such rows)
     const double cells = (double)h->M * (double)h->K;
-    return cells > 0 && (double)h->nnz >= kPanelTallDensity * cells ? 16 : 12;
+    const double d100 = cells > 0 ? 100.0 * (double)h->nnz / cells : 0.0;
+    const long long tiles = (N + 255) / 256;
+    auto cost = [&](int pr) {
+        const long long wgs = (long long)((h->M + pr - 1) / pr) * tiles;
+        return (double)((wgs + 255) / 256) * (pr * d100 + 256.0);
+    };
+    return cost(336) <= cost(256) ? PanelShape{16, 21} : PanelShape{16, 16};
 }
-inline int panel_rows_of(int waves) { return waves == 12 ? 12 * 28 : waves == 8 ? 8 * 42 : 256; }
 
 // The variant (2..6) and columns per lane that a launch of h on these
 // operands runs: the handle's configuration, or the automatic choice, after
@@ -1246,15 +1268,17 @@ void resolve_variant(const gcoo_matrix *h, const void *D, int N, const float *C,
         // against 351 us at s = 0.997, 316 against 248 us at 0.998).  BF16
         // gathers move half the bytes and the BF16 panel kernel widens every
         // D value (4 more vector instructions per entry), so for BF16 it only
-        // wins from about 0.7 % (336-row panels: 492 against 569 us at 1 %,
-        // 419 against 453 us at 0.8 %, 310 against 290 us at 0.5 %).  The density is the handle's last completed conversion (a
+        // wins from about 0.5 % (16 x 21 panels: 460 against 565 us at 1 %,
+        // 365 against 400 us at 0.7 %, 287 against 292 us at 0.5 %, 212
+        // against 194 us at 0.3 %).  The density is the handle's last completed conversion (a
         // first optimistic launch does not know the new count yet); every
         // variant gives the same bits.  It also needs a workgroup per CU at
         // least (256 x 256 tiles of C).
         const double cells = (double)h->M * (double)h->K;
         const double minDensity = E::bytes == 4 ? kPanelMinDensity : kPanelMinDensityBf16;
         const bool dense = cells > 0 && (double)h->nnz >= minDensity * cells;
-        const int pr = panel_rows_of(panel_waves<E>(h, cfg.waves));
+        const PanelShape ps = panel_shape<E>(h, N);
+        const int pr = ps.waves * ps.rows;
         const long long wgs = (long long)((h->M + pr - 1) / pr) * ((N + 255) / 256);
         variant = (dense && wgs >= kPanelMinWorkgroups) ? 6 : 2;
     }
@@ -1300,17 +1324,13 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         }
     }
     if (variant == 6) {
-        if constexpr (E::bytes == 4) {
-            switch (panel_waves<E>(h, cfg.waves)) {
-            case 12: return launch_panel<E, 75, 12, 28>(h, seg, D, N, C, guard, s, xs);
-            case 8: return launch_panel<E, 75, 8, 42>(h, seg, D, N, C, guard, s, xs);
-            default: return launch_panel<E, 75, 16, 16>(h, seg, D, N, C, guard, s, xs);
-            }
-        } else {
-            if (panel_waves<E>(h, cfg.waves) == 12)
-                return launch_panel<E, 150, 12, 28>(h, seg, D, N, C, guard, s, xs);
-            return launch_panel<E, 150, 16, 16>(h, seg, D, N, C, guard, s, xs);
-        }
+        constexpr int TK = E::bytes == 4 ? 75 : 150;
+        const PanelShape ps = panel_shape<E>(h, N);
+        if (ps.rows == 21) return launch_panel<E, TK, 16, 21>(h, seg, D, N, C, guard, s, xs);
+        if (ps.rows == 28) return launch_panel<E, TK, 12, 28>(h, seg, D, N, C, guard, s, xs);
+        if constexpr (E::bytes == 4)
+            if (ps.rows == 42) return launch_panel<E, TK, 8, 42>(h, seg, D, N, C, guard, s, xs);
+        return launch_panel<E, TK, 16, 16>(h, seg, D, N, C, guard, s, xs);
     }
     if (variant == 5) {
         // widest 16-byte-per-lane vector the operands allow
@@ -1393,9 +1413,10 @@ int launch_spdm_f32_cfg(const gcoo_matrix *h, const float *D, int N, float *C,
 }
 
 int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const float *C,
-                int *variant, int *vec, int *waves)
+                int *variant, int *vec, int *waves, int *rows)
 {
     *waves = 0;
+    *rows = 0;
     if (dtype == 0) {
         const unsigned long long dBytes = (unsigned long long)h->K * N * 4;
         if (h->spdm_cfg.variant == 1 || dBytes >= 0x7fffffffull) {
@@ -1406,10 +1427,18 @@ int spdm_choice(const gcoo_matrix *h, int dtype, const void *D, int N, const flo
             return GCOO_OK;
         }
         resolve_variant<F32>(h, D, N, C, variant, vec);
-        if (*variant == 6) *waves = panel_waves<F32>(h, h->spdm_cfg.waves);
+        if (*variant == 6) {
+            const PanelShape ps = panel_shape<F32>(h, N);
+            *waves = ps.waves;
+            *rows = ps.rows;
+        }
     } else {
         resolve_variant<BF16>(h, D, N, C, variant, vec);
-        if (*variant == 6) *waves = panel_waves<BF16>(h, h->spdm_cfg.waves);
+        if (*variant == 6) {
+            const PanelShape ps = panel_shape<BF16>(h, N);
+            *waves = ps.waves;
+            *rows = ps.rows;
+        }
     }
     return GCOO_OK;
 }

@@ -44,10 +44,12 @@ Per chunk, in order:
     the wait for them, the scalar load of the next batch's 8 entries (run
     under this batch's steps), 8 steps, the wait for the next entries.  Two
     register sets alternate; the last 1..7 entries take one guarded round.
-Entry u is two words in SGPRs: x = (LDS byte offset of its D row / 16) << 16
-| row slot, and the FP32 bits of its value.  The LDS address of an entry is
-one vector multiply-add of x's high half (v_mad_u32_u16 with op_sel: hi16 *
-16 + the lane's buffer address), so no scalar instruction goes into it.  A
+Entry u is two words in SGPRs: x = (LDS byte offset of its D row from
+buffer 0, so buffer 1's rows at + 75 KB, / 16) << 16 | row slot, and the
+FP32 bits of its value.  The LDS address of an entry is one vector
+multiply-add of x's high half (v_mad_u32_u16 with op_sel: hi16 * 16 + the
+lane's address in buffer 0), so no scalar instruction goes into it; two
+address VGPRs serve all reads in turn (a read takes its address at issue).  A
 step is acc[slot] = fmaf(value, d, acc[slot]) for the 4 columns, in VGPR-index
 mode (s_set_gpr_idx_on/_idx take the slot from bits 7:0 of x).  The wave's issue priority is 3 for a chunk of
 12 or more entries (2 below), 3 / 2 before each batch by the entries left,
@@ -55,13 +57,13 @@ and 0 at the chunk's end.
 
 Operands: %[pbase] SGPR pair; %[vpbo], %[vpn] VGPRs as above; %[sli]
 (in/out: lane of the current chunk, the segment's end on exit) and %[liend]
-SGPRs; %[par] SGPR, parity of the first chunk; %[vb0], %[vb1] VGPRs, this
-lane's LDS byte address in buffer 0 / 1; %[sdst] SGPR, the wave's staging
+SGPRs; %[par] SGPR, parity of the first chunk; %[vb0] VGPR, this lane's
+LDS byte address in buffer 0; %[sdst] SGPR, the wave's staging
 destination in buffer 0; %[sso] (in/out: D byte offset of the wave's first
 staging row of the next chunk, advanced by %[tkrowb] per chunk), %[lim],
 %[kb], %[rowb], %[tkrowb] SGPRs; %[voff] VGPR; %[rsrc] the buffer resource
 of D; the accumulators v0..v(4R-1) (component c of row slot q in
-v[R c + q]).  Scratch: v(4R)..v(4R+39), s24..s30 and s36..s99 (s32..s34 are
+v[R c + q]).  Scratch: v(4R)..v(4R+33), s24..s30 and s36..s99 (s32..s34 are
 the stack, frame and base pointers).  M0 is saved and restored.  The scalar loads
 read up to 7 entries past the end of a chunk's list, which the list capacity
 covers; no LDS read is issued for an entry past the end.  Every wave runs
@@ -73,26 +75,32 @@ ST, SREM, SOFF, SCNT, SM0, PBN, SNN = 24, 25, 26, 27, 28, 29, 30
 BANKS = ((list(range(36, 52)), list(range(52, 68))),
          (list(range(68, 84)), list(range(84, 100))))
 CHUNKB = 75 * 1024
-ROWS = (16, 28, 42)       # rows per wave of the generated FP32 blocks
-ROWS_BF16 = (16, 28)      # ... and BF16 blocks
+ROWS = (16, 21, 28, 42)   # rows per wave of the generated FP32 blocks
+ROWS_BF16 = (16, 21, 28)  # ... and BF16 blocks
+NSCR = 34                 # scratch VGPRs above the accumulators
 R = 16                    # rows per wave of the block being generated
 BF16 = False              # the block reads BF16 D (8 bytes per lane) and widens it
-DV, AV = 64, 96           # D values of the batch loop, their addresses
+DV, AV = 64, 96           # D values of the batch loop; two address VGPRs, used in turn
 G_D = (64, 80)            # D buffers of the rolling phase
-G_AV = (96, 100)          # their address VGPRs
 L = []
 
 
 def configure(rows, bf16=False):
     """Register layout for `rows` rows per wave: accumulators v0..v(4 rows - 1),
     scratch from v(4 rows) on."""
-    global R, DV, AV, G_D, G_AV, BF16
+    global R, DV, AV, G_D, BF16
     R = rows
     BF16 = bf16
     base = 4 * rows
     DV, AV = base, base + 32
     G_D = (base, base + 16)
-    G_AV = (base + 32, base + 36)
+
+
+def av(u):
+    """Address VGPR of the u-th LDS read of a group or batch: two, used in
+    turn (a read takes its address when it issues, so the next-but-one
+    address may overwrite it)."""
+    return AV + u % 2
 
 
 def e(x):
@@ -105,7 +113,7 @@ class Body:
     def __init__(self, p):
         self.p = p
         self.A, self.B = BANKS[p]
-        self.base = f"%[vb{p}]"
+        self.base = "%[vb0]"   # list offsets include the buffer
 
     # ---- D reads and their widening ----
     @staticmethod
@@ -132,8 +140,8 @@ class Body:
     # ---- batch loop (entries from 16 on) ----
     def reads(self, S, k):
         for u in range(k):
-            e(f"v_mad_u32_u16 v{AV+u}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
-            self.dread(DV + 4 * u, AV + u)
+            e(f"v_mad_u32_u16 v{av(u)}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
+            self.dread(DV + 4 * u, av(u))
 
     @staticmethod
     def entry(S, u):
@@ -153,13 +161,13 @@ class Body:
 
     def tail(self, S, t):
         p = self.p
-        e(f"v_mad_u32_u16 v{AV}, s{S[0]}, 16, {self.base} op_sel:[1,0,0,0]")
-        self.dread(DV, AV)
+        e(f"v_mad_u32_u16 v{av(0)}, s{S[0]}, 16, {self.base} op_sel:[1,0,0,0]")
+        self.dread(DV, av(0))
         for u in range(1, 7):
             e(f"s_cmp_gt_u32 s{SREM}, {u}")
             e(f"s_cbranch_scc0 .Lgw{t}{p}_%=")
-            e(f"v_mad_u32_u16 v{AV+u}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
-            self.dread(DV + 4 * u, AV + u)
+            e(f"v_mad_u32_u16 v{av(u)}, s{S[2*u]}, 16, {self.base} op_sel:[1,0,0,0]")
+            self.dread(DV + 4 * u, av(u))
         e(f".Lgw{t}{p}_%=:")
         e("s_waitcnt lgkmcnt(0)")
         if BF16:
@@ -218,8 +226,8 @@ class Body:
         b = g % 2
         for q in range(4):
             sx, _ = self.group_entry(g, q)
-            e(f"v_mad_u32_u16 v{G_AV[b]+q}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
-            self.dread(G_D[b] + 4 * q, G_AV[b] + q)
+            e(f"v_mad_u32_u16 v{av(q)}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
+            self.dread(G_D[b] + 4 * q, av(q))
 
     def group_entry_steps(self, g, q):
         _, sv = self.group_entry(g, q)
@@ -266,8 +274,8 @@ class Body:
             if q:
                 e(f"s_cmp_gt_u32 s{SCNT}, {q}")
                 e(f"s_cbranch_scc0 .L{tag}{self.p}_%=")
-            e(f"v_mad_u32_u16 v{G_AV[b]+q}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
-            self.dread(G_D[b] + 4 * q, G_AV[b] + q)
+            e(f"v_mad_u32_u16 v{av(q)}, s{sx}, 16, {self.base} op_sel:[1,0,0,0]")
+            self.dread(G_D[b] + 4 * q, av(q))
         e(f".L{tag}{self.p}_%=:")
 
     def roll16(self):
@@ -449,13 +457,13 @@ def main():
         tag = f"{'BF16_' if bf16 else ''}R{rows}"
         print(f"// {'BF16' if bf16 else 'FP32'} D, {rows} rows per wave: accumulators "
               f"v0..v{4 * rows - 1}, component c of row slot q")
-        print(f"// in v[{rows} c + q]; scratch v{4 * rows}..v{4 * rows + 39}")
+        print(f"// in v[{rows} c + q]; scratch v{4 * rows}..v{4 * rows + NSCR - 1}")
         print(f"#define GCOO_PANEL_SEGMENT_ASM_{tag} \\")
         for i, x in enumerate(lines):
             end = " \\" if i + 1 < len(lines) else ""
             print(f'    "{x}\\n\\t"{end}')
         print(f"#define GCOO_PANEL_SEGMENT_CLOBBERS_{tag} \\")
-        regs = ([f'"v{i}"' for i in range(4 * rows, 4 * rows + 40)] +
+        regs = ([f'"v{i}"' for i in range(4 * rows, 4 * rows + NSCR)] +
                 [f'"s{i}"' for i in range(24, 31)] +
                 [f'"s{i}"' for i in range(36, 100)] + ['"scc"', '"memory"'])
         for i in range(0, len(regs), 12):
@@ -463,7 +471,9 @@ def main():
             print("    " + ", ".join(regs[i:i+12]) + end)
         if bf16:
             continue
-        ops = [f'"+{{v[{8 * i}:{8 * i + 7}]}}"(a[{i}])' for i in range(rows // 2)]
+        # f32x8 blocks of accumulators, f32x4 blocks for an odd row count
+        w = 8 if rows % 2 == 0 else 4
+        ops = [f'"+{{v[{w * i}:{w * i + w - 1}]}}"(a[{i}])' for i in range(4 * rows // w)]
         print(f"#define GCOO_PANEL_ACC_OPERANDS_R{rows}(a) \\")
         for i in range(0, len(ops), 4):
             end = ", \\" if i + 4 < len(ops) else ""

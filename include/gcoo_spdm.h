@@ -181,18 +181,22 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              chain from there; the others take variant 2.
  *   variant 5  group rows: one wave walks the 4 rows of a row group at once,
  *              16 lanes per row, each lane loading 16 bytes of D per entry.
- *   variant 6  panel-staged: a 1024-thread workgroup owns 256 rows x 256
- *              columns; k-chunks of the D tile are staged in LDS once and
- *              serve all 256 rows (needs N * element bytes % 16 == 0, a
- *              16-byte aligned D and C; otherwise variant 2 runs).
+ *   variant 6  panel-staged: a workgroup owns a panel of rows (waves x
+ *              rows per wave: 16 x 16, 16 x 21, 12 x 28 or, FP32 only,
+ *              8 x 42) x 256 columns; k-chunks of the D tile are staged in
+ *              LDS once and serve all the panel's rows (needs N * element
+ *              bytes % 16 == 0, a 16-byte aligned D and C; otherwise
+ *              variant 2 runs).
  *   vec        columns per lane (1, 2, 4; 8 for BF16), at most 16 bytes of
  *              D per lane; a wave covers 64*vec columns.
- *   waves      waves per workgroup (1..16) of variant 3; for variant 6
- *              (FP32) 16, 12 or 8 waves of 16, 28 or 42 rows (0: 16 from
- *              0.9 % density on, 12 below).
+ *   waves      waves per workgroup (1..16) of variant 3; for variant 6 16,
+ *              12 or (FP32) 8 (0: automatic, 16 x 16 or 16 x 21 by the grid's
+ *              rounds of one workgroup per CU).
  *   chunk      k-chunk length of variant 3; long-row threshold of variant 4;
  *              rows per wave of variant 2 (1: one row, another nonzero value:
- *              the group's cpg rows, 0: one row once rows average 12 entries).
+ *              the group's cpg rows, 0: one row once rows average 12 entries);
+ *              rows per wave of variant 6 (16 or 21 with 16 waves, 28 with
+ *              12, 42 with 8; 0: 16, 28, 42 by the waves).
  *   unroll     entries whose D loads are in flight per wave in variant 2
  *              (8, 16 or 32; 0 means 16).
  * All variants give bit-identical C. */
@@ -206,7 +210,7 @@ typedef struct gcoo_spdm_config {
 int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg);
 int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
 /* The variant (1..6), columns per lane and, for variant 6, waves per
- * workgroup (16: 16 rows each, 12: 28, 8: 42; else 0) that a multiply of h
+ * workgroup and rows per wave (else 0 and 0) that a multiply of h
  * with these operands (dtype 0: gcoo_spdm_f32*, 1: gcoo_spdm_bf16*) runs: the
  * configuration, or the automatic choice from the last conversion's density
  * and the shape, after the fall-backs of each variant's operand
@@ -214,7 +218,7 @@ int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
  * lets measurement tools name the kernel they time.  D and C are only
  * inspected for alignment. */
 int gcoo_get_spdm_choice(gcoo_handle h, int dtype, const void *D, int N, const float *C,
-                         int *variant, int *vec, int *waves);
+                         int *variant, int *vec, int *waves, int *rows);
 
 /* Convert S then multiply: ShGCOOSpDM::mutiplyBy (src/ShGCOOSpDM.cu:131-180),
  * without the reference's per-call allocations once capacity is reached. */

@@ -130,21 +130,22 @@ def test_config1_800_s09_vs_fp64(dev):
     assert err <= FP32_REL_TOL
 
 
-@pytest.mark.parametrize("variant,waves", [(0, 0), (6, 16), (6, 12), (6, 8)])
+@pytest.mark.parametrize("variant,waves,rows", [(0, 0, 0), (6, 16, 16), (6, 16, 21), (6, 12, 0),
+                                               (6, 8, 0)])
 @pytest.mark.parametrize("sparsity", [0.99, 0.995, 0.999])
-def test_config2_n8000_full_size(dev, sparsity, variant, waves):
+def test_config2_n8000_full_size(dev, sparsity, variant, waves, rows):
     """n = 8000: conversion bit-exact at full size, and all of C bit-exact
     against the oracle's fmaf chain plus FP64 rel-Frobenius <= 1e-5 over all
     of C (the check of src/main.cu:128-142 at this size)."""
     import torch
     n = 8000
-    print(f"config2 n={n} s={sparsity} variant={variant} waves={waves}", flush=True)
+    print(f"config2 n={n} s={sparsity} variant={variant} waves={waves} rows={rows}", flush=True)
     Sd = torch.empty((n, n), device=dev)
     Dd = torch.empty((n, n), device=dev)
     G.fill_sparse(Sd, 7, 1, sparsity)
     G.fill_uniform(Dd, 7, 0)
     C = torch.empty((n, n), device=dev)
-    m = G.GCOOMatrix(n, n, 4).set_config(variant=variant, waves=waves)
+    m = G.GCOOMatrix(n, n, 4).set_config(variant=variant, waves=waves, chunk=rows)
     m.multiply_by(Sd, Dd, C)
     S = Sd.cpu().numpy()
     D = Dd.cpu().numpy()
@@ -254,18 +255,19 @@ def test_row_segment_rounds_bit_exact(dev, rows_per_wave, unroll, M, K, N, dens)
     (64, 2000, 1030, 0.05),    # N * 4 not a multiple of 16: row-segment fallback
     (300, 301, 256, 0.05),     # odd K: BF16's last staged row pair straddles K
 ])
-@pytest.mark.parametrize("dtype,waves", [("f32", 16), ("f32", 12), ("f32", 8),
-                                         ("bf16", 16), ("bf16", 12)])
-def test_panel_variant_shapes(dev, M, K, N, dens, dtype, waves):
+@pytest.mark.parametrize("dtype,waves,rows", [("f32", 16, 0), ("f32", 16, 21), ("f32", 12, 0),
+                                              ("f32", 8, 0), ("bf16", 16, 0), ("bf16", 16, 21),
+                                              ("bf16", 12, 0)])
+def test_panel_variant_shapes(dev, M, K, N, dens, dtype, waves, rows):
     """The panel-staged multiply (variant 6; workgroups of 16 waves x 16
-    rows, 12 x 28 or 8 x 42; FP32 D in 75-row chunks, BF16 D in 150-row
-    chunks) across panel, chunk and tile edges gives the reference's bits
-    (BF16: the chain over BF16-widened operands), and follows re-conversions
-    of the handle (its panel lists are rebuilt)."""
+    rows, 16 x 21, 12 x 28 or 8 x 42; FP32 D in 75-row chunks, BF16 D in
+    150-row chunks) across panel, chunk and tile edges gives the reference's
+    bits (BF16: the chain over BF16-widened operands), and follows
+    re-conversions of the handle (its panel lists are rebuilt)."""
     import torch
-    print(f"panel {dtype} waves={waves} M={M} K={K} N={N} dens={dens}", flush=True)
+    print(f"panel {dtype} waves={waves} rows={rows} M={M} K={K} N={N} dens={dens}", flush=True)
     rng = np.random.default_rng(M * 7 + K + N)
-    m = G.GCOOMatrix(M, K, 4).set_config(variant=6, waves=waves)
+    m = G.GCOOMatrix(M, K, 4).set_config(variant=6, waves=waves, chunk=rows)
     for d in (dens, dens / 2):
         S = H.rng_sparse(rng, M, K, d)
         S[M // 2, : min(K, 700)] = rng.uniform(-1, 1, min(K, 700))  # a long row

@@ -31,10 +31,10 @@ CHUNKB = 75 * 1024
 ROWB = 32000
 
 
-ROWS = (16, 28, 42)
+ROWS = (16, 21, 28, 42)
 
 
-ROWS_BF16 = (16, 28)
+ROWS_BF16 = (16, 21, 28)
 
 
 def _instructions(rows=16, bf16=False):
@@ -301,11 +301,13 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16, bf16=Fals
     per = [rng.randrange(20) for _ in range(c0 - cw)] + list(counts)
     per += [rng.randrange(20) for _ in range(64 - len(per))]
     entries, vpbo, vpn, chunks = [], [], [], []
-    for n in per:
+    for i, n in enumerate(per):
         vpbo.append(len(entries) * 8)
         vpn.append(n)
         slots = sorted(rng.randrange(rows) for _ in range(n))
-        ch = [(word(rng.randrange(tk) * rowd, slot), rng.getrandbits(32)) for slot in slots]
+        # offsets from buffer 0: chunk c's rows are in buffer c & 1
+        buf = ((cw + i) & 1) * CHUNKB
+        ch = [(word(buf + rng.randrange(tk) * rowd, slot), rng.getrandbits(32)) for slot in slots]
         chunks.append(ch)
         entries += ch
     entries += [(word(rng.randrange(tk) * rowd, rng.randrange(rows)), rng.getrandbits(32))
@@ -317,7 +319,7 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16, bf16=Fals
     lim = max(kb - 15 * rowb, 0) if stager else 0
     sso = ((c0 + 1) * tk + wave * 15 * rpl) * ROWB
     ops = {"pbase": 0, "sli": c0 - cw, "liend": c0 - cw + len(counts), "par": c0 & 1,
-           "vb0": 0x40, "vb1": 0x40 + CHUNKB, "sdst": wave * 15 * 1024, "sso": sso,
+           "vb0": 0x40, "sdst": wave * 15 * 1024, "sso": sso,
            "lim": lim, "kb": kb, "rowb": rowb, "row1": ROWB, "tkrowb": tk * ROWB}
     lane = Lane(_prog(rows, bf16), entries, ops, {"vpbo": vpbo, "vpn": vpn}).run()
     return lane, chunks[c0 - cw:c0 - cw + len(counts)], ops
@@ -325,7 +327,7 @@ def _segment(counts, c0=0, cw=0, stager=True, K=8000, seed=0, rows=16, bf16=Fals
 
 def _check_steps(lane, chunks, c0, ops, rows=16):
     for i, ch in enumerate(chunks):
-        base = ops["vb0"] if (c0 + i) % 2 == 0 else ops["vb1"]
+        base = ops["vb0"]   # the list offsets include the buffer
         expect = []
         for x, val in ch:
             slot, off = x & 0xFF, (x >> 16) * 16
@@ -439,7 +441,7 @@ def test_interpreter_catches_an_early_read():
     lane_cls = Lane
     rng = random.Random(3)
     ents = [(word(rng.randrange(75) * 1024, i % 16), i) for i in range(40)]
-    ops = {"pbase": 0, "sli": 0, "liend": 1, "par": 0, "vb0": 0, "vb1": CHUNKB, "sdst": 0,
+    ops = {"pbase": 0, "sli": 0, "liend": 1, "par": 0, "vb0": 0, "sdst": 0,
            "sso": 75 * ROWB, "lim": 0, "kb": 0, "rowb": ROWB, "tkrowb": 75 * ROWB}
     with pytest.raises(AssertionError, match="in flight"):
         lane_cls(prog, ents, ops, {"vpbo": [0] * 64, "vpn": [12] + [0] * 63}).run()

@@ -108,11 +108,12 @@ def _simulate(prog, counts, c0=2, bf16=False, ooo=False, seed=0, nwaves=6, rows=
     for w in range(nwaves):
         wr = random.Random(seed * 131 + w)
         entries, vpbo, vpn = [], [], []
-        for n in [wr.randrange(20) for _ in range(c0)] + list(counts[w]) + [0] * 8:
+        for c, n in enumerate([wr.randrange(20) for _ in range(c0)] + list(counts[w]) + [0] * 8):
             vpbo.append(len(entries) * 8)
             vpn.append(n)
-            entries += [(T.word(wr.randrange(tk) * rowd, wr.randrange(rows)), wr.getrandbits(32))
-                        for _ in range(n)]
+            # offsets from buffer 0: chunk c's rows are in buffer c & 1
+            entries += [(T.word((c & 1) * CHUNKB + wr.randrange(tk) * rowd, wr.randrange(rows)),
+                         wr.getrandbits(32)) for _ in range(n)]
         vpbo += [0] * (64 - len(vpbo))
         vpn += [0] * (64 - len(vpn))
         entries += [(T.word(0, 0), 0)] * 24
@@ -120,7 +121,7 @@ def _simulate(prog, counts, c0=2, bf16=False, ooo=False, seed=0, nwaves=6, rows=
         kb = K * T.ROWB if stager else 0
         rowb = rpl * T.ROWB
         ops = {"pbase": 0, "sli": c0, "liend": c0 + len(counts[w]), "par": c0 & 1,
-               "vb0": 0x40, "vb1": 0x40 + CHUNKB, "sdst": w * 15 * 1024,
+               "vb0": 0x40, "sdst": w * 15 * 1024,
                "sso": ((c0 + 1) * tk + w * 15 * rpl) * T.ROWB,
                "lim": max(kb - 15 * rowb, 0) if stager else 0, "kb": kb, "rowb": rowb,
                "row1": T.ROWB, "tkrowb": tk * T.ROWB}
@@ -172,7 +173,8 @@ def _counts(rng, nwaves, nchunks):
             for _ in range(nwaves)]
 
 
-@pytest.mark.parametrize("bf16,rows", [(False, 16), (False, 28), (False, 42), (True, 16), (True, 28)])
+@pytest.mark.parametrize("bf16,rows", [(False, 16), (False, 21), (False, 28), (False, 42),
+                                       (True, 16), (True, 21), (True, 28)])
 @pytest.mark.parametrize("ooo", [False, True])
 @pytest.mark.parametrize("p_land", [0.05, 0.0005])
 @pytest.mark.parametrize("seed", range(3))

@@ -38,7 +38,7 @@ def _case(dev, M, K, N, s, seed, variant=0):
 def test_noalloc_matches_plain_two_calls(dev, variant):
     import torch
     # both calls take the panel kernel (lists in the static scratch) by
-    # default: 16-wave panels at 1 %, then 12-wave panels at 0.6 %
+    # default: 16 x 21 panels at 1 % and at 0.6 %
     shapes = [(8000, 8000, 8000, 0.99), (6000, 6000, 6000, 0.994)]
     for call, (M, K, N, s) in enumerate(shapes):
         print(f"noalloc variant={variant} call={call} M={M} K={K} N={N} s={s}", flush=True)
