@@ -407,7 +407,7 @@ class ServerStats(ctypes.Structure):
 class Server:
     """Continuous-batching request loop (gcoo_server_*), operands resident."""
 
-    def __init__(self, hbm_budget_bytes: int, slots: int = 4, max_batch_jobs: int = 128):
+    def __init__(self, hbm_budget_bytes: int, slots: int = 2, max_batch_jobs: int = 64):
         cfg = ServerConfig(hbm_budget_bytes, slots, max_batch_jobs)
         s = ctypes.c_void_p()
         _check(lib().gcoo_server_create(ctypes.byref(cfg), ctypes.byref(s)), "gcoo_server_create")

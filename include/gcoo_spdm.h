@@ -290,7 +290,9 @@ int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes
 typedef struct gcoo_server *gcoo_server_t;
 typedef struct gcoo_server_config_s {
     size_t hbm_budget_bytes; /* output arena size: the admission budget */
-    int slots;               /* concurrent batches / streams, 1..8 */
+    int slots;               /* concurrent batches / streams, 1..8 (2 serves best
+                                on one MI355X: with more streams than the
+                                process's hardware queues, completions stall) */
     int max_batch_jobs;      /* requests per launched batch */
 } gcoo_server_config;
 typedef struct gcoo_server_stats_s {
