@@ -87,18 +87,26 @@ namespace {
 
 inline hipStream_t as_stream(gcoo_stream s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Internal result of read_status: the single-pass conversion stalled.
+// Internal result of status_check: the single-pass conversion stalled.
 constexpr int kStalled = -1;
 
-// Reads the conversion status word back (one small D2H copy plus a stream
-// synchronisation: the replacement of the reference's nnz readback,
-// src/ShGCOOSpDM.cu:83-90).  Returns kStalled after a stalled single-pass
-// conversion, having queued the clearing of its abort word.
-int read_status(gcoo_handle h, hipStream_t s)
+// The conversion status word's readback (the replacement of the reference's
+// nnz readback, src/ShGCOOSpDM.cu:83-90) in two halves: status_copy queues
+// the small D2H copy and an event behind it; status_check waits for that
+// event only -- work queued after the copy (the multiply) keeps running --
+// and returns kStalled after a stalled single-pass conversion, having queued
+// the clearing of its abort word.
+int status_copy(gcoo_handle h, hipStream_t s)
 {
     GCOO_HIP_TRY(hipMemcpyAsync(h->h_status, h->d_status, sizeof(ConvStatus),
                                 hipMemcpyDeviceToHost, s));
-    GCOO_HIP_TRY(hipStreamSynchronize(s));
+    GCOO_HIP_TRY(hipEventRecord(h->statusEvent, s));
+    return GCOO_OK;
+}
+
+int status_check(gcoo_handle h, hipStream_t s)
+{
+    GCOO_HIP_TRY(hipEventSynchronize(h->statusEvent));
     if (h->h_status->stall) {
         GCOO_HIP_TRY(hipMemsetAsync(&h->d_status->stall, 0, sizeof(int), s));
         return kStalled;
@@ -112,8 +120,10 @@ int read_status(gcoo_handle h, hipStream_t s)
 // Enqueue count + scan + scatter with the current capacity; when the scan's
 // total exceeds it, grow and scatter again.  On return the conversion is
 // complete and h->nnz is set.  When D is given, the multiply is enqueued
-// right behind the scatter (guarded by the status word) so that a conversion
-// that fits the existing capacity costs a single synchronisation.  When the
+// right behind the status copy (guarded by the status word) and the host
+// waits for the copy only, so a conversion that fits the existing capacity
+// returns while its multiply runs (as the reference returns with its kernel
+// launched); the caller synchronises the stream before reading C.  When the
 // single-pass kernel's look-back gives up (st->stall), the kernels queued
 // behind it skip their work and the conversion runs again on the count /
 // scan / scatter kernels, whose launches always complete (the reference's
@@ -130,11 +140,10 @@ int convert_impl(gcoo_handle h, const float *S, const float *D, int N, float *C,
     const bool fused = allow_fused && fused_conversion_ok(h, S);
     int rc = fused ? launch_convert_fused(h, S, s) : launch_count_and_scan(h, S, s);
     if (rc) return rc;
-    if (h->cap > 0) {
-        if (!fused && (rc = launch_scatter(h, S, s))) return rc;
-        if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
-    }
-    rc = read_status(h, s);
+    if (h->cap > 0 && !fused && (rc = launch_scatter(h, S, s))) return rc;
+    if ((rc = status_copy(h, s))) return rc;
+    if (h->cap > 0 && D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
+    rc = status_check(h, s);
     if (rc == kStalled) return convert_impl(h, S, D, N, C, s, false);
     if (rc) return rc;
     const int64_t total = h->h_status->total;
@@ -144,8 +153,9 @@ int convert_impl(gcoo_handle h, const float *S, const float *D, int N, float *C,
         h->nnz = total;  // the multiply's automatic variant choice reads it
         h->maxRow = h->h_status->maxRow;
         if ((rc = fused ? launch_convert_fused(h, S, s) : launch_scatter(h, S, s))) return rc;
+        if ((rc = status_copy(h, s))) return rc;
         if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
-        rc = read_status(h, s);  // synchronises
+        rc = status_check(h, s);
         if (rc == kStalled) return convert_impl(h, S, D, N, C, s, false);
         if (rc) return rc;
     }
@@ -348,7 +358,8 @@ int gcoo_create(int M, int K, int cpg, gcoo_handle *out)
         hipMalloc(&h->rowptr, sizeof(int) * ((size_t)M + 1)) != hipSuccess ||
         hipMalloc(&h->rowcnt, sizeof(int) * ((size_t)M + 1)) != hipSuccess ||
         hipMalloc(&h->d_status, sizeof(ConvStatus)) != hipSuccess ||
-        hipHostMalloc(&h->h_status, sizeof(ConvStatus), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc(&h->h_status, sizeof(ConvStatus), hipHostMallocDefault) != hipSuccess ||
+        hipEventCreateWithFlags(&h->statusEvent, hipEventDisableTiming) != hipSuccess) {
         (void)hipGetLastError();
         gcoo_destroy(h);
         return set_error(GCOO_ERR_ALLOC, "cannot allocate GCOO index arrays");
@@ -366,6 +377,7 @@ int gcoo_create(int M, int K, int cpg, gcoo_handle *out)
 int gcoo_destroy(gcoo_handle h)
 {
     if (!h) return GCOO_OK;
+    if (h->statusEvent) (void)hipEventDestroy(h->statusEvent);
     if (h->vals) (void)hipFree(h->vals);
     if (h->rowcol) (void)hipFree(h->rowcol);
     if (h->idx) (void)hipFree(h->idx);

@@ -76,6 +76,9 @@ struct gcoo_matrix {
     // arrays): a batch records it per job and refuses to run once it moved.
     uint64_t generation = 0;
 
+    // recorded behind the status word's copy to the host, so that a
+    // conversion can be checked while the multiply queued behind it runs
+    hipEvent_t statusEvent = nullptr;
     float *vals = nullptr;     // [cap]
     int *rowcol = nullptr;     // [2 * cap]: rows at 0, cols at cap
     int *idx = nullptr;        // [2 * (nGroup + 1)]: groupIndex, nnzPerGroup

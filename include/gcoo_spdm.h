@@ -221,7 +221,11 @@ int gcoo_get_spdm_choice(gcoo_handle h, int dtype, const void *D, int N, const f
                          int *variant, int *vec, int *waves, int *rows);
 
 /* Convert S then multiply: ShGCOOSpDM::mutiplyBy (src/ShGCOOSpDM.cu:131-180),
- * without the reference's per-call allocations once capacity is reached. */
+ * without the reference's per-call allocations once capacity is reached.
+ * Returns once the conversion has been checked (its status read back, as
+ * the reference reads nnz back); the multiply may still be running on s, as
+ * the reference's kernel is when mutiplyBy returns: synchronise s before
+ * reading C, and issue later calls on the handle on the same stream. */
 int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,
                      int N, float *C, gcoo_stream s);
 
