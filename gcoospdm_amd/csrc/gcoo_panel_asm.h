@@ -2124,7 +2124,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lpr20_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr0_%=\n\t" \
@@ -2603,7 +2603,7 @@
     ".Lba0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpa0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa0_%=\n\t" \
@@ -2681,7 +2681,7 @@
     ".Lbb0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpb0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb0_%=\n\t" \
@@ -3074,7 +3074,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lpr21_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr1_%=\n\t" \
@@ -3553,7 +3553,7 @@
     ".Lba1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpa1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa1_%=\n\t" \
@@ -3631,7 +3631,7 @@
     ".Lbb1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpb1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb1_%=\n\t" \
@@ -4070,7 +4070,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpr20_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr0_%=\n\t" \
@@ -4549,7 +4549,7 @@
     ".Lba0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpa0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa0_%=\n\t" \
@@ -4627,7 +4627,7 @@
     ".Lbb0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpb0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb0_%=\n\t" \
@@ -5020,7 +5020,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpr21_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr1_%=\n\t" \
@@ -5499,7 +5499,7 @@
     ".Lba1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpa1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa1_%=\n\t" \
@@ -5577,7 +5577,7 @@
     ".Lbb1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpb1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb1_%=\n\t" \
@@ -6014,7 +6014,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 32\n\t" \
     "s_cbranch_scc0 .Lpr20_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr0_%=\n\t" \
@@ -6493,7 +6493,7 @@
     ".Lba0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 42\n\t" \
     "s_cbranch_scc0 .Lpa0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa0_%=\n\t" \
@@ -6571,7 +6571,7 @@
     ".Lbb0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 42\n\t" \
     "s_cbranch_scc0 .Lpb0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb0_%=\n\t" \
@@ -6964,7 +6964,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 32\n\t" \
     "s_cbranch_scc0 .Lpr21_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr1_%=\n\t" \
@@ -7443,7 +7443,7 @@
     ".Lba1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 42\n\t" \
     "s_cbranch_scc0 .Lpa1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa1_%=\n\t" \
@@ -7521,7 +7521,7 @@
     ".Lbb1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 42\n\t" \
     "s_cbranch_scc0 .Lpb1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb1_%=\n\t" \
@@ -11010,7 +11010,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lpr20_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr0_%=\n\t" \
@@ -11741,7 +11741,7 @@
     ".Lba0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpa0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa0_%=\n\t" \
@@ -11851,7 +11851,7 @@
     ".Lbb0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpb0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb0_%=\n\t" \
@@ -12463,7 +12463,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lpr21_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr1_%=\n\t" \
@@ -13194,7 +13194,7 @@
     ".Lba1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpa1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa1_%=\n\t" \
@@ -13304,7 +13304,7 @@
     ".Lbb1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpb1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb1_%=\n\t" \
@@ -13955,7 +13955,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpr20_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr0_%=\n\t" \
@@ -14686,7 +14686,7 @@
     ".Lba0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpa0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa0_%=\n\t" \
@@ -14796,7 +14796,7 @@
     ".Lbb0_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb0_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpb0_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb0_%=\n\t" \
@@ -15408,7 +15408,7 @@
     "s_add_u32 %[sso], %[sso], %[tkrowb]\n\t" \
     "s_cmp_eq_u32 s25, 0\n\t" \
     "s_cbranch_scc1 .Lce1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 12\n\t" \
+    "s_cmp_ge_u32 s25, 21\n\t" \
     "s_cbranch_scc0 .Lpr21_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lpr1_%=\n\t" \
@@ -16139,7 +16139,7 @@
     ".Lba1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lta1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpa1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqa1_%=\n\t" \
@@ -16249,7 +16249,7 @@
     ".Lbb1_%=:\n\t" \
     "s_cmp_lt_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Ltb1_%=\n\t" \
-    "s_cmp_ge_u32 s25, 16\n\t" \
+    "s_cmp_ge_u32 s25, 28\n\t" \
     "s_cbranch_scc0 .Lpb1_%=\n\t" \
     "s_setprio 3\n\t" \
     "s_branch .Lqb1_%=\n\t" \

@@ -51,9 +51,11 @@ multiply-add of x's high half (v_mad_u32_u16 with op_sel: hi16 * 16 + the
 lane's address in buffer 0), so no scalar instruction goes into it; two
 address VGPRs serve all reads in turn (a read takes its address at issue).  A
 step is acc[slot] = fmaf(value, d, acc[slot]) for the 4 columns, in VGPR-index
-mode (s_set_gpr_idx_on/_idx take the slot from bits 7:0 of x).  The wave's issue priority is 3 for a chunk of
-12 or more entries (2 below), 3 / 2 before each batch by the entries left,
-and 0 at the chunk's end.
+mode (s_set_gpr_idx_on/_idx take the slot from bits 7:0 of x).  The wave's
+issue priority is 3 for a chunk of 3R/4 or more entries (2 below: R rows
+hold 3R/4 entries of a chunk at 1 % density), 3 / 2 before each batch by
+whether R or more entries are left, and 0 at the chunk's end (n = 8000,
+s = 0.99, 16 x 21: 403 us against 410 us with the thresholds of 16 rows).
 
 Operands: %[pbase] SGPR pair; %[vpbo], %[vpn] VGPRs as above; %[sli]
 (in/out: lane of the current chunk, the segment's end on exit) and %[liend]
@@ -78,6 +80,7 @@ CHUNKB = 75 * 1024
 ROWS = (16, 21, 28, 42)   # rows per wave of the generated FP32 blocks
 ROWS_BF16 = (16, 21, 28)  # ... and BF16 blocks
 NSCR = 34                 # scratch VGPRs above the accumulators
+PRIO_CHUNK, PRIO_BATCH = 12, 16  # entries left for issue priority 3 (see configure)
 R = 16                    # rows per wave of the block being generated
 BF16 = False              # the block reads BF16 D (8 bytes per lane) and widens it
 DV, AV = 64, 96           # D values of the batch loop; two address VGPRs, used in turn
@@ -88,9 +91,11 @@ L = []
 def configure(rows, bf16=False):
     """Register layout for `rows` rows per wave: accumulators v0..v(4 rows - 1),
     scratch from v(4 rows) on."""
-    global R, DV, AV, G_D, BF16
+    global R, DV, AV, G_D, BF16, PRIO_CHUNK, PRIO_BATCH
     R = rows
     BF16 = bf16
+    PRIO_CHUNK = (3 * rows + 2) // 4
+    PRIO_BATCH = rows
     base = 4 * rows
     DV, AV = base, base + 32
     G_D = (base, base + 16)
@@ -193,7 +198,7 @@ class Body:
         e(f".Lb{t}{p}_%=:")
         e(f"s_cmp_lt_u32 s{SREM}, 8")
         e(f"s_cbranch_scc1 .Lt{t}{p}_%=")
-        e(f"s_cmp_ge_u32 s{SREM}, 16")
+        e(f"s_cmp_ge_u32 s{SREM}, {PRIO_BATCH}")
         e(f"s_cbranch_scc0 .Lp{t}{p}_%=")
         e("s_setprio 3")
         e(f"s_branch .Lq{t}{p}_%=")
@@ -394,7 +399,7 @@ class Body:
         self.stage()
         e(f"s_cmp_eq_u32 s{SREM}, 0")
         e(f"s_cbranch_scc1 .Lce{p}_%=")
-        e(f"s_cmp_ge_u32 s{SREM}, 12")
+        e(f"s_cmp_ge_u32 s{SREM}, {PRIO_CHUNK}")
         e(f"s_cbranch_scc0 .Lpr2{p}_%=")
         e("s_setprio 3")
         e(f"s_branch .Lpr{p}_%=")
