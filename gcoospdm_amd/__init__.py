@@ -305,7 +305,10 @@ class GCOOMatrix:
         return v.value, vec.value, w.value, r.value
 
     def multiply_by(self, S, D, C, stream=None):
-        """Convert S then C = S * D (the reference's mutiplyBy)."""
+        """Convert S then C = S * D (the reference's mutiplyBy).  Returns once
+        the conversion is checked; the multiply may still run on `stream`
+        (torch's current stream by default), so work queued on that stream
+        sees C complete."""
         import torch
         _require_dev(S, "S", torch.float32)
         _require_dev(D, "D", torch.float32)

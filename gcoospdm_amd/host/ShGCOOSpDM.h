@@ -25,6 +25,9 @@ public:
                                  int *&pGroupIndex, int *&pNnzPerGroup,
                                  int nGroup);
 
+    // Returns once the conversion is checked, with the multiply possibly
+    // still running on the object's stream (as the reference's does):
+    // synchronise that stream before reading C.
     double mutiplyBy(float *A, int nRows, int nCols, float *C) override;
     double mutiply(float *B, int nRows, int nCols, float *C) override;
     void build() override;
