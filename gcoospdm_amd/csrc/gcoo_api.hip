@@ -104,6 +104,29 @@ int status_copy(gcoo_handle h, hipStream_t s)
     return GCOO_OK;
 }
 
+// The status copy and the multiply behind a conversion.  When the multiply
+// is the panel kernel, the panel-list kernel (rebuilt after every
+// conversion, first on the stream) copies the status word to the host and
+// its launch records the event, so no separate copy sits between the
+// conversion and the multiply; otherwise the copy goes first.
+int status_and_multiply(gcoo_handle h, const float *D, int N, float *C, hipStream_t s)
+{
+    int variant = 0, vec = 0, waves = 0, rows = 0;
+    if (h->h_status_dev &&
+        spdm_choice(h, 0, D, N, C, &variant, &vec, &waves, &rows) == GCOO_OK && variant == 6) {
+        h->statusExport = h->h_status_dev;
+        h->statusExported = false;
+        const int rc = launch_spdm_f32(h, D, N, C, h->d_status, s);
+        h->statusExport = nullptr;
+        if (rc) return rc;
+        // no lists kernel ran (lists still valid, or no rows): copy after
+        return h->statusExported ? GCOO_OK : status_copy(h, s);
+    }
+    int rc = status_copy(h, s);
+    if (!rc) rc = launch_spdm_f32(h, D, N, C, h->d_status, s);
+    return rc;
+}
+
 int status_check(gcoo_handle h, hipStream_t s)
 {
     GCOO_HIP_TRY(hipEventSynchronize(h->statusEvent));
@@ -141,8 +164,8 @@ int convert_impl(gcoo_handle h, const float *S, const float *D, int N, float *C,
     int rc = fused ? launch_convert_fused(h, S, s) : launch_count_and_scan(h, S, s);
     if (rc) return rc;
     if (h->cap > 0 && !fused && (rc = launch_scatter(h, S, s))) return rc;
-    if ((rc = status_copy(h, s))) return rc;
-    if (h->cap > 0 && D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
+    rc = (h->cap > 0 && D) ? status_and_multiply(h, D, N, C, s) : status_copy(h, s);
+    if (rc) return rc;
     rc = status_check(h, s);
     if (rc == kStalled) return convert_impl(h, S, D, N, C, s, false);
     if (rc) return rc;
@@ -153,8 +176,8 @@ int convert_impl(gcoo_handle h, const float *S, const float *D, int N, float *C,
         h->nnz = total;  // the multiply's automatic variant choice reads it
         h->maxRow = h->h_status->maxRow;
         if ((rc = fused ? launch_convert_fused(h, S, s) : launch_scatter(h, S, s))) return rc;
-        if ((rc = status_copy(h, s))) return rc;
-        if (D && (rc = launch_spdm_f32(h, D, N, C, h->d_status, s))) return rc;
+        rc = D ? status_and_multiply(h, D, N, C, s) : status_copy(h, s);
+        if (rc) return rc;
         rc = status_check(h, s);
         if (rc == kStalled) return convert_impl(h, S, D, N, C, s, false);
         if (rc) return rc;
@@ -363,6 +386,11 @@ int gcoo_create(int M, int K, int cpg, gcoo_handle *out)
         (void)hipGetLastError();
         gcoo_destroy(h);
         return set_error(GCOO_ERR_ALLOC, "cannot allocate GCOO index arrays");
+    }
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&h->h_status_dev), h->h_status, 0) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        h->h_status_dev = nullptr;  // the status is then always copied
     }
     if (hipMemset(h->d_status, 0, sizeof(ConvStatus)) != hipSuccess) {
         (void)hipGetLastError();

@@ -121,6 +121,13 @@ struct gcoo_matrix {
     mutable int plist_rowb = 0;               // bytes of a staged D row they encode
     mutable int plist_rpw = 0;                // rows per wave they were built for
     mutable int plist_waves = 0;              // waves per panel they were built for
+    // Set by a conversion with a panel multiply behind it: the panel-list
+    // kernel copies the status word to this device view of h_status (so no
+    // separate copy sits on the multiply's path) and the lists' launch
+    // records statusEvent; `statusExported` tells the conversion it happened.
+    mutable gcoo::ConvStatus *statusExport = nullptr;
+    mutable bool statusExported = false;
+    gcoo::ConvStatus *h_status_dev = nullptr;  // device view of h_status
 
     // single-pass conversion: per-group look-back flags (two (state, epoch,
     // value) words per group, never cleared: each conversion has its own
@@ -209,7 +216,7 @@ size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves);
 // panel_offset_elems ints, plist of h->nnz entries); touches no handle state.
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
                       int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
-                      hipStream_t s);
+                      hipStream_t s, ConvStatus *hst = nullptr, bool *exported = nullptr);
 
 // Device scratch handed to the multiply by a caller that cannot allocate:
 // the panel lists and the long-row list are carved out of it instead of

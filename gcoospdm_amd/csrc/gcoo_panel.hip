@@ -52,8 +52,20 @@ template <bool LDS, int NT>
 __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int rowb,
     int nC, int RPW, int widen, int *__restrict__ poff, int *__restrict__ gcnt,
-    uint2 *__restrict__ plist, const ConvStatus *__restrict__ guard, long long cap)
+    uint2 *__restrict__ plist, const ConvStatus *__restrict__ guard, long long cap,
+    ConvStatus *__restrict__ hst)
 {
+    // the conversion's status word to the host (its host mirror's device
+    // view), first of all, so that the host can check the conversion while
+    // the multiply queued after this kernel runs
+    if (hst && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned *src = reinterpret_cast<const unsigned *>(guard);
+        unsigned *dst = reinterpret_cast<unsigned *>(hst);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(ConvStatus) / 4); ++i)
+            __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+    }
     if (guard_skip(guard, cap)) return;
     extern __shared__ int lcnt[];
     const int pw = blockIdx.x;  // panel * waves + wave
@@ -146,8 +158,9 @@ size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves)
 
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
                       int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
-                      hipStream_t s)
+                      hipStream_t s, ConvStatus *hst, bool *exported)
 {
+    if (exported) *exported = false;
     // two chunk buffers of offsets / 16 in 16 bits
     if (2ll * tk * rowb > (1ll << 20) || rowb % 16 != 0)
         return set_error(GCOO_ERR_UNSUPPORTED, "panel chunk of %d x %d bytes", tk, rowb);
@@ -165,12 +178,14 @@ int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, i
             hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
                                rowb, nC, rpw, widen ? 1 : 0, poff, gcnt, plist, guard,
-                               (long long)h->cap);
+                               (long long)h->cap, guard ? hst : nullptr);
         else
             hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC, rpw,
-                               widen ? 1 : 0, poff, gcnt, plist, guard, (long long)h->cap);
+                               widen ? 1 : 0, poff, gcnt, plist, guard, (long long)h->cap,
+                               guard ? hst : nullptr);
         GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
+        if (exported) *exported = guard && hst;
     }
     return GCOO_OK;
 }
@@ -200,8 +215,18 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, 
             return set_error(GCOO_ERR_ALLOC, "cannot allocate the panel lists");
         }
     }
-    int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, widen, h->poff, h->plist, guard, s);
+    // a conversion queued just before asked for its status word: the lists
+    // kernel copies it to the host, and the event marks when it has
+    ConvStatus *hst = (guard && guard == h->d_status) ? h->statusExport : nullptr;
+    bool exported = false;
+    int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, widen, h->poff, h->plist, guard, s,
+                               hst, &exported);
     if (rc) return rc;
+    if (exported) {
+        GCOO_HIP_TRY(hipEventRecord(h->statusEvent, s));
+        h->statusExport = nullptr;
+        h->statusExported = true;
+    }
     h->plist_src = seg;
     h->plist_tk = tk;
     h->plist_rowb = rowb;

@@ -394,18 +394,21 @@ def test_main_driver_reference_inputs(dev):
     assert "Test passed" in r.stdout
 
 
-@pytest.mark.parametrize("M,K,N,dens", [(257, 516, 96, 0.1), (2000, 2048, 256, 0.01)])
-def test_conversion_stall_fallback(dev, M, K, N, dens):
+@pytest.mark.parametrize("M,K,N,dens,variant", [(257, 516, 96, 0.1, 0), (2000, 2048, 256, 0.01, 0),
+                                                (2000, 2048, 256, 0.01, 6)])
+def test_conversion_stall_fallback(dev, M, K, N, dens, variant):
     """The single-pass conversion's abort word is raised from the host before
     the launch (as a group whose bounded look-back wait gave up would raise
     it): every kernel still exits normally, the multiply queued behind the
     conversion skips, and the library converts again with the count / scan /
     scatter kernels.  Arrays and C must still equal the oracle's, and the
-    next conversion takes the single-pass kernel again."""
+    next conversion takes the single-pass kernel again.  With the panel
+    kernel (variant 6) the status word reaches the host through its
+    panel-list kernel."""
     import torch
-    print(f"stall fallback M={M} K={K} N={N} dens={dens}", flush=True)
+    print(f"stall fallback M={M} K={K} N={N} dens={dens} variant={variant}", flush=True)
     rng = np.random.default_rng(M + K)
-    m = G.GCOOMatrix(M, K, 4)
+    m = G.GCOOMatrix(M, K, 4).set_config(variant=variant)
     for step in range(3):
         S = H.rng_sparse(rng, M, K, dens * (1 + step))
         D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
