@@ -7784,6 +7784,14 @@
 // in v[16 c + q]; scratch v64..v97
 #define GCOO_PANEL_SEGMENT_ASM_BF16_R16 \
     "s_mov_b32 s28, m0\n\t" \
+    "v_mov_b32 v64, 0\n\t" \
+    "v_mov_b32 v68, 0\n\t" \
+    "v_mov_b32 v72, 0\n\t" \
+    "v_mov_b32 v76, 0\n\t" \
+    "v_mov_b32 v80, 0\n\t" \
+    "v_mov_b32 v84, 0\n\t" \
+    "v_mov_b32 v88, 0\n\t" \
+    "v_mov_b32 v92, 0\n\t" \
     "s_nop 4\n\t" \
     "v_readlane_b32 s29, %[vpbo], %[sli]\n\t" \
     "v_readlane_b32 s30, %[vpn], %[sli]\n\t" \
@@ -8076,43 +8084,43 @@
     "s_cbranch_scc1 .Lf00_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
     "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     ".Lgvp0s0_%=:\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
@@ -8145,40 +8153,44 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lf00_%=:\n\t" \
     "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
     "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
     "v_fma_f32 v16, s37, v65, v16\n\t" \
@@ -8206,37 +8218,37 @@
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
     "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
     "v_fma_f32 v16, s37, v65, v16\n\t" \
@@ -8258,28 +8270,24 @@
     "v_fma_f32 v32, s43, v78, v32\n\t" \
     "v_fma_f32 v48, s43, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     ".Lgvp1s0_%=:\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v80, v0\n\t" \
@@ -8312,22 +8320,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz00_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
     "v_fma_f32 v16, s37, v65, v16\n\t" \
@@ -8354,30 +8358,30 @@
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq10_%=\n\t" \
     "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v80, v0\n\t" \
     "v_fma_f32 v16, s45, v81, v16\n\t" \
@@ -8405,37 +8409,37 @@
     "s_cbranch_scc0 .Lz10_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
     "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v80, v0\n\t" \
     "v_fma_f32 v16, s45, v81, v16\n\t" \
@@ -8457,28 +8461,24 @@
     "v_fma_f32 v32, s51, v94, v32\n\t" \
     "v_fma_f32 v48, s51, v95, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     ".Lgvp2s0_%=:\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
@@ -8511,22 +8511,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz10_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v80, v0\n\t" \
     "v_fma_f32 v16, s45, v81, v16\n\t" \
@@ -8553,30 +8549,30 @@
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
     "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
     "v_fma_f32 v16, s53, v65, v16\n\t" \
@@ -8604,37 +8600,37 @@
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
     "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
     "v_fma_f32 v16, s53, v65, v16\n\t" \
@@ -8656,28 +8652,24 @@
     "v_fma_f32 v32, s59, v78, v32\n\t" \
     "v_fma_f32 v48, s59, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     ".Lgvp3s0_%=:\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v80, v0\n\t" \
@@ -8710,22 +8702,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz20_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
     "v_fma_f32 v16, s53, v65, v16\n\t" \
@@ -8750,22 +8738,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lf30_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v80, v0\n\t" \
     "v_fma_f32 v16, s61, v81, v16\n\t" \
@@ -8804,42 +8788,46 @@
     "s_setprio 2\n\t" \
     ".Lqa0_%=:\n\t" \
     "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
     "s_load_dwordx16 s[52:67], %[pbase], s26\n\t" \
     ".Lna0_%=:\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
     "v_fma_f32 v16, s37, v65, v16\n\t" \
@@ -8861,22 +8849,18 @@
     "v_fma_f32 v32, s43, v78, v32\n\t" \
     "v_fma_f32 v48, s43, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v80, v0\n\t" \
     "v_fma_f32 v16, s45, v81, v16\n\t" \
@@ -8914,42 +8898,46 @@
     "s_setprio 2\n\t" \
     ".Lqb0_%=:\n\t" \
     "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
     "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
     ".Lnb0_%=:\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
     "v_fma_f32 v16, s53, v65, v16\n\t" \
@@ -8971,22 +8959,18 @@
     "v_fma_f32 v32, s59, v78, v32\n\t" \
     "v_fma_f32 v48, s59, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v80, v0\n\t" \
     "v_fma_f32 v16, s61, v81, v16\n\t" \
@@ -9016,73 +9000,73 @@
     "s_branch .Lba0_%=\n\t" \
     ".Lta0_%=:\n\t" \
     "v_mad_u32_u16 v96, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v97, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v96, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v97, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v96, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v97, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v96, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     ".Lgva0_%=:\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v64, v0\n\t" \
@@ -9136,73 +9120,73 @@
     "s_branch .Lce0_%=\n\t" \
     ".Ltb0_%=:\n\t" \
     "v_mad_u32_u16 v96, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v97, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v96, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v97, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v96, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v97, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v96, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     ".Lgvb0_%=:\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v64, v0\n\t" \
@@ -9529,43 +9513,43 @@
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
     "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     ".Lgvp0s1_%=:\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
@@ -9598,40 +9582,44 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
     "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
     "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
     "v_fma_f32 v16, s69, v65, v16\n\t" \
@@ -9659,37 +9647,37 @@
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
     "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
     "v_fma_f32 v16, s69, v65, v16\n\t" \
@@ -9711,28 +9699,24 @@
     "v_fma_f32 v32, s75, v78, v32\n\t" \
     "v_fma_f32 v48, s75, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     ".Lgvp1s1_%=:\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v80, v0\n\t" \
@@ -9765,22 +9749,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz01_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
     "v_fma_f32 v16, s69, v65, v16\n\t" \
@@ -9807,30 +9787,30 @@
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
     "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v80, v0\n\t" \
     "v_fma_f32 v16, s77, v81, v16\n\t" \
@@ -9858,37 +9838,37 @@
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
     "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v80, v0\n\t" \
     "v_fma_f32 v16, s77, v81, v16\n\t" \
@@ -9910,28 +9890,24 @@
     "v_fma_f32 v32, s83, v94, v32\n\t" \
     "v_fma_f32 v48, s83, v95, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     ".Lgvp2s1_%=:\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
@@ -9964,22 +9940,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz11_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v80, v0\n\t" \
     "v_fma_f32 v16, s77, v81, v16\n\t" \
@@ -10006,30 +9978,30 @@
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
     "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
     "v_fma_f32 v16, s85, v65, v16\n\t" \
@@ -10057,37 +10029,37 @@
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
     "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
     "v_fma_f32 v16, s85, v65, v16\n\t" \
@@ -10109,28 +10081,24 @@
     "v_fma_f32 v32, s91, v78, v32\n\t" \
     "v_fma_f32 v48, s91, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     ".Lgvp3s1_%=:\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v80, v0\n\t" \
@@ -10163,22 +10131,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz21_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
     "v_fma_f32 v16, s85, v65, v16\n\t" \
@@ -10203,22 +10167,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lf31_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v80, v0\n\t" \
     "v_fma_f32 v16, s93, v81, v16\n\t" \
@@ -10257,42 +10217,46 @@
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
     "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
     "s_load_dwordx16 s[84:99], %[pbase], s26\n\t" \
     ".Lna1_%=:\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
     "v_fma_f32 v16, s69, v65, v16\n\t" \
@@ -10314,22 +10278,18 @@
     "v_fma_f32 v32, s75, v78, v32\n\t" \
     "v_fma_f32 v48, s75, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v80, v0\n\t" \
     "v_fma_f32 v16, s77, v81, v16\n\t" \
@@ -10367,42 +10327,46 @@
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
     "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     "v_mad_u32_u16 v97, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v97\n\t" \
+    "ds_read_b64 v[94:95], v97\n\t" \
+    "ds_read_u16_d16_hi v92, v97\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
     "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
     ".Lnb1_%=:\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
     "v_fma_f32 v16, s85, v65, v16\n\t" \
@@ -10424,22 +10388,18 @@
     "v_fma_f32 v32, s91, v78, v32\n\t" \
     "v_fma_f32 v48, s91, v79, v48\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v80, v0\n\t" \
     "v_fma_f32 v16, s93, v81, v16\n\t" \
@@ -10469,73 +10429,73 @@
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
     "v_mad_u32_u16 v96, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v97, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v96, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v97, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v96, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v97, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v96, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     ".Lgva1_%=:\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v64, v0\n\t" \
@@ -10589,73 +10549,73 @@
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
     "v_mad_u32_u16 v96, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[64:65], v96\n\t" \
+    "ds_read_b64 v[66:67], v96\n\t" \
+    "ds_read_u16_d16_hi v64, v96\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v97, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[68:69], v97\n\t" \
+    "ds_read_b64 v[70:71], v97\n\t" \
+    "ds_read_u16_d16_hi v68, v97\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v96, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[72:73], v96\n\t" \
+    "ds_read_b64 v[74:75], v96\n\t" \
+    "ds_read_u16_d16_hi v72, v96\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v97, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[76:77], v97\n\t" \
+    "ds_read_b64 v[78:79], v97\n\t" \
+    "ds_read_u16_d16_hi v76, v97\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v96, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[80:81], v96\n\t" \
+    "ds_read_b64 v[82:83], v96\n\t" \
+    "ds_read_u16_d16_hi v80, v96\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v97, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v97\n\t" \
+    "ds_read_b64 v[86:87], v97\n\t" \
+    "ds_read_u16_d16_hi v84, v97\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v96, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v96\n\t" \
+    "ds_read_b64 v[90:91], v96\n\t" \
+    "ds_read_u16_d16_hi v88, v96\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v67, 0xffff0000, v65\n\t" \
-    "v_lshlrev_b32_e32 v66, 16, v65\n\t" \
-    "v_and_b32_e32 v65, 0xffff0000, v64\n\t" \
-    "v_lshlrev_b32_e32 v64, 16, v64\n\t" \
+    "v_and_b32_e32 v65, 0xffff0000, v66\n\t" \
+    "v_lshlrev_b32_e32 v66, 16, v67\n\t" \
+    "v_and_b32_e32 v67, 0xffff0000, v67\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v71, 0xffff0000, v69\n\t" \
-    "v_lshlrev_b32_e32 v70, 16, v69\n\t" \
-    "v_and_b32_e32 v69, 0xffff0000, v68\n\t" \
-    "v_lshlrev_b32_e32 v68, 16, v68\n\t" \
+    "v_and_b32_e32 v69, 0xffff0000, v70\n\t" \
+    "v_lshlrev_b32_e32 v70, 16, v71\n\t" \
+    "v_and_b32_e32 v71, 0xffff0000, v71\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v75, 0xffff0000, v73\n\t" \
-    "v_lshlrev_b32_e32 v74, 16, v73\n\t" \
-    "v_and_b32_e32 v73, 0xffff0000, v72\n\t" \
-    "v_lshlrev_b32_e32 v72, 16, v72\n\t" \
+    "v_and_b32_e32 v73, 0xffff0000, v74\n\t" \
+    "v_lshlrev_b32_e32 v74, 16, v75\n\t" \
+    "v_and_b32_e32 v75, 0xffff0000, v75\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v79, 0xffff0000, v77\n\t" \
-    "v_lshlrev_b32_e32 v78, 16, v77\n\t" \
-    "v_and_b32_e32 v77, 0xffff0000, v76\n\t" \
-    "v_lshlrev_b32_e32 v76, 16, v76\n\t" \
+    "v_and_b32_e32 v77, 0xffff0000, v78\n\t" \
+    "v_lshlrev_b32_e32 v78, 16, v79\n\t" \
+    "v_and_b32_e32 v79, 0xffff0000, v79\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v83, 0xffff0000, v81\n\t" \
-    "v_lshlrev_b32_e32 v82, 16, v81\n\t" \
-    "v_and_b32_e32 v81, 0xffff0000, v80\n\t" \
-    "v_lshlrev_b32_e32 v80, 16, v80\n\t" \
+    "v_and_b32_e32 v81, 0xffff0000, v82\n\t" \
+    "v_lshlrev_b32_e32 v82, 16, v83\n\t" \
+    "v_and_b32_e32 v83, 0xffff0000, v83\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     ".Lgvb1_%=:\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v64, v0\n\t" \
@@ -10729,6 +10689,14 @@
 // in v[21 c + q]; scratch v84..v117
 #define GCOO_PANEL_SEGMENT_ASM_BF16_R21 \
     "s_mov_b32 s28, m0\n\t" \
+    "v_mov_b32 v84, 0\n\t" \
+    "v_mov_b32 v88, 0\n\t" \
+    "v_mov_b32 v92, 0\n\t" \
+    "v_mov_b32 v96, 0\n\t" \
+    "v_mov_b32 v100, 0\n\t" \
+    "v_mov_b32 v104, 0\n\t" \
+    "v_mov_b32 v108, 0\n\t" \
+    "v_mov_b32 v112, 0\n\t" \
     "s_nop 4\n\t" \
     "v_readlane_b32 s29, %[vpbo], %[sli]\n\t" \
     "v_readlane_b32 s30, %[vpn], %[sli]\n\t" \
@@ -11021,43 +10989,43 @@
     "s_cbranch_scc1 .Lf00_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
     "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     ".Lgvp0s0_%=:\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v84, v0\n\t" \
@@ -11090,40 +11058,44 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lf00_%=:\n\t" \
     "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
     "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v84, v0\n\t" \
     "v_fma_f32 v21, s37, v85, v21\n\t" \
@@ -11151,37 +11123,37 @@
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
     "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v84, v0\n\t" \
     "v_fma_f32 v21, s37, v85, v21\n\t" \
@@ -11203,28 +11175,24 @@
     "v_fma_f32 v42, s43, v98, v42\n\t" \
     "v_fma_f32 v63, s43, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     ".Lgvp1s0_%=:\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v100, v0\n\t" \
@@ -11257,22 +11225,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz00_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v84, v0\n\t" \
     "v_fma_f32 v21, s37, v85, v21\n\t" \
@@ -11299,30 +11263,30 @@
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq10_%=\n\t" \
     "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v100, v0\n\t" \
     "v_fma_f32 v21, s45, v101, v21\n\t" \
@@ -11350,37 +11314,37 @@
     "s_cbranch_scc0 .Lz10_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
     "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v100, v0\n\t" \
     "v_fma_f32 v21, s45, v101, v21\n\t" \
@@ -11402,28 +11366,24 @@
     "v_fma_f32 v42, s51, v114, v42\n\t" \
     "v_fma_f32 v63, s51, v115, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     ".Lgvp2s0_%=:\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v84, v0\n\t" \
@@ -11456,22 +11416,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz10_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v100, v0\n\t" \
     "v_fma_f32 v21, s45, v101, v21\n\t" \
@@ -11498,30 +11454,30 @@
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
     "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v84, v0\n\t" \
     "v_fma_f32 v21, s53, v85, v21\n\t" \
@@ -11549,37 +11505,37 @@
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
     "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v84, v0\n\t" \
     "v_fma_f32 v21, s53, v85, v21\n\t" \
@@ -11601,28 +11557,24 @@
     "v_fma_f32 v42, s59, v98, v42\n\t" \
     "v_fma_f32 v63, s59, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     ".Lgvp3s0_%=:\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v100, v0\n\t" \
@@ -11655,22 +11607,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz20_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v84, v0\n\t" \
     "v_fma_f32 v21, s53, v85, v21\n\t" \
@@ -11695,22 +11643,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lf30_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v100, v0\n\t" \
     "v_fma_f32 v21, s61, v101, v21\n\t" \
@@ -11749,42 +11693,46 @@
     "s_setprio 2\n\t" \
     ".Lqa0_%=:\n\t" \
     "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
     "s_load_dwordx16 s[52:67], %[pbase], s26\n\t" \
     ".Lna0_%=:\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v84, v0\n\t" \
     "v_fma_f32 v21, s37, v85, v21\n\t" \
@@ -11806,22 +11754,18 @@
     "v_fma_f32 v42, s43, v98, v42\n\t" \
     "v_fma_f32 v63, s43, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v100, v0\n\t" \
     "v_fma_f32 v21, s45, v101, v21\n\t" \
@@ -11859,42 +11803,46 @@
     "s_setprio 2\n\t" \
     ".Lqb0_%=:\n\t" \
     "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
     "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
     ".Lnb0_%=:\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v84, v0\n\t" \
     "v_fma_f32 v21, s53, v85, v21\n\t" \
@@ -11916,22 +11864,18 @@
     "v_fma_f32 v42, s59, v98, v42\n\t" \
     "v_fma_f32 v63, s59, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v100, v0\n\t" \
     "v_fma_f32 v21, s61, v101, v21\n\t" \
@@ -11961,73 +11905,73 @@
     "s_branch .Lba0_%=\n\t" \
     ".Lta0_%=:\n\t" \
     "v_mad_u32_u16 v116, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v117, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v116, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v117, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v116, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v117, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v116, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     ".Lgva0_%=:\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v84, v0\n\t" \
@@ -12081,73 +12025,73 @@
     "s_branch .Lce0_%=\n\t" \
     ".Ltb0_%=:\n\t" \
     "v_mad_u32_u16 v116, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v117, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v116, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v117, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v116, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v117, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v116, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     ".Lgvb0_%=:\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v84, v0\n\t" \
@@ -12474,43 +12418,43 @@
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
     "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     ".Lgvp0s1_%=:\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v84, v0\n\t" \
@@ -12543,40 +12487,44 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
     "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
     "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v84, v0\n\t" \
     "v_fma_f32 v21, s69, v85, v21\n\t" \
@@ -12604,37 +12552,37 @@
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
     "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v84, v0\n\t" \
     "v_fma_f32 v21, s69, v85, v21\n\t" \
@@ -12656,28 +12604,24 @@
     "v_fma_f32 v42, s75, v98, v42\n\t" \
     "v_fma_f32 v63, s75, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     ".Lgvp1s1_%=:\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v100, v0\n\t" \
@@ -12710,22 +12654,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz01_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v84, v0\n\t" \
     "v_fma_f32 v21, s69, v85, v21\n\t" \
@@ -12752,30 +12692,30 @@
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
     "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v100, v0\n\t" \
     "v_fma_f32 v21, s77, v101, v21\n\t" \
@@ -12803,37 +12743,37 @@
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
     "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v100, v0\n\t" \
     "v_fma_f32 v21, s77, v101, v21\n\t" \
@@ -12855,28 +12795,24 @@
     "v_fma_f32 v42, s83, v114, v42\n\t" \
     "v_fma_f32 v63, s83, v115, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     ".Lgvp2s1_%=:\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v84, v0\n\t" \
@@ -12909,22 +12845,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz11_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v100, v0\n\t" \
     "v_fma_f32 v21, s77, v101, v21\n\t" \
@@ -12951,30 +12883,30 @@
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
     "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v84, v0\n\t" \
     "v_fma_f32 v21, s85, v85, v21\n\t" \
@@ -13002,37 +12934,37 @@
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
     "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v84, v0\n\t" \
     "v_fma_f32 v21, s85, v85, v21\n\t" \
@@ -13054,28 +12986,24 @@
     "v_fma_f32 v42, s91, v98, v42\n\t" \
     "v_fma_f32 v63, s91, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     ".Lgvp3s1_%=:\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v100, v0\n\t" \
@@ -13108,22 +13036,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz21_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v84, v0\n\t" \
     "v_fma_f32 v21, s85, v85, v21\n\t" \
@@ -13148,22 +13072,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lf31_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v100, v0\n\t" \
     "v_fma_f32 v21, s93, v101, v21\n\t" \
@@ -13202,42 +13122,46 @@
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
     "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
     "s_load_dwordx16 s[84:99], %[pbase], s26\n\t" \
     ".Lna1_%=:\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v84, v0\n\t" \
     "v_fma_f32 v21, s69, v85, v21\n\t" \
@@ -13259,22 +13183,18 @@
     "v_fma_f32 v42, s75, v98, v42\n\t" \
     "v_fma_f32 v63, s75, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v100, v0\n\t" \
     "v_fma_f32 v21, s77, v101, v21\n\t" \
@@ -13312,42 +13232,46 @@
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
     "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     "v_mad_u32_u16 v117, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v117\n\t" \
+    "ds_read_b64 v[114:115], v117\n\t" \
+    "ds_read_u16_d16_hi v112, v117\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
     "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
     ".Lnb1_%=:\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v84, v0\n\t" \
     "v_fma_f32 v21, s85, v85, v21\n\t" \
@@ -13369,22 +13293,18 @@
     "v_fma_f32 v42, s91, v98, v42\n\t" \
     "v_fma_f32 v63, s91, v99, v63\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v100, v0\n\t" \
     "v_fma_f32 v21, s93, v101, v21\n\t" \
@@ -13414,73 +13334,73 @@
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
     "v_mad_u32_u16 v116, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v117, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v116, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v117, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v116, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v117, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v116, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     ".Lgva1_%=:\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v84, v0\n\t" \
@@ -13534,73 +13454,73 @@
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
     "v_mad_u32_u16 v116, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[84:85], v116\n\t" \
+    "ds_read_b64 v[86:87], v116\n\t" \
+    "ds_read_u16_d16_hi v84, v116\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v117, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[88:89], v117\n\t" \
+    "ds_read_b64 v[90:91], v117\n\t" \
+    "ds_read_u16_d16_hi v88, v117\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v116, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[92:93], v116\n\t" \
+    "ds_read_b64 v[94:95], v116\n\t" \
+    "ds_read_u16_d16_hi v92, v116\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v117, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[96:97], v117\n\t" \
+    "ds_read_b64 v[98:99], v117\n\t" \
+    "ds_read_u16_d16_hi v96, v117\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v116, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[100:101], v116\n\t" \
+    "ds_read_b64 v[102:103], v116\n\t" \
+    "ds_read_u16_d16_hi v100, v116\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v117, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[104:105], v117\n\t" \
+    "ds_read_b64 v[106:107], v117\n\t" \
+    "ds_read_u16_d16_hi v104, v117\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v116, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[108:109], v116\n\t" \
+    "ds_read_b64 v[110:111], v116\n\t" \
+    "ds_read_u16_d16_hi v108, v116\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v87, 0xffff0000, v85\n\t" \
-    "v_lshlrev_b32_e32 v86, 16, v85\n\t" \
-    "v_and_b32_e32 v85, 0xffff0000, v84\n\t" \
-    "v_lshlrev_b32_e32 v84, 16, v84\n\t" \
+    "v_and_b32_e32 v85, 0xffff0000, v86\n\t" \
+    "v_lshlrev_b32_e32 v86, 16, v87\n\t" \
+    "v_and_b32_e32 v87, 0xffff0000, v87\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v91, 0xffff0000, v89\n\t" \
-    "v_lshlrev_b32_e32 v90, 16, v89\n\t" \
-    "v_and_b32_e32 v89, 0xffff0000, v88\n\t" \
-    "v_lshlrev_b32_e32 v88, 16, v88\n\t" \
+    "v_and_b32_e32 v89, 0xffff0000, v90\n\t" \
+    "v_lshlrev_b32_e32 v90, 16, v91\n\t" \
+    "v_and_b32_e32 v91, 0xffff0000, v91\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v95, 0xffff0000, v93\n\t" \
-    "v_lshlrev_b32_e32 v94, 16, v93\n\t" \
-    "v_and_b32_e32 v93, 0xffff0000, v92\n\t" \
-    "v_lshlrev_b32_e32 v92, 16, v92\n\t" \
+    "v_and_b32_e32 v93, 0xffff0000, v94\n\t" \
+    "v_lshlrev_b32_e32 v94, 16, v95\n\t" \
+    "v_and_b32_e32 v95, 0xffff0000, v95\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v99, 0xffff0000, v97\n\t" \
-    "v_lshlrev_b32_e32 v98, 16, v97\n\t" \
-    "v_and_b32_e32 v97, 0xffff0000, v96\n\t" \
-    "v_lshlrev_b32_e32 v96, 16, v96\n\t" \
+    "v_and_b32_e32 v97, 0xffff0000, v98\n\t" \
+    "v_lshlrev_b32_e32 v98, 16, v99\n\t" \
+    "v_and_b32_e32 v99, 0xffff0000, v99\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v103, 0xffff0000, v101\n\t" \
-    "v_lshlrev_b32_e32 v102, 16, v101\n\t" \
-    "v_and_b32_e32 v101, 0xffff0000, v100\n\t" \
-    "v_lshlrev_b32_e32 v100, 16, v100\n\t" \
+    "v_and_b32_e32 v101, 0xffff0000, v102\n\t" \
+    "v_lshlrev_b32_e32 v102, 16, v103\n\t" \
+    "v_and_b32_e32 v103, 0xffff0000, v103\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v107, 0xffff0000, v105\n\t" \
-    "v_lshlrev_b32_e32 v106, 16, v105\n\t" \
-    "v_and_b32_e32 v105, 0xffff0000, v104\n\t" \
-    "v_lshlrev_b32_e32 v104, 16, v104\n\t" \
+    "v_and_b32_e32 v105, 0xffff0000, v106\n\t" \
+    "v_lshlrev_b32_e32 v106, 16, v107\n\t" \
+    "v_and_b32_e32 v107, 0xffff0000, v107\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v111, 0xffff0000, v109\n\t" \
-    "v_lshlrev_b32_e32 v110, 16, v109\n\t" \
-    "v_and_b32_e32 v109, 0xffff0000, v108\n\t" \
-    "v_lshlrev_b32_e32 v108, 16, v108\n\t" \
+    "v_and_b32_e32 v109, 0xffff0000, v110\n\t" \
+    "v_lshlrev_b32_e32 v110, 16, v111\n\t" \
+    "v_and_b32_e32 v111, 0xffff0000, v111\n\t" \
     ".Lgvb1_%=:\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v84, v0\n\t" \
@@ -13674,6 +13594,14 @@
 // in v[28 c + q]; scratch v112..v145
 #define GCOO_PANEL_SEGMENT_ASM_BF16_R28 \
     "s_mov_b32 s28, m0\n\t" \
+    "v_mov_b32 v112, 0\n\t" \
+    "v_mov_b32 v116, 0\n\t" \
+    "v_mov_b32 v120, 0\n\t" \
+    "v_mov_b32 v124, 0\n\t" \
+    "v_mov_b32 v128, 0\n\t" \
+    "v_mov_b32 v132, 0\n\t" \
+    "v_mov_b32 v136, 0\n\t" \
+    "v_mov_b32 v140, 0\n\t" \
     "s_nop 4\n\t" \
     "v_readlane_b32 s29, %[vpbo], %[sli]\n\t" \
     "v_readlane_b32 s30, %[vpn], %[sli]\n\t" \
@@ -13966,43 +13894,43 @@
     "s_cbranch_scc1 .Lf00_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
     "v_mad_u32_u16 v144, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r0_%=\n\t" \
     "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     ".Lp0r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp0s0_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     ".Lgvp0s0_%=:\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
@@ -14035,40 +13963,44 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lf00_%=:\n\t" \
     "v_mad_u32_u16 v144, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq00_%=\n\t" \
     "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
     "v_fma_f32 v28, s37, v113, v28\n\t" \
@@ -14096,37 +14028,37 @@
     "s_cbranch_scc0 .Lz00_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
     "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r0_%=\n\t" \
     "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     ".Lp1r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
     "v_fma_f32 v28, s37, v113, v28\n\t" \
@@ -14148,28 +14080,24 @@
     "v_fma_f32 v56, s43, v126, v56\n\t" \
     "v_fma_f32 v84, s43, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp1s0_%=\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     ".Lgvp1s0_%=:\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v128, v0\n\t" \
@@ -14202,22 +14130,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz00_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
     "v_fma_f32 v28, s37, v113, v28\n\t" \
@@ -14244,30 +14168,30 @@
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq10_%=\n\t" \
     "v_mad_u32_u16 v144, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v128, v0\n\t" \
     "v_fma_f32 v28, s45, v129, v28\n\t" \
@@ -14295,37 +14219,37 @@
     "s_cbranch_scc0 .Lz10_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
     "v_mad_u32_u16 v144, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r0_%=\n\t" \
     "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     ".Lp2r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v128, v0\n\t" \
     "v_fma_f32 v28, s45, v129, v28\n\t" \
@@ -14347,28 +14271,24 @@
     "v_fma_f32 v56, s51, v142, v56\n\t" \
     "v_fma_f32 v84, s51, v143, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp2s0_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     ".Lgvp2s0_%=:\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
@@ -14401,22 +14321,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz10_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v128, v0\n\t" \
     "v_fma_f32 v28, s45, v129, v28\n\t" \
@@ -14443,30 +14359,30 @@
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq20_%=\n\t" \
     "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
     "v_fma_f32 v28, s53, v113, v28\n\t" \
@@ -14494,37 +14410,37 @@
     "s_cbranch_scc0 .Lz20_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
     "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r0_%=\n\t" \
     "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     ".Lp3r0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
     "v_fma_f32 v28, s53, v113, v28\n\t" \
@@ -14546,28 +14462,24 @@
     "v_fma_f32 v56, s59, v126, v56\n\t" \
     "v_fma_f32 v84, s59, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp3s0_%=\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     ".Lgvp3s0_%=:\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v128, v0\n\t" \
@@ -14600,22 +14512,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lz20_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
     "v_fma_f32 v28, s53, v113, v28\n\t" \
@@ -14640,22 +14548,18 @@
     "s_branch .Lce0_%=\n\t" \
     ".Lf30_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v128, v0\n\t" \
     "v_fma_f32 v28, s61, v129, v28\n\t" \
@@ -14694,42 +14598,46 @@
     "s_setprio 2\n\t" \
     ".Lqa0_%=:\n\t" \
     "v_mad_u32_u16 v144, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s50, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna0_%=\n\t" \
     "s_load_dwordx16 s[52:67], %[pbase], s26\n\t" \
     ".Lna0_%=:\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
     "v_fma_f32 v28, s37, v113, v28\n\t" \
@@ -14751,22 +14659,18 @@
     "v_fma_f32 v56, s43, v126, v56\n\t" \
     "v_fma_f32 v84, s43, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s44, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s45, v128, v0\n\t" \
     "v_fma_f32 v28, s45, v129, v28\n\t" \
@@ -14804,42 +14708,46 @@
     "s_setprio 2\n\t" \
     ".Lqb0_%=:\n\t" \
     "v_mad_u32_u16 v144, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s66, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb0_%=\n\t" \
     "s_load_dwordx16 s[36:51], %[pbase], s26\n\t" \
     ".Lnb0_%=:\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
     "v_fma_f32 v28, s53, v113, v28\n\t" \
@@ -14861,22 +14769,18 @@
     "v_fma_f32 v56, s59, v126, v56\n\t" \
     "v_fma_f32 v84, s59, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s60, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s61, v128, v0\n\t" \
     "v_fma_f32 v28, s61, v129, v28\n\t" \
@@ -14906,73 +14810,73 @@
     "s_branch .Lba0_%=\n\t" \
     ".Lta0_%=:\n\t" \
     "v_mad_u32_u16 v144, s36, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v145, s38, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v144, s40, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v145, s42, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v144, s44, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v145, s46, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa0_%=\n\t" \
     "v_mad_u32_u16 v144, s48, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     ".Lgwa0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgva0_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     ".Lgva0_%=:\n\t" \
     "s_set_gpr_idx_on s36, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s37, v112, v0\n\t" \
@@ -15026,73 +14930,73 @@
     "s_branch .Lce0_%=\n\t" \
     ".Ltb0_%=:\n\t" \
     "v_mad_u32_u16 v144, s52, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v145, s54, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v144, s56, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v145, s58, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v144, s60, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v145, s62, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb0_%=\n\t" \
     "v_mad_u32_u16 v144, s64, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     ".Lgwb0_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgvb0_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     ".Lgvb0_%=:\n\t" \
     "s_set_gpr_idx_on s52, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s53, v112, v0\n\t" \
@@ -15419,43 +15323,43 @@
     "s_cbranch_scc1 .Lf01_%=\n\t" \
     "s_mov_b32 s27, s25\n\t" \
     "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp0r1_%=\n\t" \
     "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     ".Lp0r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp0s1_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     ".Lgvp0s1_%=:\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
@@ -15488,40 +15392,44 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lf01_%=:\n\t" \
     "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "s_cmp_ge_u32 s25, 8\n\t" \
     "s_cbranch_scc0 .Lq01_%=\n\t" \
     "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
     "v_fma_f32 v28, s69, v113, v28\n\t" \
@@ -15549,37 +15457,37 @@
     "s_cbranch_scc0 .Lz01_%=\n\t" \
     "s_sub_u32 s27, s25, 4\n\t" \
     "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp1r1_%=\n\t" \
     "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     ".Lp1r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
     "v_fma_f32 v28, s69, v113, v28\n\t" \
@@ -15601,28 +15509,24 @@
     "v_fma_f32 v56, s75, v126, v56\n\t" \
     "v_fma_f32 v84, s75, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp1s1_%=\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     ".Lgvp1s1_%=:\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v128, v0\n\t" \
@@ -15655,22 +15559,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz01_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
     "v_fma_f32 v28, s69, v113, v28\n\t" \
@@ -15697,30 +15597,30 @@
     "s_cmp_ge_u32 s25, 12\n\t" \
     "s_cbranch_scc0 .Lq11_%=\n\t" \
     "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v128, v0\n\t" \
     "v_fma_f32 v28, s77, v129, v28\n\t" \
@@ -15748,37 +15648,37 @@
     "s_cbranch_scc0 .Lz11_%=\n\t" \
     "s_sub_u32 s27, s25, 8\n\t" \
     "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp2r1_%=\n\t" \
     "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     ".Lp2r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v128, v0\n\t" \
     "v_fma_f32 v28, s77, v129, v28\n\t" \
@@ -15800,28 +15700,24 @@
     "v_fma_f32 v56, s83, v142, v56\n\t" \
     "v_fma_f32 v84, s83, v143, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp2s1_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     ".Lgvp2s1_%=:\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \
@@ -15854,22 +15750,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz11_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v128, v0\n\t" \
     "v_fma_f32 v28, s77, v129, v28\n\t" \
@@ -15896,30 +15788,30 @@
     "s_cmp_ge_u32 s25, 16\n\t" \
     "s_cbranch_scc0 .Lq21_%=\n\t" \
     "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
-    "s_waitcnt lgkmcnt(4)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
+    "s_waitcnt lgkmcnt(8)\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \
     "v_fma_f32 v28, s85, v113, v28\n\t" \
@@ -15947,37 +15839,37 @@
     "s_cbranch_scc0 .Lz21_%=\n\t" \
     "s_sub_u32 s27, s25, 12\n\t" \
     "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lp3r1_%=\n\t" \
     "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     ".Lp3r1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \
     "v_fma_f32 v28, s85, v113, v28\n\t" \
@@ -15999,28 +15891,24 @@
     "v_fma_f32 v56, s91, v126, v56\n\t" \
     "v_fma_f32 v84, s91, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s27, 1\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s27, 2\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     "s_cmp_gt_u32 s27, 3\n\t" \
     "s_cbranch_scc0 .Lgvp3s1_%=\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     ".Lgvp3s1_%=:\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v128, v0\n\t" \
@@ -16053,22 +15941,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lz21_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \
     "v_fma_f32 v28, s85, v113, v28\n\t" \
@@ -16093,22 +15977,18 @@
     "s_branch .Lce1_%=\n\t" \
     ".Lf31_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v128, v0\n\t" \
     "v_fma_f32 v28, s93, v129, v28\n\t" \
@@ -16147,42 +16027,46 @@
     "s_setprio 2\n\t" \
     ".Lqa1_%=:\n\t" \
     "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s82, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lna1_%=\n\t" \
     "s_load_dwordx16 s[84:99], %[pbase], s26\n\t" \
     ".Lna1_%=:\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
     "v_fma_f32 v28, s69, v113, v28\n\t" \
@@ -16204,22 +16088,18 @@
     "v_fma_f32 v56, s75, v126, v56\n\t" \
     "v_fma_f32 v84, s75, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s76, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s77, v128, v0\n\t" \
     "v_fma_f32 v28, s77, v129, v28\n\t" \
@@ -16257,42 +16137,46 @@
     "s_setprio 2\n\t" \
     ".Lqb1_%=:\n\t" \
     "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     "v_mad_u32_u16 v145, s98, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[140:141], v145\n\t" \
+    "ds_read_b64 v[142:143], v145\n\t" \
+    "ds_read_u16_d16_hi v140, v145\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
     "s_cmp_le_u32 s25, 8\n\t" \
     "s_cbranch_scc1 .Lnb1_%=\n\t" \
     "s_load_dwordx16 s[68:83], %[pbase], s26\n\t" \
     ".Lnb1_%=:\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \
     "v_fma_f32 v28, s85, v113, v28\n\t" \
@@ -16314,22 +16198,18 @@
     "v_fma_f32 v56, s91, v126, v56\n\t" \
     "v_fma_f32 v84, s91, v127, v84\n\t" \
     "s_set_gpr_idx_off\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
-    "v_and_b32_e32 v143, 0xffff0000, v141\n\t" \
-    "v_lshlrev_b32_e32 v142, 16, v141\n\t" \
-    "v_and_b32_e32 v141, 0xffff0000, v140\n\t" \
-    "v_lshlrev_b32_e32 v140, 16, v140\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
+    "v_and_b32_e32 v141, 0xffff0000, v142\n\t" \
+    "v_lshlrev_b32_e32 v142, 16, v143\n\t" \
+    "v_and_b32_e32 v143, 0xffff0000, v143\n\t" \
     "s_set_gpr_idx_on s92, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s93, v128, v0\n\t" \
     "v_fma_f32 v28, s93, v129, v28\n\t" \
@@ -16359,73 +16239,73 @@
     "s_branch .Lba1_%=\n\t" \
     ".Lta1_%=:\n\t" \
     "v_mad_u32_u16 v144, s68, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v145, s70, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v144, s72, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v145, s74, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v144, s76, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v145, s78, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwa1_%=\n\t" \
     "v_mad_u32_u16 v144, s80, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     ".Lgwa1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgva1_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     ".Lgva1_%=:\n\t" \
     "s_set_gpr_idx_on s68, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s69, v112, v0\n\t" \
@@ -16479,73 +16359,73 @@
     "s_branch .Lce1_%=\n\t" \
     ".Ltb1_%=:\n\t" \
     "v_mad_u32_u16 v144, s84, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[112:113], v144\n\t" \
+    "ds_read_b64 v[114:115], v144\n\t" \
+    "ds_read_u16_d16_hi v112, v144\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v145, s86, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[116:117], v145\n\t" \
+    "ds_read_b64 v[118:119], v145\n\t" \
+    "ds_read_u16_d16_hi v116, v145\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v144, s88, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[120:121], v144\n\t" \
+    "ds_read_b64 v[122:123], v144\n\t" \
+    "ds_read_u16_d16_hi v120, v144\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v145, s90, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[124:125], v145\n\t" \
+    "ds_read_b64 v[126:127], v145\n\t" \
+    "ds_read_u16_d16_hi v124, v145\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v144, s92, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[128:129], v144\n\t" \
+    "ds_read_b64 v[130:131], v144\n\t" \
+    "ds_read_u16_d16_hi v128, v144\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v145, s94, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[132:133], v145\n\t" \
+    "ds_read_b64 v[134:135], v145\n\t" \
+    "ds_read_u16_d16_hi v132, v145\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgwb1_%=\n\t" \
     "v_mad_u32_u16 v144, s96, 16, %[vb0] op_sel:[1,0,0,0]\n\t" \
-    "ds_read_b64 v[136:137], v144\n\t" \
+    "ds_read_b64 v[138:139], v144\n\t" \
+    "ds_read_u16_d16_hi v136, v144\n\t" \
     ".Lgwb1_%=:\n\t" \
     "s_waitcnt lgkmcnt(0)\n\t" \
-    "v_and_b32_e32 v115, 0xffff0000, v113\n\t" \
-    "v_lshlrev_b32_e32 v114, 16, v113\n\t" \
-    "v_and_b32_e32 v113, 0xffff0000, v112\n\t" \
-    "v_lshlrev_b32_e32 v112, 16, v112\n\t" \
+    "v_and_b32_e32 v113, 0xffff0000, v114\n\t" \
+    "v_lshlrev_b32_e32 v114, 16, v115\n\t" \
+    "v_and_b32_e32 v115, 0xffff0000, v115\n\t" \
     "s_cmp_gt_u32 s25, 1\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v119, 0xffff0000, v117\n\t" \
-    "v_lshlrev_b32_e32 v118, 16, v117\n\t" \
-    "v_and_b32_e32 v117, 0xffff0000, v116\n\t" \
-    "v_lshlrev_b32_e32 v116, 16, v116\n\t" \
+    "v_and_b32_e32 v117, 0xffff0000, v118\n\t" \
+    "v_lshlrev_b32_e32 v118, 16, v119\n\t" \
+    "v_and_b32_e32 v119, 0xffff0000, v119\n\t" \
     "s_cmp_gt_u32 s25, 2\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v123, 0xffff0000, v121\n\t" \
-    "v_lshlrev_b32_e32 v122, 16, v121\n\t" \
-    "v_and_b32_e32 v121, 0xffff0000, v120\n\t" \
-    "v_lshlrev_b32_e32 v120, 16, v120\n\t" \
+    "v_and_b32_e32 v121, 0xffff0000, v122\n\t" \
+    "v_lshlrev_b32_e32 v122, 16, v123\n\t" \
+    "v_and_b32_e32 v123, 0xffff0000, v123\n\t" \
     "s_cmp_gt_u32 s25, 3\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v127, 0xffff0000, v125\n\t" \
-    "v_lshlrev_b32_e32 v126, 16, v125\n\t" \
-    "v_and_b32_e32 v125, 0xffff0000, v124\n\t" \
-    "v_lshlrev_b32_e32 v124, 16, v124\n\t" \
+    "v_and_b32_e32 v125, 0xffff0000, v126\n\t" \
+    "v_lshlrev_b32_e32 v126, 16, v127\n\t" \
+    "v_and_b32_e32 v127, 0xffff0000, v127\n\t" \
     "s_cmp_gt_u32 s25, 4\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v131, 0xffff0000, v129\n\t" \
-    "v_lshlrev_b32_e32 v130, 16, v129\n\t" \
-    "v_and_b32_e32 v129, 0xffff0000, v128\n\t" \
-    "v_lshlrev_b32_e32 v128, 16, v128\n\t" \
+    "v_and_b32_e32 v129, 0xffff0000, v130\n\t" \
+    "v_lshlrev_b32_e32 v130, 16, v131\n\t" \
+    "v_and_b32_e32 v131, 0xffff0000, v131\n\t" \
     "s_cmp_gt_u32 s25, 5\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v135, 0xffff0000, v133\n\t" \
-    "v_lshlrev_b32_e32 v134, 16, v133\n\t" \
-    "v_and_b32_e32 v133, 0xffff0000, v132\n\t" \
-    "v_lshlrev_b32_e32 v132, 16, v132\n\t" \
+    "v_and_b32_e32 v133, 0xffff0000, v134\n\t" \
+    "v_lshlrev_b32_e32 v134, 16, v135\n\t" \
+    "v_and_b32_e32 v135, 0xffff0000, v135\n\t" \
     "s_cmp_gt_u32 s25, 6\n\t" \
     "s_cbranch_scc0 .Lgvb1_%=\n\t" \
-    "v_and_b32_e32 v139, 0xffff0000, v137\n\t" \
-    "v_lshlrev_b32_e32 v138, 16, v137\n\t" \
-    "v_and_b32_e32 v137, 0xffff0000, v136\n\t" \
-    "v_lshlrev_b32_e32 v136, 16, v136\n\t" \
+    "v_and_b32_e32 v137, 0xffff0000, v138\n\t" \
+    "v_lshlrev_b32_e32 v138, 16, v139\n\t" \
+    "v_and_b32_e32 v139, 0xffff0000, v139\n\t" \
     ".Lgvb1_%=:\n\t" \
     "s_set_gpr_idx_on s84, gpr_idx(SRC2,DST)\n\t" \
     "v_fma_f32 v0, s85, v112, v0\n\t" \

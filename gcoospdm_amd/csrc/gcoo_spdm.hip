@@ -654,7 +654,9 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_lds_kernel(
 //     address, and the index instructions take the slot from its low byte.
 //   * BF16 D is staged as it is (150-row chunks of 512-byte rows, two rows
 //     per wave load), read 8 bytes per lane and widened exactly to FP32
-//     before the steps; the list values are stored widened.
+//     before the steps, the lane's first column by a second, 16-bit read into
+//     the high half of a register with a zero low half; the list values are
+//     stored widened.
 // Every row's chain is its entries in ascending k, as in all variants.
 // ---------------------------------------------------------------------------
 typedef float f32x16 __attribute__((ext_vector_type(16)));

@@ -7,10 +7,12 @@ GCOO_PANEL_SEGMENT_CLOBBERS_R<R>, and GCOO_PANEL_ACC_OPERANDS_R<R>(a), the
 operands pinning the accumulators (an array of 4 R / 8 f32x8) to v0..v(4R-1);
 and for BF16 D (ROWS_BF16) GCOO_PANEL_SEGMENT_ASM_BF16_R<R> with its
 clobbers.  The BF16 block stages 150-row chunks of 512-byte D rows (two rows
-per 1 KB wave load, %[rowb] the two rows' bytes, %[row1] one row's), reads
-an entry's 4 values with ds_read_b64 and widens them exactly to FP32 (the
-BF16 bits as the high half) before the steps; the list values are already
-widened.
+per 1 KB wave load, %[rowb] the two rows' bytes, %[row1] one row's) and
+reads an entry's 4 values with ds_read_b64, widening columns 1..3 exactly
+to FP32 (the BF16 bits as the high half: 2 ANDs, 1 shift), and column 0
+with a second, 16-bit read into the high half of a register whose low half
+the block zeroes once per segment (so that column costs an LDS read instead
+of a vector instruction); the list values are already widened.
 
     python3 gen_panel_asm.py > gcoo_panel_asm.h
 
@@ -38,7 +40,8 @@ Per chunk, in order:
     g + 1 are issued before the steps of group g, and lgkmcnt(4) waits for
     group g (LDS reads complete in order, so with 4 later reads issued the
     count is reached only once group g is in, whatever scalar loads are
-    still outstanding).  A last group of 1..3 entries reads and steps only
+    still outstanding; BF16 issues 8 reads per group and waits for
+    lgkmcnt(8)).  A last group of 1..3 entries reads and steps only
     its valid entries, after lgkmcnt(0);
   * entries from 16 on: the batch loop, 8 entries per batch: 8 LDS reads,
     the wait for them, the scalar load of the next batch's 8 entries (run
@@ -83,6 +86,7 @@ NSCR = 34                 # scratch VGPRs above the accumulators
 PRIO_CHUNK, PRIO_BATCH = 12, 16  # entries left for issue priority 3 (see configure)
 R = 16                    # rows per wave of the block being generated
 BF16 = False              # the block reads BF16 D (8 bytes per lane) and widens it
+RPE = 1                   # LDS read instructions per entry (BF16: 2)
 DV, AV = 64, 96           # D values of the batch loop; two address VGPRs, used in turn
 G_D = (64, 80)            # D buffers of the rolling phase
 L = []
@@ -91,9 +95,10 @@ L = []
 def configure(rows, bf16=False):
     """Register layout for `rows` rows per wave: accumulators v0..v(4 rows - 1),
     scratch from v(4 rows) on."""
-    global R, DV, AV, G_D, BF16, PRIO_CHUNK, PRIO_BATCH
+    global R, DV, AV, G_D, BF16, RPE, PRIO_CHUNK, PRIO_BATCH
     R = rows
     BF16 = bf16
+    RPE = 2 if bf16 else 1
     PRIO_CHUNK = (3 * rows + 2) // 4
     PRIO_BATCH = rows
     base = 4 * rows
@@ -123,24 +128,27 @@ class Body:
     # ---- D reads and their widening ----
     @staticmethod
     def dread(d, a):
-        """The LDS read of an entry's 4 D values into v[d..d+3] from the
-        address in v{a} (BF16: two dwords into v[d], v[d+1])."""
+        """The LDS reads of an entry's 4 D values (columns 0..3 of the lane)
+        into v[d..d+3] from the address in v{a}.  BF16: the 8 bytes into
+        v[d+2], v[d+3] (two values per dword, low half first), and column 0
+        again with a 16-bit read into the high half of v[d], whose low half
+        stays zero: its exact FP32 value."""
         if BF16:
-            e(f"ds_read_b64 v[{d}:{d+1}], v{a}")
+            e(f"ds_read_b64 v[{d+2}:{d+3}], v{a}")
+            e(f"ds_read_u16_d16_hi v{d}, v{a}")
         else:
             e(f"ds_read_b128 v[{d}:{d+3}], v{a}")
 
     @staticmethod
     def widen(d):
-        """BF16: the 4 values in v[d], v[d+1] (two per dword, low half first)
-        widened to FP32 in v[d..d+3], exactly (the BF16 bits become the FP32
-        high half).  Outside VGPR-index mode, which would offset the
-        destinations."""
+        """BF16: columns 1..3 of an entry widened exactly to FP32 (the BF16
+        bits as the high half) from the pairs in v[d+2], v[d+3]; column 0 came
+        widened from its 16-bit read.  Outside VGPR-index mode, which would
+        offset the destinations."""
         if BF16:
-            e(f"v_and_b32_e32 v{d+3}, 0xffff0000, v{d+1}")
-            e(f"v_lshlrev_b32_e32 v{d+2}, 16, v{d+1}")
-            e(f"v_and_b32_e32 v{d+1}, 0xffff0000, v{d}")
-            e(f"v_lshlrev_b32_e32 v{d}, 16, v{d}")
+            e(f"v_and_b32_e32 v{d+1}, 0xffff0000, v{d+2}")
+            e(f"v_lshlrev_b32_e32 v{d+2}, 16, v{d+3}")
+            e(f"v_and_b32_e32 v{d+3}, 0xffff0000, v{d+3}")
 
     # ---- batch loop (entries from 16 on) ----
     def reads(self, S, k):
@@ -303,7 +311,8 @@ class Body:
             e(f"s_cmp_ge_u32 s{SREM}, {4*g+8}")
             e(f"s_cbranch_scc0 .Lq{g}{p}_%=")
             self.group_reads(g + 1)
-            e("s_waitcnt lgkmcnt(4)")
+            # group g is in once at most the next group's reads are out
+            e(f"s_waitcnt lgkmcnt({4 * RPE})")
             self.group_steps_full(g)
             e(f"s_branch .Lf{g+1}{p}_%=")
             e(f".Lq{g}{p}_%=:")
@@ -434,6 +443,10 @@ def generate(rows, bf16=False):
     body0, body1 = Body(0), Body(1)
     # prologue: the segment's first chunk, its entries loaded before its barrier
     e(f"s_mov_b32 s{SM0}, m0")
+    if BF16:
+        # column 0 registers: 16-bit reads write their high halves only
+        for i in range(0, 32, 4):
+            e(f"v_mov_b32 v{DV + i}, 0")
     e("s_nop 4")  # %[sli] may have just been written by a vector instruction
     e(f"v_readlane_b32 s{PBN}, %[vpbo], %[sli]")
     e(f"v_readlane_b32 s{SNN}, %[vpn], %[sli]")

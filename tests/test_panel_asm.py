@@ -231,7 +231,7 @@ class Lane:
                 # two dwords of BF16 pairs: dword i holds columns 2i (low
                 # half) and 2i + 1 (high half)
                 kind, regs = _regs(args[0])
-                assert len(regs) == 2
+                assert len(regs) == 2 and regs[0] % 2 == 0
                 akind, addr = self.vread(args[1])
                 assert akind == "addr"
                 self._check("v", regs, "load into")
@@ -239,6 +239,24 @@ class Lane:
                 for i in range(2):
                     self.v[regs[i]] = ("d64", addr, i)
                 self.inflight.append(("lds", {("v", r) for r in regs}))
+            elif op == "ds_read_u16_d16_hi":
+                # BF16: 16 bits into the register's high half, the low half
+                # kept; the result is the exact FP32 value only if that low
+                # half is zero (a zeroed register, or one only such reads
+                # have written).  Byte offset 2 j is column j of the lane.
+                d = int(args[0][1:])
+                a, _, off = args[1].partition(" ")
+                off = int(off.split(":")[1]) if off else 0
+                assert off in (0, 2, 4, 6), ln
+                akind, addr = self.vread(a)
+                assert akind == "addr"
+                self._check("v", [d], "load into")
+                prev = self.v.get(d)
+                assert prev == ("zero",) or (prev is not None and prev[0] == "d16"), \
+                    f"16-bit read into v{d} whose low half is not known zero"
+                self.on_lds_read(addr + off, 2)
+                self.v[d] = ("d16", addr, off // 2)
+                self.inflight.append(("lds", {("v", d)}))
             elif op in ("v_and_b32_e32", "v_lshlrev_b32_e32"):
                 # BF16 widening: mask keeps the high half (odd column), a
                 # shift by 16 moves the low half up (even column)
@@ -254,6 +272,11 @@ class Lane:
                     col = 2 * i
                 self._check("v", [d], "write")
                 self.v[d] = ("d", addr, col)
+            elif op == "v_mov_b32":
+                assert not self.idx_on and args[1] == "0", ln
+                d = int(args[0][1:])
+                self._check("v", [d], "write")
+                self.v[d] = ("zero",)
             elif op == "buffer_load_dwordx4":
                 assert args[0] in ("%[voff]", "%[voffz]") and args[1] == "%[rsrc]"
                 soff = self.sval(args[2].split()[0])
@@ -265,6 +288,9 @@ class Lane:
                 s2 = int(args[3][1:]) + self.idx
                 assert d == s2
                 src = self.vread(args[2])
+                if src[0] == "d16":
+                    src = ("d",) + src[1:]
+                assert src[0] == "d", f"step on {src}"
                 self._check("v", [d], "accumulate into")
                 self.fmas.append((self.barriers, d, self.sval(args[1]), src))
             else:
@@ -445,6 +471,20 @@ def test_interpreter_catches_an_early_read():
            "sso": 75 * ROWB, "lim": 0, "kb": 0, "rowb": ROWB, "tkrowb": 75 * ROWB}
     with pytest.raises(AssertionError, match="in flight"):
         lane_cls(prog, ents, ops, {"vpbo": [0] * 64, "vpn": [12] + [0] * 63}).run()
+
+
+def test_interpreter_catches_a_bf16_read_without_a_zero_low_half():
+    # BF16 column 0 is exact only if its register's low half is zero: a block
+    # that does not zero those registers first must fail
+    prog = [ln for ln in _prog(21, True) if not re.fullmatch(r"v_mov_b32 v\d+, 0", ln)]
+    assert len(prog) < len(_prog(21, True))
+    rng = random.Random(4)
+    ents = [(word(rng.randrange(150) * 512, i % 21), i) for i in range(40)]
+    ops = {"pbase": 0, "sli": 0, "liend": 1, "par": 0, "vb0": 0, "sdst": 0,
+           "sso": 150 * ROWB, "lim": 0, "kb": 0, "rowb": 2 * ROWB, "row1": ROWB,
+           "tkrowb": 150 * ROWB}
+    with pytest.raises(AssertionError, match="low half"):
+        Lane(prog, ents, ops, {"vpbo": [0] * 64, "vpn": [12] + [0] * 63}).run()
 
 
 def test_committed_asm_header_is_the_generator_output():
