@@ -10,10 +10,13 @@ reference's own results time (src/bench.py:9 KERNEL_ONLY=False); `value` is
 `roofline` is the multiply kernel alone (the dominant kernel of the step):
 algorithmic bytes 12*nnz + 8*(nGroup+1) + 4*K*N + 4*M*N per launch over its
 mean duration, measured with HIP events on the stream it is launched on,
-against MI355X's 8 TB/s.  `traffic` is the HBM bytes per launch measured in
-the run by two rocprofv3 counter passes over `bench.py --profile-kernel` in a
-child process (FETCH_SIZE, then WRITE_SIZE, each scaled by what it reads for
-a 256 MiB copy); null, with the reason, when rocprofv3 is missing or fails.
+against MI355X's 8 TB/s.  `traffic` is the HBM-side bytes per launch
+measured in the run by rocprofv3 counter passes over `bench.py
+--profile-kernel` in a child process: reads from the L2's read requests to
+memory by size, writes from WRITE_SIZE scaled by what it reads for a 256 MiB
+copy; FETCH_SIZE scaled the same way is reported beside it
+(`traffic_copy_scaled`).  Null, with the reason, when rocprofv3 is missing
+or fails.
 
 `cpu_baseline` times the oracle's C restatement (OpenMP, and on one core) on
 a bounded sample.  Extra objects report the other workloads: the kernel alone
@@ -218,9 +221,9 @@ def _short_kernel_name(name):
     return name.replace("gcoo::(anonymous namespace)::", "").replace("void ", "").split("(")[0]
 
 
-def _counter_pass(counter, n, s, seed, timeout_s):
+def _counter_pass(counters, n, s, seed, timeout_s):
     """One rocprofv3 counter pass over `bench.py --profile-kernel` in a child
-    process: {short kernel name: [KiB per dispatch, ...]}."""
+    process: {counter: {short kernel name: [value per dispatch, ...]}}."""
     import csv
     import shutil
     import signal
@@ -230,7 +233,8 @@ def _counter_pass(counter, n, s, seed, timeout_s):
         raise RuntimeError("rocprofv3 not found")
     out = tempfile.mkdtemp(prefix="gcoo_pmc_", dir="/tmp")
     env = dict(os.environ, TMPDIR="/tmp", PYTHONPATH=ROOT)
-    cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+    counter = " ".join(counters)
+    cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
            sys.executable, os.path.join(ROOT, "bench.py"), "--profile-kernel", "--n", str(n),
            "--sparsity", str(s), "--seed", str(seed)]
     p = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
@@ -249,7 +253,8 @@ def _counter_pass(counter, n, s, seed, timeout_s):
             for fn in files:
                 if fn.endswith("counter_collection.csv"):
                     for r in csv.DictReader(open(os.path.join(root, fn))):
-                        vals.setdefault(_short_kernel_name(r["Kernel_Name"]), []).append(
+                        vals.setdefault(r["Counter_Name"], {}).setdefault(
+                            _short_kernel_name(r["Kernel_Name"]), []).append(
                             float(r["Counter_Value"]))
         if not vals:
             raise RuntimeError(f"{counter} pass wrote no counters")
@@ -258,26 +263,54 @@ def _counter_pass(counter, n, s, seed, timeout_s):
         shutil.rmtree(out, ignore_errors=True)
 
 
+REQ_COUNTERS = ("TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum",
+                "TCC_EA0_RDREQ_sum")
+
+
 def measure_traffic(n, s, seed, kernels, timeout_s=180):
-    """HBM bytes per launch of the given kernels, measured now: FETCH_SIZE,
-    then WRITE_SIZE, each in its own rocprofv3 pass, each scaled by what the
-    counter reads for a copy of a 256 MiB tensor (256 MiB read, 256 MiB
-    written; the largest reading of its ten dispatches, which stream
-    through the caches).  Returns {kernel: bytes} plus the two factors, or
+    """HBM-side bytes per launch of the given kernels, measured now in three
+    rocprofv3 passes: FETCH_SIZE, WRITE_SIZE, and the L2's read requests to
+    memory by size (32, 64 and 128 bytes, and their total).
+
+    FETCH_SIZE and WRITE_SIZE are each scaled by what they read for a copy of
+    a 256 MiB tensor (the largest of its ten dispatches).  FETCH_SIZE counts
+    every request that is not a 32-byte one as 64 bytes, and the copy's reads
+    are all 128-byte requests (factor 2), so the scaled figure is exact only
+    for kernels whose requests are all 128-byte ones; the multiply's scalar
+    list loads are 64-byte requests.  The read bytes are therefore taken
+    from the request counts (32 n32 + 64 n64 + 128 n128), the write bytes
+    from the scaled WRITE_SIZE (factor 1), and the copy-scaled fetch is
+    reported beside them.  Returns {kernel: {...}} plus the factors, or
     {"error": ...}."""
     try:
-        fe = _counter_pass("FETCH_SIZE", n, s, seed, timeout_s)
-        wr = _counter_pass("WRITE_SIZE", n, s, seed, timeout_s)
+        fe = _counter_pass(["FETCH_SIZE"], n, s, seed, timeout_s)["FETCH_SIZE"]
+        wr = _counter_pass(["WRITE_SIZE"], n, s, seed, timeout_s)["WRITE_SIZE"]
+        rq = _counter_pass(list(REQ_COUNTERS), n, s, seed, timeout_s)
         copy = "__amd_rocclr_copyBuffer"
         ff = 262144.0 / max(fe[copy])
         wf = 262144.0 / max(wr[copy])
-        out = {"fetch_factor": round(ff, 4), "write_factor": round(wf, 4)}
-        for k in kernels:
-            if k not in fe or k not in wr:
+
+        def mean(d, k):
+            if k not in d:
                 raise RuntimeError(f"no counters for {k}")
-            f_b = sum(fe[k]) / len(fe[k]) * 1024 * ff
-            w_b = sum(wr[k]) / len(wr[k]) * 1024 * wf
-            out[k] = {"fetch_bytes": int(f_b), "write_bytes": int(w_b), "bytes": int(f_b + w_b)}
+            return sum(d[k]) / len(d[k])
+
+        def read_bytes(k, pick=mean):
+            n32, n64, n128, tot = (pick(rq[c], k) for c in REQ_COUNTERS)
+            return 32 * n32 + 64 * n64 + 128 * n128, (n32, n64, n128, tot)
+
+        copy_read, _ = read_bytes(copy, lambda d, k: max(d[k]))
+        out = {"fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+               "copy_read_check": round(copy_read / 268435456.0, 4)}
+        for k in kernels:
+            f_b, (n32, n64, n128, tot) = read_bytes(k)
+            if abs(n32 + n64 + n128 - tot) > 0.005 * tot + 1:
+                raise RuntimeError(f"{k}: read requests by size do not add up to the total")
+            f_cal = mean(fe, k) * 1024 * ff
+            w_b = mean(wr, k) * 1024 * wf
+            out[k] = {"fetch_bytes": int(f_b), "write_bytes": int(w_b), "bytes": int(f_b + w_b),
+                      "read_requests": {"32B": int(n32), "64B": int(n64), "128B": int(n128)},
+                      "fetch_bytes_copy_scaled": int(f_cal)}
         return out
     except Exception as e:  # counters are reported, not required
         return {"error": str(e)[:400]}
@@ -625,8 +658,13 @@ def main():
                      "variant": kvariant, "vec": kvec,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE passes in this run, "
+                     "traffic_source": ("rocprofv3 passes in this run: reads from the L2's "
+                                        "32/64/128-byte request counts, writes from WRITE_SIZE "
                                         "scaled by a 256 MiB copy" if traffic else traffic_meas),
+                     "traffic_detail": traffic_meas.get(kname) if traffic else None,
+                     "traffic_copy_scaled": (traffic_meas[kname]["fetch_bytes_copy_scaled"] +
+                                             traffic_meas[kname]["write_bytes"]) if traffic else None,
+                     "copy_read_check": traffic_meas.get("copy_read_check") if traffic else None,
                      "traffic_over_algorithmic": round(traffic / bytes_k, 3) if traffic else None,
                      "bf16_kernel": kname16, "bf16_traffic": traffic16,
                      "algorithmic_bytes": bytes_k, "mean_launch_us": round(t_kern * 1e6, 2),

@@ -107,6 +107,45 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.stdout == ""
 
 
+def test_bench_traffic_from_request_sizes(monkeypatch):
+    """bench.py's `traffic`: read bytes from the L2's request counts by size,
+    write bytes from WRITE_SIZE scaled by the 256 MiB copy, and FETCH_SIZE
+    scaled by the copy beside them.  Counter readings are stubbed with the
+    shape of a real pass: the copy's reads are all 128-byte requests, which
+    FETCH_SIZE counts as 64 bytes (factor 2); a kernel with 64-byte requests
+    as well is overstated by the scaled FETCH_SIZE and exact from the counts."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    copy, k = "__amd_rocclr_copyBuffer", "spdm_panel_kernel<F32, 75, 16, 21>"
+    n128, n64 = 3_000_000, 2_000_000
+    counts = {
+        "FETCH_SIZE": {copy: [131072.0, 10.0], k: [(n128 + n64) * 64 / 1024]},
+        "WRITE_SIZE": {copy: [262144.0], k: [250000.0]},
+        "TCC_EA0_RDREQ_32B_sum": {copy: [0.0], k: [0.0]},
+        "TCC_EA0_RDREQ_64B_sum": {copy: [0.0], k: [float(n64)]},
+        "TCC_EA0_RDREQ_128B_sum": {copy: [2097152.0], k: [float(n128)]},
+        "TCC_EA0_RDREQ_sum": {copy: [2097152.0], k: [float(n128 + n64)]},
+    }
+
+    def fake_pass(counters, n, s, seed, timeout_s):
+        return {c: counts[c] for c in counters}
+
+    monkeypatch.setattr(bench, "_counter_pass", fake_pass)
+    out = bench.measure_traffic(8000, 0.99, 1, (k,))
+    assert out["fetch_factor"] == 2.0 and out["write_factor"] == 1.0
+    assert out["copy_read_check"] == 1.0
+    r = out[k]
+    assert r["fetch_bytes"] == 128 * n128 + 64 * n64
+    assert r["write_bytes"] == 250000 * 1024
+    assert r["bytes"] == r["fetch_bytes"] + r["write_bytes"]
+    assert r["fetch_bytes_copy_scaled"] == 2 * 64 * (n128 + n64)
+    # request counts that do not add up to the total are an error, reported
+    counts["TCC_EA0_RDREQ_sum"][k] = [float(n128 + 2 * n64)]
+    assert "do not add up" in bench.measure_traffic(8000, 0.99, 1, (k,))["error"]
+
+
 @pytest.mark.skipif(not os.path.exists(MAIN), reason="main not built")
 @pytest.mark.parametrize("seed,sparsity", [(None, 0.7), (7, 0.9), (12345, 0.0)])
 def test_main_ref_inputs_match_reference_randomInit(tmp_path, seed, sparsity):
