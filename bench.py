@@ -447,14 +447,14 @@ def zipf_case(G, dev):
     handles = {4: m}
     for dt in ("f32", "bf16"):
         tuned = autotune.skew_config(N, per_row, max_row, dt == "bf16")
-        cfg = dict(cpg=4, variant=0, vec=0, unroll=0, chunk=0)
+        cfg = dict(cpg=4, variant=0, vec=0, unroll=0, chunk=0, waves=0)
         if tuned:
-            cfg.update({k: tuned[k] for k in cfg})
+            cfg.update({k: tuned[k] for k in cfg if k in tuned})
         if cfg["cpg"] not in handles:
             handles[cfg["cpg"]] = G.GCOOMatrix(M, K, cfg["cpg"]).convert_coo(r, c, v)
         h = handles[cfg["cpg"]]
         h.set_config(variant=cfg["variant"], vec=cfg["vec"], unroll=cfg["unroll"],
-                     chunk=cfg["chunk"])
+                     chunk=cfg["chunk"], waves=cfg["waves"])
         fn = (lambda: h.spdm(D, C, sync=False)) if dt == "f32" else \
              (lambda: h.spdm_bf16(D16, C, sync=False))
         fn()

@@ -10,7 +10,7 @@ Three tables:
   batch[dtype][class] = {rows, vec}   for gcoo_batch_add, where
       class = (N bucket, entries-per-row bucket) of a job;
   plain[dtype][class] = {variant, vec} for gcoo_set_spdm_config;
-  skew[dtype][class]  = {cpg, variant, vec, unroll, chunk} for matrices
+  skew[dtype][class]  = {cpg, variant, vec, unroll, chunk, waves} for matrices
       with skewed rows, class = (N bucket, entries-per-row bucket, skew
       bucket = longest row over the mean row): the group width (the
       reference's COLUMN_PER_GROUP, src/constants.h.t:4, grid-searched by
@@ -69,7 +69,7 @@ def skew_key(N: int, per_row: float, max_row: float) -> str:
 
 
 def skew_config(N: int, per_row: float, max_row: float, bf16: bool, table: dict | None = None):
-    """Tuned {cpg, variant, vec, unroll, chunk} for a skewed matrix, or None."""
+    """Tuned {cpg, variant, vec, unroll, chunk, waves} for a skewed matrix, or None."""
     t = (table if table is not None else load()).get("skew", {}).get("bf16" if bf16 else "f32", {})
     return t.get(skew_key(N, per_row, max_row))
 
@@ -205,10 +205,14 @@ def tune_skew(bf16: bool, log, device="cuda:0", seed=11):
     for vec in vecs:
         for unroll in (8, 16, 32):
             for chunk in (0, 1):
-                cands.append(dict(variant=2, vec=vec, unroll=unroll, chunk=chunk))
+                # waves: the long-rows-first threshold of variant 2 (0
+                # automatic, 1 off, w: rows of 2^w entries or more first)
+                for waves in (0, 1, 8):
+                    cands.append(dict(variant=2, vec=vec, unroll=unroll, chunk=chunk,
+                                      waves=waves))
         for chunk in (64, 256, 1024):
-            cands.append(dict(variant=4, vec=vec, unroll=16, chunk=chunk))
-    cands.append(dict(variant=6, vec=4, unroll=0, chunk=0))
+            cands.append(dict(variant=4, vec=vec, unroll=16, chunk=chunk, waves=0))
+    cands.append(dict(variant=6, vec=4, unroll=0, chunk=0, waves=0))
     best, ref = None, None
     for cpg in (1, 2, 4):
         m = G.GCOOMatrix(M, K, cpg).convert_coo(r, c, v)

@@ -295,29 +295,61 @@ __device__ __forceinline__ void lds_load(float (&d)[VEC], const unsigned char *p
 // conversion scatter) with one accumulator set: no per-entry row dispatch.  D is read with buffer loads: the entry's row
 // offset k*N*bytes is a scalar offset, the lane's column offset a vector
 // offset, and lanes past N read zeros from the hardware range check.
+//
+// Long rows first.  A row's chain is serial, so on a skewed matrix the
+// longest rows set the launch's end when they start late.  With longBlocks
+// > 0 the first workgroups of each column tile take the rows listed in
+// longList (longCtr[0] of them, one row per wave, listed on the device by
+// length >= the threshold), and the row units skip those rows (skipLen).
+// With xpt > 0 (8 / column tiles) XCD x serves tile x / xpt: its first
+// workgroups are its share of the long rows (interleaved over the tile's
+// xpt XCDs), the rest a contiguous range of the row units.
 // ---------------------------------------------------------------------------
 template <typename E, int VEC, int U>
 __global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int nGroup, int cpg,
     int M, const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C,
-    int nGroupBlocks, const ConvStatus *__restrict__ guard, long long cap, int skipLen)
+    int nGroupBlocks, const ConvStatus *__restrict__ guard, long long cap, int skipLen,
+    const int *__restrict__ longList, const unsigned *__restrict__ longCtr, int longBlocks,
+    int xpt)
 {
     if (guard_skip(guard, cap)) return;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int ct = L / nGroupBlocks;
-    const int gb = L - ct * nGroupBlocks;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = __lane_id();
-    const int g = gb * kSpdmWaves + wave;
-    if (g >= nGroup) return;
+    int ct, gb;  // column tile; row block, or long block when negative (-1 - index)
+    if (xpt) {
+        const int x = blockIdx.x % kNumXcd, i = blockIdx.x / kNumXcd;
+        const int lbx = (longBlocks + xpt - 1) / xpt;  // long blocks per XCD
+        const int rbx = (nGroupBlocks + xpt - 1) / xpt;
+        ct = x / xpt;
+        gb = i < lbx ? -1 - (i * xpt + x % xpt) : (x % xpt) * rbx + (i - lbx);
+        if (gb < 0 ? -1 - gb >= longBlocks : gb >= nGroupBlocks) return;
+    } else {
+        const int L = xcd_remap(blockIdx.x, gridDim.x);
+        const int per = longBlocks + nGroupBlocks;
+        ct = L / per;
+        gb = L - ct * per;
+        gb = gb < longBlocks ? -1 - gb : gb - longBlocks;
+    }
+    int r0, r1;
+    if (gb < 0) {
+        const int u = (-1 - gb) * kSpdmWaves + wave;
+        if (u >= (int)longCtr[0]) return;
+        r0 = longList[u];
+        r1 = r0 + 1;
+        skipLen = INT_MAX;
+    } else {
+        const int g = gb * kSpdmWaves + wave;
+        if (g >= nGroup) return;
+        r0 = g * cpg;
+        r1 = min(r0 + cpg, M);
+    }
     const int j = ct * (64 * VEC) + lane * VEC;
     const bool active = j < N;
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(D), (short)0, (int)dBytes, kBufferFlags);
     const int voff = active ? j * E::bytes : (int)dBytes;  // out of range -> zeros
     const int rowBytes = N * E::bytes;
-    const int r0 = g * cpg;
-    const int r1 = min(r0 + cpg, M);
     for (int r = r0; r < r1; ++r) {
         float acc[VEC];
 #pragma unroll
@@ -1073,9 +1105,29 @@ int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
     return GCOO_OK;
 }
 
+// Threshold of the long-rows-first order of variant 2 (entries; 0: off).
+// Configured as 2^waves (waves 2..16, 1: off), else automatic: on when the
+// last conversion's longest row holds at least 16 times the mean and 512
+// entries, with the power of two at or above 16 times the mean (at least
+// 256).  On the 65536^2 Zipf matrix (mean 16, longest 1334 rows, 476 rows
+// of 512 or more holding 37 % of the entries) the threshold 512 took 146
+// against 164 us (FP32, 1 column per lane, rounds of 16).
+int long_first_threshold(const gcoo_matrix *h)
+{
+    const int w = h->spdm_cfg.waves;
+    if (w == 1) return 0;
+    if (w >= 2) return 1 << w;
+    if (h->M <= 0 || h->nnz <= 0) return 0;
+    const double mean = (double)h->nnz / h->M;
+    if (h->maxRow < 512 || h->maxRow < 16.0 * mean) return 0;
+    int t = 256;
+    while (t < 16.0 * mean && t < (1 << 30)) t <<= 1;
+    return t;
+}
+
 template <typename E, int VEC, int U>
 int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                    const ConvStatus *guard, hipStream_t s, int skipLen)
+                    const ConvStatus *guard, hipStream_t s, int skipLen, const ExtScratch *xs)
 {
     // A wave walks `rpw` consecutive rows: one row, or the group's cpg rows
     // (the segments are per row, so any split gives the same bits).  One row
@@ -1093,13 +1145,56 @@ int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N,
     const int nGroupBlocks = (nUnits + kSpdmWaves - 1) / kSpdmWaves;
     const int tileCols = 64 * VEC;
     const int nColTiles = (N + tileCols - 1) / tileCols;
-    const long long nblocks = (long long)nGroupBlocks * nColTiles;
+    // long rows first: listed on the device (rows with >= lf entries), in
+    // the handle's long-row scratch or, for the no-allocation entry point,
+    // in xs (too little of it: the natural order, same bits)
+    const int lf = skipLen == INT_MAX ? long_first_threshold(h) : 0;
+    int longBlocks = 0;
+    int *longList = nullptr;
+    unsigned *longCtr = nullptr;
+    if (lf > 0) {
+        const size_t listB = align256(sizeof(int) * ((size_t)h->M + 1));
+        if (xs) {
+            if (listB + 256 <= xs->bytes) {
+                longList = reinterpret_cast<int *>(xs->base);
+                longCtr = reinterpret_cast<unsigned *>(xs->base + listB);
+            }
+        } else {
+            if (!h->longList) {
+                if (hipMalloc(&h->longList, sizeof(int) * ((size_t)h->M + 1)) != hipSuccess ||
+                    hipMalloc(&h->longCtr, 256) != hipSuccess) {
+                    (void)hipGetLastError();
+                    if (h->longList) (void)hipFree(h->longList);
+                    h->longList = nullptr;
+                    return set_error(GCOO_ERR_ALLOC, "long-row scratch");
+                }
+            }
+            longList = h->longList;
+            longCtr = h->longCtr;
+        }
+    }
+    if (longList) {
+        const long long maxLong = std::min<long long>(h->M, h->nnz / lf);
+        longBlocks = (int)((maxLong + kSpdmWaves - 1) / kSpdmWaves);
+        hipLaunchKernelGGL(longrow_reset_kernel, dim3(1), dim3(64), 0, s, longCtr);
+        GCOO_CHECK_LAUNCH("longrow_reset_kernel");
+        hipLaunchKernelGGL(longrow_classify_kernel, dim3((h->M + 255) / 256), dim3(256), 0, s,
+                           h->rowptr, h->M, lf, longList, longCtr, guard, (long long)h->cap);
+        GCOO_CHECK_LAUNCH("longrow_classify_kernel");
+        skipLen = lf;
+    }
+    const int xpt = (longBlocks > 0 && nColTiles <= kNumXcd && kNumXcd % nColTiles == 0)
+                        ? kNumXcd / nColTiles : 0;
+    const long long nblocks =
+        xpt ? (long long)kNumXcd * ((longBlocks + xpt - 1) / xpt + (nGroupBlocks + xpt - 1) / xpt)
+            : ((long long)longBlocks + nGroupBlocks) * nColTiles;
     if (nblocks > INT32_MAX)
         return set_error(GCOO_ERR_OVERFLOW, "grid of %lld workgroups too large", nblocks);
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
     hipLaunchKernelGGL((spdm_rowseg_kernel<E, VEC, U>), dim3((unsigned)nblocks),
                        dim3(kSpdmThreads), 0, s, h->rowptr, seg, nUnits, rpw, h->M, D,
-                       (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap, skipLen);
+                       (unsigned)dBytes, N, C, nGroupBlocks, guard, (long long)h->cap, skipLen,
+                       (const int *)longList, (const unsigned *)longCtr, longBlocks, xpt);
     GCOO_CHECK_LAUNCH("spdm_rowseg_kernel");
     return GCOO_OK;
 }
@@ -1110,13 +1205,14 @@ int launch_rowseg_u(const gcoo_matrix *h, const int2 *seg, const void *D, int N,
 // within +-5 us of 8 at n = 8000 for s = 0.99 .. 0.999.
 template <typename E, int VEC>
 int launch_rowseg(const gcoo_matrix *h, const int2 *seg, const void *D, int N, float *C,
-                  const ConvStatus *guard, hipStream_t s, int skipLen = INT_MAX)
+                  const ConvStatus *guard, hipStream_t s, int skipLen = INT_MAX,
+                  const ExtScratch *xs = nullptr)
 {
     const int u = h->spdm_cfg.unroll ? h->spdm_cfg.unroll : 16;
-    if (u <= 8) return launch_rowseg_u<E, VEC, 8>(h, seg, D, N, C, guard, s, skipLen);
+    if (u <= 8) return launch_rowseg_u<E, VEC, 8>(h, seg, D, N, C, guard, s, skipLen, xs);
     if (u <= 16 || VEC * E::bytes > 8)
-        return launch_rowseg_u<E, VEC, 16>(h, seg, D, N, C, guard, s, skipLen);
-    return launch_rowseg_u<E, VEC, 32>(h, seg, D, N, C, guard, s, skipLen);
+        return launch_rowseg_u<E, VEC, 16>(h, seg, D, N, C, guard, s, skipLen, xs);
+    return launch_rowseg_u<E, VEC, 32>(h, seg, D, N, C, guard, s, skipLen, xs);
 }
 
 // Rows with at least minLen entries through the long-row kernel (listed on
@@ -1364,14 +1460,14 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         }
     }
     if constexpr (E::bytes == 4) {
-        if (vec >= 4) return launch_rowseg<E, 4>(h, seg, D, N, C, guard, s);
-        if (vec == 2) return launch_rowseg<E, 2>(h, seg, D, N, C, guard, s);
-        return launch_rowseg<E, 1>(h, seg, D, N, C, guard, s);
+        if (vec >= 4) return launch_rowseg<E, 4>(h, seg, D, N, C, guard, s, INT_MAX, xs);
+        if (vec == 2) return launch_rowseg<E, 2>(h, seg, D, N, C, guard, s, INT_MAX, xs);
+        return launch_rowseg<E, 1>(h, seg, D, N, C, guard, s, INT_MAX, xs);
     } else {
-        if (vec >= 8) return launch_rowseg<E, 8>(h, seg, D, N, C, guard, s);
-        if (vec == 4) return launch_rowseg<E, 4>(h, seg, D, N, C, guard, s);
-        if (vec == 2) return launch_rowseg<E, 2>(h, seg, D, N, C, guard, s);
-        return launch_rowseg<E, 1>(h, seg, D, N, C, guard, s);
+        if (vec >= 8) return launch_rowseg<E, 8>(h, seg, D, N, C, guard, s, INT_MAX, xs);
+        if (vec == 4) return launch_rowseg<E, 4>(h, seg, D, N, C, guard, s, INT_MAX, xs);
+        if (vec == 2) return launch_rowseg<E, 2>(h, seg, D, N, C, guard, s, INT_MAX, xs);
+        return launch_rowseg<E, 1>(h, seg, D, N, C, guard, s, INT_MAX, xs);
     }
 }
 

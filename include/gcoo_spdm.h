@@ -191,7 +191,12 @@ int gcoo_spdm_bf16(gcoo_handle h, const uint16_t *D, int K, int N, float *C,
  *              D per lane; a wave covers 64*vec columns.
  *   waves      waves per workgroup (1..16) of variant 3; for variant 6 16,
  *              12 or (FP32) 8 (0: automatic, 16 x 16 or 16 x 21 by the grid's
- *              rounds of one workgroup per CU).
+ *              rounds of one workgroup per CU); for variant 2 the long-rows-
+ *              first order: rows of 2^waves entries or more (waves 2..16)
+ *              are listed on the device and walked by the first workgroups
+ *              of each column tile, so that the longest serial chains start
+ *              first (1: natural row order; 0: automatic, on when the
+ *              longest row holds 512 entries and 16 times the mean).
  *   chunk      k-chunk length of variant 3; long-row threshold of variant 4;
  *              rows per wave of variant 2 (1: one row, another nonzero value:
  *              the group's cpg rows, 0: one row once rows average 12 entries);

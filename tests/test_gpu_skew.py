@@ -101,6 +101,41 @@ def test_long_row_kernel_shapes(dev, bf16, N):
     assert H.bits_equal(C.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+@pytest.mark.parametrize("N", [64, 256, 520])
+def test_long_rows_first_order(dev, bf16, N):
+    """Variant 2 with its longest rows walked first (rows with >= 2^waves
+    entries listed on the device and taken by the first workgroups of each
+    column tile), against the natural order (waves = 1) and the oracle:
+    thresholds that list most rows, a few, or none; one row per wave and the
+    group's rows per wave; column tiles that divide the 8 XCDs (the
+    interleaved placement) and that do not."""
+    import torch
+    M, K = 301, 900
+    rng = np.random.default_rng(N + (5 if bf16 else 0))
+    lens = np.minimum(rng.zipf(1.6, M), 600)
+    lens[::11] = 0
+    S = np.zeros((M, K), np.float32)
+    for r in range(M):
+        ks = rng.choice(K, size=lens[r], replace=False)
+        S[r, ks] = rng.uniform(-1, 1, lens[r])
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    g = oracle.convert(S, 4)
+    m = G.GCOOMatrix(M, K, 4).convert_dense(to_dev(S, dev))
+    Dd = G.to_bf16(to_dev(D, dev)) if bf16 else to_dev(D, dev)
+    ref = (oracle.spdm_bf16(g, oracle.to_bf16(g["vals"]), oracle.to_bf16(D)) if bf16
+           else oracle.spdm_f32(g, D))
+    for vec in ((1, 2, 4, 8) if bf16 else (1, 2, 4)):
+        for waves in (1, 2, 4, 6, 9, 0):
+            for chunk in (1, 2):
+                print(f"long-first bf16={bf16} N={N} vec={vec} waves={waves} chunk={chunk}",
+                      flush=True)
+                m.set_config(variant=2, vec=vec, waves=waves, chunk=chunk)
+                C = torch.full((M, N), float("nan"), device=dev)
+                (m.spdm_bf16 if bf16 else m.spdm)(Dd, C)
+                assert H.bits_equal(C.cpu().numpy(), ref), (vec, waves, chunk)
+
+
 def test_multiply_by_with_long_rows(dev):
     """mutiplyBy (conversion + multiply enqueued together) on a skewed dense
     matrix, twice (the second call reuses capacity)."""
