@@ -147,8 +147,9 @@ int gcoo_spdm_f32(gcoo_handle h, const float *D, int K, int N, float *C,
  * long-row list and counter -- lives in one 16 MiB device array per GPU,
  * held by the library until the process ends.  A multiply whose scratch
  * would exceed 16 MiB (panel lists take 8 bytes per entry, so about 2 M
- * entries; variant 2 needs none) returns GCOO_ERR_BUDGET and enqueues
- * nothing.  All calls on one GPU share that array: calls that overlap on a
+ * entries) returns GCOO_ERR_BUDGET and enqueues nothing.  Variant 2 needs
+ * only the long-row list when its long rows go first (4 bytes per row);
+ * without room for it, it keeps the natural row order (the same bits).  All calls on one GPU share that array: calls that overlap on a
  * GPU (other streams, other threads) race, so the caller serialises them. */
 int gcoo_spdm_f32_noalloc_async(gcoo_handle h, const float *D, int K, int N, float *C,
                                 gcoo_stream s);

@@ -1,9 +1,10 @@
 """The no-allocation multiply (gcoo_spdm_f32_noalloc_async) through the C
 ABI against the existing entry point (gcoo_spdm_f32): bit-equal C on two
 calls in a row, the second on a smaller input, for the panel-staged kernel
-(panel lists in the library's static scratch) and the long-row split (its
-row list and counter there); a multiply whose lists exceed the scratch is
-refused before anything is enqueued."""
+(panel lists in the library's static scratch), the long-row split and
+variant 2's long-rows-first order (their row list and counter there); a
+multiply whose lists exceed the scratch is refused before anything is
+enqueued."""
 import numpy as np
 import pytest
 
@@ -56,6 +57,33 @@ def test_noalloc_matches_plain_two_calls(dev, variant):
             assert H.bits_equal(got.cpu().numpy(),
                                 oracle.spdm_f32(g, D.cpu().numpy(), threads=oracle.max_threads()))
         del S, D, m, want, got
+
+
+def test_noalloc_long_rows_first(dev):
+    """Variant 2 with its long rows first: the row list and its counter in
+    the static scratch, two calls in a row (the second smaller), bit-equal
+    to the existing entry point and to the oracle."""
+    import torch
+    for call, (M, K, N) in enumerate([(4096, 3000, 256), (1500, 2000, 128)]):
+        print(f"noalloc long-first call={call} M={M} K={K} N={N}", flush=True)
+        rng = np.random.default_rng(40 + call)
+        lens = np.minimum(rng.zipf(1.5, M), K // 2)
+        S = np.zeros((M, K), np.float32)
+        for r in range(M):
+            S[r, rng.choice(K, size=lens[r], replace=False)] = rng.uniform(-1, 1, lens[r])
+        D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+        Sd = torch.from_numpy(S).to(dev)
+        Dd = torch.from_numpy(D).to(dev)
+        m = G.GCOOMatrix(M, K, 4).convert_dense(Sd)
+        m.set_config(variant=2, vec=1, waves=6, chunk=1)
+        want = torch.empty((M, N), device=dev)
+        m.spdm(Dd, want)
+        got = torch.full((M, N), float("nan"), device=dev)
+        _noalloc(m, Dd, got)
+        assert torch.equal(got.view(torch.int32), want.view(torch.int32)), call
+        g = oracle.convert(S, 4)
+        assert H.bits_equal(got.cpu().numpy(), oracle.spdm_f32(g, D, threads=oracle.max_threads()))
+        del Sd, Dd, m, want, got
 
 
 def test_noalloc_rejects_oversized_lists(dev):
