@@ -92,6 +92,9 @@ _SIGNATURES = [
     ("gcoo_batch_graph_launch", _I, [_P, _P]),
     ("gcoo_batch_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I64), _c.POINTER(_I)]),
     ("gcoo_batch_job_tile", _I, [_P, _I, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I)]),
+    ("gcoo_batch_set_panel_mode", _I, [_P, _I, _c.c_double, _c.c_double]),
+    ("gcoo_batch_panel_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I)]),
+    ("gcoo_batch_job_is_panel", _I, [_P, _I, _c.POINTER(_I)]),
     ("gcoo_server_create", _I, [_P, _PP]),
     ("gcoo_server_destroy", _I, [_P]),
     ("gcoo_server_register", _I, [_P, _P, _P, _I, _I, _I, _I, _c.POINTER(_I)]),
@@ -386,6 +389,28 @@ class Batch:
         _check(lib().gcoo_batch_info(self._b, ctypes.byref(j), ctypes.byref(it),
                                      ctypes.byref(g)), "gcoo_batch_info")
         return dict(jobs=j.value, items=it.value, grid_blocks=g.value)
+
+    def set_panel_mode(self, mode: int, ratio_f32: float = 0.0, ratio_bf16: float = 0.0):
+        """0: no panel jobs, 1: the cost model's choice (default; a job is a
+        panel job when its panel estimate is below ratio x its persistent
+        estimate, per element type, 0 = the built-in ratio), 2: every job
+        whose operands allow the panel kernel.  Takes effect at commit."""
+        _check(lib().gcoo_batch_set_panel_mode(self._b, mode, float(ratio_f32),
+                                               float(ratio_bf16)),
+               "gcoo_batch_set_panel_mode")
+        return self
+
+    def panel_info(self):
+        pj, wg = ctypes.c_int(), ctypes.c_int()
+        _check(lib().gcoo_batch_panel_info(self._b, ctypes.byref(pj), ctypes.byref(wg)),
+               "gcoo_batch_panel_info")
+        return dict(panel_jobs=pj.value, workgroups=wg.value)
+
+    def job_is_panel(self, job) -> bool:
+        p = ctypes.c_int()
+        _check(lib().gcoo_batch_job_is_panel(self._b, job, ctypes.byref(p)),
+               "gcoo_batch_job_is_panel")
+        return bool(p.value)
 
     def job_tile(self, job):
         r, v, ln = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -3,8 +3,10 @@
 512 independent jobs (M, K log-uniform in [256, 8192], N in {32..256},
 sparsity in [0.95, 0.999], see workloads.batch_jobs), FP32 or BF16:
   sequential  one plain multiply launch per job, back to back on one stream;
-  batched     one HIP-graph replay of the persistent batch kernel with the
-              autotuned tiles.
+  batched     one HIP-graph replay of the batch with the autotuned tiles:
+              the panel-staged launch of the jobs the batch's cost model
+              gives it, then the persistent kernel for the others
+              (panel_mode 0: every job on the persistent kernel).
 Both are timed with HIP events on the stream they run on; GFLOP/s counts
 2 * nnz * N per job.  Used by bench.py; also runnable alone:
     python -m gcoospdm_amd.batchbench [--jobs 512] [--bf16]
@@ -16,7 +18,7 @@ import json
 import time
 
 
-def run(count=512, bf16=False, reps=10, seed=2024, table=None):
+def run(count=512, bf16=False, reps=10, seed=2024, table=None, panel_mode=1, panel_ratio=0.0):
     import torch
     import gcoospdm_amd as G
     from . import autotune, workloads as W
@@ -62,7 +64,7 @@ def run(count=512, bf16=False, reps=10, seed=2024, table=None):
     t_seq = timed(seq)
     ref = [C.clone() for _, _, C in jobs]
 
-    b = G.Batch()
+    b = G.Batch().set_panel_mode(panel_mode, panel_ratio, panel_ratio)
     for (m, D, C), sp in zip(jobs, specs):
         rows, vec = autotune.batch_tile(sp.N, sp.K * (1 - sp.sparsity), bf16, table)
         b.add(m, D, C, rows=rows, vec=vec)
@@ -72,6 +74,8 @@ def run(count=512, bf16=False, reps=10, seed=2024, table=None):
     same = all(torch.equal(C.view(torch.int32), r.view(torch.int32))
                for (_, _, C), r in zip(jobs, ref))
     info = b.info()
+    pinfo = b.panel_info()
+    pflops = sum(2.0 * z * sp.N for i, (z, sp) in enumerate(zip(nnz, specs)) if b.job_is_panel(i))
     return {
         "workload": f"batch_{count}_jobs", "dtype": "bf16" if bf16 else "f32",
         "jobs": count, "nnz": int(sum(nnz)), "gflop": round(flops / 1e9, 3),
@@ -82,6 +86,9 @@ def run(count=512, bf16=False, reps=10, seed=2024, table=None):
         "speedup": round(t_seq / t_batch, 3),
         "batched_hbm_frac": round(algo_bytes / t_batch / 8e12, 4),
         "work_items": info["items"], "grid_blocks": info["grid_blocks"],
+        "panel_mode": panel_mode, "panel_ratio": panel_ratio, "panel_jobs": pinfo["panel_jobs"],
+        "panel_workgroups": pinfo["workgroups"],
+        "panel_flop_share": round(pflops / flops, 3) if flops else 0.0,
         "bit_identical_to_alone": bool(same),
     }
 
@@ -91,10 +98,15 @@ def main():
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--bf16", action="store_true")
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--panel-mode", type=int, default=1,
+                    help="0: no panel jobs, 1: the cost model's (default), 2: all eligible")
+    ap.add_argument("--panel-ratio", type=float, default=0.0,
+                    help="mode 1: panel when its estimate is below ratio x the other (0: 1.0)")
     ap.add_argument("--calibrate", action="store_true",
                     help="also copy a 256 MiB tensor 3 times (known bytes for counter runs)")
     args = ap.parse_args()
-    print(json.dumps(run(args.jobs, args.bf16, args.reps)), flush=True)
+    print(json.dumps(run(args.jobs, args.bf16, args.reps, panel_mode=args.panel_mode,
+                         panel_ratio=args.panel_ratio)), flush=True)
     if args.calibrate:
         import torch
         X = torch.empty(64 << 20, device="cuda:0")   # 256 MiB

@@ -10,8 +10,15 @@
 // the same fmaf chain (row r's entries, k ascending) as in the plain kernels,
 // so a job's C is bit-identical whether it runs alone or in any batch.
 //
-// The run is one counter-reset kernel followed by the persistent kernel; it
-// is graph-capturable (no memset, no host synchronisation, no allocation).
+// Jobs dense enough for the panel-staged multiply (gcoo_spdm.hip, variant 6)
+// run it instead, all of them in one launch per element type ahead of the
+// persistent kernel: a cost model (below) picks them at commit, their panel
+// lists are built then, and a table gives each workgroup of the launch one
+// (job, panel, column tile) unit, whole jobs per XCD with balanced work.
+//
+// The run is one counter-reset kernel, the panel launches and the persistent
+// kernel; it is graph-capturable (no memset, no host synchronisation, no
+// allocation).
 #include "gcoo_internal.h"
 
 #include <type_traits>
@@ -235,6 +242,13 @@ __device__ __forceinline__ void run_item(const JobDesc &jd, __amdgpu_buffer_rsrc
 // One item queue per XCD: queue x holds the items [start[x], start[x + 1]),
 // a contiguous run of whole jobs (mostly) chosen so that the queues carry
 // about equal work, and its counter sits on a cache line of its own.
+// Default ratios of the panel-job choice (see plan_panels): on the 512-job
+// workload FP32 was fastest at 1.0 (1.71 ms; 0.5: 1.96, 2.0: 1.80) and BF16
+// with nearly every job on the panel kernel (the BF16 persistent kernel
+// widens every gathered value: 2.00 ms at 2.0 against 2.07 with none).
+constexpr double kPanelRatioF32 = 1.0;
+constexpr double kPanelRatioBf16 = 8.0;
+
 struct BatchQueues {
     int start[kNumXcd + 1];
 };
@@ -312,7 +326,8 @@ using namespace gcoo;
 
 struct gcoo_batch {
     std::vector<JobDesc> jobs;
-    std::vector<int> itemStart;  // nJobs + 1
+    std::vector<int> itemStart;  // nJobs + 1 (panel jobs have none once committed)
+    std::vector<int> jobItems;   // work items of each job on the persistent kernel
     std::vector<gcoo_handle> handles;
     // each handle's conversion generation when the job table was committed:
     // a run refuses to start once one moved (the table would point at
@@ -331,6 +346,19 @@ struct gcoo_batch {
     int gridBlocks = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+
+    // panel jobs (chosen at commit): 1 = by the cost model, 0 = none, 2 =
+    // every job whose operands allow it
+    int panelMode = 1;
+    double panelRatio[2] = {kPanelRatioF32, kPanelRatioBf16};  // FP32, BF16 jobs
+    std::vector<int> panelOf;          // per job: index into pjobs, or -1
+    std::vector<uint64_t> plistGens;   // per panel job: its handle's list generation
+    std::vector<PanelJob> pjobs;
+    std::vector<PanelUnit> units;      // FP32 launch's units, then BF16's
+    int nUnitsF32 = 0, nUnitsBf16 = 0;
+    PanelJob *d_pjobs = nullptr, *h_pjobs = nullptr;
+    PanelUnit *d_units = nullptr, *h_units = nullptr;
+    size_t capPJobs = 0, capUnits = 0;
 };
 
 namespace {
@@ -361,18 +389,177 @@ int check_generations(const gcoo_batch *b)
             return set_error(GCOO_ERR_STATE,
                              "job %zu: its matrix was converted again after the batch was "
                              "committed; commit (and capture) the batch again", i);
+        const int pj = i < b->panelOf.size() ? b->panelOf[i] : -1;
+        if (pj >= 0 && (h->plist_gen != b->plistGens[pj] || h->plist != b->pjobs[pj].plist ||
+                        h->poff != b->pjobs[pj].poff))
+            return set_error(GCOO_ERR_STATE,
+                             "job %zu: its matrix's panel lists were rebuilt (another multiply "
+                             "of it used another panel shape) after the batch was committed; "
+                             "commit (and capture) the batch again", i);
     }
     return GCOO_OK;
 }
 
+bool has_work(const gcoo_batch *b)
+{
+    return !b->jobs.empty() && (b->itemStart.back() > 0 || b->nUnitsF32 + b->nUnitsBf16 > 0);
+}
+
+// The panel launches, then the persistent kernel, on s.  (Run beside each
+// other on two streams, the persistent kernel's workgroups held their CUs
+// until its queues drained and the panel units, which need a whole CU each,
+// waited: the panel launch stretched from 0.8 to 1.8 ms and the batch took
+// 1.85 ms either way.)
 int enqueue_run(gcoo_batch *b, hipStream_t s)
 {
-    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
-    GCOO_CHECK_LAUNCH("batch_reset_kernel");
-    const int nJobs = (int)b->jobs.size();
-    hipLaunchKernelGGL(batch_persistent_kernel, dim3(b->gridBlocks), dim3(kBatchThreads), 0, s,
-                       b->d_jobs, b->d_itemStart, nJobs, b->queues, b->d_counter);
-    GCOO_CHECK_LAUNCH("batch_persistent_kernel");
+    const bool items = b->itemStart.back() > 0;
+    if (int rc = launch_panel_batch(false, b->d_units, b->nUnitsF32, b->d_pjobs, s)) return rc;
+    if (int rc = launch_panel_batch(true, b->d_units + b->nUnitsF32, b->nUnitsBf16, b->d_pjobs, s))
+        return rc;
+    if (items) {
+        hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
+        GCOO_CHECK_LAUNCH("batch_reset_kernel");
+        const int nJobs = (int)b->jobs.size();
+        hipLaunchKernelGGL(batch_persistent_kernel, dim3(b->gridBlocks), dim3(kBatchThreads), 0,
+                           s, b->d_jobs, b->d_itemStart, nJobs, b->queues, b->d_counter);
+        GCOO_CHECK_LAUNCH("batch_persistent_kernel");
+    }
+    return GCOO_OK;
+}
+
+// Cost model of the choice, in GPU-seconds.  Panel kernel: one workgroup
+// per CU; a unit takes about a ns per entry of its panel, b ns per D row
+// staged and c us of fixed costs (C tile store, first chunk), fitted on
+// single-job launches of the 512-job workload (FP32: 3.1 ns, 6.85 ns,
+// 10.3 us; BF16: 4.2 ns, 2.84 ns, 12.4 us), times 1.3 for running beside
+// the other jobs, over the CUs.  Persistent kernel: bound by the D gathers
+// into the CUs (one element per FMA), about 8.4 TFLOP/s over the whole GPU
+// on that workload, FP32 and BF16 alike.  A job is a panel job when its
+// panel estimate is below `panelRatio` times its persistent-kernel estimate.
+constexpr double kPanelBatchFactor = 1.3;
+constexpr double kGatherFlops = 8.4e12;
+
+double panel_unit_seconds(const gcoo_matrix *h, bool bf16)
+{
+    const double panelRows = std::min(h->M, kBatchPanelWaves * kBatchPanelRpw);
+    const double entries = h->M > 0 ? (double)h->nnz * panelRows / h->M : 0.0;
+    return bf16 ? 4.2e-9 * entries + 2.84e-9 * h->K + 12.4e-6
+                : 3.1e-9 * entries + 6.85e-9 * h->K + 10.3e-6;
+}
+
+long long panel_units(const gcoo_matrix *h, int N)
+{
+    const int pr = kBatchPanelWaves * kBatchPanelRpw;
+    return (long long)((h->M + pr - 1) / pr) * ((N + 255) / 256);
+}
+
+double gather_seconds(const JobDesc &jd, const gcoo_matrix *h)
+{
+    return 2.0 * (double)h->nnz * jd.N / kGatherFlops;
+}
+
+// Choose the panel jobs, build their lists (the handles keep them) and lay
+// out the units of the two launches.
+int plan_panels(gcoo_batch *b, hipStream_t s)
+{
+    const size_t n = b->jobs.size();
+    b->panelOf.assign(n, -1);
+    b->plistGens.clear();
+    b->pjobs.clear();
+    b->units.clear();
+    b->nUnitsF32 = b->nUnitsBf16 = 0;
+    if (b->panelMode == 0) return GCOO_OK;
+    // the segments whose lists a handle holds for this batch: a second job of
+    // the same matrix in the other element type would rebuild them, so it
+    // takes the gather kernel
+    std::vector<std::pair<const gcoo_matrix *, const int2 *>> claimed;
+    std::vector<double> unitSec;
+    std::vector<bool> pjBf16;
+    for (size_t i = 0; i < n; ++i) {
+        const JobDesc &jd = b->jobs[i];
+        const gcoo_matrix *h = b->handles[i];
+        if (jd.N <= 0 || h->M <= 0 || h->K <= 0 || h->nnz <= 0) continue;
+        if (!panel_operands_ok(jd.D, jd.N, jd.C, jd.bytes)) continue;
+        const bool bf16 = jd.bytes == 2;
+        const long long units = panel_units(h, jd.N);
+        const double us = panel_unit_seconds(h, bf16);
+        if (b->panelMode == 1 &&
+            !(kPanelBatchFactor * (double)units * us / b->cus <
+              b->panelRatio[bf16 ? 1 : 0] * gather_seconds(jd, h)))
+            continue;
+        bool clash = false;
+        for (const auto &c : claimed)
+            if (c.first == h && c.second != jd.seg) clash = true;
+        if (clash) continue;
+        int rc = ensure_panel_lists(h, jd.seg, batch_panel_tk(bf16), batch_panel_rowb(bf16),
+                                    kBatchPanelRpw, kBatchPanelWaves, bf16, nullptr, s);
+        if (rc) return rc;
+        claimed.emplace_back(h, jd.seg);
+        PanelJob pj{};
+        pj.poff = h->poff;
+        pj.plist = h->plist;
+        pj.D = jd.D;
+        pj.C = jd.C;
+        pj.M = h->M;
+        pj.K = h->K;
+        pj.N = jd.N;
+        pj.nC = (h->K + batch_panel_tk(bf16) - 1) / batch_panel_tk(bf16);
+        pj.nPanels = (h->M + kBatchPanelWaves * kBatchPanelRpw - 1) /
+                     (kBatchPanelWaves * kBatchPanelRpw);
+        pj.dBytes = jd.dBytes;
+        b->panelOf[i] = (int)b->pjobs.size();
+        b->pjobs.push_back(pj);
+        b->plistGens.push_back(h->plist_gen);
+        unitSec.push_back(us);
+        pjBf16.push_back(bf16);
+    }
+    // per element type: the units of the jobs with the longest units first,
+    // each to the XCD with the least estimated work so far (so a large job
+    // spreads over the XCDs and no XCD idles while another runs a long job);
+    // on an XCD a job's units of one column tile follow each other and take
+    // turns at the L2 touch of the D tile they stage together.  Workgroup b
+    // runs on XCD b % 8, so XCD x's k-th unit goes to workgroup 8 k + x; XCDs
+    // with fewer units get empty ones.
+    for (int bf = 0; bf < 2; ++bf) {
+        std::vector<int> js;
+        for (size_t p = 0; p < b->pjobs.size(); ++p)
+            if (pjBf16[p] == (bf == 1)) js.push_back((int)p);
+        if (js.empty()) continue;
+        std::stable_sort(js.begin(), js.end(), [&](int x, int y) { return unitSec[x] > unitSec[y]; });
+        std::vector<std::vector<PanelUnit>> lists(kNumXcd);
+        double load[kNumXcd] = {};
+        for (int p : js) {
+            const int tiles = (b->pjobs[p].N + 255) / 256;
+            const int np = b->pjobs[p].nPanels;
+            for (int ct = 0; ct < tiles; ++ct) {
+                size_t first[kNumXcd];
+                for (int x = 0; x < kNumXcd; ++x) first[x] = lists[x].size();
+                for (int pn = 0; pn < np; ++pn) {
+                    int x = 0;
+                    for (int y = 1; y < kNumXcd; ++y)
+                        if (load[y] < load[x]) x = y;
+                    lists[x].push_back(PanelUnit{p, ct * np + pn, 1, 0});
+                    load[x] += unitSec[p];
+                }
+                for (int x = 0; x < kNumXcd; ++x) {
+                    const int cnt = (int)(lists[x].size() - first[x]);
+                    for (int k = 0; k < cnt; ++k) {
+                        lists[x][first[x] + k].tmod = cnt;
+                        lists[x][first[x] + k].tidx = k;
+                    }
+                }
+            }
+        }
+        size_t most = 0;
+        for (int x = 0; x < kNumXcd; ++x) most = std::max(most, lists[x].size());
+        if (most * kNumXcd > (size_t)INT32_MAX)
+            return set_error(GCOO_ERR_OVERFLOW, "too many panel workgroups");
+        const size_t first = b->units.size();
+        b->units.resize(first + most * kNumXcd, PanelUnit{-1, 0, 1, 0});
+        for (int x = 0; x < kNumXcd; ++x)
+            for (size_t k = 0; k < lists[x].size(); ++k) b->units[first + k * kNumXcd + x] = lists[x][k];
+        (bf ? b->nUnitsBf16 : b->nUnitsF32) = (int)(most * kNumXcd);
+    }
     return GCOO_OK;
 }
 
@@ -417,6 +604,10 @@ int gcoo_batch_destroy(gcoo_batch_t b)
     if (b->d_counter) (void)hipFree(b->d_counter);
     if (b->h_jobs) (void)hipHostFree(b->h_jobs);
     if (b->h_itemStart) (void)hipHostFree(b->h_itemStart);
+    if (b->d_pjobs) (void)hipFree(b->d_pjobs);
+    if (b->h_pjobs) (void)hipHostFree(b->h_pjobs);
+    if (b->d_units) (void)hipFree(b->d_units);
+    if (b->h_units) (void)hipHostFree(b->h_units);
     delete b;
     return GCOO_OK;
 }
@@ -429,6 +620,12 @@ int gcoo_batch_clear(gcoo_batch_t b)
     b->handles.clear();
     b->gens.clear();
     b->itemStart.assign(1, 0);
+    b->jobItems.clear();
+    b->panelOf.clear();
+    b->plistGens.clear();
+    b->pjobs.clear();
+    b->units.clear();
+    b->nUnitsF32 = b->nUnitsBf16 = 0;
     b->committed = false;
     return GCOO_OK;
 }
@@ -475,6 +672,7 @@ int gcoo_batch_add(gcoo_batch_t b, gcoo_handle h, const void *D, int dtype, int 
     b->jobs.push_back(jd);
     b->handles.push_back(h);
     b->itemStart.push_back((int)(start + items));
+    b->jobItems.push_back((int)items);
     b->committed = false;
     destroy_graph(b);
     if (job_index) *job_index = (int)b->jobs.size() - 1;
@@ -516,6 +714,39 @@ int gcoo::batch_commit_async(gcoo_batch_t b, gcoo_stream stream)
         b->gens[i] = h->generation;
     }
     const size_t n = b->jobs.size();
+    // the panel jobs leave the persistent kernel's items
+    if (int rc = plan_panels(b, s)) return rc;
+    b->itemStart.assign(1, 0);
+    for (size_t i = 0; i < n; ++i)
+        b->itemStart.push_back(b->itemStart.back() + (b->panelOf[i] >= 0 ? 0 : b->jobItems[i]));
+    const size_t np = b->pjobs.size(), nu = b->units.size();
+    if (np > b->capPJobs || nu > b->capUnits) {
+        if (b->d_pjobs) GCOO_HIP_TRY(hipFree(b->d_pjobs));
+        if (b->h_pjobs) GCOO_HIP_TRY(hipHostFree(b->h_pjobs));
+        if (b->d_units) GCOO_HIP_TRY(hipFree(b->d_units));
+        if (b->h_units) GCOO_HIP_TRY(hipHostFree(b->h_units));
+        b->d_pjobs = b->h_pjobs = nullptr;
+        b->d_units = b->h_units = nullptr;
+        b->capPJobs = b->capUnits = 0;
+        const size_t cj = std::max<size_t>(np, 16), cu = std::max<size_t>(nu, 256);
+        if (hipMalloc(&b->d_pjobs, sizeof(PanelJob) * cj) != hipSuccess ||
+            hipHostMalloc(&b->h_pjobs, sizeof(PanelJob) * cj, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&b->d_units, sizeof(PanelUnit) * cu) != hipSuccess ||
+            hipHostMalloc(&b->h_units, sizeof(PanelUnit) * cu, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return set_error(GCOO_ERR_ALLOC, "batch panel tables");
+        }
+        b->capPJobs = cj;
+        b->capUnits = cu;
+    }
+    if (np > 0) {
+        std::copy(b->pjobs.begin(), b->pjobs.end(), b->h_pjobs);
+        std::copy(b->units.begin(), b->units.end(), b->h_units);
+        GCOO_HIP_TRY(hipMemcpyAsync(b->d_pjobs, b->h_pjobs, sizeof(PanelJob) * np,
+                                    hipMemcpyHostToDevice, s));
+        GCOO_HIP_TRY(hipMemcpyAsync(b->d_units, b->h_units, sizeof(PanelUnit) * nu,
+                                    hipMemcpyHostToDevice, s));
+    }
     if (n > b->capJobs) {
         if (b->d_jobs) GCOO_HIP_TRY(hipFree(b->d_jobs));
         if (b->d_itemStart) GCOO_HIP_TRY(hipFree(b->d_itemStart));
@@ -595,7 +826,7 @@ int gcoo_batch_run_async(gcoo_batch_t b, gcoo_stream stream)
     if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
     if (!b->committed) return set_error(GCOO_ERR_STATE, "batch not committed");
     if (int rc = check_generations(b)) return rc;
-    if (b->jobs.empty() || b->itemStart.back() == 0) return GCOO_OK;
+    if (!has_work(b)) return GCOO_OK;
     return enqueue_run(b, reinterpret_cast<hipStream_t>(stream));
 }
 
@@ -609,7 +840,7 @@ int gcoo_batch_graph_capture(gcoo_batch_t b, gcoo_stream stream)
     destroy_graph(b);
     GCOO_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int rc = GCOO_OK;
-    if (!b->jobs.empty() && b->itemStart.back() > 0) rc = enqueue_run(b, s);
+    if (has_work(b)) rc = enqueue_run(b, s);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(s, &g);
     if (rc) {
@@ -641,6 +872,36 @@ int gcoo_batch_info(gcoo_batch_t b, int *jobs, int64_t *items, int *grid_blocks)
     if (jobs) *jobs = (int)b->jobs.size();
     if (items) *items = b->itemStart.back();
     if (grid_blocks) *grid_blocks = b->gridBlocks;
+    return GCOO_OK;
+}
+
+int gcoo_batch_set_panel_mode(gcoo_batch_t b, int mode, double ratio_f32, double ratio_bf16)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    if (mode < 0 || mode > 2) return set_error(GCOO_ERR_INVALID, "panel mode %d (0, 1 or 2)", mode);
+    if (!(ratio_f32 >= 0.0 && ratio_f32 <= 1e6 && ratio_bf16 >= 0.0 && ratio_bf16 <= 1e6))
+        return set_error(GCOO_ERR_INVALID, "panel ratios %g, %g", ratio_f32, ratio_bf16);
+    b->panelMode = mode;
+    b->panelRatio[0] = ratio_f32 > 0.0 ? ratio_f32 : kPanelRatioF32;
+    b->panelRatio[1] = ratio_bf16 > 0.0 ? ratio_bf16 : kPanelRatioBf16;
+    b->committed = false;
+    destroy_graph(b);
+    return GCOO_OK;
+}
+
+int gcoo_batch_panel_info(gcoo_batch_t b, int *panel_jobs, int *workgroups)
+{
+    if (!b) return set_error(GCOO_ERR_INVALID, "null batch");
+    if (panel_jobs) *panel_jobs = b->committed ? (int)b->pjobs.size() : 0;
+    if (workgroups) *workgroups = b->committed ? b->nUnitsF32 + b->nUnitsBf16 : 0;
+    return GCOO_OK;
+}
+
+int gcoo_batch_job_is_panel(gcoo_batch_t b, int job, int *panel)
+{
+    if (!b || job < 0 || job >= (int)b->jobs.size() || !panel)
+        return set_error(GCOO_ERR_INVALID, "bad batch, job index or output");
+    *panel = b->committed && job < (int)b->panelOf.size() && b->panelOf[job] >= 0;
     return GCOO_OK;
 }
 

@@ -121,6 +121,9 @@ struct gcoo_matrix {
     mutable int plist_rowb = 0;               // bytes of a staged D row they encode
     mutable int plist_rpw = 0;                // rows per wave they were built for
     mutable int plist_waves = 0;              // waves per panel they were built for
+    // Bumped whenever the lists are rebuilt or forgotten: a batch that runs
+    // panel jobs from these lists records it and refuses to run once it moved.
+    mutable uint64_t plist_gen = 0;
     // Set by a conversion with a panel multiply behind it: the panel-list
     // kernel copies the status word to this device view of h_status (so no
     // separate copy sits on the multiply's path) and the lists' launch
@@ -207,7 +210,11 @@ int ensure_seg16(gcoo_matrix *h, hipStream_t s);
 int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
                        int waves, bool widen, const ConvStatus *guard, hipStream_t s);
 // Forget the panel lists (after a conversion or a capacity change).
-inline void invalidate_panel_lists(const gcoo_matrix *h) { h->plist_src = nullptr; }
+inline void invalidate_panel_lists(const gcoo_matrix *h)
+{
+    h->plist_src = nullptr;
+    ++h->plist_gen;
+}
 // Ints of the offset (and count) area the panel lists of h need for k
 // chunks of tk rows and panels of waves x rpw rows; the lists themselves
 // need h->nnz uint2 entries.
@@ -230,6 +237,36 @@ struct ExtScratch {
 // chosen kernel needs more than xs->bytes.
 int launch_spdm_f32_ext(const gcoo_matrix *h, const float *D, int N, float *C,
                         const ExtScratch *xs, hipStream_t s);
+
+// ---- the panel multiply of many jobs in one launch (gcoo_spdm.hip) ----
+// A batch's panel jobs: one descriptor each (16-byte aligned), the same
+// arguments as a single panel launch.
+struct PanelJob {
+    const int *poff;
+    const uint2 *plist;
+    const void *D;
+    float *C;
+    int M, K, N, nC, nPanels;
+    unsigned dBytes;
+    int pad[2];
+};
+// The one workgroup shape batches use: 16 waves x 21 rows (336-row panels),
+// chunks of 75 FP32 or 150 BF16 D rows of 256 columns.
+constexpr int kBatchPanelWaves = 16, kBatchPanelRpw = 21;
+inline int batch_panel_tk(bool bf16) { return bf16 ? 150 : 75; }
+inline int batch_panel_rowb(bool bf16) { return bf16 ? 512 : 1024; }
+// True when D, N and C allow the panel kernel (16-byte pieces of D rows, 4
+// columns per lane).
+bool panel_operands_ok(const void *D, int N, const float *C, int elemBytes);
+// Workgroup b of a launch: job (-1: none), tile = column tile * nPanels +
+// panel, and its turn tidx of tmod at the L2 touch among the units of the
+// same job and column tile on its XCD.
+struct PanelUnit {
+    int job, tile, tmod, tidx;
+};
+// Enqueue one launch of nUnits workgroups over the jobs (FP32 or BF16 D).
+int launch_panel_batch(bool bf16, const PanelUnit *units, int nUnits, const PanelJob *jobs,
+                       hipStream_t s);
 
 // ---- batches (gcoo_batch.hip) ----
 // gcoo_batch_commit without the final stream synchronisation.

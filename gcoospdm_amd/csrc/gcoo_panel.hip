@@ -219,6 +219,7 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, 
     // kernel copies it to the host, and the event marks when it has
     ConvStatus *hst = (guard && guard == h->d_status) ? h->statusExport : nullptr;
     bool exported = false;
+    ++h->plist_gen;
     int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, widen, h->poff, h->plist, guard, s,
                                hst, &exported);
     if (rc) return rc;

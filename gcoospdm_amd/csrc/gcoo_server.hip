@@ -386,7 +386,8 @@ int gcoo_server_create(const gcoo_server_config *cfg, gcoo_server_t *out)
         Slot *slot = s->slots.back().get();
         bool ok = slot != nullptr &&
                   hipStreamCreateWithFlags(&slot->stream, hipStreamNonBlocking) == hipSuccess &&
-                  (s->plain || gcoo_batch_create(&slot->batch) == GCOO_OK);
+                  (s->plain || (gcoo_batch_create(&slot->batch) == GCOO_OK &&
+                                gcoo_batch_set_panel_mode(slot->batch, 0, 0.0, 0.0) == GCOO_OK));
         for (int e = 0; ok && e < s->depth; ++e) {
             hipEvent_t ev = nullptr;
             ok = hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;

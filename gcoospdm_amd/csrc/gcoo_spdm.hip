@@ -700,11 +700,16 @@ constexpr long long kPanelMinWorkgroups = 256;  // ... with at least one workgro
 
 typedef float f32x8 __attribute__((ext_vector_type(8)));
 
+// One workgroup's tile: panel `panel` of the row panels, column tile ct.
+// The workgroups that stage the same D tile at the same time on one XCD
+// take turns at the L2 touch: this one touches chunk c when c % tmod ==
+// tidx (a single launch: all panels of a tile, tmod = nPanels, tidx = panel).
 template <typename E, int TK, int WAVES, int RPW>
-__global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
-    const int *__restrict__ poff, const uint2 *__restrict__ plist, int M, int K, int nC,
-    const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C, int nPanels,
-    const ConvStatus *__restrict__ guard, long long cap)
+__device__ __forceinline__ void panel_tile(const int *__restrict__ poff,
+                                           const uint2 *__restrict__ plist, int M, int K,
+                                           int nC, const void *__restrict__ D, unsigned dBytes,
+                                           int N, float *__restrict__ C, int ct, int panel,
+                                           int tmod, int tidx)
 {
     constexpr int VEC = 4;
     constexpr int PR = WAVES * RPW;           // panel rows
@@ -730,10 +735,6 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
     static_assert(2 * CHUNKB + 256 <= 160 * 1024, "LDS");
     __shared__ f32x4 sD[DUMMY + 16];
 
-    if (guard_skip(guard, cap)) return;
-    const int L = xcd_remap(blockIdx.x, gridDim.x);
-    const int ct = L / nPanels;
-    const int panel = L - ct * nPanels;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = __lane_id();
     const int col0 = ct * TN;
@@ -862,9 +863,9 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
             while (c < wend) {
                 int cend = wend;
                 if (wave == TW) {
-                    if (c % nPanels == panel && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
-                    int t = (panel - (c + 1)) % nPanels;
-                    t = c + 1 + (t < 0 ? t + nPanels : t);  // next chunk that is touched at
+                    if (c % tmod == tidx && c + kPanelPrefetch < nC) touch(c + kPanelPrefetch);
+                    int t = (tidx - (c + 1)) % tmod;
+                    t = c + 1 + (t < 0 ? t + tmod : t);  // next chunk that is touched at
                     if (t < cend && t + kPanelPrefetch < nC) cend = t;
                 }
                 unsigned sli = __builtin_amdgcn_readfirstlane((unsigned)(c - cw));
@@ -911,6 +912,39 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
             if (r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
         }
     }
+}
+
+template <typename E, int TK, int WAVES, int RPW>
+__global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
+    const int *__restrict__ poff, const uint2 *__restrict__ plist, int M, int K, int nC,
+    const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C, int nPanels,
+    const ConvStatus *__restrict__ guard, long long cap)
+{
+    if (guard_skip(guard, cap)) return;
+    const int L = xcd_remap(blockIdx.x, gridDim.x);
+    const int ct = L / nPanels;
+    const int panel = L - ct * nPanels;
+    panel_tile<E, TK, WAVES, RPW>(poff, plist, M, K, nC, D, dBytes, N, C, ct, panel, nPanels,
+                                  panel);
+}
+
+// The panel multiply of many jobs in one launch (the panel jobs of a batch):
+// workgroup b takes tile units[b] = (job, the job's tile index as in the
+// single-job launch: column tile * panels + panel, L2-touch turn tidx of
+// tmod), or nothing for job -1.  Workgroups are dispatched to the XCDs
+// round-robin (b % 8); the host deals the units over the XCDs by estimated
+// work and keeps a job's units on one XCD next to each other, so that they
+// stage the same D tile together through that XCD's L2.
+template <typename E, int TK, int WAVES, int RPW>
+__global__ __launch_bounds__(WAVES * 64) void spdm_panel_batch_kernel(
+    const PanelUnit *__restrict__ units, const PanelJob *__restrict__ jobs)
+{
+    const PanelUnit u = units[blockIdx.x];
+    if (u.job < 0) return;
+    const PanelJob &j = jobs[u.job];
+    const int ct = u.tile / j.nPanels;
+    panel_tile<E, TK, WAVES, RPW>(j.poff, j.plist, j.M, j.K, j.nC, j.D, j.dBytes, j.N, j.C, ct,
+                                  u.tile - ct * j.nPanels, u.tmod, u.tidx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1472,6 +1506,27 @@ int launch_typed(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
 }
 
 }  // namespace
+
+bool panel_operands_ok(const void *D, int N, const float *C, int elemBytes)
+{
+    return (N * elemBytes) % 16 == 0 && (reinterpret_cast<uintptr_t>(D) & 15u) == 0 &&
+           max_vec(D, N, C, elemBytes) >= 4;
+}
+
+int launch_panel_batch(bool bf16, const PanelUnit *units, int nUnits, const PanelJob *jobs,
+                       hipStream_t s)
+{
+    static_assert(kBatchPanelWaves == 16 && kBatchPanelRpw == 21, "batch panel shape");
+    if (nUnits <= 0) return GCOO_OK;
+    if (bf16)
+        hipLaunchKernelGGL((spdm_panel_batch_kernel<BF16, 150, 16, 21>), dim3((unsigned)nUnits),
+                           dim3(16 * 64), 0, s, units, jobs);
+    else
+        hipLaunchKernelGGL((spdm_panel_batch_kernel<F32, 75, 16, 21>), dim3((unsigned)nUnits),
+                           dim3(16 * 64), 0, s, units, jobs);
+    GCOO_CHECK_LAUNCH("spdm_panel_batch_kernel");
+    return GCOO_OK;
+}
 
 int launch_spdm_f32_cfg(const gcoo_matrix *h, const float *D, int N, float *C,
                         const ConvStatus *guard, hipStream_t s, const ExtScratch *xs);

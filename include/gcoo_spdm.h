@@ -250,8 +250,9 @@ int gcoo_test_set_conversion_abort(gcoo_handle h, gcoo_stream s);
  * BF16 values are derived at commit), any shapes.  Each job is cut into work
  * items of `rows` rows x one column tile of `vec` columns per lane (0 = the
  * built-in choice; tuned values come from gcoospdm_amd/autotune.py).  A run
- * is a counter-reset kernel plus one persistent kernel whose waves pull work
- * items until none are left; it allocates nothing and synchronises nothing,
+ * is the panel jobs' launches (below), a counter-reset kernel and one
+ * persistent kernel whose waves pull work items until none are left; it
+ * allocates nothing and synchronises nothing,
  * so it can be captured into a HIP graph and replayed.  Every C element is
  * the same fmaf chain as in gcoo_spdm_f32 / gcoo_spdm_bf16: results are
  * bit-identical to running each job alone.  The batch keeps the pointers it
@@ -272,8 +273,28 @@ int gcoo_batch_run_async(gcoo_batch_t b, gcoo_stream s);
 /* Capture one run into a HIP graph on stream s (non-null), then replay it. */
 int gcoo_batch_graph_capture(gcoo_batch_t b, gcoo_stream s);
 int gcoo_batch_graph_launch(gcoo_batch_t b, gcoo_stream s);
+/* items: work items of the persistent kernel (panel jobs have none). */
 int gcoo_batch_info(gcoo_batch_t b, int *jobs, int64_t *items, int *grid_blocks);
 int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes);
+/* Panel jobs.  Jobs dense enough for the panel-staged multiply (the LDS
+ * kernel gcoo_spdm_f32 / gcoo_spdm_bf16 use from about 0.25 % / 0.5 %
+ * density; 336-row panels x 256-column tiles, so N % 4 == 0 and 16-byte
+ * aligned D and C) run it instead of the persistent kernel: one launch per
+ * element type ahead of it, its units dealt over the XCDs by estimated work.
+ * Commit chooses them (mode 1, the default: where a cost model puts the panel
+ * kernel's time below `ratio_f32` / `ratio_bf16` times the persistent
+ * kernel's, 0 meaning the built-in 1.0 / 8.0; 0: none; 2: every job whose
+ * operands allow it) and builds their panel lists, which the handle keeps; a
+ * later multiply of
+ * the same matrix with another panel shape rebuilds them, after which run,
+ * capture and graph launch return GCOO_ERR_STATE until the batch is
+ * committed again.  A matrix in two jobs of different element types runs
+ * the second on the persistent kernel.  Same bits either way. */
+int gcoo_batch_set_panel_mode(gcoo_batch_t b, int mode, double ratio_f32,
+                              double ratio_bf16);
+/* After commit: panel jobs and the workgroups of their launches. */
+int gcoo_batch_panel_info(gcoo_batch_t b, int *panel_jobs, int *workgroups);
+int gcoo_batch_job_is_panel(gcoo_batch_t b, int job, int *panel);
 
 /* ---- Serving: continuous batching of requests on resident operands -------
  * No counterpart in the reference.  Operands (a converted handle and its D,

@@ -209,3 +209,111 @@ def test_batch_refuses_after_reconvert(dev):
     for (m, D, C), sp in zip(jobs, specs):
         ref = alone(m, D, sp.N, D.dtype != torch.float32)
         assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), sp
+
+
+def _panel_specs():
+    """Jobs that the panel kernel takes (dense enough, N a multiple of 4) next
+    to jobs it does not: ragged M (not multiples of the 336-row panel), one
+    and two 256-column tiles (N = 256, 260, 512), short K, FP32 and BF16."""
+    shapes = [(700, 900, 256, 0.97), (336, 400, 260, 0.95), (1000, 75, 512, 0.9),
+              (50, 1500, 256, 0.98), (1200, 2000, 128, 0.96), (400, 600, 48, 0.95),
+              (673, 151, 256, 0.99), (9, 5, 256, 0.5), (800, 800, 100, 0.97)]
+    return [W.JobSpec(i, M, K, N, s, 7000 + i) for i, (M, K, N, s) in enumerate(shapes)]
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_batch_panel_jobs_match_gather(dev, mode):
+    """The same jobs with no panel jobs (mode 0) and with panel jobs (mode 1:
+    the cost model's choice, 2: every job the operands allow), FP32 and BF16
+    mixed in one batch: identical bits, and the panel jobs bit-equal to the
+    oracle's chain."""
+    import torch
+    specs = _panel_specs()
+    jobs = [W.materialize(sp, dev, i % 2 == 1) for i, sp in enumerate(specs)]
+    results = {}
+    for md in (0, mode):
+        b = G.Batch().set_panel_mode(md)
+        for m, D, C in jobs:
+            C.fill_(float("nan"))
+            b.add(m, D, C)
+        b.commit()
+        b.run()
+        torch.cuda.synchronize()
+        info = b.panel_info()
+        print(f"panel mode {md}: {info}", flush=True)
+        if md == 0:
+            assert info["panel_jobs"] == 0
+        else:
+            assert info["panel_jobs"] >= 1
+        if md == 2:
+            # every job with entries whose D rows are whole 16-byte pieces
+            # (the operands are aligned)
+            assert info["panel_jobs"] == sum(
+                1 for i, (sp, (m, _, _)) in enumerate(zip(specs, jobs))
+                if (sp.N * (2 if i % 2 else 4)) % 16 == 0 and m.nnz > 0)
+        results[md] = ([C.clone() for _, _, C in jobs], [b.job_is_panel(i) for i in range(len(jobs))])
+    for i, sp in enumerate(specs):
+        assert torch.equal(results[0][0][i].view(torch.int32),
+                           results[mode][0][i].view(torch.int32)), sp
+    for i, sp in enumerate(specs):
+        if not results[mode][1][i]:
+            continue
+        S, Dh = _host_job(sp, dev)
+        g = oracle.convert(S, 4)
+        if i % 2 == 1:
+            want = oracle.spdm_bf16(g, oracle.to_bf16(g["vals"]), oracle.to_bf16(Dh))
+        else:
+            want = oracle.spdm_f32(g, Dh)
+        assert H.bits_equal(results[mode][0][i].cpu().numpy(), want), sp
+
+
+def test_batch_panel_same_matrix_two_types(dev):
+    """One matrix in an FP32 and a BF16 job of the same batch: the panel lists
+    are per element type, so the second job takes the persistent kernel; both
+    results equal the jobs run alone."""
+    import torch
+    sp = W.JobSpec(0, 900, 700, 256, 0.97, 4242)
+    m, D, C = W.materialize(sp, dev, False)
+    D16 = G.to_bf16(D)
+    C16 = torch.empty_like(C)
+    b = G.Batch().set_panel_mode(2)
+    b.add(m, D, C)
+    b.add(m, D16, C16)
+    b.commit()
+    b.run()
+    torch.cuda.synchronize()
+    assert b.job_is_panel(0) and not b.job_is_panel(1)
+    assert torch.equal(C.view(torch.int32), alone(m, D, 256, False).view(torch.int32))
+    assert torch.equal(C16.view(torch.int32), alone(m, D16, 256, True).view(torch.int32))
+
+
+def test_batch_panel_lists_rebuilt_refuses(dev):
+    """After commit, a plain multiply of a panel job's matrix with another
+    panel shape rebuilds the handle's lists: run, capture and replay refuse
+    with GCOO_ERR_STATE, launching nothing; a new commit runs again."""
+    import torch
+    sp = W.JobSpec(0, 1000, 1200, 256, 0.97, 4343)
+    m, D, C = W.materialize(sp, dev, False)
+    b = G.Batch().set_panel_mode(2)
+    b.add(m, D, C)
+    b.commit()
+    assert b.job_is_panel(0)
+    s = torch.cuda.Stream()
+    b.capture(s)
+    b.replay(s)
+    s.synchronize()
+    ref = C.clone()
+    m.set_config(variant=6, waves=16, chunk=16)   # 256-row panels: other lists
+    C2 = torch.empty_like(C)
+    m.spdm(D, C2)
+    m.set_config()
+    assert torch.equal(C2.view(torch.int32), ref.view(torch.int32))
+    for call in (b.run, lambda: b.capture(s), lambda: b.replay(s)):
+        with pytest.raises(G.GCOOError) as e:
+            call()
+        assert e.value.code == G.GCOO_ERR_STATE
+    C.fill_(float("nan"))
+    b.commit()
+    b.run()
+    torch.cuda.synchronize()
+    assert torch.equal(C.view(torch.int32), ref.view(torch.int32))
