@@ -93,6 +93,12 @@ _SIGNATURES = [
     ("gcoo_batch_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I64), _c.POINTER(_I)]),
     ("gcoo_batch_job_tile", _I, [_P, _I, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I)]),
     ("gcoo_batch_set_panel_mode", _I, [_P, _I, _c.c_double, _c.c_double]),
+    ("gcoo_csr_create", _I, [_I, _I, _PP]),
+    ("gcoo_csr_destroy", _I, [_P]),
+    ("gcoo_csr_convert_dense", _I, [_P, _P, _P]),
+    ("gcoo_csr_get_arrays", _I, [_P, _PP, _PP, _PP, _c.POINTER(_I64)]),
+    ("gcoo_csr_spmm_f32", _I, [_P, _P, _I, _P, _P]),
+    ("gcoo_csr_spmm_f32_async", _I, [_P, _P, _I, _P, _P]),
     ("gcoo_batch_panel_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I)]),
     ("gcoo_batch_job_is_panel", _I, [_P, _I, _c.POINTER(_I)]),
     ("gcoo_server_create", _I, [_P, _PP]),
@@ -556,14 +562,101 @@ class ShGCOOSpDM(ShGemm):
         return self._m
 
 
-class ShCsrSpGemm(ShGCOOSpDM):
+class CSRMatrix:
+    """A CSR matrix on the device (gcoo_csr_*): rowPtr, colInd, vals."""
+
+    def __init__(self, M: int, K: int):
+        c = ctypes.c_void_p()
+        _check(lib().gcoo_csr_create(M, K, ctypes.byref(c)), "gcoo_csr_create")
+        self._c = c
+        self.M, self.K = M, K
+
+    def close(self):
+        if getattr(self, "_c", None) is not None and self._c.value:
+            lib().gcoo_csr_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def convert_dense(self, S, stream=None):
+        import torch
+        _require_dev(S, "S", torch.float32)
+        if tuple(S.shape) != (self.M, self.K):
+            raise ValueError("S shape mismatch")
+        _check(lib().gcoo_csr_convert_dense(self._c, _ptr(S), _stream(stream)),
+               "gcoo_csr_convert_dense")
+        return self
+
+    def spmm(self, D, C, stream=None, sync=True):
+        import torch
+        _require_dev(D, "D", torch.float32)
+        _require_dev(C, "C", torch.float32)
+        K, N = D.shape
+        if K != self.K or tuple(C.shape) != (self.M, N):
+            raise ValueError("shape mismatch")
+        fn = lib().gcoo_csr_spmm_f32 if sync else lib().gcoo_csr_spmm_f32_async
+        _check(fn(self._c, _ptr(D), N, _ptr(C), _stream(stream)), "gcoo_csr_spmm_f32")
+        return C
+
+    def arrays_host(self):
+        """(rowPtr, colInd, vals) copied to the host as numpy arrays."""
+        import numpy as np
+        rp, ci, va = (ctypes.c_void_p() for _ in range(3))
+        n = ctypes.c_int64()
+        _check(lib().gcoo_csr_get_arrays(self._c, ctypes.byref(rp), ctypes.byref(ci),
+                                         ctypes.byref(va), ctypes.byref(n)), "gcoo_csr_get_arrays")
+        out = {}
+        for key, ptr, count, dt in (("rowPtr", rp, self.M + 1, np.int32),
+                                    ("colInd", ci, n.value, np.int32),
+                                    ("vals", va, n.value, np.float32)):
+            host = np.empty(count, dt)
+            if count:
+                _check(lib().gcoo_memcpy_d2h(host.ctypes.data_as(ctypes.c_void_p), ptr,
+                                             host.nbytes, None), "gcoo_memcpy_d2h")
+            out[key] = host
+        out["nnz"] = n.value
+        return out
+
+
+class ShCsrSpGemm(ShGemm):
     """Factory key "csr": the reference's row-wise CSR SpDM ablation
-    (src/ShCsrSpGemm.cu:9-146) as the GCOO path with one row per group --
-    whose row segments are CSR -- and the row-segment kernel."""
+    (src/ShCsrSpGemm.h, src/ShCsrSpGemm.cu:9-146) over gcoo_csr_*: the
+    three-array CSR of S and the reference kernel's structure (a workgroup
+    per row and column tile, the row's entries staged in LDS)."""
 
     def __init__(self, denseMatrix, nRows: int, nCols: int):
-        super().__init__(denseMatrix, nRows, nCols, cpg=1)
-        self._m.set_config(variant=2)
+        super().__init__(nRows, nCols)
+        self.m_pDenseMatrix = denseMatrix
+        self._c = CSRMatrix(nRows, nCols)
+
+    def _shape_check(self, A, nRows, nCols):
+        if tuple(A.shape) != (nRows, nCols):
+            raise ValueError(f"A shape {tuple(A.shape)} != {(nRows, nCols)}")
+
+    def mutiplyBy(self, A, nRows, nCols, C):
+        self._shape_check(A, nRows, nCols)
+        self._c.convert_dense(self.m_pDenseMatrix)
+        self._c.spmm(A, C)
+        return 0.0
+
+    def mutiply(self, B, nRows, nCols, C):
+        return self.mutiplyBy(B, nRows, nCols, C)
+
+    def build(self):
+        self._c.convert_dense(self.m_pDenseMatrix)
+
+    def runKernelTest(self, A, nRows, nCols, C):
+        self._shape_check(A, nRows, nCols)
+        self._c.spmm(A, C)
+        return 0.0
+
+    @property
+    def matrix(self) -> CSRMatrix:
+        return self._c
 
 
 class SpGemmBuilder:

@@ -122,6 +122,10 @@ Result bench(const float *A, size_t nRowsA, size_t nColsA, const float *B, size_
     Result res;
     res.us_per_iter = iters > 0 ? ms * 1000.0 / iters : 0.0;
     if (auto *g = dynamic_cast<ShGCOOSpDM *>(algoObj.get())) res.nnz = g->nnz();
+    if (auto *c = dynamic_cast<ShCsrSpGemm *>(algoObj.get())) {
+        int64_t n = 0;
+        if (gcoo_csr_get_arrays(c->csr(), nullptr, nullptr, nullptr, &n) == GCOO_OK) res.nnz = n;
+    }
     CHECK_GCOO(gcoo_memcpy_d2h(C, dC, nC * sizeof(float), nullptr));
     std::printf("Time:%f\n", res.us_per_iter);
     algoObj.reset();

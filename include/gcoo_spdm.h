@@ -243,6 +243,31 @@ int gcoo_multiply_by(gcoo_handle h, const float *S, const float *D, int K,
  * kernel still exits normally. */
 int gcoo_test_set_conversion_abort(gcoo_handle h, gcoo_stream s);
 
+/* ---- The "csr" algorithm -------------------------------------------------
+ * The reference's row-wise CSR SpDM ablation, factory key "csr" (reference
+ * src/ShCsrSpGemm.h, src/ShCsrSpGemm.cu:9-146, src/SpGemmBuilder.h:12-23).
+ * A CSR matrix holds the standard three arrays of S (M x K): rowPtr[M + 1],
+ * colInd[nnz], vals[nnz], each row's entries in ascending column order (what
+ * the reference's cusparseSnnz + cusparseSdense2csr produce,
+ * src/ShCsrSpGemm.cu:67-110; the same entries as the GCOO conversion:
+ * S[r][k] != 0.0).  The multiply follows the reference kernel
+ * (src/ShCsrSpGemm.cu:9-43): one workgroup per (row, 256-column tile), the
+ * row's entries staged in LDS, one output column per thread; each C element
+ * is the fmaf chain over the row's entries in ascending k, the same bits as
+ * gcoo_spdm_f32.  No transposes of D and C (the reference makes them with
+ * cublasSgeam because its kernel reads column-major operands). */
+typedef struct gcoo_csr *gcoo_csr_t;
+int gcoo_csr_create(int M, int K, gcoo_csr_t *out);
+int gcoo_csr_destroy(gcoo_csr_t c);
+/* Dense S (device, row-major M x K) -> CSR.  Synchronous. */
+int gcoo_csr_convert_dense(gcoo_csr_t c, const float *S, gcoo_stream s);
+/* Device arrays of the last conversion (owned by c). */
+int gcoo_csr_get_arrays(gcoo_csr_t c, const int **rowPtr, const int **colInd,
+                        const float **vals, int64_t *nnz);
+/* C = S * D with D K x N and C M x N (device, row-major). */
+int gcoo_csr_spmm_f32(gcoo_csr_t c, const float *D, int N, float *C, gcoo_stream s);
+int gcoo_csr_spmm_f32_async(gcoo_csr_t c, const float *D, int N, float *C, gcoo_stream s);
+
 /* ---- Batches: many independent multiplies in one persistent launch -------
  * No counterpart in the reference (one multiply per launch,
  * src/ShGCOOSpDM.cu:153-167).  A batch holds jobs C_i = S_i * D_i, each S_i

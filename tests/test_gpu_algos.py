@@ -1,6 +1,6 @@
 """The reference's SpGemmBuilder algorithms beside "gcoospdm": "csr" (the
-row-wise CSR ablation, run by this library with one row per group) gives the
-reference's bits; the vendor baselines "cusparse" (rocSPARSE) and "cublas"
+row-wise CSR ablation: three-array CSR and the reference kernel's
+structure) gives the reference's bits; the vendor baselines "cusparse" (rocSPARSE) and "cublas"
 (rocBLAS) sum in their own order, so they are held to the FP32 tolerance
 against the FP64 product (relative Frobenius <= 1e-5, src/main.cu:136)."""
 import os
@@ -161,3 +161,31 @@ def test_harness_tsv(dev, tmp_path):
     rows = [ln.split("\t") for ln in buf.getvalue().strip().splitlines()]
     assert rows[1][0] == str(n) and rows[1][2] == "m.mtx"
     assert all(0.0 < float(t) < harness.FAILED for t in rows[1][3:])
+
+
+@pytest.mark.parametrize("M,K,N,dens", [(130, 257, 520, 0.05), (1, 1, 1, 1.0), (7, 300, 1, 0.5),
+                                        (40, 33, 256, 0.0), (700, 900, 300, 0.02)])
+def test_csr_format_and_kernel(dev, M, K, N, dens):
+    """The CSR arrays equal the oracle's conversion with one row per group
+    (whose groups are rows: groupIndex is rowPtr, the (k, r) order within a
+    one-row group is ascending k), edge values included (-0.0 dropped, NaN,
+    denormals and inf kept); the CSR kernel's C is bit-equal to the oracle's
+    fmaf chain; the product of a converted matrix can be re-run."""
+    import torch
+    rng = np.random.default_rng(M + 3 * K + 7 * N)
+    S = H.rng_sparse(rng, M, K, dens)
+    if M * K >= 8:
+        S.flat[rng.integers(0, M * K, 4)] = np.array([-0.0, np.nan, 1e-40, np.inf], np.float32)
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    c = G.CSRMatrix(M, K).convert_dense(to_dev(S, dev))
+    got = c.arrays_host()
+    g = oracle.convert(S, 1)
+    assert np.array_equal(got["rowPtr"], g["groupIndex"])
+    assert got["nnz"] == g["nnz"]
+    assert np.array_equal(got["colInd"], g["cols"])
+    assert np.array_equal(got["vals"].view(np.uint32), g["vals"].view(np.uint32))
+    want = oracle.spdm_f32(oracle.convert(S, 4), D)
+    for rep in range(2):
+        C = torch.full((M, N), float("nan"), device=dev)
+        c.spmm(to_dev(D, dev), C)
+        assert H.bits_equal(C.cpu().numpy(), want), rep
