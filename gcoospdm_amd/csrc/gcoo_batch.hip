@@ -513,13 +513,15 @@ int plan_panels(gcoo_batch *b, hipStream_t s)
         unitSec.push_back(us);
         pjBf16.push_back(bf16);
     }
-    // per element type: the units of the jobs with the longest units first,
-    // each to the XCD with the least estimated work so far (so a large job
-    // spreads over the XCDs and no XCD idles while another runs a long job);
-    // on an XCD a job's units of one column tile follow each other and take
-    // turns at the L2 touch of the D tile they stage together.  Workgroup b
+    // per element type: the jobs with the longest units first, each column
+    // tile's units in groups of at most one XCD's CUs (32), each group to the
+    // XCD with the least estimated work so far.  A group's units run together
+    // on one XCD, stage the same D tile through its L2 (taking turns at the
+    // L2 touch) and so fetch it once; a job larger than one XCD spreads over
+    // several, and no XCD idles while another runs a long job.  Workgroup b
     // runs on XCD b % 8, so XCD x's k-th unit goes to workgroup 8 k + x; XCDs
     // with fewer units get empty ones.
+    constexpr int kGroupUnits = 32;
     for (int bf = 0; bf < 2; ++bf) {
         std::vector<int> js;
         for (size_t p = 0; p < b->pjobs.size(); ++p)
@@ -532,21 +534,14 @@ int plan_panels(gcoo_batch *b, hipStream_t s)
             const int tiles = (b->pjobs[p].N + 255) / 256;
             const int np = b->pjobs[p].nPanels;
             for (int ct = 0; ct < tiles; ++ct) {
-                size_t first[kNumXcd];
-                for (int x = 0; x < kNumXcd; ++x) first[x] = lists[x].size();
-                for (int pn = 0; pn < np; ++pn) {
+                for (int p0 = 0; p0 < np; p0 += kGroupUnits) {
+                    const int cnt = std::min(kGroupUnits, np - p0);
                     int x = 0;
                     for (int y = 1; y < kNumXcd; ++y)
                         if (load[y] < load[x]) x = y;
-                    lists[x].push_back(PanelUnit{p, ct * np + pn, 1, 0});
-                    load[x] += unitSec[p];
-                }
-                for (int x = 0; x < kNumXcd; ++x) {
-                    const int cnt = (int)(lists[x].size() - first[x]);
-                    for (int k = 0; k < cnt; ++k) {
-                        lists[x][first[x] + k].tmod = cnt;
-                        lists[x][first[x] + k].tidx = k;
-                    }
+                    for (int k = 0; k < cnt; ++k)
+                        lists[x].push_back(PanelUnit{p, ct * np + p0 + k, cnt, k});
+                    load[x] += cnt * unitSec[p];
                 }
             }
         }
