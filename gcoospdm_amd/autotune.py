@@ -6,9 +6,13 @@ substituting them into templates, recompiling and timing `main`
 are runtime arguments, so the search needs no rebuild: each candidate is
 timed with HIP events on the device and the fastest is written to a table.
 
-Three tables:
+Four tables:
   batch[dtype][class] = {rows, vec}   for gcoo_batch_add, where
-      class = (N bucket, entries-per-row bucket) of a job;
+      class = (N bucket, entries-per-row bucket) of a job (searched with
+      every job on the persistent kernel, the kernel these tiles are for);
+  batch_panel[dtype] = {mode, ratio}  for gcoo_batch_set_panel_mode: which
+      jobs of a batch run the panel-staged kernel instead, searched on the
+      whole many-jobs workload with the tuned tiles;
   plain[dtype][class] = {variant, vec} for gcoo_set_spdm_config;
   skew[dtype][class]  = {cpg, variant, vec, unroll, chunk, waves} for matrices
       with skewed rows, class = (N bucket, entries-per-row bucket, skew
@@ -95,6 +99,14 @@ def batch_tile(N: int, per_row: float, bf16: bool, table: dict | None = None):
     return (e["rows"], e["vec"]) if e else (0, 0)
 
 
+def batch_panel(bf16: bool, table: dict | None = None):
+    """(mode, ratio) for Batch.set_panel_mode; (1, 0.0) = the library's
+    built-in choice."""
+    t = (table if table is not None else load()).get("batch_panel", {}).get(
+        "bf16" if bf16 else "f32")
+    return (t["mode"], t["ratio"]) if t else (1, 0.0)
+
+
 def plain_config(N: int, per_row: float, bf16: bool, table: dict | None = None):
     t = (table if table is not None else load()).get("plain", {}).get("bf16" if bf16 else "f32", {})
     e = t.get(class_key(N, per_row))
@@ -130,7 +142,7 @@ def tune_batch(jobs_by_class, bf16: bool, log):
         best = None
         for rows in (1, 2, 4, 8, 16):
             for vec in vecs:
-                b = G.Batch()
+                b = G.Batch().set_panel_mode(0)
                 for m, D, C, N in jobs:
                     b.add(m, D, C, rows=rows, vec=vec)
                 b.commit()
@@ -142,6 +154,41 @@ def tune_batch(jobs_by_class, bf16: bool, log):
                 b.close()
         out[key] = {"rows": best[1], "vec": best[2], "ms": round(best[0], 4)}
     return out
+
+
+def tune_batch_panel(specs, bf16: bool, log, table: dict):
+    """The panel-job choice of a batch, searched on the whole workload
+    (every job materialised, tuned tiles): mode 0 (none), mode 1 at several
+    ratios, mode 2 (every eligible job); -> {mode, ratio, ms}."""
+    import torch
+    import gcoospdm_amd as G
+    from . import workloads as W
+    jobs = [W.materialize(sp, bf16=bf16) for sp in specs]
+    cands = [(0, 0.0)] + [(1, r) for r in (0.5, 0.7, 1.0, 1.4, 2.0, 4.0, 8.0, 16.0)] + [(2, 0.0)]
+    best, ref = None, None
+    stream = torch.cuda.Stream()
+    for mode, ratio in cands:
+        b = G.Batch().set_panel_mode(mode, ratio, ratio)
+        for (m, D, C), sp in zip(jobs, specs):
+            rows, vec = batch_tile(sp.N, sp.K * (1 - sp.sparsity), bf16, table)
+            b.add(m, D, C, rows=rows, vec=vec)
+        b.commit()
+        b.capture(stream)
+        with torch.cuda.stream(stream):
+            ms = _time(lambda: b.replay(stream), reps=5)
+        out = [C.clone() for _, _, C in jobs]
+        if ref is None:
+            ref = out
+        same = all(torch.equal(x.view(torch.int32), y.view(torch.int32)) for x, y in zip(out, ref))
+        info = b.panel_info()
+        log(dict(kind="batch_panel", dtype="bf16" if bf16 else "f32", mode=mode, ratio=ratio,
+                 ms=round(ms, 4), panel_jobs=info["panel_jobs"], same=same))
+        if not same:
+            raise RuntimeError(f"batch panel mode {mode} ratio {ratio} changed the bits")
+        if best is None or ms < best[0]:
+            best = (ms, mode, ratio)
+        b.close()
+    return {"mode": best[1], "ratio": best[2], "ms": round(best[0], 4)}
 
 
 def tune_plain(bf16: bool, log, device="cuda:0"):
@@ -258,6 +305,8 @@ def main():
                     help="rerun only the skew search, keeping the other tables")
     ap.add_argument("--batch-only", action="store_true",
                     help="rerun only the batch tile search, keeping the other tables")
+    ap.add_argument("--batch-panel-only", action="store_true",
+                    help="rerun only the batch panel-job search, keeping the other tables")
     args = ap.parse_args()
     from . import workloads as W
 
@@ -278,6 +327,14 @@ def main():
         print(f"wrote {args.out}", flush=True)
         return
     specs = W.batch_jobs(args.jobs, 2024)
+    if args.batch_panel_only:
+        table = dict(load(args.out)) if os.path.exists(args.out) else {}
+        table["batch_panel"] = {d: tune_batch_panel(specs, d == "bf16", log, table)
+                                for d in ("f32", "bf16")}
+        with open(args.out, "w") as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        print(f"wrote {args.out}", flush=True)
+        return
     if args.batch_only:
         table = dict(load(args.out)) if os.path.exists(args.out) else {}
         table["batch"] = {}
@@ -288,11 +345,12 @@ def main():
             json.dump(table, f, indent=1, sort_keys=True)
         print(f"wrote {args.out}", flush=True)
         return
-    table = {"device": "MI355X (gfx950)", "batch": {}, "plain": {}, "skew": {}}
+    table = {"device": "MI355X (gfx950)", "batch": {}, "batch_panel": {}, "plain": {}, "skew": {}}
     for bf16 in (False, True):
         by_class = _jobs_by_class(specs, bf16, args.per_class)
         table["batch"]["bf16" if bf16 else "f32"] = tune_batch(by_class, bf16, log)
         del by_class
+        table["batch_panel"]["bf16" if bf16 else "f32"] = tune_batch_panel(specs, bf16, log, table)
         table["plain"]["bf16" if bf16 else "f32"] = tune_plain(bf16, log)
         table["skew"]["bf16" if bf16 else "f32"] = tune_skew(bf16, log)
     table["seconds"] = round(time.time() - t0, 1)

@@ -18,7 +18,7 @@ import json
 import time
 
 
-def run(count=512, bf16=False, reps=10, seed=2024, table=None, panel_mode=1, panel_ratio=0.0):
+def run(count=512, bf16=False, reps=10, seed=2024, table=None, panel_mode=None, panel_ratio=0.0):
     import torch
     import gcoospdm_amd as G
     from . import autotune, workloads as W
@@ -64,6 +64,8 @@ def run(count=512, bf16=False, reps=10, seed=2024, table=None, panel_mode=1, pan
     t_seq = timed(seq)
     ref = [C.clone() for _, _, C in jobs]
 
+    if panel_mode is None:  # the tuned choice
+        panel_mode, panel_ratio = autotune.batch_panel(bf16, table)
     b = G.Batch().set_panel_mode(panel_mode, panel_ratio, panel_ratio)
     for (m, D, C), sp in zip(jobs, specs):
         rows, vec = autotune.batch_tile(sp.N, sp.K * (1 - sp.sparsity), bf16, table)
@@ -98,8 +100,9 @@ def main():
     ap.add_argument("--jobs", type=int, default=512)
     ap.add_argument("--bf16", action="store_true")
     ap.add_argument("--reps", type=int, default=10)
-    ap.add_argument("--panel-mode", type=int, default=1,
-                    help="0: no panel jobs, 1: the cost model's (default), 2: all eligible")
+    ap.add_argument("--panel-mode", type=int, default=None,
+                    help="0: no panel jobs, 1: the cost model's, 2: all eligible "
+                         "(default: the tuned choice)")
     ap.add_argument("--panel-ratio", type=float, default=0.0,
                     help="mode 1: panel when its estimate is below ratio x the other (0: 1.0)")
     ap.add_argument("--calibrate", action="store_true",
