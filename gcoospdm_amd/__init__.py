@@ -93,6 +93,7 @@ _SIGNATURES = [
     ("gcoo_batch_info", _I, [_P, _c.POINTER(_I), _c.POINTER(_I64), _c.POINTER(_I)]),
     ("gcoo_batch_job_tile", _I, [_P, _I, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I)]),
     ("gcoo_batch_set_panel_mode", _I, [_P, _I, _c.c_double, _c.c_double]),
+    ("gcoo_set_panel_balance", _I, [_P, _I]),
     ("gcoo_csr_create", _I, [_I, _I, _PP]),
     ("gcoo_csr_destroy", _I, [_P]),
     ("gcoo_csr_convert_dense", _I, [_P, _P, _P]),
@@ -287,6 +288,13 @@ class GCOOMatrix:
         fn = lib().gcoo_spdm_bf16 if sync else lib().gcoo_spdm_bf16_async
         _check(fn(self._h, _ptr(D16), K, N, _ptr(C), _stream(stream)), "gcoo_spdm_bf16")
         return C
+
+    def set_panel_balance(self, mode: int):
+        """Row order of the panel-staged multiply's lists (gcoo_set_panel_balance):
+        0 in order (default), 1 balanced where the lists outlive the call, 2
+        always balanced."""
+        _check(lib().gcoo_set_panel_balance(self._h, mode), "gcoo_set_panel_balance")
+        return self
 
     def set_config(self, variant=0, vec=0, waves=0, chunk=0, unroll=0):
         """Multiply-kernel variant / tile shape (0 = automatic)."""

@@ -415,6 +415,7 @@ int gcoo_destroy(gcoo_handle h)
     if (h->seg16) (void)hipFree(h->seg16);
     if (h->plist) (void)hipFree(h->plist);
     if (h->poff) (void)hipFree(h->poff);
+    if (h->prow) (void)hipFree(h->prow);
     if (h->rowptr) (void)hipFree(h->rowptr);
     if (h->rowcnt) (void)hipFree(h->rowcnt);
     if (h->mask) (void)hipFree(h->mask);
@@ -532,6 +533,14 @@ int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg)
     if (cfg->unroll != 0 && cfg->unroll != 8 && cfg->unroll != 16 && cfg->unroll != 32)
         return set_error(GCOO_ERR_INVALID, "unroll %d (0, 8, 16 or 32)", cfg->unroll);
     h->spdm_cfg = *cfg;
+    return GCOO_OK;
+}
+
+int gcoo_set_panel_balance(gcoo_handle h, int mode)
+{
+    if (!h) return set_error(GCOO_ERR_INVALID, "null handle");
+    if (mode < 0 || mode > 2) return set_error(GCOO_ERR_INVALID, "panel balance %d (0, 1 or 2)", mode);
+    h->panel_balance = mode;
     return GCOO_OK;
 }
 

@@ -391,7 +391,7 @@ int check_generations(const gcoo_batch *b)
                              "committed; commit (and capture) the batch again", i);
         const int pj = i < b->panelOf.size() ? b->panelOf[i] : -1;
         if (pj >= 0 && (h->plist_gen != b->plistGens[pj] || h->plist != b->pjobs[pj].plist ||
-                        h->poff != b->pjobs[pj].poff))
+                        h->poff != b->pjobs[pj].poff || panel_rowmap(h) != b->pjobs[pj].rowmap))
             return set_error(GCOO_ERR_STATE,
                              "job %zu: its matrix's panel lists were rebuilt (another multiply "
                              "of it used another panel shape) after the batch was committed; "
@@ -491,13 +491,15 @@ int plan_panels(gcoo_batch *b, hipStream_t s)
         for (const auto &c : claimed)
             if (c.first == h && c.second != jd.seg) clash = true;
         if (clash) continue;
+        // the lists serve every run of the batch: balanced row order
         int rc = ensure_panel_lists(h, jd.seg, batch_panel_tk(bf16), batch_panel_rowb(bf16),
-                                    kBatchPanelRpw, kBatchPanelWaves, bf16, nullptr, s);
+                                    kBatchPanelRpw, kBatchPanelWaves, bf16, nullptr, s, true);
         if (rc) return rc;
         claimed.emplace_back(h, jd.seg);
         PanelJob pj{};
         pj.poff = h->poff;
         pj.plist = h->plist;
+        pj.rowmap = panel_rowmap(h);
         pj.D = jd.D;
         pj.C = jd.C;
         pj.M = h->M;

@@ -124,6 +124,13 @@ struct gcoo_matrix {
     // Bumped whenever the lists are rebuilt or forgotten: a batch that runs
     // panel jobs from these lists records it and refuses to run once it moved.
     mutable uint64_t plist_gen = 0;
+    // Balanced row order of the lists' panels (see gcoo_panel.hip): when
+    // plist_balanced, slot q of wave w of panel p is row prow[p * PR + w *
+    // rpw + q] (-1: none) instead of p * PR + w * rpw + q.
+    mutable int *prow = nullptr;
+    mutable size_t prow_elems = 0;
+    mutable bool plist_balanced = false;
+    int panel_balance = 0;  // gcoo_set_panel_balance: balanced lists for this handle's multiplies
     // Set by a conversion with a panel multiply behind it: the panel-list
     // kernel copies the status word to this device view of h_status (so no
     // separate copy sits on the multiply's path) and the lists' launch
@@ -208,7 +215,14 @@ int ensure_seg16(gcoo_matrix *h, hipStream_t s);
 // plist[off[c] .. off[c + 1]), ordered by (row, k).  Kernels exit early when
 // guard_skip(guard, h->cap) holds.
 int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
-                       int waves, bool widen, const ConvStatus *guard, hipStream_t s);
+                       int waves, bool widen, const ConvStatus *guard, hipStream_t s,
+                       bool balance = false);
+// The row order of the lists' panels for the multiply's C stores: h->prow
+// when the current lists are balanced, else null (rows in order).
+inline const int *panel_rowmap(const gcoo_matrix *h) { return h->plist_balanced ? h->prow : nullptr; }
+// True when the balanced order is available for these lists (at most 128
+// chunks, 512 panel rows, 16 waves).
+bool panel_balance_supported(const gcoo_matrix *h, int tk, int rpw, int waves);
 // Forget the panel lists (after a conversion or a capacity change).
 inline void invalidate_panel_lists(const gcoo_matrix *h)
 {
@@ -221,9 +235,12 @@ inline void invalidate_panel_lists(const gcoo_matrix *h)
 size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves);
 // Enqueue the build of the panel lists into caller-given buffers (poff of
 // panel_offset_elems ints, plist of h->nnz entries); touches no handle state.
+// With rowmap (waves * rpw ints per panel), the panels' rows are dealt to
+// the waves in the balanced order, written to rowmap.
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
                       int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
-                      hipStream_t s, ConvStatus *hst = nullptr, bool *exported = nullptr);
+                      hipStream_t s, ConvStatus *hst = nullptr, bool *exported = nullptr,
+                      int *rowmap = nullptr);
 
 // Device scratch handed to the multiply by a caller that cannot allocate:
 // the panel lists and the long-row list are carved out of it instead of
@@ -246,9 +263,9 @@ struct PanelJob {
     const uint2 *plist;
     const void *D;
     float *C;
+    const int *rowmap;  // balanced row order of the lists, or null
     int M, K, N, nC, nPanels;
     unsigned dBytes;
-    int pad[2];
 };
 // The one workgroup shape batches use: 16 waves x 21 rows (336-row panels),
 // chunks of 75 FP32 or 150 BF16 D rows of 256 columns.

@@ -30,6 +30,8 @@
 // scratch area after the offsets.
 #include "gcoo_internal.h"
 
+#include <climits>
+
 namespace gcoo {
 namespace {
 
@@ -53,7 +55,7 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int rowb,
     int nC, int RPW, int widen, int *__restrict__ poff, int *__restrict__ gcnt,
     uint2 *__restrict__ plist, const ConvStatus *__restrict__ guard, long long cap,
-    ConvStatus *__restrict__ hst)
+    ConvStatus *__restrict__ hst, const int *__restrict__ rowmap, int waves)
 {
     // the conversion's status word to the host (its host mirror's device
     // view), first of all, so that the host can check the conversion while
@@ -77,13 +79,45 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     const long long r0 = (long long)pw * RPW;
     const int R0 = (int)(r0 < M ? r0 : M);
     const int R1 = (int)(r0 + RPW < M ? r0 + RPW : M);
-    const int base = rowptr[R0];
+    // Slot rr of this wave is row R0 + rr, or with a balanced order row
+    // rowmap[pw * RPW + rr] (-1: none).  The panel's rows are the same
+    // either way, so its waves' lists tile the same range of the segments;
+    // a wave's list then starts after its panel's earlier waves' entries.
+    auto rowof = [&](int rr) { return rowmap ? rowmap[(size_t)pw * RPW + rr] : R0 + rr; };
+    const int nslots = rowmap ? RPW : R1 - R0;
+    int base = rowptr[R0];
+    int wend = rowptr[R1];
+    if (rowmap) {
+        __shared__ int sb[2];
+        const int wv = pw % waves;
+        const long long p0 = (long long)(pw - wv) * RPW;  // the panel's first row
+        if (threadIdx.x < 64) {
+            int before = 0, own = 0;
+            for (int i = threadIdx.x; i < (wv + 1) * RPW; i += 64) {
+                const int r = rowmap[p0 + i];
+                const int len = r >= 0 ? rowptr[r + 1] - rowptr[r] : 0;
+                if (i < wv * RPW) before += len;
+                else own += len;
+            }
+            before = wave_sum(before);
+            own = wave_sum(own);
+            if (threadIdx.x == 0) {
+                sb[0] = before;
+                sb[1] = own;
+            }
+        }
+        __syncthreads();
+        base = rowptr[(int)(p0 < M ? p0 : M)] + sb[0];
+        wend = base + sb[1];
+    }
 
     for (int i = tid; i < L; i += NT) cnt[i] = 0;
     __syncthreads();
     // 1. run lengths per (chunk, row)
-    for (int rr = wave; rr < R1 - R0; rr += NT / 64) {
-        const int b = rowptr[R0 + rr], e = rowptr[R0 + rr + 1];
+    for (int rr = wave; rr < nslots; rr += NT / 64) {
+        const int rw = rowof(rr);
+        if (rw < 0) continue;
+        const int b = rowptr[rw], e = rowptr[rw + 1];
         int carry = b;
         for (int i0 = b; i0 < e; i0 += 64) {
             const int i = i0 + lane;
@@ -121,10 +155,12 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     }
     __syncthreads();
     for (int c = tid; c < nC; c += NT) op[c] = cnt[c * RPW];
-    if (tid == 0) op[nC] = rowptr[R1];
+    if (tid == 0) op[nC] = wend;
     // 3. entries to their slots
-    for (int rr = wave; rr < R1 - R0; rr += NT / 64) {
-        const int b = rowptr[R0 + rr], e = rowptr[R0 + rr + 1];
+    for (int rr = wave; rr < nslots; rr += NT / 64) {
+        const int rw = rowof(rr);
+        if (rw < 0) continue;
+        const int b = rowptr[rw], e = rowptr[rw + 1];
         int carry = b;
         for (int i0 = b; i0 < e; i0 += 64) {
             const int i = i0 + lane;
@@ -142,6 +178,193 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
     }
 }
 
+// ---- balanced order of a panel's rows over its waves -------------------
+// The multiply meets at a workgroup barrier every chunk, so a chunk takes as
+// long as the wave with the most work in it.  With the panel's rows dealt
+// to the waves in order, a wave's share of a chunk is a sum of 21 random
+// rows' counts: at 1 % density the slowest of 16 waves carries about 1.45
+// times the mean.  This kernel deals the rows instead so that every chunk's
+// counts come out even: rows longest first, 16 at a time; each wave, the
+// most loaded first, takes the row of the 16 that raises the sum over
+// chunks of the per-chunk maximum least (the batched form of the greedy
+// that picks one row at a time: the same result to 1 %, 21 steps instead of
+// 336).  The waves that also stage D (kAssignStagers) start each chunk with
+// kAssignHandicap entries' worth of load.  Measured at n = 8000, s = 0.99
+// (16 x 21): 403.7 -> 389.3 us with the host's greedy order, 405.6 -> 397.0
+// us with this kernel's.  Any order gives the same bits: each row's chain is
+// still its entries in ascending k.  One workgroup per panel;
+// rowmap[p * PR + w * rpw + q] = the row in slot q of wave w (-1: none).
+// The picks are serial (one wave, 16 per step, each a readlane scan of 16
+// costs), about 0.5 ms per build at that size: the order pays only for
+// lists that serve many multiplies (batches; gcoo_set_panel_balance).
+constexpr int kAssignThreads = 1024;
+constexpr int kAssignMaxChunks = 128;  // longer K keeps the plain order
+constexpr int kAssignMaxRows = 512;    // panel rows (waves * rpw) at most
+constexpr int kAssignMaxWaves = 16;
+constexpr int kAssignStagers = 5;      // the multiply's staging waves (0..4)
+constexpr int kAssignHandicap = 4;
+
+__global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
+    const int *__restrict__ rowptr, const int2 *__restrict__ seg, int M, int tk, int nC,
+    int RPW, int waves, int *__restrict__ rowmap, const ConvStatus *__restrict__ guard,
+    long long cap, ConvStatus *__restrict__ hst)
+{
+    constexpr int NT = kAssignThreads;
+    constexpr int CW = kAssignMaxChunks / 4;  // count words per row (4 chunks each)
+    __shared__ unsigned cnt[kAssignMaxRows * CW];               // bytes: entries per (row, chunk)
+    __shared__ int wc[kAssignMaxWaves * kAssignMaxChunks];      // per (wave, chunk) load
+    __shared__ int cur[kAssignMaxChunks];                       // per-chunk maximum over waves
+    __shared__ int key[kAssignMaxRows], idx[kAssignMaxRows];    // rows by length
+    __shared__ int part[NT / 256][256];                         // cost partial sums
+    __shared__ int load[kAssignMaxWaves], pick[kAssignMaxWaves], worder[kAssignMaxWaves];
+    __shared__ int rp[kAssignMaxRows + 1];                      // the panel's row offsets
+    if (hst && blockIdx.x == 0 && threadIdx.x == 0) {
+        const unsigned *src = reinterpret_cast<const unsigned *>(guard);
+        unsigned *dst = reinterpret_cast<unsigned *>(hst);
+#pragma unroll
+        for (int i = 0; i < (int)(sizeof(ConvStatus) / 4); ++i)
+            __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+    }
+    if (guard_skip(guard, cap)) return;
+    const int tid = threadIdx.x, lane = __lane_id();
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int PR = waves * RPW;
+    const long long r0l = (long long)blockIdx.x * PR;
+    const int r0 = (int)r0l;
+    const int nr = (int)(r0l + PR <= M ? PR : M - r0l);
+    auto count = [&](int q, int c) -> int {
+        return q < nr ? (int)((cnt[q * CW + (c >> 2)] >> ((c & 3) * 8)) & 255u) : 0;
+    };
+
+    // 1. entries per (row, chunk), and the rows by length
+    for (int i = tid; i < PR * CW; i += NT) cnt[i] = 0;
+    for (int i = tid; i <= nr; i += NT) rp[i] = rowptr[r0 + i];
+    __syncthreads();
+    for (int i = tid; i < kAssignMaxRows; i += NT) {
+        key[i] = i < nr ? rp[i + 1] - rp[i] : (i < PR ? -1 : -2);
+        idx[i] = i;
+    }
+    // the panel's entries are one range of the segments: every thread takes
+    // every NT-th entry (independent loads) and finds its row in rp
+    {
+        const int e0 = rp[0], e1 = rp[nr];
+#pragma unroll 4
+        for (int i = e0 + tid; i < e1; i += NT) {
+            const int k = seg[i].x;
+            int lo = 0, hi = nr - 1;  // the last row whose start is <= i
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (rp[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            const int c = k / tk;
+            atomicAdd(&cnt[lo * CW + (c >> 2)], 1u << ((c & 3) * 8));
+        }
+    }
+    // bitonic sort of the 512 (key, index) pairs: key descending, index
+    // ascending among equal keys
+    for (int k = 2; k <= kAssignMaxRows; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            __syncthreads();
+            if (tid < kAssignMaxRows) {
+                const int l = tid ^ j;
+                if (l > tid) {
+                    const bool first = (key[tid] > key[l]) || (key[tid] == key[l] && idx[tid] < idx[l]);
+                    // sections with (tid & k) == 0 end in order, the others reversed
+                    if (((tid & k) == 0) != first) {
+                        const int tk2 = key[tid], ti = idx[tid];
+                        key[tid] = key[l];
+                        idx[tid] = idx[l];
+                        key[l] = tk2;
+                        idx[l] = ti;
+                    }
+                }
+            }
+        }
+    }
+    for (int i = tid; i < waves * nC; i += NT)
+        wc[(i / nC) * kAssignMaxChunks + i % nC] = i / nC < kAssignStagers ? kAssignHandicap : 0;
+    if (tid < waves) load[tid] = (tid < kAssignStagers ? kAssignHandicap : 0) * nC;
+    for (int c = tid; c < nC; c += NT) cur[c] = waves > 0 ? kAssignHandicap : 0;
+    __syncthreads();
+
+    // 2. RPW steps: the next `waves` rows by length, one to each wave
+    const int pairs = waves * waves;
+    for (int step = 0; step < RPW; ++step) {
+        const int *brow = idx + step * waves;  // this step's rows (panel-relative)
+        {
+            const int pr = tid & 255, sl = tid >> 8;
+            int sum = 0;
+            if (pr < pairs) {
+                const int w = pr / waves, q = brow[pr % waves];
+                for (int c = sl; c < nC; c += NT / 256)
+                    sum += max(cur[c], wc[w * kAssignMaxChunks + c] + count(q, c));
+            }
+            part[sl][pr] = sum;
+        }
+        __syncthreads();
+        if (wv == 0) {
+            // the waves by load, most loaded first (index ascending on ties)
+            const int ld = lane < waves ? load[lane] : 0;
+            int rank = 0;
+            for (int o = 0; o < waves; ++o) {
+                const int lo = __shfl(ld, o, 64);
+                rank += (lo > ld) || (lo == ld && o < lane);
+            }
+            if (lane < waves) worder[rank] = lane;
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the LDS writes above are done
+            __builtin_amdgcn_wave_barrier();
+            // lane j holds the costs of batch row j for every wave; the picks
+            // run in scalar registers (one readlane per candidate)
+            int costs[kAssignMaxWaves];
+#pragma unroll
+            for (int w = 0; w < kAssignMaxWaves; ++w) {
+                costs[w] = 0;
+                if (w < waves && lane < waves)
+#pragma unroll
+                    for (int sl = 0; sl < NT / 256; ++sl) costs[w] += part[sl][w * waves + lane];
+            }
+            unsigned taken = 0;  // batch rows already picked (bit j)
+            for (int t = 0; t < waves; ++t) {
+                const int w = worder[t];
+                int mine = 0;
+#pragma unroll
+                for (int u = 0; u < kAssignMaxWaves; ++u)
+                    if (u == w) mine = costs[u];
+                int best = INT_MAX, bj = 0;
+                for (int jj = 0; jj < waves; ++jj) {
+                    const int cj = __builtin_amdgcn_readlane(mine, jj);
+                    if (!((taken >> jj) & 1u) && cj < best) {
+                        best = cj;
+                        bj = jj;
+                    }
+                }
+                taken |= 1u << bj;
+                if (lane == 0) pick[w] = bj;
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < waves * nC; i += NT) {
+            const int w = i / nC, c = i - w * nC;
+            wc[w * kAssignMaxChunks + c] += count(brow[pick[w]], c);
+        }
+        if (tid < waves) {
+            const int pos = step * waves + pick[tid];  // sorted position: key = its length
+            const int q = idx[pos];
+            load[tid] += q < nr ? key[pos] : 0;
+            rowmap[r0l + (long long)tid * RPW + step] = q < nr ? r0 + q : -1;
+        }
+        __syncthreads();
+        for (int c = tid; c < nC; c += NT) {
+            int m = 0;
+            for (int w = 0; w < waves; ++w) m = max(m, wc[w * kAssignMaxChunks + c]);
+            cur[c] = m;
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace
 
 size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves)
@@ -156,9 +379,16 @@ size_t panel_offset_elems(const gcoo_matrix *h, int tk, int rpw, int waves)
     return nW * ((size_t)nC + 1) + (lds ? 0 : nW * L);
 }
 
+bool panel_balance_supported(const gcoo_matrix *h, int tk, int rpw, int waves)
+{
+    const int nC = (h->K + tk - 1) / tk;
+    return nC <= kAssignMaxChunks && rpw * waves <= kAssignMaxRows && waves <= kAssignMaxWaves &&
+           tk <= 255;
+}
+
 int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
                       int waves, bool widen, int *poff, uint2 *plist, const ConvStatus *guard,
-                      hipStream_t s, ConvStatus *hst, bool *exported)
+                      hipStream_t s, ConvStatus *hst, bool *exported, int *rowmap)
 {
     if (exported) *exported = false;
     // two chunk buffers of offsets / 16 in 16 bits
@@ -172,18 +402,30 @@ int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, i
     const size_t nW = nP * waves;
     const size_t L = (size_t)nC * rpw;
     const bool lds = L <= (size_t)kListLdsInts;
+    if (rowmap && !panel_balance_supported(h, tk, rpw, waves))
+        return set_error(GCOO_ERR_UNSUPPORTED, "balanced panel order for %d chunks of %d x %d rows",
+                         nC, waves, rpw);
     if (nW > 0) {
+        // the first kernel behind a conversion exports its status word
+        ConvStatus *hx = guard ? hst : nullptr;
+        if (rowmap) {
+            hipLaunchKernelGGL(panel_assign_kernel, dim3((unsigned)nP), dim3(kAssignThreads), 0, s,
+                               h->rowptr, seg, h->M, tk, nC, rpw, waves, rowmap, guard,
+                               (long long)h->cap, hx);
+            GCOO_CHECK_LAUNCH("panel_assign_kernel");
+            hx = nullptr;
+        }
         int *gcnt = poff + nW * ((size_t)nC + 1);
         if (lds)
             hipLaunchKernelGGL((panel_wave_lists_kernel<true, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), sizeof(int) * L, s, h->rowptr, seg, h->M, tk,
                                rowb, nC, rpw, widen ? 1 : 0, poff, gcnt, plist, guard,
-                               (long long)h->cap, guard ? hst : nullptr);
+                               (long long)h->cap, hx, rowmap, waves);
         else
             hipLaunchKernelGGL((panel_wave_lists_kernel<false, kListThreads>), dim3((unsigned)nW),
                                dim3(kListThreads), 0, s, h->rowptr, seg, h->M, tk, rowb, nC, rpw,
-                               widen ? 1 : 0, poff, gcnt, plist, guard, (long long)h->cap,
-                               guard ? hst : nullptr);
+                               widen ? 1 : 0, poff, gcnt, plist, guard, (long long)h->cap, hx,
+                               rowmap, waves);
         GCOO_CHECK_LAUNCH("panel_wave_lists_kernel");
         if (exported) *exported = guard && hst;
     }
@@ -191,11 +433,28 @@ int build_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, i
 }
 
 int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, int rpw,
-                       int waves, bool widen, const ConvStatus *guard, hipStream_t s)
+                       int waves, bool widen, const ConvStatus *guard, hipStream_t s,
+                       bool balance)
 {
+    balance = balance && panel_balance_supported(h, tk, rpw, waves);
     if (h->plist_src == seg && h->plist_tk == tk && h->plist_rowb == rowb &&
-        h->plist_rpw == rpw && h->plist_waves == waves)
+        h->plist_rpw == rpw && h->plist_waves == waves && h->plist_balanced == balance)
         return GCOO_OK;
+    if (balance) {
+        const size_t rows = ((size_t)h->M + (size_t)rpw * waves - 1) / ((size_t)rpw * waves) *
+                            ((size_t)rpw * waves);
+        if (rows > h->prow_elems) {
+            if (h->prow) GCOO_HIP_TRY(hipFree(h->prow));
+            h->prow = nullptr;
+            h->prow_elems = 0;
+            if (hipMalloc(&h->prow, sizeof(int) * rows) != hipSuccess) {
+                (void)hipGetLastError();
+                h->prow = nullptr;
+                return set_error(GCOO_ERR_ALLOC, "cannot allocate the panel row order");
+            }
+            h->prow_elems = rows;
+        }
+    }
     const size_t elems = panel_offset_elems(h, tk, rpw, waves);
     if (elems > h->poff_elems) {
         if (h->poff) GCOO_HIP_TRY(hipFree(h->poff));
@@ -221,7 +480,7 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, 
     bool exported = false;
     ++h->plist_gen;
     int rc = build_panel_lists(h, seg, tk, rowb, rpw, waves, widen, h->poff, h->plist, guard, s,
-                               hst, &exported);
+                               hst, &exported, balance ? h->prow : nullptr);
     if (rc) return rc;
     if (exported) {
         GCOO_HIP_TRY(hipEventRecord(h->statusEvent, s));
@@ -233,6 +492,7 @@ int ensure_panel_lists(const gcoo_matrix *h, const int2 *seg, int tk, int rowb, 
     h->plist_rowb = rowb;
     h->plist_rpw = rpw;
     h->plist_waves = waves;
+    h->plist_balanced = balance;
     return GCOO_OK;
 }
 

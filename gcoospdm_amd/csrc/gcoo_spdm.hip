@@ -709,7 +709,7 @@ __device__ __forceinline__ void panel_tile(const int *__restrict__ poff,
                                            const uint2 *__restrict__ plist, int M, int K,
                                            int nC, const void *__restrict__ D, unsigned dBytes,
                                            int N, float *__restrict__ C, int ct, int panel,
-                                           int tmod, int tidx)
+                                           int tmod, int tidx, const int *__restrict__ rowmap)
 {
     constexpr int VEC = 4;
     constexpr int PR = WAVES * RPW;           // panel rows
@@ -905,11 +905,12 @@ __device__ __forceinline__ void panel_tile(const int *__restrict__ poff,
     if (j < N) {
 #pragma unroll
         for (int q = 0; q < RPW; ++q) {
-            const int r = panel * PR + wave * RPW + q;
+            // slot q's row: in order, or the lists' balanced order (-1: none)
+            const int r = rowmap ? rowmap[panel * PR + wave * RPW + q] : panel * PR + wave * RPW + q;
             float out[VEC];
 #pragma unroll
             for (int c = 0; c < VEC; ++c) out[c] = acc[(RPW * c + q) / AW][(RPW * c + q) % AW];
-            if (r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
+            if (r >= 0 && r < M) store_vec<VEC>(C + (size_t)r * N + j, out);
         }
     }
 }
@@ -918,14 +919,14 @@ template <typename E, int TK, int WAVES, int RPW>
 __global__ __launch_bounds__(WAVES * 64) void spdm_panel_kernel(
     const int *__restrict__ poff, const uint2 *__restrict__ plist, int M, int K, int nC,
     const void *__restrict__ D, unsigned dBytes, int N, float *__restrict__ C, int nPanels,
-    const ConvStatus *__restrict__ guard, long long cap)
+    const ConvStatus *__restrict__ guard, long long cap, const int *__restrict__ rowmap)
 {
     if (guard_skip(guard, cap)) return;
     const int L = xcd_remap(blockIdx.x, gridDim.x);
     const int ct = L / nPanels;
     const int panel = L - ct * nPanels;
     panel_tile<E, TK, WAVES, RPW>(poff, plist, M, K, nC, D, dBytes, N, C, ct, panel, nPanels,
-                                  panel);
+                                  panel, rowmap);
 }
 
 // The panel multiply of many jobs in one launch (the panel jobs of a batch):
@@ -944,7 +945,7 @@ __global__ __launch_bounds__(WAVES * 64) void spdm_panel_batch_kernel(
     const PanelJob &j = jobs[u.job];
     const int ct = u.tile / j.nPanels;
     panel_tile<E, TK, WAVES, RPW>(j.poff, j.plist, j.M, j.K, j.nC, j.D, j.dBytes, j.N, j.C, ct,
-                                  u.tile - ct * j.nPanels, u.tmod, u.tidx);
+                                  u.tile - ct * j.nPanels, u.tmod, u.tidx, j.rowmap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1103,6 +1104,7 @@ int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
     constexpr int PR = WAVES * RPW;       // panel rows
     const int *poff = h->poff;
     const uint2 *plist = h->plist;
+    const int *rowmap = nullptr;
     if (xs) {
         // lists rebuilt into the caller's scratch: offsets, then the entries
         // (plus the slack the kernel's batched entry loads may read past a
@@ -1121,10 +1123,15 @@ int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
         poff = po;
         plist = pl;
     } else {
-        int rc = ensure_panel_lists(h, seg, TK, ROWB, RPW, WAVES, E::bytes == 2, guard, s);
+        // the balanced row order (gcoo_panel.hip) costs a kernel per list
+        // build: only when asked for (mode 1: where the lists outlive the
+        // call, not right behind a conversion in the same call; 2: always)
+        const bool balance = h->panel_balance == 2 || (h->panel_balance == 1 && !guard);
+        int rc = ensure_panel_lists(h, seg, TK, ROWB, RPW, WAVES, E::bytes == 2, guard, s, balance);
         if (rc) return rc;
         poff = h->poff;
         plist = h->plist;
+        rowmap = panel_rowmap(h);
     }
     const unsigned long long dBytes = (unsigned long long)h->K * N * E::bytes;
     const int nPanels = (h->M + PR - 1) / PR;
@@ -1134,7 +1141,7 @@ int launch_panel(const gcoo_matrix *h, const int2 *seg, const void *D, int N, fl
     hipLaunchKernelGGL((spdm_panel_kernel<E, TK, WAVES, RPW>), dim3((unsigned)nblocks),
                        dim3(WAVES * 64), 0, s,
                        poff, plist, h->M, h->K, nC, D, (unsigned)dBytes, N, C, nPanels,
-                       guard, (long long)h->cap);
+                       guard, (long long)h->cap, rowmap);
     GCOO_CHECK_LAUNCH("spdm_panel_kernel");
     return GCOO_OK;
 }

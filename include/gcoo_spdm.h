@@ -215,6 +215,18 @@ typedef struct gcoo_spdm_config {
 } gcoo_spdm_config;
 int gcoo_set_spdm_config(gcoo_handle h, const gcoo_spdm_config *cfg);
 int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
+/* Row order of the panel-staged multiply (variant 6).  Its workgroup meets
+ * at a barrier every k chunk, so a chunk lasts as long as its busiest wave;
+ * the lists can deal each panel's rows to the waves so that every chunk's
+ * entry counts come out even (the multiply at n = 8000, s = 0.99: 405.6 ->
+ * 397.0 us), at the price of an extra kernel per list build (about 0.5 ms
+ * there), so it pays only for lists that serve many multiplies.
+ * mode 0 (default): rows in order; 1: balanced when the lists are built for
+ * a multiply of an already converted matrix (gcoo_spdm_*), whose lists
+ * outlive the call, in order inside gcoo_multiply_by; 2: always balanced.
+ * Batches always balance their panel jobs' lists (built once at commit).
+ * The bits of C do not depend on it.  No counterpart in the reference. */
+int gcoo_set_panel_balance(gcoo_handle h, int mode);
 /* The variant (1..6), columns per lane and, for variant 6, waves per
  * workgroup and rows per wave (else 0 and 0) that a multiply of h
  * with these operands (dtype 0: gcoo_spdm_f32*, 1: gcoo_spdm_bf16*) runs: the

@@ -425,3 +425,35 @@ def test_conversion_stall_fallback(dev, M, K, N, dens, variant):
         g = oracle.convert(S, 4)
         check_arrays(m, g)
         assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D)), step
+
+
+@pytest.mark.parametrize("M,K,N,s", [(8000, 8000, 512, 0.99), (700, 900, 260, 0.97),
+                                      (50, 1500, 256, 0.98), (1000, 75, 256, 0.9),
+                                      (400, 12000, 256, 0.99)])
+def test_panel_balanced_order_same_bits(dev, M, K, N, s):
+    """The panel kernel's lists with each panel's rows dealt to the waves in
+    the balanced order (gcoo_set_panel_balance 2) give the same bits as in
+    order, for FP32 and BF16: ragged and partial panels, one chunk, and K past
+    the balanced order's 128-chunk limit (which keeps the plain order)."""
+    import torch
+    print(f"balanced order M={M} K={K} N={N} s={s}", flush=True)
+    S = torch.empty((M, K), device=dev)
+    D = torch.empty((K, N), device=dev)
+    G.fill_sparse(S, 31, 1, s)
+    G.fill_uniform(D, 31, 0)
+    m = G.GCOOMatrix(M, K, 4).convert_dense(S)
+    m.set_config(variant=6, waves=16, chunk=21)
+    D16 = G.to_bf16(D)
+    out = {}
+    for mode in (0, 2):
+        m.set_panel_balance(mode)
+        C = torch.full((M, N), float("nan"), device=dev)
+        m.spdm(D, C)
+        C16 = torch.full((M, N), float("nan"), device=dev)
+        m.spdm_bf16(D16, C16)
+        out[mode] = (C, C16)
+    assert torch.equal(out[0][0].view(torch.int32), out[2][0].view(torch.int32))
+    assert torch.equal(out[0][1].view(torch.int32), out[2][1].view(torch.int32))
+    if M * K <= 1_000_000:
+        Sh, Dh = S.cpu().numpy(), D.cpu().numpy()
+        assert H.bits_equal(out[2][0].cpu().numpy(), oracle.spdm_f32(oracle.convert(Sh, 4), Dh))
