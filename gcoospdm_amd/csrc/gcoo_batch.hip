@@ -52,6 +52,56 @@ struct JobDesc {
 __device__ __forceinline__ float lo16(unsigned w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float hi16(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
 
+// The bits of VEC elements of BYTES bytes each (at most 16 bytes), as
+// loaded; 2-byte loads zero-extended.
+template <int BYTES, int VEC>
+struct ElemBits {
+    unsigned w[(BYTES * VEC + 3) / 4];
+};
+
+// Those bits at byte offset voff (per lane) + soff (wave-uniform) of the
+// buffer; out-of-range lanes read zeros.
+template <int BYTES, int VEC>
+__device__ __forceinline__ ElemBits<BYTES, VEC> load_bits(__amdgpu_buffer_rsrc_t r, int voff,
+                                                          int soff)
+{
+    constexpr int B = BYTES * VEC;
+    static_assert(B == 2 || B == 4 || B == 8 || B == 16, "at most 16 bytes per lane");
+    ElemBits<BYTES, VEC> x;
+    if constexpr (B == 16) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x.w[i] = (unsigned)v[i];
+    } else if constexpr (B == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        x.w[0] = (unsigned)v[0];
+        x.w[1] = (unsigned)v[1];
+    } else if constexpr (B == 4) {
+        x.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    } else {
+        x.w[0] = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+    }
+    return x;
+}
+
+// The loaded bits widened to FP32 (exact: a BF16 is the high half of an FP32).
+template <int BYTES, int VEC>
+__device__ __forceinline__ void widen_elems(float (&d)[VEC], const ElemBits<BYTES, VEC> &x)
+{
+    if constexpr (BYTES == 4) {
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) d[i] = __uint_as_float(x.w[i]);
+    } else if constexpr (VEC == 1) {
+        d[0] = lo16(x.w[0]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < VEC / 2; ++i) {
+            d[2 * i] = lo16(x.w[i]);
+            d[2 * i + 1] = hi16(x.w[i]);
+        }
+    }
+}
+
 // VEC widened elements of BYTES bytes each at byte offset voff (per lane) +
 // soff (wave-uniform) of the buffer; out-of-range lanes read zeros.  The
 // width is a template parameter, so the entry loops carry no branches on it.
@@ -59,39 +109,7 @@ template <int BYTES, int VEC>
 __device__ __forceinline__ void load_elems(float (&d)[VEC], __amdgpu_buffer_rsrc_t r, int voff,
                                            int soff)
 {
-    constexpr int B = BYTES * VEC;
-    static_assert(B == 2 || B == 4 || B == 8 || B == 16, "at most 16 bytes per lane");
-    if constexpr (B == 16) {
-        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-        if constexpr (BYTES == 4) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) d[i] = __uint_as_float(x[i]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                d[2 * i] = lo16(x[i]);
-                d[2 * i + 1] = hi16(x[i]);
-            }
-        }
-    } else if constexpr (B == 8) {
-        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        if constexpr (BYTES == 4) {
-            d[0] = __uint_as_float(x[0]);
-            d[1] = __uint_as_float(x[1]);
-        } else {
-            d[0] = lo16(x[0]); d[1] = hi16(x[0]); d[2] = lo16(x[1]); d[3] = hi16(x[1]);
-        }
-    } else if constexpr (B == 4) {
-        const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-        if constexpr (BYTES == 4) {
-            d[0] = __uint_as_float(x);
-        } else {
-            d[0] = lo16(x);
-            d[1] = hi16(x);
-        }
-    } else {
-        d[0] = lo16(__builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
-    }
+    widen_elems<BYTES, VEC>(d, load_bits<BYTES, VEC>(r, voff, soff));
 }
 
 // An entry's value word as FP32 (BF16 segments hold the bits in the low half).
@@ -203,22 +221,27 @@ __device__ __forceinline__ void run_row_wave(const JobDesc &jd, __amdgpu_buffer_
     if constexpr (R > 8)
         for (; p + 8 <= pend; p += 8) round(std::integral_constant<int, 8>{}, p);
     if (p < pend) {
-        // the last 1..7 entries
+        // the last 1..7 entries; the loads keep their raw bits and are
+        // widened next to their FMAs (a widening beside its guarded load
+        // would wait for that load before the next one is issued)
         const int n = pend - p;
         float a[7];
-        float d[7][VEC];
+        ElemBits<BYTES, VEC> w[7];
 #pragma unroll
         for (int u = 0; u < 7; ++u) {
             if (u < n) {
                 a[u] = entry_val<BYTES>(sg[2 * (p + u) + 1]);
-                load_elems<BYTES, VEC>(d[u], rsrc, voffc, sg[2 * (p + u)] * rowBytes);
+                w[u] = load_bits<BYTES, VEC>(rsrc, voffc, sg[2 * (p + u)] * rowBytes);
             }
         }
 #pragma unroll
         for (int u = 0; u < 7; ++u)
-            if (u < n)
+            if (u < n) {
+                float d[VEC];
+                widen_elems<BYTES, VEC>(d, w[u]);
 #pragma unroll
-                for (int v = 0; v < VEC; ++v) fmac_s(acc[v], a[u], d[u][v]);
+                for (int v = 0; v < VEC; ++v) fmac_s(acc[v], a[u], d[v]);
+            }
     }
     if (valid && active) {
         float *dst = jd.C + (size_t)r * jd.N + j;

@@ -198,52 +198,61 @@ constexpr int kBufferFlags = 0x00020000;  // raw 32-bit buffer, gfx9 dword3
 __device__ __forceinline__ float lo_bf16(unsigned w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float hi_bf16(unsigned w) { return __uint_as_float(w & 0xffff0000u); }
 
+// The bits of VEC consecutive elements of type E (at most 16 bytes), as
+// loaded; 2-byte loads zero-extended.
+template <typename E, int VEC>
+struct RawBits {
+    unsigned w[(VEC * E::bytes + 3) / 4];
+};
+
+// Those bits at byte offset voff + soff of the buffer.
+template <typename E, int VEC>
+__device__ __forceinline__ RawBits<E, VEC> buf_load_raw(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    constexpr int B = VEC * E::bytes;
+    RawBits<E, VEC> x;
+    if constexpr (B == 2) {
+        x.w[0] = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+    } else if constexpr (B == 4) {
+        x.w[0] = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+    } else if constexpr (B == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        x.w[0] = (unsigned)v[0];
+        x.w[1] = (unsigned)v[1];
+    } else {
+        static_assert(B == 16, "16-byte loads at most");
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x.w[i] = (unsigned)v[i];
+    }
+    return x;
+}
+
+// The loaded bits widened to FP32 (exact: a BF16 is the high half of an FP32).
+template <typename E, int VEC>
+__device__ __forceinline__ void widen(float (&d)[VEC], const RawBits<E, VEC> &x)
+{
+    if constexpr (E::bytes == 4) {
+#pragma unroll
+        for (int i = 0; i < VEC; ++i) d[i] = __uint_as_float(x.w[i]);
+    } else if constexpr (VEC == 1) {
+        d[0] = lo_bf16(x.w[0]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < VEC / 2; ++i) {
+            d[2 * i] = lo_bf16(x.w[i]);
+            d[2 * i + 1] = hi_bf16(x.w[i]);
+        }
+    }
+}
+
 // VEC consecutive elements of type E at byte offset voff + soff of the
 // buffer, widened to FP32.
 template <typename E, int VEC>
 __device__ __forceinline__ void buf_load(float (&d)[VEC], __amdgpu_buffer_rsrc_t r, int voff,
                                          int soff)
 {
-    constexpr int B = VEC * E::bytes;
-    if constexpr (B == 2) {
-        const unsigned short x = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
-        d[0] = lo_bf16(x);
-    } else if constexpr (B == 4) {
-        const unsigned x = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-        if constexpr (VEC == 1) {
-            d[0] = __uint_as_float(x);
-        } else {
-            d[0] = lo_bf16(x);
-            d[VEC - 1] = hi_bf16(x);
-        }
-    } else if constexpr (B == 8) {
-        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        const unsigned w[2] = {(unsigned)x[0], (unsigned)x[1]};
-        if constexpr (VEC == 2) {
-            d[0] = __uint_as_float(w[0]);
-            d[1] = __uint_as_float(w[1]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                d[2 * i] = lo_bf16(w[i]);
-                d[2 * i + 1] = hi_bf16(w[i]);
-            }
-        }
-    } else {
-        static_assert(B == 16, "16-byte loads at most");
-        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
-        const unsigned w[4] = {(unsigned)x[0], (unsigned)x[1], (unsigned)x[2], (unsigned)x[3]};
-        if constexpr (VEC == 4) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) d[i] = __uint_as_float(w[i]);
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                d[2 * i] = lo_bf16(w[i]);
-                d[2 * i + 1] = hi_bf16(w[i]);
-            }
-        }
-    }
+    widen<E, VEC>(d, buf_load_raw<E, VEC>(r, voff, soff));
 }
 
 // Same, from LDS.
@@ -358,24 +367,30 @@ __global__ __launch_bounds__(kSpdmThreads) void spdm_rowseg_kernel(
         const int p1 = rowptr[r + 1];
         if (p1 - p0 >= skipLen) continue;  // a long row: the long-row kernel owns it
         // B entries (the first n of them when guarded): all their D loads
-        // are issued before the first FMA, then the chain runs in order
+        // are issued before the first FMA, then the chain runs in order.  The
+        // loads keep their raw bits and are widened next to their FMAs: in a
+        // guarded round, a widening beside its load would wait for that load
+        // before the next one is issued.
         auto round = [&](auto bc, int p, int n) {
             constexpr int B = decltype(bc)::value;
             float a[B];
-            float d[B][VEC];
+            RawBits<E, VEC> w[B];
 #pragma unroll
             for (int u = 0; u < B; ++u) {
                 if (u < n) {
                     const int2 kv = seg[p + u];
                     a[u] = E::val(kv.y);
-                    buf_load<E, VEC>(d[u], rsrc, voff, kv.x * rowBytes);
+                    w[u] = buf_load_raw<E, VEC>(rsrc, voff, kv.x * rowBytes);
                 }
             }
 #pragma unroll
             for (int u = 0; u < B; ++u)
-                if (u < n)
+                if (u < n) {
+                    float d[VEC];
+                    widen<E, VEC>(d, w[u]);
 #pragma unroll
-                    for (int v = 0; v < VEC; ++v) fmac(acc[v], a[u], d[u][v]);
+                    for (int v = 0; v < VEC; ++v) fmac(acc[v], a[u], d[v]);
+                }
         };
         int p = p0;
         for (; p + U <= p1; p += U) round(std::integral_constant<int, U>{}, p, U);
