@@ -195,7 +195,7 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
 // still its entries in ascending k.  One workgroup per panel;
 // rowmap[p * PR + w * rpw + q] = the row in slot q of wave w (-1: none).
 // The picks are serial (one wave, 16 per step, each a readlane scan of 16
-// costs), about 0.5 ms per build at that size: the order pays only for
+// costs), about 0.42 ms per build at that size: the order pays only for
 // lists that serve many multiplies (batches; gcoo_set_panel_balance).
 constexpr int kAssignThreads = 1024;
 constexpr int kAssignMaxChunks = 128;  // longer K keeps the plain order
@@ -210,9 +210,12 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
     long long cap, ConvStatus *__restrict__ hst)
 {
     constexpr int NT = kAssignThreads;
-    constexpr int CW = kAssignMaxChunks / 4;  // count words per row (4 chunks each)
+    // row strides one word past a multiple of the bank count, so that the
+    // rows and waves a wave's lanes read at once fall in different banks
+    constexpr int CW = kAssignMaxChunks / 4 + 1;  // count words per row (4 chunks each)
+    constexpr int WS = kAssignMaxChunks + 1;      // words per wave of wc
     __shared__ unsigned cnt[kAssignMaxRows * CW];               // bytes: entries per (row, chunk)
-    __shared__ int wc[kAssignMaxWaves * kAssignMaxChunks];      // per (wave, chunk) load
+    __shared__ int wc[kAssignMaxWaves * WS];                    // per (wave, chunk) load
     __shared__ int cur[kAssignMaxChunks];                       // per-chunk maximum over waves
     __shared__ int key[kAssignMaxRows], idx[kAssignMaxRows];    // rows by length
     __shared__ int part[NT / 256][256];                         // cost partial sums
@@ -284,7 +287,7 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
         }
     }
     for (int i = tid; i < waves * nC; i += NT)
-        wc[(i / nC) * kAssignMaxChunks + i % nC] = i / nC < kAssignStagers ? kAssignHandicap : 0;
+        wc[(i / nC) * WS + i % nC] = i / nC < kAssignStagers ? kAssignHandicap : 0;
     if (tid < waves) load[tid] = (tid < kAssignStagers ? kAssignHandicap : 0) * nC;
     for (int c = tid; c < nC; c += NT) cur[c] = waves > 0 ? kAssignHandicap : 0;
     __syncthreads();
@@ -299,7 +302,7 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
             if (pr < pairs) {
                 const int w = pr / waves, q = brow[pr % waves];
                 for (int c = sl; c < nC; c += NT / 256)
-                    sum += max(cur[c], wc[w * kAssignMaxChunks + c] + count(q, c));
+                    sum += max(cur[c], wc[w * WS + c] + count(q, c));
             }
             part[sl][pr] = sum;
         }
@@ -347,7 +350,7 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
         __syncthreads();
         for (int i = tid; i < waves * nC; i += NT) {
             const int w = i / nC, c = i - w * nC;
-            wc[w * kAssignMaxChunks + c] += count(brow[pick[w]], c);
+            wc[w * WS + c] += count(brow[pick[w]], c);
         }
         if (tid < waves) {
             const int pos = step * waves + pick[tid];  // sorted position: key = its length
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
         __syncthreads();
         for (int c = tid; c < nC; c += NT) {
             int m = 0;
-            for (int w = 0; w < waves; ++w) m = max(m, wc[w * kAssignMaxChunks + c]);
+            for (int w = 0; w < waves; ++w) m = max(m, wc[w * WS + c]);
             cur[c] = m;
         }
         __syncthreads();

@@ -244,6 +244,40 @@ def test_row_segment_rounds_bit_exact(dev, rows_per_wave, unroll, M, K, N, dens)
     assert H.bits_equal(C.cpu().numpy(), ref16)
 
 
+@pytest.mark.parametrize("vec", [1, 2, 4, 8])
+def test_row_segment_tails_every_width(dev, vec):
+    """Variant 2, one row per wave, at every column width: the guarded last
+    1..7 entries of a row load 2, 4, 8 or 16 bytes per lane (BF16) and are
+    widened next to their FMAs; rows of 0..40 entries cover every tail
+    length after rounds of 16 and of 8.  FP32 runs up to 4 columns per lane."""
+    import torch
+    M, K, N = 41, 300, 64
+    print(f"rowseg tails vec={vec}", flush=True)
+    rng = np.random.default_rng(4100 + vec)
+    S = np.zeros((M, K), np.float32)
+    for r in range(M):
+        cols = rng.choice(K, size=r, replace=False)
+        S[r, cols] = rng.uniform(-1, 1, r).astype(np.float32)
+    D = rng.uniform(-1, 1, (K, N)).astype(np.float32)
+    g = oracle.convert(S, 4)
+    m = G.GCOOMatrix(M, K, 4).convert_dense(to_dev(S, dev))
+    m.set_config(variant=2, chunk=1, unroll=16, vec=vec)
+    Dd = to_dev(D, dev)
+    C = torch.full((M, N), float("nan"), device=dev)
+    D16 = G.to_bf16(Dd)
+    ch = m.spdm_choice(D16, C)
+    assert ch[:2] == (2, vec), ch
+    m.spdm_bf16(D16, C)
+    ref16 = oracle.spdm_bf16(g, oracle.to_bf16(g["vals"]), oracle.to_bf16(D))
+    assert H.bits_equal(C.cpu().numpy(), ref16)
+    if vec <= 4:
+        C.fill_(float("nan"))
+        ch = m.spdm_choice(Dd, C)
+        assert ch[:2] == (2, vec), ch
+        m.spdm(Dd, C)
+        assert H.bits_equal(C.cpu().numpy(), oracle.spdm_f32(g, D))
+
+
 @pytest.mark.parametrize("M,K,N,dens", [
     (300, 1000, 512, 0.02),    # 2 panels (one partial), 16 chunks
     (257, 129, 260, 0.3),      # dense chunks: the 8-entry batches and every tail size
