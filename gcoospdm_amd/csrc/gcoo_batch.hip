@@ -147,27 +147,76 @@ __device__ __forceinline__ void run_row(const JobDesc &jd, __amdgpu_buffer_rsrc_
     // of its loads is out of range and returns zeros
     const unsigned rowBytes = (unsigned)jd.N * BYTES;
     const unsigned voffc = active ? (unsigned)j * BYTES : jd.dBytes;
-    while (__builtin_amdgcn_ballot_w64(p < pend) != 0) {
+    if constexpr (BYTES == 4) {
+        // FP32: rounds of U entries, each loaded and then its D loads issued
+        // (a software-pipelined form measured slower for FP32: more vector
+        // registers, fewer waves)
+        while (__builtin_amdgcn_ballot_w64(p < pend) != 0) {
+            int2 e[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                e[u] = (p + u < pend) ? jd.seg[p + u] : make_int2(0, 0);
+            float d[U][VEC];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned voff =
+                    (p + u < pend) ? voffc + (unsigned)e[u].x * rowBytes : jd.dBytes;
+                load_elems<BYTES, VEC>(d[u], rsrc, (int)voff, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (p + u < pend) {
+                    const float a = entry_val<BYTES>(e[u].y);
+#pragma unroll
+                    for (int v = 0; v < VEC; ++v) fmac_v(acc[v], a, d[u][v]);
+                }
+            }
+            p += U;
+        }
+    } else {
+        // BF16: each round issues the next round's entry loads and its own
+        // D loads before its FMAs, so it waits for one load latency, not for
+        // an entry load and then the D load it addresses.  The loads are
+        // unconditional, so the waits count exactly: an entry past the row's
+        // end reads entry pend - 1 (entry 0 when pend is 0: the segments'
+        // capacity is never zero), its D load goes out of range and returns
+        // +0.0, and its multiplier is -0.0: fmaf(-0.0, +0.0, acc) == acc for
+        // every acc, so the chain is the row's own entries in ascending k.
+        // An entry is read as one 64-bit word (k, value bits) through a
+        // global pointer: a vector-memory load only.
+        typedef __attribute__((address_space(1))) const long long gword;
+        gword *sg = (gword *)jd.seg;  // a C-style cast may change the address space
+        const int last = pend > 0 ? pend - 1 : 0;
+        auto entry = [&](int q) {
+            const long long w = sg[min(q, last)];
+            return make_int2((int)w, (int)(w >> 32));
+        };
         int2 e[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
-            e[u] = (p + u < pend) ? jd.seg[p + u] : make_int2(0, 0);
-        float d[U][VEC];
+        for (int u = 0; u < U; ++u) e[u] = entry(p + u);
+        while (__builtin_amdgcn_ballot_w64(p < pend) != 0) {
+            // the next round's entries first: loads complete in issue order,
+            // so the wait for a D load of this round also covers them
+            int2 en[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const unsigned voff =
-                (p + u < pend) ? voffc + (unsigned)e[u].x * rowBytes : jd.dBytes;
-            load_elems<BYTES, VEC>(d[u], rsrc, (int)voff, 0);
-        }
+            for (int u = 0; u < U; ++u) en[u] = entry(p + U + u);
+            float d[U][VEC];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (p + u < pend) {
-                const float a = entry_val<BYTES>(e[u].y);
+            for (int u = 0; u < U; ++u) {
+                const unsigned voff =
+                    (p + u < pend) ? voffc + (unsigned)e[u].x * rowBytes : jd.dBytes;
+                load_elems<BYTES, VEC>(d[u], rsrc, (int)voff, 0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const float a = (p + u < pend) ? entry_val<BYTES>(e[u].y) : -0.0f;
 #pragma unroll
                 for (int v = 0; v < VEC; ++v) fmac_v(acc[v], a, d[u][v]);
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u) e[u] = en[u];
+            p += U;
         }
-        p += U;
     }
     if (valid && active) {
         float *dst = jd.C + (size_t)r * jd.N + j;
@@ -266,26 +315,63 @@ __device__ __forceinline__ void run_item(const JobDesc &jd, __amdgpu_buffer_rsrc
 // a contiguous run of whole jobs (mostly) chosen so that the queues carry
 // about equal work, and its counter sits on a cache line of its own.
 // Default ratios of the panel-job choice (see plan_panels): on the 512-job
-// workload FP32 was fastest at 1.0 (1.71 ms; 0.5: 1.96, 2.0: 1.80) and BF16
-// with nearly every job on the panel kernel (the BF16 persistent kernel
-// widens every gathered value: 2.00 ms at 2.0 against 2.07 with none).
+// workload FP32 was fastest at 1.0 (1.63 ms; 0.5: 1.98, 2.0: 1.71, none:
+// 2.04) and BF16 at 1.0 among the ratios (1.61 ms; 0.5: 1.75, 2.0: 1.69,
+// 8.0: 1.81), though with no panel jobs at all it takes 1.60 ms.
 constexpr double kPanelRatioF32 = 1.0;
-constexpr double kPanelRatioBf16 = 8.0;
+constexpr double kPanelRatioBf16 = 1.0;
 
 struct BatchQueues {
     int start[kNumXcd + 1];
 };
 constexpr int kCounterStride = 32;  // unsigned ints between two queue counters
+// Resident persistent workgroups per CU at most: the FP32 kernel fits 7, but
+// on the 512-job workload (every job persistent) 4 / 5 / 6 / 7 / 8 per CU
+// took 2.07 / 2.00 / 2.00 / 2.02 / 2.02 ms; the BF16 kernel fits 5 (1.59 ms;
+// 6 and 8: 1.61, 1.63 ms).
+constexpr int kBatchMaxPerCu = 6;
 
 __global__ __launch_bounds__(64) void batch_reset_kernel(unsigned *counter)
 {
     if (threadIdx.x < kNumXcd) counter[threadIdx.x * kCounterStride] = 0;
 }
 
+// Element types a persistent kernel serves (a bit set): the kernel holds the
+// row loops of those types only, since its register count, and so the waves
+// per SIMD, is the largest any of its loops needs.
+constexpr int kTypeF32 = 1, kTypeBf16 = 2;
+
+// One dispatch per item on the element size and width.
+template <int TYPES>
+__device__ __forceinline__ void dispatch_item(const JobDesc &jd, __amdgpu_buffer_rsrc_t rsrc,
+                                              int r0, int sub, int j, bool active, int streams)
+{
+    if constexpr ((TYPES & kTypeF32) != 0) {
+        if (TYPES == kTypeF32 || jd.bytes == 4) {
+            switch (jd.vec) {
+            case 1: run_item<4, 1>(jd, rsrc, r0, sub, j, active, streams); break;
+            case 2: run_item<4, 2>(jd, rsrc, r0, sub, j, active, streams); break;
+            default: run_item<4, 4>(jd, rsrc, r0, sub, j, active, streams); break;
+            }
+            return;
+        }
+    }
+    if constexpr ((TYPES & kTypeBf16) != 0) {
+        switch (jd.vec) {
+        case 1: run_item<2, 1>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 2: run_item<2, 2>(jd, rsrc, r0, sub, j, active, streams); break;
+        case 4: run_item<2, 4>(jd, rsrc, r0, sub, j, active, streams); break;
+        default: run_item<2, 8>(jd, rsrc, r0, sub, j, active, streams); break;
+        }
+    }
+}
+
 // Workgroup b runs on XCD b % kNumXcd (the hardware dispatches workgroups
 // round-robin over the XCDs) and takes items from that XCD's queue first, so
 // a job's D and C tiles are fetched into one XCD's L2 instead of all eight;
-// when its queue is empty it takes items from the others in turn.
+// when its queue is empty it takes items from the others in turn.  TYPES:
+// the element types of the batch's items (kTypeF32 | kTypeBf16).
+template <int TYPES>
 __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
     const JobDesc *__restrict__ jobs, const int *__restrict__ itemStart, int nJobs,
     BatchQueues queues, unsigned *__restrict__ counter)
@@ -328,16 +414,7 @@ __global__ __launch_bounds__(kBatchThreads) void batch_persistent_kernel(
         const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<void *>(jd.D), (short)0, (int)jd.dBytes, 0x00020000);
         const int r0 = rb * jd.rows;
-        // one dispatch per item on the element size and width
-        switch (jd.bytes * 16 + jd.vec) {
-        case 4 * 16 + 1: run_item<4, 1>(jd, rsrc, r0, sub, j, active, streams); break;
-        case 4 * 16 + 2: run_item<4, 2>(jd, rsrc, r0, sub, j, active, streams); break;
-        case 4 * 16 + 4: run_item<4, 4>(jd, rsrc, r0, sub, j, active, streams); break;
-        case 2 * 16 + 1: run_item<2, 1>(jd, rsrc, r0, sub, j, active, streams); break;
-        case 2 * 16 + 2: run_item<2, 2>(jd, rsrc, r0, sub, j, active, streams); break;
-        case 2 * 16 + 4: run_item<2, 4>(jd, rsrc, r0, sub, j, active, streams); break;
-        default: run_item<2, 8>(jd, rsrc, r0, sub, j, active, streams); break;
-        }
+        dispatch_item<TYPES>(jd, rsrc, r0, sub, j, active, streams);
         }
     }
 }
@@ -356,8 +433,10 @@ struct gcoo_batch {
     // a run refuses to start once one moved (the table would point at
     // re-converted or freed entry arrays)
     std::vector<uint64_t> gens;
-    // resident workgroups of the persistent kernel, queried once at creation
-    int cus = 256, perCu = 4;
+    // resident workgroups per CU of each persistent kernel (by its element
+    // types, 1..3), queried once at creation
+    int cus = 256, perCu[4] = {4, 4, 4, 4};
+    int types = kTypeF32;          // element types of the items, set at commit
     BatchQueues queues{};          // per-XCD item ranges, set at commit
     JobDesc *d_jobs = nullptr;
     int *d_itemStart = nullptr;
@@ -443,8 +522,18 @@ int enqueue_run(gcoo_batch *b, hipStream_t s)
         hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
         GCOO_CHECK_LAUNCH("batch_reset_kernel");
         const int nJobs = (int)b->jobs.size();
-        hipLaunchKernelGGL(batch_persistent_kernel, dim3(b->gridBlocks), dim3(kBatchThreads), 0,
-                           s, b->d_jobs, b->d_itemStart, nJobs, b->queues, b->d_counter);
+        if (b->types == kTypeF32)
+            hipLaunchKernelGGL(batch_persistent_kernel<kTypeF32>, dim3(b->gridBlocks),
+                               dim3(kBatchThreads), 0, s, b->d_jobs, b->d_itemStart, nJobs,
+                               b->queues, b->d_counter);
+        else if (b->types == kTypeBf16)
+            hipLaunchKernelGGL(batch_persistent_kernel<kTypeBf16>, dim3(b->gridBlocks),
+                               dim3(kBatchThreads), 0, s, b->d_jobs, b->d_itemStart, nJobs,
+                               b->queues, b->d_counter);
+        else
+            hipLaunchKernelGGL(batch_persistent_kernel<kTypeF32 | kTypeBf16>, dim3(b->gridBlocks),
+                               dim3(kBatchThreads), 0, s, b->d_jobs, b->d_itemStart, nJobs,
+                               b->queues, b->d_counter);
         GCOO_CHECK_LAUNCH("batch_persistent_kernel");
     }
     return GCOO_OK;
@@ -604,11 +693,16 @@ int gcoo_batch_create(gcoo_batch_t *out)
         (void)hipGetLastError();
         b->cus = 256;
     }
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->perCu, batch_persistent_kernel,
-                                                     kBatchThreads, 0) != hipSuccess ||
-        b->perCu < 1) {
-        (void)hipGetLastError();
-        b->perCu = 4;
+    const void *kernels[4] = {nullptr, (const void *)batch_persistent_kernel<kTypeF32>,
+                              (const void *)batch_persistent_kernel<kTypeBf16>,
+                              (const void *)batch_persistent_kernel<kTypeF32 | kTypeBf16>};
+    for (int t = 1; t <= 3; ++t) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b->perCu[t], kernels[t], kBatchThreads,
+                                                         0) != hipSuccess ||
+            b->perCu[t] < 1) {
+            (void)hipGetLastError();
+            b->perCu[t] = 4;
+        }
     }
     b->itemStart.push_back(0);
     *out = b;
@@ -797,9 +891,16 @@ int gcoo::batch_commit_async(gcoo_batch_t b, gcoo_stream stream)
                                     hipMemcpyHostToDevice, s));
     }
     const long long items = b->itemStart.back();
-    // persistent: as many workgroups as can be resident at once, a multiple
-    // of the XCD count so that every XCD has the same number
-    long long blocks = (long long)b->cus * b->perCu;
+    // the element types among the jobs with items: the kernel launched
+    b->types = 0;
+    for (size_t i = 0; i < b->jobs.size(); ++i)
+        if (b->itemStart[i + 1] > b->itemStart[i])
+            b->types |= b->jobs[i].bytes == 4 ? kTypeF32 : kTypeBf16;
+    if (b->types == 0) b->types = kTypeF32;
+    // persistent: as many workgroups as can be resident at once, at most
+    // kBatchMaxPerCu per CU, a multiple of the XCD count so that every XCD
+    // has the same number
+    long long blocks = (long long)b->cus * std::min(b->perCu[b->types], kBatchMaxPerCu);
     const long long needed = (items + kBatchWaves - 1) / kBatchWaves;
     if (blocks > needed) blocks = needed;
     blocks = (blocks + kNumXcd - 1) / kNumXcd * kNumXcd;
