@@ -64,16 +64,21 @@ int ensure_capacity(gcoo_matrix *h, int64_t need)
 
 int ensure_seg16(gcoo_matrix *h, hipStream_t s)
 {
-    if (!h->seg16_valid && h->nnz > 0) {
-        if (!h->seg16) {
+    if (!h->seg16_valid) {
+        // allocated even without entries (a converted handle's capacity is
+        // never zero): the batch's BF16 row streams read entry 0 of an
+        // empty row's array, unused, so that their loads need no guard
+        if (!h->seg16 && h->cap > 0) {
             if (hipMalloc(&h->seg16, sizeof(int2) * (size_t)h->cap) != hipSuccess) {
                 (void)hipGetLastError();
                 h->seg16 = nullptr;
                 return set_error(GCOO_ERR_ALLOC, "cannot allocate the BF16 segments");
             }
         }
-        int rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, s);
-        if (rc) return rc;
+        if (h->nnz > 0) {
+            int rc = launch_seg_to_bf16(h->seg, h->seg16, h->nnz, s);
+            if (rc) return rc;
+        }
     }
     h->seg16_valid = true;
     return GCOO_OK;

@@ -52,6 +52,40 @@ def test_batch_matches_alone(dev, bf16, rows, vec):
         assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), sp
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+def test_batch_empty_rows_narrow_streams(dev, bf16):
+    """Jobs with no entries at all, and jobs whose first rows are empty, on
+    16- and 32-lane row streams (N = 16, 32): the streams' loads past a
+    row's end read entries they do not use; every C is bit-equal to the job
+    alone (+0.0 where a row has no entries)."""
+    import torch
+    print(f"batch empty rows bf16={bf16}", flush=True)
+    rng = np.random.default_rng(5)
+    b = G.Batch().set_panel_mode(0)
+    jobs = []
+    for M, K, N, kind in [(37, 50, 16, "zero"), (64, 90, 32, "zero"), (45, 70, 32, "head"),
+                          (9, 20, 16, "head"), (130, 40, 32, "all")]:
+        S = np.zeros((M, K), np.float32)
+        if kind == "head":
+            S[M // 2:] = H.rng_sparse(rng, M - M // 2, K, 0.3)
+        elif kind == "all":
+            S = H.rng_sparse(rng, M, K, 0.4)
+        D = torch.from_numpy(rng.uniform(-1, 1, (K, N)).astype(np.float32)).to(dev)
+        if bf16:
+            D = G.to_bf16(D)
+        m = G.GCOOMatrix(M, K, 4).convert_dense(torch.from_numpy(S).to(dev))
+        C = torch.full((M, N), float("nan"), device=dev)
+        b.add(m, D, C)
+        jobs.append((m, D, C, N))
+    b.commit()
+    b.run()
+    torch.cuda.synchronize()
+    for m, D, C, N in jobs:
+        ref = alone(m, D, N, bf16)
+        assert torch.equal(C.view(torch.int32), ref.view(torch.int32)), (m.M, m.K, N)
+    b.close()
+
+
 def test_batch_vs_oracle(dev):
     """A few jobs checked directly against the oracle's fmaf chain."""
     import torch
