@@ -320,10 +320,10 @@ int gcoo_batch_job_tile(gcoo_batch_t b, int job, int *rows, int *vec, int *lanes
  * element type ahead of it, its units dealt over the XCDs by estimated work.
  * Commit chooses them (mode 1, the default: where a cost model puts the panel
  * kernel's time below `ratio_f32` / `ratio_bf16` times the persistent
- * kernel's, 0 meaning the built-in 1.0 / 8.0; 0: none; 2: every job whose
+ * kernel's, 0 meaning the built-in 1.0 / 1.0; 0: none; 2: every job whose
  * operands allow it) and builds their panel lists, which the handle keeps; a
- * later multiply of
- * the same matrix with another panel shape rebuilds them, after which run,
+ * later multiply of the same matrix with another panel shape rebuilds them,
+ * after which run,
  * capture and graph launch return GCOO_ERR_STATE until the batch is
  * committed again.  A matrix in two jobs of different element types runs
  * the second on the persistent kernel.  Same bits either way. */
