@@ -194,8 +194,8 @@ __global__ __launch_bounds__(NT) void panel_wave_lists_kernel(
 // us with this kernel's.  Any order gives the same bits: each row's chain is
 // still its entries in ascending k.  One workgroup per panel;
 // rowmap[p * PR + w * rpw + q] = the row in slot q of wave w (-1: none).
-// The picks are serial (one wave, 16 per step, each a readlane scan of 16
-// costs), about 0.42 ms per build at that size: the order pays only for
+// The picks are serial (one wave, 16 per step, each four lane exchanges),
+// about 0.25 ms per build at that size: the order pays only for
 // lists that serve many multiplies (batches; gcoo_set_panel_balance).
 constexpr int kAssignThreads = 1024;
 constexpr int kAssignMaxChunks = 128;  // longer K keeps the plain order
@@ -318,8 +318,11 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
             if (lane < waves) worder[rank] = lane;
             __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the LDS writes above are done
             __builtin_amdgcn_wave_barrier();
-            // lane j holds the costs of batch row j for every wave; the picks
-            // run in scalar registers (one readlane per candidate)
+            // lane j holds the costs of batch row j for every wave.  A pick
+            // is the untaken row of least cost, the lowest row among equal
+            // costs: the least (cost << 5 | row) over lanes 0..15, found by
+            // four exchanges (a cost is at most the panel's entries plus the
+            // handicaps, below 2^24: 512 rows of at most 128 x 150 entries)
             int costs[kAssignMaxWaves];
 #pragma unroll
             for (int w = 0; w < kAssignMaxWaves; ++w) {
@@ -328,21 +331,19 @@ __global__ __launch_bounds__(kAssignThreads) void panel_assign_kernel(
 #pragma unroll
                     for (int sl = 0; sl < NT / 256; ++sl) costs[w] += part[sl][w * waves + lane];
             }
+            const int wo = lane < waves ? worder[lane] : 0;
             unsigned taken = 0;  // batch rows already picked (bit j)
             for (int t = 0; t < waves; ++t) {
-                const int w = worder[t];
+                const int w = __builtin_amdgcn_readlane(wo, t);
                 int mine = 0;
 #pragma unroll
                 for (int u = 0; u < kAssignMaxWaves; ++u)
                     if (u == w) mine = costs[u];
-                int best = INT_MAX, bj = 0;
-                for (int jj = 0; jj < waves; ++jj) {
-                    const int cj = __builtin_amdgcn_readlane(mine, jj);
-                    if (!((taken >> jj) & 1u) && cj < best) {
-                        best = cj;
-                        bj = jj;
-                    }
-                }
+                const bool cand = lane < waves && !((taken >> lane) & 1u);
+                int key = cand ? (mine << 5) | lane : INT_MAX;
+#pragma unroll
+                for (int d = 8; d >= 1; d >>= 1) key = min(key, __shfl_xor(key, d, 64));
+                const int bj = __builtin_amdgcn_readfirstlane(key) & 31;
                 taken |= 1u << bj;
                 if (lane == 0) pick[w] = bj;
             }

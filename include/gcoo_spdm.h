@@ -219,7 +219,7 @@ int gcoo_get_spdm_config(gcoo_handle h, gcoo_spdm_config *cfg);
  * at a barrier every k chunk, so a chunk lasts as long as its busiest wave;
  * the lists can deal each panel's rows to the waves so that every chunk's
  * entry counts come out even (the multiply at n = 8000, s = 0.99: 405.6 ->
- * 397.0 us), at the price of an extra kernel per list build (about 0.42 ms
+ * 397.0 us), at the price of an extra kernel per list build (about 0.25 ms
  * there), so it pays only for lists that serve many multiplies.
  * mode 0 (default): rows in order; 1: balanced when the lists are built for
  * a multiply of an already converted matrix (gcoo_spdm_*), whose lists
